@@ -1,0 +1,1009 @@
+/*
+ * fourier_oracle.c -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of the reference's direct Fourier inversion path
+ * (EMAN2 libEM "fourier" Reconstructor with the gauss_2 and trilinear
+ * inserters, its particle prep and finish()).  It is the checker the GPU
+ * library is compared against, and the CPU baseline bench.py times.  Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it;
+ * the shipped library never links or calls it.
+ *
+ * Arithmetic is written in the reference's source order and types (float
+ * where the reference uses float, double where it uses double) so that a
+ * deterministic build (-O2 -ffp-contract=off) reproduces the reference's
+ * IEEE semantics.  FFTs are computed in double (a textbook mixed-radix DFT)
+ * and rounded to float, where the reference calls FFTW in single precision;
+ * the FFT is pinned against numpy.fft and the reference's known answers.
+ *
+ * Parity pins (tests/test_oracle.py):
+ *   fast_exp / hypot tables  - closed forms + reference loop bounds (N_pix)
+ *   FFT conventions          - rt/pyem/test_emdata.py:722-760 known answers,
+ *                              numpy.fft.rfftn/irfftn
+ *   phase origin             - numpy.fft.(i)fftshift, round trip
+ *                              (rt/pyem/test_processor.py:1595-1650)
+ *   Transform / symmetry     - closed forms, orthonormality, group closure
+ *   insertion                - single-pixel known answer computed by hand,
+ *                              insert(w)+insert(-w)=0, linearity, Hermitian
+ *                              symmetry of the finished map, brute-force
+ *                              scatter in numpy on tiny boxes
+ *   finish                   - noiseless phantom reconstruction
+ *   slice agreement          - self-consistency (absqual ~ 1 for noiseless)
+ */
+#include "fourier_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <ctype.h>
+#include <stdio.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+#include <time.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+/* EMConsts (libEM/emobject.cpp:53-57) */
+static const double OR_PI = 3.141592653589793238462643383279502884197169399;
+#define OR_DEG2RAD (OR_PI / 180.0)
+#define OR_I2G ((float)(4.0 / (M_PI * M_PI)))
+/* Transform::ERR_LIMIT (libEM/transform.cpp:57) */
+static const float OR_ERR_LIMIT = 0.000001f;
+
+/* ======================================================================== */
+/* Util (libEM/util.cpp:742-800, libEM/util.h:465-471,816-819,874-880)      */
+/* ======================================================================== */
+
+static float g_fast_exp[1000];
+static int g_fast_exp_init = 0;
+
+void or_fast_exp_table(float *out)
+{
+    /* libEM/util.cpp:792-794: mem[i]=(float)exp(-i/50.0) */
+    for (int i = 0; i < 1000; i++) out[i] = (float)exp(-i / 50.0);
+}
+
+float or_fast_exp(float f)
+{
+    /* libEM/util.cpp:788-800 */
+    if (!g_fast_exp_init) {
+        or_fast_exp_table(g_fast_exp);
+        g_fast_exp_init = 1;
+    }
+    if (f > 0 || f < -19.98) return expf(f);
+    int g = (int)((double)(-f) * 50.0 + 0.5);
+    return g_fast_exp[g];
+}
+
+int or_round_f(float x)
+{
+    /* libEM/util.h:465-471 */
+    if (x < 0) return (int)(x - 0.5f);
+    return (int)(x + 0.5f);
+}
+
+int or_hypot_fast_int(int x, int y)
+{
+    /* libEM/util.cpp:764-785: (short)Util::round(hypot((float)x,(float)y)).
+     * For |x|,|y| <= 4095 sqrt(x^2+y^2) is never within rounding distance
+     * of k+1/2, so the rounding direction is unambiguous. */
+    x = abs(x);
+    y = abs(y);
+    double h = sqrt((double)x * x + (double)y * y);
+    return (int)(h + 0.5);
+}
+
+float or_hypot_fast(int x, int y)
+{
+    /* libEM/util.cpp:742-762: (float)hypot((float)x,(float)y) */
+    x = abs(x);
+    y = abs(y);
+    return (float)sqrt((double)x * x + (double)y * y);
+}
+
+static inline float hypot3sq(float x, float y, float z)
+{
+    /* libEM/util.h:816-819 */
+    return (x * x + y * y + z * z);
+}
+
+static inline int fast_floor(float x)
+{
+    /* libEM/util.h:874-880 */
+    if (x < 0) return ((int)x - 1);
+    return (int)x;
+}
+
+static inline void apply_precision(float *value, float precision)
+{
+    /* libEM/util.h:1309-1314 */
+    float c = ceilf(*value);
+    float f = (float)fast_floor(*value);
+    if (fabsf(*value - c) < precision) *value = c;
+    else if (fabsf(*value - f) < precision) *value = f;
+}
+
+/* ======================================================================== */
+/* Transform                                                                 */
+/* ======================================================================== */
+
+void or_euler_eman(double az, double alt, double phi, float m[12])
+{
+    /* libEM/transform.cpp:650-667 (eman convention, FP64 trig, cast) */
+    double azp = az * OR_DEG2RAD;
+    double altp = alt * OR_DEG2RAD;
+    double phip = phi * OR_DEG2RAD;
+    memset(m, 0, 12 * sizeof(float));
+    m[0 * 4 + 0] = (float)(cos(phip) * cos(azp) - cos(altp) * sin(azp) * sin(phip));
+    m[0 * 4 + 1] = (float)(cos(phip) * sin(azp) + cos(altp) * cos(azp) * sin(phip));
+    m[0 * 4 + 2] = (float)(sin(altp) * sin(phip));
+    m[1 * 4 + 0] = (float)(-sin(phip) * cos(azp) - cos(altp) * sin(azp) * cos(phip));
+    m[1 * 4 + 1] = (float)(-sin(phip) * sin(azp) + cos(altp) * cos(azp) * cos(phip));
+    m[1 * 4 + 2] = (float)(sin(altp) * cos(phip));
+    m[2 * 4 + 0] = (float)(sin(altp) * sin(azp));
+    m[2 * 4 + 1] = (float)(-sin(altp) * cos(azp));
+    m[2 * 4 + 2] = (float)cos(altp);
+}
+
+void or_transform_mult(const float a[12], const float b[12], float out[12])
+{
+    /* libEM/transform.cpp:1384-1395: result = M2 * M1 in FP32 */
+    float r[12];
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 4; j++) {
+            r[i * 4 + j] = a[i * 4 + 0] * b[0 * 4 + j] + a[i * 4 + 1] * b[1 * 4 + j] +
+                           a[i * 4 + 2] * b[2 * 4 + j];
+        }
+        r[i * 4 + 3] += a[i * 4 + 3];
+    }
+    memcpy(out, r, sizeof(r));
+}
+
+static float get_determinant(const float m[12])
+{
+    /* libEM/transform.cpp:1310-1322 */
+    double det2;
+    det2 = m[0] * ((double)m[5] * m[10] - (double)m[9] * m[6]);
+    det2 -= m[1] * ((double)m[4] * m[10] - (double)m[8] * m[6]);
+    det2 += m[2] * ((double)m[4] * m[9] - (double)m[8] * m[5]);
+    float det = (float)det2;
+    apply_precision(&det, OR_ERR_LIMIT);
+    return det;
+}
+
+static float get_scale(const float m[12])
+{
+    /* libEM/transform.cpp:1176-1187 */
+    float determinant = get_determinant(m);
+    if (determinant < 0) determinant *= -1;
+    float scale = powf(determinant, 1.0f / 3.0f);
+    int int_scale = (int)scale;
+    float scale_residual = scale - (float)int_scale;
+    if (scale_residual < OR_ERR_LIMIT) scale = (float)int_scale;
+    apply_precision(&scale, OR_ERR_LIMIT);
+    return scale;
+}
+
+void or_rotation_only(float m[12])
+{
+    /* insert_slice's set_scale(1.0); set_mirror(false); set_trans(0,0,0)
+     * (libEM/reconstructor.cpp:1104-1106; libEM/transform.cpp:1154-1174,
+     *  1269-1289, 1067-1075) */
+    float old_scale = get_scale(m);
+    float n_scale = 1.0f;
+    apply_precision(&n_scale, OR_ERR_LIMIT);
+    float corrected = n_scale / old_scale;
+    if (corrected != 1.0) {
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) m[i * 4 + j] *= corrected;
+    }
+    if (get_determinant(m) < 0) {
+        for (int j = 0; j < 4; j++) m[j] *= -1;
+    }
+    int mir = get_determinant(m) < 0;
+    m[3] = mir ? -0.0f : 0.0f;
+    m[7] = 0.0f;
+    m[11] = 0.0f;
+}
+
+void or_vec_times_transform(float x, float y, float z, const float m[12], float out[3])
+{
+    /* libEM/transform.h:547-553 (transpose product) */
+    out[0] = x * m[0 * 4 + 0] + y * m[1 * 4 + 0] + z * m[2 * 4 + 0];
+    out[1] = x * m[0 * 4 + 1] + y * m[1 * 4 + 1] + z * m[2 * 4 + 1];
+    out[2] = x * m[0 * 4 + 2] + y * m[1 * 4 + 2] + z * m[2 * 4 + 2];
+}
+
+/* ======================================================================== */
+/* Symmetry3D                                                                */
+/* ======================================================================== */
+
+static int parse_sym(const char *sym, char *kind, int *n)
+{
+    /* libEM/symmetry.cpp:75-142 (c<n>, d<n>) */
+    if (!sym || !sym[0]) return -1;
+    char c = (char)tolower((unsigned char)sym[0]);
+    if (c != 'c' && c != 'd') return -1;
+    if (!sym[1]) return -1;
+    for (const char *p = sym + 1; *p; p++)
+        if (!isdigit((unsigned char)*p)) return -1;
+    int v = atoi(sym + 1);
+    if (v <= 0) return -1;
+    *kind = c;
+    *n = v;
+    return 0;
+}
+
+int or_sym_nsym(const char *sym)
+{
+    /* libEM/symmetry.h:289 (C: n), :397 (D: 2n) */
+    char k;
+    int n;
+    if (parse_sym(sym, &k, &n)) return -1;
+    return k == 'c' ? n : 2 * n;
+}
+
+int or_sym_get(const char *sym, float *out)
+{
+    char k;
+    int n;
+    if (parse_sym(sym, &k, &n)) return -1;
+    if (k == 'c') {
+        /* CSym::get_sym libEM/symmetry.cpp:1407-1418 */
+        for (int i = 0; i < n; i++) {
+            float az = (float)(i % n) * 360.0f / (float)n;
+            or_euler_eman((double)az, 0.0, 0.0, out + 12 * i);
+        }
+        return n;
+    }
+    /* DSym::get_sym libEM/symmetry.cpp:1452-1470 */
+    int nsym = 2 * n;
+    for (int i = 0; i < nsym; i++) {
+        float az, alt;
+        if (i >= nsym / 2) {
+            az = (float)((i % nsym) - nsym / 2) * 360.0f / (float)(nsym / 2);
+            alt = 180.0f;
+        } else {
+            az = (float)(i % nsym) * 360.0f / (float)(nsym / 2);
+            alt = 0.0f;
+        }
+        or_euler_eman((double)az, (double)alt, 0.0, out + 12 * i);
+    }
+    return nsym;
+}
+
+/* ======================================================================== */
+/* caller prep                                                               */
+/* ======================================================================== */
+
+float or_edge_mean(const float *data, int nx, int ny)
+{
+    /* libEM/emdata_metadata.cpp:713-736 (nz == 1 branch) */
+    double edge_sum = 0;
+    size_t nxy = (size_t)nx * ny;
+    for (int i = 0, j = (ny - 1) * nx; i < nx; ++i, ++j) edge_sum += data[i] + data[j];
+    for (size_t i = 0, j = nx - 1; i < nxy; i += nx, j += nx) edge_sum += data[i] + data[j];
+    return (float)edge_sum / (nx * 2 + ny * 2);
+}
+
+void or_clip_2d(const float *in, int nx, int ny, int x0, int y0, int onx, int ony, float *out)
+{
+    /* EMData::clip_inplace with Region(x0,y0,onx,ony), fill 0 */
+    for (int y = 0; y < ony; y++)
+        for (int x = 0; x < onx; x++) {
+            int sx = x + x0, sy = y + y0;
+            out[(size_t)y * onx + x] =
+                (sx >= 0 && sx < nx && sy >= 0 && sy < ny) ? in[(size_t)sy * nx + sx] : 0.0f;
+        }
+}
+
+/* ======================================================================== */
+/* FFT: textbook mixed-radix DFT in double                                   */
+/* ======================================================================== */
+
+typedef struct { double re, im; } cplx;
+
+static int smallest_factor(int n)
+{
+    if (n % 4 == 0) return 4;
+    if (n % 2 == 0) return 2;
+    for (int p = 3; p * p <= n; p += 2)
+        if (n % p == 0) return p;
+    return n;
+}
+
+/* out[k] = sum_j in[j*istride] * exp(sign*2*pi*i*j*k/n); tw holds
+ * exp(sign*2*pi*i*j/N) for the top-level length N (n divides N). */
+static void fft_rec(const cplx *in, int istride, cplx *out, int n, int N, const cplx *tw)
+{
+    if (n == 1) {
+        out[0] = in[0];
+        return;
+    }
+    int p = smallest_factor(n);
+    int m = n / p;
+    int step = N / n;
+    for (int r = 0; r < p; r++) fft_rec(in + (size_t)r * istride, istride * p, out + (size_t)r * m, m, N, tw);
+    cplx t[64];
+    cplx *tt = p <= 64 ? t : (cplx *)malloc(sizeof(cplx) * p);
+    for (int k = 0; k < m; k++) {
+        for (int r = 0; r < p; r++) {
+            cplx y = out[(size_t)r * m + k];
+            cplx w = tw[(size_t)((long)r * k % n) * step];
+            tt[r].re = y.re * w.re - y.im * w.im;
+            tt[r].im = y.re * w.im + y.im * w.re;
+        }
+        for (int q = 0; q < p; q++) {
+            double sr = 0, si = 0;
+            for (int r = 0; r < p; r++) {
+                cplx w = tw[(size_t)((long)r * q % p) * (N / p)];
+                sr += tt[r].re * w.re - tt[r].im * w.im;
+                si += tt[r].re * w.im + tt[r].im * w.re;
+            }
+            out[(size_t)k + (size_t)q * m].re = sr;
+            out[(size_t)k + (size_t)q * m].im = si;
+        }
+    }
+    if (tt != t) free(tt);
+}
+
+static cplx *make_twiddles(int n, int sign)
+{
+    cplx *tw = (cplx *)malloc(sizeof(cplx) * n);
+    for (int j = 0; j < n; j++) {
+        double ang = sign * 2.0 * OR_PI * (double)j / n;
+        tw[j].re = cos(ang);
+        tw[j].im = sin(ang);
+    }
+    return tw;
+}
+
+/* in-place transform of n elements at stride `stride` in a */
+static void fft_strided(cplx *a, int n, size_t stride, const cplx *tw, cplx *tmp_in, cplx *tmp_out)
+{
+    for (int i = 0; i < n; i++) tmp_in[i] = a[(size_t)i * stride];
+    fft_rec(tmp_in, 1, tmp_out, n, n, tw);
+    for (int i = 0; i < n; i++) a[(size_t)i * stride] = tmp_out[i];
+}
+
+/* complex N-d transform over a full (nz,ny,nx) complex array */
+static void fft_full_nd(cplx *a, int nx, int ny, int nz, int sign, int skip_x)
+{
+    int mx = nx > ny ? nx : ny;
+    if (nz > mx) mx = nz;
+    cplx *ti = (cplx *)malloc(sizeof(cplx) * mx), *to = (cplx *)malloc(sizeof(cplx) * mx);
+    size_t nxy = (size_t)nx * ny;
+    if (!skip_x && nx > 1) {
+        cplx *tw = make_twiddles(nx, sign);
+        for (size_t r = 0; r < (size_t)ny * nz; r++) fft_strided(a + r * nx, nx, 1, tw, ti, to);
+        free(tw);
+    }
+    if (ny > 1) {
+        cplx *tw = make_twiddles(ny, sign);
+        for (int z = 0; z < nz; z++)
+            for (int x = 0; x < nx; x++) fft_strided(a + z * nxy + x, ny, nx, tw, ti, to);
+        free(tw);
+    }
+    if (nz > 1) {
+        cplx *tw = make_twiddles(nz, sign);
+        for (size_t xy = 0; xy < nxy; xy++) fft_strided(a + xy, nz, nxy, tw, ti, to);
+        free(tw);
+    }
+    free(ti);
+    free(to);
+}
+
+static void r2c_nd(const float *in, int nx, int ny, int nz, float *out)
+{
+    /* FFTW r2c convention (libEM/emfft.cpp:462-505): unnormalised forward
+     * exp(-2 pi i k.x / n); output rows of nx/2+1 complex, (nx+2) floats for
+     * even nx. */
+    size_t n = (size_t)nx * ny * nz;
+    cplx *a = (cplx *)malloc(sizeof(cplx) * n);
+    for (size_t i = 0; i < n; i++) {
+        a[i].re = in[i];
+        a[i].im = 0;
+    }
+    fft_full_nd(a, nx, ny, nz, -1, 0);
+    int hx = nx / 2 + 1;
+    int rowf = 2 * hx;
+    for (size_t r = 0; r < (size_t)ny * nz; r++)
+        for (int x = 0; x < hx; x++) {
+            out[r * rowf + 2 * x] = (float)a[r * nx + x].re;
+            out[r * rowf + 2 * x + 1] = (float)a[r * nx + x].im;
+        }
+    free(a);
+}
+
+static void c2r_nd(const float *in, int nx, int ny, int nz, float *out)
+{
+    /* unnormalised inverse (FFTW c2r, libEM/emfft.cpp:507-550): inverse
+     * complex transforms along z and y, then a Hermitian-completed inverse
+     * along x taking the real part (imaginary parts of the x=0 and x=nx/2
+     * terms are ignored, as in a c2r of the last dimension). */
+    int hx = nx / 2 + 1;
+    int rowf = 2 * hx;
+    size_t nrows = (size_t)ny * nz;
+    cplx *h = (cplx *)malloc(sizeof(cplx) * hx * nrows);
+    for (size_t r = 0; r < nrows; r++)
+        for (int x = 0; x < hx; x++) {
+            h[r * hx + x].re = in[r * rowf + 2 * x];
+            h[r * hx + x].im = in[r * rowf + 2 * x + 1];
+        }
+    fft_full_nd(h, hx, ny, nz, +1, 1);
+    cplx *row = (cplx *)malloc(sizeof(cplx) * nx), *ro = (cplx *)malloc(sizeof(cplx) * nx);
+    cplx *twx = make_twiddles(nx, +1);
+    for (size_t r = 0; r < nrows; r++) {
+        for (int x = 0; x < hx; x++) row[x] = h[r * hx + x];
+        row[0].im = 0;
+        if (nx % 2 == 0) row[nx / 2].im = 0;
+        for (int x = hx; x < nx; x++) {
+            row[x].re = h[r * hx + (nx - x)].re;
+            row[x].im = -h[r * hx + (nx - x)].im;
+        }
+        fft_rec(row, 1, ro, nx, nx, twx);
+        for (int x = 0; x < nx; x++) out[r * nx + x] = (float)ro[x].re;
+    }
+    free(twx);
+    free(row);
+    free(ro);
+    free(h);
+}
+
+void or_fft_r2c_1d(const float *in, int nx, float *out) { r2c_nd(in, nx, 1, 1, out); }
+void or_fft_r2c_2d(const float *in, int nx, int ny, float *out) { r2c_nd(in, nx, ny, 1, out); }
+void or_fft_r2c_3d(const float *in, int nx, int ny, int nz, float *out) { r2c_nd(in, nx, ny, nz, out); }
+
+void or_fft_c2r_2d(const float *in, int nx, int ny, float *out)
+{
+    /* do_ift_inplace: c2r then mult(1.0f/(nx*ny)) (libEM/emdata_transform.cpp:392-425) */
+    c2r_nd(in, nx, ny, 1, out);
+    float scale = 1.0f / (float)((size_t)nx * ny);
+    for (size_t i = 0; i < (size_t)nx * ny; i++) out[i] *= scale;
+}
+
+void or_fft_c2r_3d(const float *in, int nx, int ny, int nz, float *out)
+{
+    c2r_nd(in, nx, ny, nz, out);
+    float scale = 1.0f / (float)((size_t)nx * ny * nz);
+    for (size_t i = 0; i < (size_t)nx * ny * nz; i++) out[i] *= scale;
+}
+
+/* ======================================================================== */
+/* phase origin (libEM/processor.cpp:6730-6960)                             */
+/* tocorner == ifftshift, tocenter == fftshift on every axis                 */
+/* ======================================================================== */
+
+static void shift_nd(float *img, int nx, int ny, int nz, int to_corner)
+{
+    size_t n = (size_t)nx * ny * nz;
+    float *t = (float *)malloc(sizeof(float) * n);
+    int dims[3] = {nx, ny, nz};
+    int sh[3];
+    for (int d = 0; d < 3; d++) sh[d] = to_corner ? dims[d] / 2 : (dims[d] - dims[d] / 2);
+    for (int z = 0; z < nz; z++)
+        for (int y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++) {
+                int sx = (x + sh[0]) % nx, sy = (y + sh[1]) % ny, sz = (z + sh[2]) % nz;
+                t[((size_t)z * ny + y) * nx + x] = img[((size_t)sz * ny + sy) * nx + sx];
+            }
+    memcpy(img, t, sizeof(float) * n);
+    free(t);
+}
+
+void or_phaseorigin_tocorner(float *img, int nx, int ny, int nz) { shift_nd(img, nx, ny, nz, 1); }
+void or_phaseorigin_tocenter(float *img, int nx, int ny, int nz) { shift_nd(img, nx, ny, nz, 0); }
+
+/* ======================================================================== */
+/* CTF                                                                       */
+/* ======================================================================== */
+
+void or_ctf_constants(const or_ctf *c, int ny, float *g1, float *g2, float *acac, float *ds)
+{
+    /* lambda(): libEM/ctf.h:291-296 */
+    float lambda = 12.2639f / sqrtf(c->voltage * 1000.0f + 0.97845f * c->voltage * c->voltage);
+    /* get_phase(): libEM/ctf.cpp:671-675 */
+    float phase;
+    if (c->ampcont > -100.0 && c->ampcont <= 100.0) phase = (float)asin(c->ampcont / 100.0);
+    else if (c->ampcont > 100.0) phase = (float)(M_PI - asin(2.0f - c->ampcont / 100.0));
+    else phase = (float)(-M_PI - asin(-2.0 - c->ampcont / 100.0));
+    /* libEM/ctf.cpp:985-992 */
+    *ds = 1.0f / (c->apix * ny);
+    *g1 = (float)(M_PI / 2.0 * c->cs * 1.0e7 * powf(lambda, 3.0f));
+    *g2 = (float)(M_PI * lambda * 10000.0);
+    *acac = (float)(M_PI / 2.0 - phase);
+}
+
+void or_ctf_sign_2d(const or_ctf *c, int ny, float *d)
+{
+    /* EMAN2Ctf::compute_2d_complex(CTF_SIGN), libEM/ctf.cpp:1076-1090 */
+    int nx = ny + 2;
+    float g1, g2, acac, ds;
+    or_ctf_constants(c, ny, &g1, &g2, &acac, &ds);
+    for (int y = -ny / 2; y < ny / 2; y++) {
+        int y2 = (y + ny) % ny;
+        int ynx = y2 * nx;
+        for (int x = 0; x < nx / 2; x++) {
+            float s = (float)or_hypot_fast(x, y) * ds;
+            float gam = -g1 * s * s * s * s + g2 * c->defocus * s * s;
+            double v = cos((double)(gam - acac));
+            d[x * 2 + ynx] = v < 0 ? -1.0f : 1.0f;
+            d[x * 2 + ynx + 1] = 0;
+        }
+    }
+}
+
+void or_ctf_amp_2d(const or_ctf *c, int ny, float *d)
+{
+    /* CTF_AMP, libEM/ctf.cpp:1006-1021 (no astigmatism) */
+    int nx = ny + 2;
+    float g1, g2, acac, ds;
+    or_ctf_constants(c, ny, &g1, &g2, &acac, &ds);
+    for (int y = -ny / 2; y < ny / 2; y++) {
+        int y2 = (y + ny) % ny;
+        int ynx = y2 * nx;
+        for (int x = 0; x < nx / 2; x++) {
+            float s = (float)or_hypot_fast(x, y) * ds;
+            float gam = -g1 * s * s * s * s + g2 * c->defocus * s * s;
+            float v = (float)(cos((double)(gam - acac)) * exp(-(c->bfactor / 4.0f * s * s)));
+            d[x * 2 + ynx] = v;
+            d[x * 2 + ynx + 1] = 0;
+        }
+    }
+}
+
+/* ======================================================================== */
+/* preprocess                                                                */
+/* ======================================================================== */
+
+static float slice_scale(int P)
+{
+    /* libEM/reconstructor.cpp:1054: sqrt(1.0f/ny * nx) with nx = P+2 */
+    float t = 1.0f / (float)P * (float)(P + 2);
+    return (float)sqrt((double)t);
+}
+
+void or_preprocess_slice(const float *img, int P, float *out)
+{
+    /* FourierReconstructor::preprocess_slice (libEM/reconstructor.cpp:1022-1061)
+     * for an identity 2-D transform: tocorner, r2c, scale. */
+    float *t = (float *)malloc(sizeof(float) * P * P);
+    memcpy(t, img, sizeof(float) * P * P);
+    or_phaseorigin_tocorner(t, P, P, 1);
+    or_fft_r2c_2d(t, P, P, out);
+    float sc = slice_scale(P);
+    for (size_t i = 0; i < (size_t)(P + 2) * P; i++) out[i] *= sc;
+    free(t);
+}
+
+static int floordiv2(int a)
+{
+    /* python old_div(a,2) for ints == floor division */
+    return a >= 0 ? a / 2 : -((-a + 1) / 2);
+}
+
+void or_prep_particle(const float *raw, int B, int P, const or_ctf *ctf, float *out)
+{
+    /* programs/e2make3dpar.py:630-654 (edge-mean subtract, clip/pad),
+     * :704-714 (CTF_SIGN flip: fft, mult, ift), :716 preprocess_slice */
+    float *img = (float *)malloc(sizeof(float) * B * B);
+    memcpy(img, raw, sizeof(float) * B * B);
+    float em = or_edge_mean(img, B, B);
+    for (int i = 0; i < B * B; i++) img[i] -= em;
+    float *pad = (float *)malloc(sizeof(float) * P * P);
+    or_clip_2d(img, B, B, floordiv2(B - P), floordiv2(B - P), P, P, pad);
+    if (ctf) {
+        float *F = (float *)malloc(sizeof(float) * (P + 2) * P);
+        float *sg = (float *)malloc(sizeof(float) * (P + 2) * P);
+        or_fft_r2c_2d(pad, P, P, F);
+        or_ctf_sign_2d(ctf, P, sg);
+        for (size_t i = 0; i < (size_t)(P + 2) * P; i += 2) {
+            /* complex multiply by (s + 0i): (a*s - b*0, a*0 + b*s) */
+            float a = F[i], b = F[i + 1], s = sg[i];
+            F[i] = a * s - b * 0.0f;
+            F[i + 1] = a * 0.0f + b * s;
+        }
+        or_fft_c2r_2d(F, P, P, pad);
+        free(F);
+        free(sg);
+    }
+    or_preprocess_slice(pad, P, out);
+    free(pad);
+    free(img);
+}
+
+/* ======================================================================== */
+/* insertion                                                                 */
+/* ======================================================================== */
+
+/* EMData::get_complex_at(x,y) on a 2-D slice (libEM/emdata_core.cpp:126-132) */
+static inline void slice_get(const float *s, int P, int x, int y, float *re, float *im)
+{
+    int nx = P + 2, ny = P;
+    if (abs(x) >= nx / 2 || abs(y) > ny / 2) {
+        *re = 0;
+        *im = 0;
+        return;
+    }
+    if (x >= 0 && y >= 0) {
+        *re = s[x * 2 + y * nx];
+        *im = s[x * 2 + y * nx + 1];
+        return;
+    }
+    if (x > 0 && y < 0) {
+        *re = s[x * 2 + (ny + y) * nx];
+        *im = s[x * 2 + (ny + y) * nx + 1];
+        return;
+    }
+    if (x < 0 && y > 0) {
+        *re = s[-x * 2 + (ny - y) * nx];
+        *im = -s[-x * 2 + (ny - y) * nx + 1];
+        return;
+    }
+    *re = s[-x * 2 - y * nx];
+    *im = -s[-x * 2 - y * nx + 1];
+}
+
+#define DEFINE_INSERT(SUFFIX, ACC)                                                                 \
+    /* EMData::add_complex_at_fast (libEM/emdata_core.h:368-413); returns float index or nxyz */   \
+    static inline size_t add_complex_at_fast##SUFFIX(ACC *rdata, int P, int x, int y, int z,       \
+                                                     float vr, float vi)                           \
+    {                                                                                              \
+        const int nx = P + 2, ny = P, nz = P;                                                      \
+        const size_t nxy = (size_t)nx * ny;                                                        \
+        const size_t nxyz = nxy * nz;                                                              \
+        if (abs(x) >= nx / 2 || abs(y) > ny / 2 || abs(z) > nz / 2) return nxyz;                  \
+        size_t idx;                                                                                \
+        if (x == 0) {                                                                              \
+            if (y == 0 && z == 0) {                                                                \
+                rdata[0] += (ACC)vr;                                                               \
+                rdata[1] = 0;                                                                      \
+                return 0;                                                                          \
+            }                                                                                      \
+            size_t i2 = (y <= 0 ? -y : ny - y) * (size_t)nx + (z <= 0 ? -z : nz - z) * nxy;        \
+            rdata[i2] += (ACC)vr;                                                                  \
+            rdata[i2 + 1] += (ACC)(-vi);                                                           \
+        }                                                                                          \
+        if (abs(x) == nx / 2 - 1) {                                                                \
+            if (y == 0 && z == 0) {                                                                \
+                rdata[nx - 2] += (ACC)vr;                                                          \
+                rdata[nx - 1] = 0;                                                                 \
+                return nx - 2;                                                                     \
+            }                                                                                      \
+            size_t i2 = nx - 2 + (y <= 0 ? -y : ny - y) * (size_t)nx + (z <= 0 ? -z : nz - z) * nxy; \
+            rdata[i2] += (ACC)vr;                                                                  \
+            rdata[i2 + 1] += (ACC)(-vi);                                                           \
+        }                                                                                          \
+        if (x < 0) {                                                                               \
+            idx = -x * 2 + (y <= 0 ? -y : ny - y) * (size_t)nx + (z <= 0 ? -z : nz - z) * nxy;     \
+            rdata[idx] += (ACC)vr;                                                                 \
+            rdata[idx + 1] += (ACC)(-vi);                                                          \
+            return idx;                                                                            \
+        }                                                                                          \
+        idx = x * 2 + (y < 0 ? ny + y : y) * (size_t)nx + (z < 0 ? nz + z : z) * nxy;             \
+        rdata[idx] += (ACC)vr;                                                                     \
+        rdata[idx + 1] += (ACC)vi;                                                                 \
+        return idx;                                                                                \
+    }                                                                                              \
+                                                                                                   \
+    /* FourierInserter3DMode2/Mode2l::insert_pixel (libEM/reconstructor_tools.cpp:128-167,191-239) */ \
+    static inline void insert_pixel##SUFFIX(ACC *data, ACC *norm, int P, int mode, float xx,       \
+                                            float yy, float zz, float dr, float di, float weight)  \
+    {                                                                                              \
+        const int nx2 = P / 2, ny2 = P / 2, nz2 = P / 2;                                           \
+        const size_t nxyz = (size_t)(P + 2) * P * P;                                               \
+        int x0 = (int)floorf(xx);                                                                  \
+        int y0 = (int)floorf(yy);                                                                  \
+        int z0 = (int)floorf(zz);                                                                  \
+        if (x0 < -nx2 - 1 || y0 < -ny2 - 1 || z0 < -nz2 - 1 || x0 > nx2 || y0 > ny2 || z0 > nz2)  \
+            return;                                                                                \
+        int x1 = x0 + 1, y1 = y0 + 1, z1 = z0 + 1;                                                 \
+        float h = 1.0f / OR_I2G;                                                                   \
+        for (int k = z0; k <= z1; k++)                                                             \
+            for (int j = y0; j <= y1; j++)                                                         \
+                for (int i = x0; i <= x1; i++) {                                                   \
+                    float gg;                                                                      \
+                    if (mode == OR_MODE_GAUSS2) {                                                  \
+                        float r = hypot3sq((float)i - xx, (float)j - yy, (float)k - zz);           \
+                        gg = or_fast_exp(-r * h) * weight;                                         \
+                    } else {                                                                       \
+                        gg = (float)((1.0 - (double)fabsf((float)i - xx)) *                        \
+                                     (1.0 - (double)fabsf((float)j - yy)) *                        \
+                                     (1.0 - (double)fabsf((float)k - zz)) * (double)weight);       \
+                    }                                                                              \
+                    size_t off = add_complex_at_fast##SUFFIX(data, P, i, j, k, dr * gg, di * gg);  \
+                    if (off != nxyz) norm[off / 2] += (ACC)gg;                                     \
+                }                                                                                  \
+    }                                                                                              \
+                                                                                                   \
+    /* FourierReconstructor::do_insert_slice_work (libEM/reconstructor.cpp:1122-1210) */          \
+    static void insert_work##SUFFIX(int P, ACC *data, ACC *norm, const float *slice,               \
+                                    const float rot[12], const float *syms, int nsym,              \
+                                    float weight, int mode, int corners)                           \
+    {                                                                                              \
+        const float inx = (float)(P + 2), iny = (float)P;                                          \
+        const int nx = P + 2, ny = P, nz = P;                                                      \
+        for (int s = 0; s < nsym; s++) {                                                           \
+            float t3d[12];                                                                         \
+            or_transform_mult(rot, syms + 12 * s, t3d);                                            \
+            for (int y = (int)(-iny / 2); y < iny / 2; y++) {                                      \
+                for (int x = 0; x < inx / 2; x++) {                                                \
+                    float rx = (float)x / (inx - 2.0f);                                            \
+                    float ry = (float)y / iny;                                                     \
+                    if (!corners) {                                                                \
+                        int r = or_hypot_fast_int(x, y);                                           \
+                        if (r > iny / 2 && fabsf(inx - iny) < 3) continue;                         \
+                    }                                                                              \
+                    float c[3];                                                                    \
+                    or_vec_times_transform(rx, ry, 0.0f, t3d, c);                                  \
+                    float xx = c[0] * (float)(nx - 2);                                             \
+                    float yy = c[1] * (float)ny;                                                   \
+                    float zz = c[2] * (float)nz;                                                   \
+                    float dr, di;                                                                  \
+                    slice_get(slice, P, x, y, &dr, &di);                                           \
+                    insert_pixel##SUFFIX(data, norm, P, mode, xx, yy, zz, dr, di, weight);         \
+                }                                                                                  \
+            }                                                                                      \
+        }                                                                                          \
+    }
+
+DEFINE_INSERT(_f32, float)
+DEFINE_INSERT(_f64, double)
+
+void or_insert_slice_work(int P, float *data, float *norm, const float *slice, const float rot[12],
+                          const float *syms, int nsym, float weight, int mode, int corners)
+{
+    /* FourierReconstructor::insert_slice: weight==0 -> skipped (libEM/reconstructor.cpp:1088) */
+    if (weight == 0) return;
+    float r[12];
+    memcpy(r, rot, sizeof(r));
+    or_rotation_only(r);
+    insert_work_f32(P, data, norm, slice, r, syms, nsym, weight, mode, corners);
+}
+
+void or_insert_slice_work_f64(int P, double *data, double *norm, const float *slice,
+                              const float rot[12], const float *syms, int nsym, float weight,
+                              int mode, int corners)
+{
+    if (weight == 0) return;
+    float r[12];
+    memcpy(r, rot, sizeof(r));
+    or_rotation_only(r);
+    insert_work_f64(P, data, norm, slice, r, syms, nsym, weight, mode, corners);
+}
+
+static double now_s(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+double or_insert_batch_mt(int P, float *data, float *norm, const float *slices, const float *rots,
+                          const float *weights, int n, const float *syms, int nsym, int mode,
+                          int nthreads)
+{
+    /* private volumes per worker, summed at the end
+     * (programs/e2spa_make3d.py:175-192, programs/e2make3dpar.py:331-350) */
+    size_t nd = (size_t)(P + 2) * P * P, nn = (size_t)(P / 2 + 1) * P * P;
+    size_t sl = (size_t)(P + 2) * P;
+    if (nthreads < 1) nthreads = 1;
+    (void)or_fast_exp(-1.0f); /* build the table before threads start */
+    double t0 = now_s();
+    float **pd = (float **)calloc(nthreads, sizeof(float *));
+    float **pn = (float **)calloc(nthreads, sizeof(float *));
+#pragma omp parallel num_threads(nthreads)
+    {
+#ifdef _OPENMP
+        int t = omp_get_thread_num();
+#else
+        int t = 0;
+#endif
+        pd[t] = (float *)calloc(nd, sizeof(float));
+        pn[t] = (float *)calloc(nn, sizeof(float));
+        for (int i = t; i < n; i += nthreads) {
+            float r[12];
+            memcpy(r, rots + 12 * (size_t)i, sizeof(r));
+            if (weights[i] == 0) continue;
+            or_rotation_only(r);
+            insert_work_f32(P, pd[t], pn[t], slices + sl * i, r, syms, nsym, weights[i], mode, 0);
+        }
+    }
+    for (int t = 0; t < nthreads; t++) {
+        for (size_t i = 0; i < nd; i++) data[i] += pd[t][i];
+        for (size_t i = 0; i < nn; i++) norm[i] += pn[t][i];
+        free(pd[t]);
+        free(pn[t]);
+    }
+    free(pd);
+    free(pn);
+    return now_s() - t0;
+}
+
+/* ======================================================================== */
+/* finish                                                                    */
+/* ======================================================================== */
+
+void or_normalize_threed(int P, float *rdata, float *norm, int sqrtnorm)
+{
+    /* ReconstructorVolumeData::normalize_threed (libEM/reconstructor.cpp:315-388) */
+    const int nx = P + 2, ny = P, nz = P;
+    const size_t nnx = (size_t)(P / 2 + 1);
+    const size_t nnxy = (size_t)ny * nnx;
+    for (int z = -nz / 2; z < nz / 2; z++) {
+        for (int y = 0; y <= ny / 2; y++) {
+            if (z <= 0 && (y == 0 || y == ny / 2)) continue;
+            size_t idx1 = (y == 0 ? 0 : ny - y) * nnx + (z <= 0 ? -z : nz - z) * nnxy;
+            size_t idx2 = y * nnx + (z < 0 ? nz + z : z) * nnxy;
+            if (idx1 == idx2) continue;
+            float snorm = norm[idx1] + norm[idx2];
+            norm[idx1] = norm[idx2] = snorm;
+            idx1 += nnx - 1;
+            idx2 += nnx - 1;
+            snorm = norm[idx1] + norm[idx2];
+            norm[idx1] = norm[idx2] = snorm;
+        }
+    }
+    norm[0 + 0 * nnx + nz / 2 * nnxy] *= 2;
+    norm[0 + ny / 2 * nnx + 0 * nnxy] *= 2;
+    norm[0 + ny / 2 * nnx + nz / 2 * nnxy] *= 2;
+    norm[nx / 2 - 1 + 0 * nnx + nz / 2 * nnxy] *= 2;
+    norm[nx / 2 - 1 + ny / 2 * nnx + 0 * nnxy] *= 2;
+    float wfilt = 0.0f;
+    size_t tot = (size_t)nx * ny * nz;
+    for (size_t i = 0; i < tot; i += 2) {
+        float d = norm[i / 2];
+        if (sqrtnorm) d = sqrtf(d);
+        if (d == 0) {
+            rdata[i] = 0;
+            rdata[i + 1] = 0;
+        } else {
+            rdata[i] /= d + wfilt;
+            rdata[i + 1] /= d + wfilt;
+        }
+    }
+}
+
+void or_finish(int P, float *data, float *norm, int sqrtnorm, int doift, float *out)
+{
+    /* FourierReconstructor::finish (libEM/reconstructor.cpp:1552-1641) */
+    or_normalize_threed(P, data, norm, sqrtnorm);
+    if (!doift) return;
+    /* do_ift_inplace (c2r, x1/N), depad, xform.phaseorigin.tocenter */
+    or_fft_c2r_3d(data, P, P, P, out);
+    or_phaseorigin_tocenter(out, P, P, P);
+}
+
+/* ======================================================================== */
+/* slice agreement                                                           */
+/* ======================================================================== */
+
+/* EMData::get_complex_at(x,y,z) (libEM/emdata_core.cpp:135-144) */
+static inline void vol_get(const float *r, int P, int x, int y, int z, float *re, float *im)
+{
+    const int nx = P + 2, ny = P, nz = P;
+    const size_t nxy = (size_t)nx * ny;
+    if (abs(x) >= nx / 2 || abs(y) > ny / 2 || abs(z) > nz / 2) {
+        *re = 0;
+        *im = 0;
+        return;
+    }
+    if (x < 0) {
+        size_t idx = -x * 2 + (y <= 0 ? -y : ny - y) * (size_t)nx + (z <= 0 ? -z : nz - z) * nxy;
+        *re = r[idx];
+        *im = -r[idx + 1];
+        return;
+    }
+    size_t idx = x * 2 + (y < 0 ? ny + y : y) * (size_t)nx + (z < 0 ? nz + z : z) * nxy;
+    *re = r[idx];
+    *im = r[idx + 1];
+}
+
+/* get_complex_index_fast (libEM/emdata_core.h:302-308) */
+static inline size_t vol_index_fast(int P, int x, int y, int z)
+{
+    const int nx = P + 2, ny = P, nz = P;
+    const size_t nxy = (size_t)nx * ny;
+    if (x < 0) return (size_t)x * -2 + (y <= 0 ? -y : ny - y) * (size_t)nx + (z <= 0 ? -z : nz - z) * nxy;
+    return x * 2 + (y < 0 ? ny + y : y) * (size_t)nx + (z < 0 ? nz + z : z) * nxy;
+}
+
+void or_determine_slice_agreement(int P, float *data, float *norm, float *slice, const float rot_in[12],
+                                  const float *syms, int nsym, float weight, int sub, float apix,
+                                  float out[5])
+{
+    /* libEM/reconstructor.cpp:1212-1257 */
+    float rot[12];
+    memcpy(rot, rot_in, sizeof(rot));
+    or_rotation_only(rot);
+    if (sub) insert_work_f32(P, data, norm, slice, rot, syms, nsym, -weight, OR_MODE_GAUSS2, 0);
+
+    /* do_compare_slice_work, libEM/reconstructor.cpp:1259-1381 */
+    const float inx = (float)(P + 2), iny = (float)P;
+    const int nx = P + 2, ny = P, nz = P;
+    const size_t slen = (size_t)(P + 2) * P;
+    float rmax = (float)(iny * apix / 12.0);
+    if (rmax > iny / 2.0 - 1.0) rmax = (float)(iny / 2.0 - 1.0);
+    if (rmax < 5.0) rmax = 5.0f;
+    double dotlow = 0, dlpow = 0, dlpow2 = 0, dot = 0, vweight = 0, power = 0, power2 = 0;
+    int nval = 0;
+    for (int s = 0; s < nsym; s++) {
+        float t3d[12];
+        or_transform_mult(rot, syms + 12 * s, t3d);
+        for (int y = (int)(-iny / 2); y < iny / 2; y++) {
+            for (int x = 0; x <= inx / 2; x++) {
+                if (x == 0 && y == 0) continue;
+                float rx = (float)x / (inx - 2);
+                float ry = (float)y / iny;
+                float c[3];
+                or_vec_times_transform(rx, ry, 0.0f, t3d, c);
+                float xx = c[0], yy = c[1], zz = c[2];
+                if (fabsf(xx) > 0.5 || fabsf(yy) >= 0.5 || fabsf(zz) >= 0.5) continue;
+                xx = xx * (float)(nx - 2);
+                yy = yy * (float)ny;
+                zz = zz * (float)nz;
+                int idx = (int)((float)(x * 2) + inx * (float)(y < 0 ? iny + y : y));
+                /* x = P/2+1 reads one complex past the row end
+                 * (libEM/reconstructor.cpp:1312,1336); past the end of the
+                 * whole slice the reference reads beyond its buffer, which is
+                 * taken as 0 here. */
+                float dt2_0 = ((size_t)idx < slen) ? slice[idx] : 0.0f;
+                float dt2_1 = ((size_t)idx + 1 < slen) ? slice[idx + 1] : 0.0f;
+                /* pixel_at: libEM/reconstructor.cpp:1448-1461 */
+                int x0 = or_round_f(xx), y0 = or_round_f(yy), z0 = or_round_f(zz);
+                float vr, vi;
+                vol_get(data, P, x0, y0, z0, &vr, &vi);
+                size_t nidx = vol_index_fast(P, x0, y0, z0) / 2;
+                float nv = norm[nidx];
+                float dt0 = vr / nv, dt1 = vi / nv, dt2 = nv;
+                if (dt2 == 0) continue;
+                float r = (float)or_hypot_fast(x, y);
+                if (r > 3 && r < rmax) {
+                    dotlow += (dt0 * dt2_0 + dt1 * dt2_1) * dt2;
+                    dlpow += (dt0 * dt0 + dt1 * dt1) * dt2;
+                    dlpow2 += (dt2_0 * dt2_0 + dt2_1 * dt2_1) * dt2;
+                }
+                vweight += dt2;
+                dot += (dt0 * dt2_0 + dt1 * dt2_1) * dt2;
+                power += (dt0 * dt0 + dt1 * dt1) * dt2;
+                power2 += (dt2_0 * dt2_0 + dt2_1 * dt2_1) * dt2;
+                nval++;
+            }
+        }
+    }
+    dot /= sqrt(power * power2);
+    if (dlpow * dlpow2 > 0) dotlow /= sqrt(dlpow * dlpow2);
+    out[0] = (float)(power2 <= 0 ? 1.0 : sqrt(power / power2));
+    out[1] = (float)dot;
+    out[2] = (float)dotlow;
+    float rw = weight <= 0 ? 1.0f : 1.0f / weight;
+    out[3] = (float)(dot * rw / ((rw - 1.0) * dot + 1.0));
+    out[4] = (float)(vweight / (float)(nval));
+
+    if (sub) insert_work_f32(P, data, norm, slice, rot, syms, nsym, weight, OR_MODE_GAUSS2, 0);
+}
+
+/* ======================================================================== */
+/* seed                                                                      */
+/* ======================================================================== */
+
+void or_setup_seed(int P, float *data, float *norm, const float *seed, float seed_weight)
+{
+    /* FourierReconstructor::setup_seed (libEM/reconstructor.cpp:820-918) */
+    size_t nd = (size_t)(P + 2) * P * P;
+    const int nnx = P / 2 + 1, ny = P, nz = P;
+    for (size_t i = 0; i < nd; i++) data[i] = seed[i] * seed_weight;
+    for (size_t i = 0; i < (size_t)nnx * ny * nz; i++) norm[i] = seed_weight;
+#define NV(x, y, z) norm[(x) + (size_t)(y) * nnx + (size_t)(z) * nnx * ny]
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++) NV(0, j, k) = (float)(seed_weight / 2.0);
+    NV(0, ny / 2, 0) = seed_weight;
+    NV(0, 0, nz / 2) = seed_weight;
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++) NV(nnx - 1, j, k) = (float)(seed_weight / 2.0);
+    NV(nnx - 1, ny / 2, 0) = seed_weight;
+    NV(nnx - 1, 0, nz / 2) = seed_weight;
+#undef NV
+}

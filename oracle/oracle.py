@@ -1,0 +1,333 @@
+"""ctypes front end of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+The oracle is a C restatement of EMAN2's "fourier" Reconstructor path
+(libEM/reconstructor.cpp, libEM/reconstructor_tools.cpp, libEM/emdata_core.h,
+libEM/util.cpp, libEM/transform.cpp, libEM/symmetry.cpp, libEM/ctf.cpp,
+libEM/emdata_transform.cpp, libEM/processor.cpp).  Only tests/, the smoke
+check and bench.py's cpu_baseline leg import this module; the shipped
+library never does.
+
+Array conventions follow the reference:
+  slice  : float32 (P, P+2)  rows in wrap order, interleaved re/im
+  data   : float32 (P, P, P+2)  [z][y][x re/im]
+  norm   : float32 (P, P, P//2+1)
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBS = {}
+
+GAUSS2 = 0
+TRILINEAR = 1
+MODES = {"gauss_2": GAUSS2, "trilinear": TRILINEAR}
+
+_fp = ctypes.POINTER(ctypes.c_float)
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class CtfStruct(ctypes.Structure):
+    _fields_ = [("defocus", ctypes.c_float), ("bfactor", ctypes.c_float),
+                ("voltage", ctypes.c_float), ("cs", ctypes.c_float),
+                ("ampcont", ctypes.c_float), ("apix", ctypes.c_float)]
+
+
+def build():
+    """Compile both oracle flavours (deterministic and -O3 -ffast-math)."""
+    subprocess.check_call(["make", "-s", "-C", _HERE], stdout=subprocess.DEVNULL)
+
+
+def lib(fast=False):
+    name = "liboracle_fast.so" if fast else "liboracle.so"
+    if name in _LIBS:
+        return _LIBS[name]
+    path = os.path.join(_HERE, name)
+    if not os.path.exists(path):
+        build()
+    L = ctypes.CDLL(path)
+    i, f, d, v = ctypes.c_int, ctypes.c_float, ctypes.c_double, None
+    sig = {
+        "or_fast_exp_table": (v, [_fp]),
+        "or_fast_exp": (f, [f]),
+        "or_hypot_fast_int": (i, [i, i]),
+        "or_hypot_fast": (f, [i, i]),
+        "or_round_f": (i, [f]),
+        "or_euler_eman": (v, [d, d, d, _fp]),
+        "or_transform_mult": (v, [_fp, _fp, _fp]),
+        "or_rotation_only": (v, [_fp]),
+        "or_sym_nsym": (i, [ctypes.c_char_p]),
+        "or_sym_get": (i, [ctypes.c_char_p, _fp]),
+        "or_edge_mean": (f, [_fp, i, i]),
+        "or_clip_2d": (v, [_fp, i, i, i, i, i, i, _fp]),
+        "or_fft_r2c_1d": (v, [_fp, i, _fp]),
+        "or_fft_r2c_2d": (v, [_fp, i, i, _fp]),
+        "or_fft_c2r_2d": (v, [_fp, i, i, _fp]),
+        "or_fft_r2c_3d": (v, [_fp, i, i, i, _fp]),
+        "or_fft_c2r_3d": (v, [_fp, i, i, i, _fp]),
+        "or_phaseorigin_tocorner": (v, [_fp, i, i, i]),
+        "or_phaseorigin_tocenter": (v, [_fp, i, i, i]),
+        "or_ctf_constants": (v, [ctypes.POINTER(CtfStruct), i, _fp, _fp, _fp, _fp]),
+        "or_ctf_sign_2d": (v, [ctypes.POINTER(CtfStruct), i, _fp]),
+        "or_ctf_amp_2d": (v, [ctypes.POINTER(CtfStruct), i, _fp]),
+        "or_preprocess_slice": (v, [_fp, i, _fp]),
+        "or_prep_particle": (v, [_fp, i, i, ctypes.POINTER(CtfStruct), _fp]),
+        "or_insert_slice_work": (v, [i, _fp, _fp, _fp, _fp, _fp, i, f, i, i]),
+        "or_insert_slice_work_f64": (v, [i, _dp, _dp, _fp, _fp, _fp, i, f, i, i]),
+        "or_insert_batch_mt": (d, [i, _fp, _fp, _fp, _fp, _fp, i, _fp, i, i, i]),
+        "or_normalize_threed": (v, [i, _fp, _fp, i]),
+        "or_finish": (v, [i, _fp, _fp, i, i, _fp]),
+        "or_determine_slice_agreement": (v, [i, _fp, _fp, _fp, _fp, _fp, i, f, i, f, _fp]),
+        "or_setup_seed": (v, [i, _fp, _fp, _fp, f]),
+    }
+    for k, (r, a) in sig.items():
+        fn = getattr(L, k)
+        fn.restype = r
+        fn.argtypes = a
+    _LIBS[name] = L
+    return L
+
+
+def _f(a):
+    assert a.dtype == np.float32 and a.flags.c_contiguous, (a.dtype, a.flags)
+    return a.ctypes.data_as(_fp)
+
+
+def _d(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+# ---------------------------------------------------------------- Util
+def fast_exp_table():
+    out = np.zeros(1000, np.float32)
+    lib().or_fast_exp_table(_f(out))
+    return out
+
+
+def fast_exp(v):
+    return lib().or_fast_exp(float(v))
+
+
+def hypot_fast_int(x, y):
+    return lib().or_hypot_fast_int(int(x), int(y))
+
+
+def hypot_fast(x, y):
+    return lib().or_hypot_fast(int(x), int(y))
+
+
+# ---------------------------------------------------------------- Transform
+def euler_eman(az, alt, phi):
+    m = np.zeros(12, np.float32)
+    lib().or_euler_eman(float(az), float(alt), float(phi), _f(m))
+    return m.reshape(3, 4)
+
+
+def transform_mult(a, b):
+    a, b = f32(a).reshape(12), f32(b).reshape(12)
+    out = np.zeros(12, np.float32)
+    lib().or_transform_mult(_f(a), _f(b), _f(out))
+    return out.reshape(3, 4)
+
+
+def rotation_only(m):
+    m = f32(m).reshape(12).copy()
+    lib().or_rotation_only(_f(m))
+    return m.reshape(3, 4)
+
+
+def sym_get(sym):
+    n = lib().or_sym_nsym(sym.encode())
+    if n <= 0:
+        raise ValueError("unsupported symmetry %r" % sym)
+    out = np.zeros(n * 12, np.float32)
+    lib().or_sym_get(sym.encode(), _f(out))
+    return out.reshape(n, 3, 4)
+
+
+# ---------------------------------------------------------------- prep / FFT
+def edge_mean(img):
+    img = f32(img)
+    ny, nx = img.shape
+    return lib().or_edge_mean(_f(img), nx, ny)
+
+
+def fft_r2c_1d(v):
+    v = f32(v)
+    n = v.shape[0]
+    out = np.zeros(2 * (n // 2 + 1), np.float32)
+    lib().or_fft_r2c_1d(_f(v), n, _f(out))
+    return out
+
+
+def fft_r2c_2d(img):
+    img = f32(img)
+    ny, nx = img.shape
+    out = np.zeros((ny, 2 * (nx // 2 + 1)), np.float32)
+    lib().or_fft_r2c_2d(_f(img), nx, ny, _f(out))
+    return out
+
+
+def fft_c2r_2d(F, nx):
+    F = f32(F)
+    ny = F.shape[0]
+    out = np.zeros((ny, nx), np.float32)
+    lib().or_fft_c2r_2d(_f(F), nx, ny, _f(out))
+    return out
+
+
+def fft_r2c_3d(vol):
+    vol = f32(vol)
+    nz, ny, nx = vol.shape
+    out = np.zeros((nz, ny, 2 * (nx // 2 + 1)), np.float32)
+    lib().or_fft_r2c_3d(_f(vol), nx, ny, nz, _f(out))
+    return out
+
+
+def fft_c2r_3d(F, nx):
+    F = f32(F)
+    nz, ny = F.shape[:2]
+    out = np.zeros((nz, ny, nx), np.float32)
+    lib().or_fft_c2r_3d(_f(F), nx, ny, nz, _f(out))
+    return out
+
+
+def phaseorigin_tocorner(img):
+    img = f32(img).copy()
+    if img.ndim == 3:
+        nz, ny, nx = img.shape
+    elif img.ndim == 2:
+        nz, (ny, nx) = 1, img.shape
+    else:
+        nz, ny, nx = 1, 1, img.shape[0]
+    lib().or_phaseorigin_tocorner(_f(img), nx, ny, nz)
+    return img
+
+
+def phaseorigin_tocenter(img):
+    img = f32(img).copy()
+    if img.ndim == 3:
+        nz, ny, nx = img.shape
+    elif img.ndim == 2:
+        nz, (ny, nx) = 1, img.shape
+    else:
+        nz, ny, nx = 1, 1, img.shape[0]
+    lib().or_phaseorigin_tocenter(_f(img), nx, ny, nz)
+    return img
+
+
+def ctf_struct(ctf):
+    return CtfStruct(float(ctf["defocus"]), float(ctf.get("bfactor", 0.0)),
+                     float(ctf["voltage"]), float(ctf["cs"]),
+                     float(ctf["ampcont"]), float(ctf["apix"]))
+
+
+def ctf_constants(ctf, ny):
+    c = ctf_struct(ctf)
+    vals = [ctypes.c_float() for _ in range(4)]
+    lib().or_ctf_constants(ctypes.byref(c), ny, *[ctypes.byref(v) for v in vals])
+    return tuple(v.value for v in vals)  # g1, g2, acac, ds
+
+
+def ctf_sign_2d(ctf, P):
+    out = np.zeros((P, P + 2), np.float32)
+    c = ctf_struct(ctf)
+    lib().or_ctf_sign_2d(ctypes.byref(c), P, _f(out))
+    return out
+
+
+def ctf_amp_2d(ctf, P):
+    out = np.zeros((P, P + 2), np.float32)
+    c = ctf_struct(ctf)
+    lib().or_ctf_amp_2d(ctypes.byref(c), P, _f(out))
+    return out
+
+
+def preprocess_slice(img):
+    img = f32(img)
+    P = img.shape[0]
+    out = np.zeros((P, P + 2), np.float32)
+    lib().or_preprocess_slice(_f(img), P, _f(out))
+    return out
+
+
+def prep_particle(raw, P, ctf=None):
+    raw = f32(raw)
+    B = raw.shape[0]
+    out = np.zeros((P, P + 2), np.float32)
+    cp = ctypes.byref(ctf_struct(ctf)) if ctf is not None else None
+    lib().or_prep_particle(_f(raw), B, P, cp, _f(out))
+    return out
+
+
+# ---------------------------------------------------------------- volume
+class Volume:
+    """Reference-layout reconstruction state (image + tmp_data)."""
+
+    def __init__(self, P, sym="c1", mode="gauss_2", fp64=False):
+        assert P % 2 == 0
+        self.P = P
+        self.sym = sym
+        self.syms = f32(sym_get(sym))
+        self.mode = MODES[mode]
+        self.fp64 = fp64
+        dt = np.float64 if fp64 else np.float32
+        self.data = np.zeros((P, P, P + 2), dt)
+        self.norm = np.zeros((P, P, P // 2 + 1), dt)
+
+    def insert(self, slice_, rot, weight=1.0, corners=False):
+        s = f32(slice_)
+        r = f32(rot).reshape(12)
+        if self.fp64:
+            lib().or_insert_slice_work_f64(self.P, _d(self.data), _d(self.norm), _f(s), _f(r),
+                                          _f(self.syms), len(self.syms), float(weight),
+                                          self.mode, int(corners))
+        else:
+            lib().or_insert_slice_work(self.P, _f(self.data), _f(self.norm), _f(s), _f(r),
+                                      _f(self.syms), len(self.syms), float(weight),
+                                      self.mode, int(corners))
+
+    def setup_seed(self, seed, seed_weight):
+        assert not self.fp64
+        lib().or_setup_seed(self.P, _f(self.data), _f(self.norm), _f(f32(seed)), float(seed_weight))
+
+    def agreement(self, slice_, rot, weight=1.0, sub=True, apix=1.0):
+        assert not self.fp64 and self.mode == GAUSS2
+        s = f32(slice_).copy()
+        r = f32(rot).reshape(12)
+        out = np.zeros(5, np.float32)
+        lib().or_determine_slice_agreement(self.P, _f(self.data), _f(self.norm), _f(s), _f(r),
+                                          _f(self.syms), len(self.syms), float(weight),
+                                          int(sub), float(apix), _f(out))
+        return dict(reconstruct_norm=float(out[0]), reconstruct_absqual=float(out[1]),
+                    reconstruct_absqual_lowres=float(out[2]), reconstruct_qual=float(out[3]),
+                    reconstruct_weight=float(out[4]))
+
+    def finish(self, doift=True, sqrtnorm=False):
+        """Returns the real P^3 map (doift) or the normalised complex volume."""
+        data = f32(self.data).copy()
+        norm = f32(self.norm).copy()
+        out = np.zeros((self.P,) * 3, np.float32) if doift else np.zeros(1, np.float32)
+        lib().or_finish(self.P, _f(data), _f(norm), int(sqrtnorm), int(doift), _f(out))
+        self.norm_fixed = norm
+        return out if doift else data
+
+
+def insert_batch_mt(P, slices, rots, weights, sym="c1", mode="gauss_2", nthreads=1, fast=True):
+    """Multi-threaded batch insertion with private volumes (CPU baseline)."""
+    slices = f32(slices)
+    rots = f32(rots).reshape(-1, 12)
+    weights = f32(weights)
+    syms = f32(sym_get(sym))
+    data = np.zeros((P, P, P + 2), np.float32)
+    norm = np.zeros((P, P, P // 2 + 1), np.float32)
+    secs = lib(fast).or_insert_batch_mt(P, _f(data), _f(norm), _f(slices), _f(rots), _f(weights),
+                                        len(weights), _f(syms), len(syms), MODES[mode], int(nthreads))
+    return data, norm, secs
