@@ -1,0 +1,72 @@
+"""Builds libfourier_hip.so (HIP kernels + C ABI) for gfx950, in-tree.
+
+    python -m eman2_amd.build            # build (always recompiles)
+
+Every translation unit is compiled with hipcc for gfx950 with packed-FP32
+instructions disabled; the insertion kernels and the host engine are built
+with FP contraction off so their coordinate/weight arithmetic keeps the
+reference's IEEE operation order.
+"""
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libfourier_hip.so")
+ARCH = os.environ.get("FH_OFFLOAD_ARCH", "gfx950")
+
+COMMON = [
+    "-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-munsafe-fp-atomics", "-Wall", "-Wno-unused-result",
+    "-I" + os.path.join(os.path.dirname(HERE), "include"),
+]
+
+UNITS = [
+    ("prep_kernels.hip", []),
+    ("insert_kernels.hip", ["-ffp-contract=off"]),
+    ("engine.cpp", ["-x", "hip", "-ffp-contract=off"]),
+]
+
+
+def hipcc():
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def _compile(unit):
+    src, extra = unit
+    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+    cmd = [hipcc()] + COMMON + extra + ["-c", os.path.join(CSRC, src), "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("compile failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    return obj, r.stderr
+
+
+def build(verbose=False):
+    os.makedirs(BUILD, exist_ok=True)
+    with ThreadPoolExecutor(max_workers=len(UNITS)) as ex:
+        results = list(ex.map(_compile, UNITS))
+    objs = [o for o, _ in results]
+    if verbose:
+        for _, w in results:
+            if w.strip():
+                print(w, file=sys.stderr)
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed: %s\n%s" % (r.stdout, r.stderr))
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,831 @@
+// fourier_hip host engine: the "fourier" Reconstructor surface
+// (libEM/reconstructor.h:108-206, libEM/reconstructor.cpp:686-1641) over the
+// HIP kernels, exported through the C ABI of include/fourier_hip.h.
+//
+// The handle owns the device-resident volume (complex sums + weights in the
+// reference layout), the particle staging buffers of one batched chunk and
+// the binning buffers.  Every call on a handle is serialised by its mutex and
+// issued on the handle's own HIP stream (default priority).
+#include "../../include/fourier_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "fh_launch.hpp"
+
+using namespace fh;
+
+namespace {
+
+// ------------------------------------------------------------------ math
+// Util::round / fast_floor / apply_precision (libEM/util.h:465-471,874-880,1309-1314)
+int fast_floor(float x) { return x < 0 ? ((int)x - 1) : (int)x; }
+void apply_precision(float &value, float precision)
+{
+    float c = ceilf(value);
+    float f = (float)fast_floor(value);
+    if (fabsf(value - c) < precision) value = c;
+    else if (fabsf(value - f) < precision) value = f;
+}
+const float ERR_LIMIT = 0.000001f;  // Transform::ERR_LIMIT (libEM/transform.cpp:57)
+
+// Transform::get_determinant (libEM/transform.cpp:1310-1322)
+float determinant(const float *m)
+{
+    double det2;
+    det2 = m[0] * ((double)m[5] * m[10] - (double)m[9] * m[6]);
+    det2 -= m[1] * ((double)m[4] * m[10] - (double)m[8] * m[6]);
+    det2 += m[2] * ((double)m[4] * m[9] - (double)m[8] * m[5]);
+    float det = (float)det2;
+    apply_precision(det, ERR_LIMIT);
+    return det;
+}
+
+// Transform::get_scale (libEM/transform.cpp:1176-1187)
+float get_scale(const float *m)
+{
+    float d = determinant(m);
+    if (d < 0) d *= -1;
+    float scale = powf(d, 1.0f / 3.0f);
+    int is = (int)scale;
+    float res = scale - (float)is;
+    if (res < ERR_LIMIT) scale = (float)is;
+    apply_precision(scale, ERR_LIMIT);
+    return scale;
+}
+
+// set_scale(1); set_mirror(false); set_trans(0,0,0) as insert_slice does
+// (libEM/reconstructor.cpp:1104-1106); returns the 3x3 rotation.
+void rotation_only(const float *in12, float *out9)
+{
+    float m[12];
+    memcpy(m, in12, sizeof(m));
+    float old_scale = get_scale(m);
+    float n_scale = 1.0f;
+    apply_precision(n_scale, ERR_LIMIT);
+    float corrected = n_scale / old_scale;
+    if (corrected != 1.0)
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) m[i * 4 + j] *= corrected;
+    if (determinant(m) < 0)
+        for (int j = 0; j < 4; j++) m[j] *= -1;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) out9[3 * i + j] = m[4 * i + j];
+}
+
+// preprocess_slice applies any 2-D translation / scale / mirror to the image
+// (libEM/reconstructor.cpp:1026-1035); only the identity case is supported.
+bool pure_rotation(const float *m)
+{
+    if (fabsf(m[3]) > ERR_LIMIT || fabsf(m[7]) > ERR_LIMIT || fabsf(m[11]) > ERR_LIMIT) return false;
+    float d = determinant(m);
+    return d == 1.0f;
+}
+
+// eman Euler -> 3x3 (libEM/transform.cpp:650-667)
+void euler_eman(double az, double alt, double phi, float *m9)
+{
+    const double d2r = 3.141592653589793238462643383279502884197169399 / 180.0;
+    double azp = az * d2r, altp = alt * d2r, phip = phi * d2r;
+    m9[0] = (float)(cos(phip) * cos(azp) - cos(altp) * sin(azp) * sin(phip));
+    m9[1] = (float)(cos(phip) * sin(azp) + cos(altp) * cos(azp) * sin(phip));
+    m9[2] = (float)(sin(altp) * sin(phip));
+    m9[3] = (float)(-sin(phip) * cos(azp) - cos(altp) * sin(azp) * cos(phip));
+    m9[4] = (float)(-sin(phip) * sin(azp) + cos(altp) * cos(azp) * cos(phip));
+    m9[5] = (float)(sin(altp) * cos(phip));
+    m9[6] = (float)(sin(altp) * sin(azp));
+    m9[7] = (float)(-sin(altp) * cos(azp));
+    m9[8] = (float)cos(altp);
+}
+
+// Symmetry3D::get_symmetries for c<n>/d<n> (libEM/symmetry.cpp:75-142,
+// 1407-1418 CSym::get_sym, 1452-1470 DSym::get_sym)
+bool sym_ops(const std::string &name, std::vector<float> &out)
+{
+    if (name.size() < 2) return false;
+    char k = (char)tolower((unsigned char)name[0]);
+    if (k != 'c' && k != 'd') return false;
+    for (size_t i = 1; i < name.size(); i++)
+        if (!isdigit((unsigned char)name[i])) return false;
+    int n = atoi(name.c_str() + 1);
+    if (n <= 0) return false;
+    out.clear();
+    if (k == 'c') {
+        for (int i = 0; i < n; i++) {
+            float az = (float)(i % n) * 360.0f / (float)n;
+            float m[9];
+            euler_eman((double)az, 0.0, 0.0, m);
+            out.insert(out.end(), m, m + 9);
+        }
+        return true;
+    }
+    int nsym = 2 * n;
+    for (int i = 0; i < nsym; i++) {
+        float az, alt;
+        if (i >= nsym / 2) {
+            az = (float)((i % nsym) - nsym / 2) * 360.0f / (float)(nsym / 2);
+            alt = 180.0f;
+        } else {
+            az = (float)(i % nsym) * 360.0f / (float)(nsym / 2);
+            alt = 0.0f;
+        }
+        float m[9];
+        euler_eman((double)az, (double)alt, 0.0, m);
+        out.insert(out.end(), m, m + 9);
+    }
+    return true;
+}
+
+// t3d = a * b in FP32, reference order (libEM/transform.cpp:1384-1395)
+void compose9(const float *a, const float *b, float *r)
+{
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) r[3 * i + j] = a[3 * i + 0] * b[0 * 3 + j] + a[3 * i + 1] * b[1 * 3 + j] + a[3 * i + 2] * b[2 * 3 + j];
+}
+
+// CTF constants (libEM/ctf.h:291-296, libEM/ctf.cpp:671-675, :985-992)
+CtfConst ctf_constants(const fh_ctf &c, int ny)
+{
+    CtfConst k;
+    float lambda = 12.2639f / sqrtf(c.voltage * 1000.0f + 0.97845f * c.voltage * c.voltage);
+    float phase;
+    if (c.ampcont > -100.0 && c.ampcont <= 100.0) phase = (float)asin(c.ampcont / 100.0);
+    else if (c.ampcont > 100.0) phase = (float)(M_PI - asin(2.0f - c.ampcont / 100.0));
+    else phase = (float)(-M_PI - asin(-2.0 - c.ampcont / 100.0));
+    k.ds = 1.0f / (c.apix * ny);
+    k.g1 = (float)(M_PI / 2.0 * c.cs * 1.0e7 * powf(lambda, 3.0f));
+    k.g2 = (float)(M_PI * lambda * 10000.0);
+    k.acac = (float)(M_PI / 2.0 - phase);
+    return k;
+}
+
+bool smooth_235(int n)
+{
+    if (n < 2) return false;
+    for (int p : {2, 3, 5})
+        while (n % p == 0) n /= p;
+    return n == 1;
+}
+
+thread_local std::string g_create_error;
+
+}  // namespace
+
+// ====================================================================== handle
+struct fh_reconstructor {
+    std::mutex mu;
+    std::string err;
+    // parameters
+    int P = 0, H = 0, device = 0;
+    std::string sym, mode_name;
+    int mode = 0, corners = 0, sqrtnorm = 0, verbose = 0;
+    std::vector<float> syms;  // nsym x 9
+    int nsym = 1;
+    // device state
+    hipStream_t stream = nullptr;
+    bool ready = false;  // setup() called and finish() not yet
+    float2 *data = nullptr;
+    float *norm = nullptr;
+    float *exptab = nullptr, *rtab = nullptr, *syms_d = nullptr;
+    float2 *tw = nullptr;
+    FftPlan plan{};
+    float scale = 1.f;
+    int cus = 256;
+    // chunk staging
+    int cap_n = 0, cap_B = 0;
+    float *imgs = nullptr, *means = nullptr, *rots = nullptr, *wts = nullptr, *dfs = nullptr;
+    float2 *slices = nullptr;
+    float *t3d = nullptr, *wq = nullptr;
+    // binning
+    BrickGeom geom{};
+    BinBuffers bins{};
+    long long bins_cap_slices = 0;
+    int bins_brick = 0;
+    fh_tuning tune{16, 64, 2048, 3};
+    fh_stats stats{};
+    hipEvent_t ev[6] = {};
+};
+
+#define FH_TRY(h, expr)                                                                    \
+    do {                                                                                   \
+        hipError_t _e = (expr);                                                            \
+        if (_e != hipSuccess) {                                                            \
+            (h)->err = std::string(#expr) + ": " + hipGetErrorString(_e);                  \
+            return FH_ERR_DEVICE;                                                          \
+        }                                                                                  \
+    } while (0)
+
+static int fail(fh_reconstructor *h, int code, const std::string &msg)
+{
+    h->err = msg;
+    return code;
+}
+
+static size_t vol_elems(int P) { return (size_t)P * P * (P / 2 + 1); }
+
+static void free_chunk(fh_reconstructor *h)
+{
+    for (float **p : {&h->imgs, &h->means, &h->rots, &h->wts, &h->dfs, &h->t3d, &h->wq})
+        if (*p) { (void)hipFree(*p); *p = nullptr; }
+    if (h->slices) { (void)hipFree(h->slices); h->slices = nullptr; }
+    h->cap_n = h->cap_B = 0;
+}
+
+static void free_bins(fh_reconstructor *h)
+{
+    BinBuffers &b = h->bins;
+    for (int **p : {&b.counts, &b.offsets, &b.cursors, &b.item_off, &b.list, &b.ctl})
+        if (*p) { (void)hipFree(*p); *p = nullptr; }
+    if (b.items) { (void)hipFree(b.items); b.items = nullptr; }
+    h->bins_cap_slices = 0;
+}
+
+static void free_volume(fh_reconstructor *h)
+{
+    if (h->data) { (void)hipFree(h->data); h->data = nullptr; }
+    if (h->norm) { (void)hipFree(h->norm); h->norm = nullptr; }
+    h->ready = false;
+}
+
+static int ensure_chunk(fh_reconstructor *h, int n, int B)
+{
+    if (n <= h->cap_n && B <= h->cap_B) return FH_OK;
+    free_chunk(h);
+    const int P = h->P, H = h->H;
+    const size_t ns = (size_t)n * h->nsym;
+    FH_TRY(h, hipMalloc(&h->imgs, sizeof(float) * (size_t)n * B * B));
+    FH_TRY(h, hipMalloc(&h->means, sizeof(float) * n));
+    FH_TRY(h, hipMalloc(&h->rots, sizeof(float) * 9 * (size_t)n));
+    FH_TRY(h, hipMalloc(&h->wts, sizeof(float) * n));
+    FH_TRY(h, hipMalloc(&h->dfs, sizeof(float) * n));
+    FH_TRY(h, hipMalloc(&h->slices, sizeof(float2) * (size_t)n * P * (H + 1)));
+    FH_TRY(h, hipMalloc(&h->t3d, sizeof(float) * 9 * ns));
+    FH_TRY(h, hipMalloc(&h->wq, sizeof(float) * ns));
+    h->cap_n = n;
+    h->cap_B = B;
+    return FH_OK;
+}
+
+static int ensure_bins(fh_reconstructor *h, long long nslices)
+{
+    const int Bs = h->tune.brick;
+    if (nslices <= h->bins_cap_slices && Bs == h->bins_brick) return FH_OK;
+    free_bins(h);
+    BrickGeom g;
+    g.Bs = Bs;
+    g.nb = (h->P + 1 + Bs - 1) / Bs;
+    g.nbricks = g.nb * g.nb * g.nb;
+    h->geom = g;
+    BinBuffers &b = h->bins;
+    // at most 5 bricks per brick column per slice (see k_bin)
+    b.list_cap = 5LL * g.nb * g.nb * nslices;
+    b.item_cap = b.list_cap / std::max(1, h->tune.slices_per_item) + g.nbricks + 1;
+    FH_TRY(h, hipMalloc(&b.counts, sizeof(int) * g.nbricks));
+    FH_TRY(h, hipMalloc(&b.offsets, sizeof(int) * (g.nbricks + 1)));
+    FH_TRY(h, hipMalloc(&b.cursors, sizeof(int) * g.nbricks));
+    FH_TRY(h, hipMalloc(&b.item_off, sizeof(int) * (g.nbricks + 1)));
+    FH_TRY(h, hipMalloc(&b.list, sizeof(int) * b.list_cap));
+    FH_TRY(h, hipMalloc(&b.items, sizeof(WorkItem) * b.item_cap));
+    FH_TRY(h, hipMalloc(&b.ctl, sizeof(int) * 4));
+    h->bins_cap_slices = nslices;
+    h->bins_brick = Bs;
+    return FH_OK;
+}
+
+static InsertConst iconst(const fh_reconstructor *h)
+{
+    InsertConst c;
+    c.P = h->P;
+    c.H = h->H;
+    c.Pf = (float)h->P;
+    c.h = 1.0f / (float)(4.0 / (M_PI * M_PI));  // 1.0f/EMConsts::I2G
+    c.mode = h->mode;
+    c.corners = h->corners;
+    return c;
+}
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char *fh_last_error(const fh_reconstructor *h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+int fh_create(const fh_params *p, fh_reconstructor **out)
+{
+    if (!p || !out) {
+        g_create_error = "NullPointerException: null argument to fh_create";
+        return FH_ERR_NULL;
+    }
+    *out = nullptr;
+    if (p->nx != p->ny || p->ny != p->nz) {
+        g_create_error = "ImageDimensionException: fourier_hip requires a cubic size";
+        return FH_ERR_IMAGE_DIM;
+    }
+    if (p->nx < 8 || p->nx % 2 || !smooth_235(p->nx)) {
+        g_create_error = "ImageDimensionException: size must be even, >= 8 and 2^a 3^b 5^c (got " +
+                         std::to_string(p->nx) + ")";
+        return FH_ERR_IMAGE_DIM;
+    }
+    if (p->nx > 1024) {
+        g_create_error = "ImageDimensionException: size above 1024 is not supported";
+        return FH_ERR_IMAGE_DIM;
+    }
+    std::string mode = p->mode ? p->mode : "gauss_2";
+    int imode;
+    if (mode == "gauss_2") imode = 0;
+    else if (mode == "trilinear") imode = 1;
+    else {
+        g_create_error = "NotExistingObjectException: inserter mode '" + mode + "' (fourier_hip supports gauss_2, trilinear)";
+        return FH_ERR_NOT_EXISTING;
+    }
+    if (p->usessnr) {
+        g_create_error = "InvalidParameterException: usessnr is not supported by fourier_hip";
+        return FH_ERR_INVALID_PARAM;
+    }
+    std::string sym = p->sym ? p->sym : "c1";
+    std::vector<float> ops;
+    if (!sym_ops(sym, ops)) {
+        g_create_error = "NotExistingObjectException: symmetry '" + sym + "' (fourier_hip supports c<n>, d<n>)";
+        return FH_ERR_NOT_EXISTING;
+    }
+    fh_reconstructor *h = new fh_reconstructor();
+    h->P = p->nx;
+    h->H = p->nx / 2;
+    h->device = p->device;
+    h->sym = sym;
+    h->mode_name = mode;
+    h->mode = imode;
+    h->corners = p->corners;
+    h->sqrtnorm = p->sqrtnorm;
+    h->verbose = p->verbose;
+    h->syms = ops;
+    h->nsym = (int)ops.size() / 9;
+    hipError_t e;
+    if ((e = hipSetDevice(h->device)) != hipSuccess || (e = hipStreamCreate(&h->stream)) != hipSuccess) {
+        g_create_error = std::string("device error: ") + hipGetErrorString(e);
+        delete h;
+        return FH_ERR_DEVICE;
+    }
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device) == hipSuccess && cus > 0) h->cus = cus;
+    for (auto &ev : h->ev) (void)hipEventCreate(&ev);
+    // constant tables: fast_exp (util.cpp:792-794), fl(i/P), twiddles, syms
+    const int P = h->P, H = h->H;
+    std::vector<float> et(400), rt(P + 1);
+    for (int i = 0; i < 400; i++) et[i] = (float)exp(-i / 50.0);
+    for (int i = 0; i <= P; i++) rt[i] = (float)(i - H) / (float)P;
+    std::vector<float2> tw(P);
+    for (int m = 0; m < P; m++) {
+        double a = -2.0 * M_PI * (double)m / P;
+        tw[m] = make_float2((float)cos(a), (float)sin(a));
+    }
+    make_plan(P, &h->plan);
+    h->scale = (float)sqrt((double)(1.0f / (float)P * (float)(P + 2)));
+    if ((e = hipMalloc(&h->exptab, sizeof(float) * 400)) != hipSuccess ||
+        (e = hipMalloc(&h->rtab, sizeof(float) * (P + 1))) != hipSuccess ||
+        (e = hipMalloc(&h->tw, sizeof(float2) * P)) != hipSuccess ||
+        (e = hipMalloc(&h->syms_d, sizeof(float) * ops.size())) != hipSuccess ||
+        (e = hipMemcpy(h->exptab, et.data(), sizeof(float) * 400, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(h->rtab, rt.data(), sizeof(float) * (P + 1), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(h->tw, tw.data(), sizeof(float2) * P, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(h->syms_d, ops.data(), sizeof(float) * ops.size(), hipMemcpyHostToDevice)) != hipSuccess) {
+        g_create_error = std::string("device error: ") + hipGetErrorString(e);
+        fh_destroy(h);
+        return FH_ERR_DEVICE;
+    }
+    *out = h;
+    return FH_OK;
+}
+
+int fh_destroy(fh_reconstructor *h)
+{
+    if (!h) return FH_OK;
+    {
+        std::lock_guard<std::mutex> lk(h->mu);
+        (void)hipSetDevice(h->device);
+        if (h->stream) (void)hipStreamSynchronize(h->stream);
+        free_chunk(h);
+        free_bins(h);
+        free_volume(h);
+        for (void *p : {(void *)h->exptab, (void *)h->rtab, (void *)h->tw, (void *)h->syms_d})
+            if (p) (void)hipFree(p);
+        for (auto &ev : h->ev)
+            if (ev) (void)hipEventDestroy(ev);
+        if (h->stream) (void)hipStreamDestroy(h->stream);
+    }
+    delete h;
+    return FH_OK;
+}
+
+static int setup_locked(fh_reconstructor *h)
+{
+    FH_TRY(h, hipSetDevice(h->device));
+    const size_t n = vol_elems(h->P);
+    if (!h->data) {
+        hipError_t e = hipMalloc(&h->data, sizeof(float2) * n);
+        if (e != hipSuccess) return fail(h, FH_ERR_NOMEM, std::string("volume allocation: ") + hipGetErrorString(e));
+        e = hipMalloc(&h->norm, sizeof(float) * n);
+        if (e != hipSuccess) {
+            (void)hipFree(h->data);
+            h->data = nullptr;
+            return fail(h, FH_ERR_NOMEM, std::string("weight allocation: ") + hipGetErrorString(e));
+        }
+    }
+    FH_TRY(h, launch_fill_f32((float *)h->data, 0.f, 2 * n, h->stream));
+    FH_TRY(h, launch_fill_f32(h->norm, 0.f, n, h->stream));
+    h->ready = true;
+    if (h->verbose)
+        printf("3D Fourier dimensions are %d %d %d\nYou will require approximately %1.3g GB of memory to reconstruct this volume\n",
+               h->P + 2, h->P, h->P, (double)(sizeof(float) * 3 * n) / 1e9);
+    return FH_OK;
+}
+
+int fh_setup(fh_reconstructor *h)
+{
+    if (!h) return FH_ERR_NULL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    return setup_locked(h);
+}
+
+int fh_setup_seed(fh_reconstructor *h, const float *seed, float seed_weight)
+{
+    if (!h) return FH_ERR_NULL;
+    if (!seed) return fail(h, FH_ERR_NULL, "NullPointerException: seed volume is NULL");
+    std::lock_guard<std::mutex> lk(h->mu);
+    int rc = setup_locked(h);
+    if (rc) return rc;
+    const size_t n = vol_elems(h->P);
+    FH_TRY(h, hipMemcpyAsync(h->data, seed, sizeof(float2) * n, hipMemcpyHostToDevice, h->stream));
+    FH_TRY(h, launch_scale_f32((float *)h->data, seed_weight, 2 * n, h->stream));
+    FH_TRY(h, launch_seed_weights(h->norm, h->P, seed_weight, h->stream));
+    FH_TRY(h, hipStreamSynchronize(h->stream));
+    return FH_OK;
+}
+
+int fh_setup_seedandweights(fh_reconstructor *h, const float *seed, const float *weights)
+{
+    if (!h) return FH_ERR_NULL;
+    if (!seed || !weights) return fail(h, FH_ERR_NULL, "NullPointerException: seed or weight volume is NULL");
+    std::lock_guard<std::mutex> lk(h->mu);
+    int rc = setup_locked(h);
+    if (rc) return rc;
+    const size_t n = vol_elems(h->P);
+    FH_TRY(h, hipMemcpyAsync(h->data, seed, sizeof(float2) * n, hipMemcpyHostToDevice, h->stream));
+    FH_TRY(h, hipMemcpyAsync(h->norm, weights, sizeof(float) * n, hipMemcpyHostToDevice, h->stream));
+    FH_TRY(h, hipStreamSynchronize(h->stream));
+    return FH_OK;
+}
+
+int fh_clear(fh_reconstructor *h)
+{
+    if (!h) return FH_ERR_NULL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (!h->ready) return fail(h, FH_ERR_STATE, "clear() before setup()");
+    const size_t n = vol_elems(h->P);
+    FH_TRY(h, hipSetDevice(h->device));
+    FH_TRY(h, launch_fill_f32((float *)h->data, 0.f, 2 * n, h->stream));
+    FH_TRY(h, launch_fill_f32(h->norm, 0.f, n, h->stream));
+    return FH_OK;
+}
+
+// prep of n particles already in h->imgs (box B) into h->slices
+static int prep_chunk(fh_reconstructor *h, int n, int B, const float *dev_imgs, const float *defocus_host,
+                      const fh_ctf *ctf)
+{
+    FH_TRY(h, launch_edge_mean(dev_imgs, n, B, h->means, h->stream));
+    FH_TRY(h, launch_prep_rows(dev_imgs, n, B, h->P, h->means, h->slices, h->plan, h->tw, h->stream));
+    const float *dfd = nullptr;
+    CtfConst cc{};
+    if (defocus_host && ctf) {
+        FH_TRY(h, hipMemcpyAsync(h->dfs, defocus_host, sizeof(float) * n, hipMemcpyHostToDevice, h->stream));
+        dfd = h->dfs;
+        cc = ctf_constants(*ctf, h->P);
+    }
+    FH_TRY(h, launch_prep_cols(n, B, h->P, h->slices, dfd, cc, h->scale, h->plan, h->tw, h->stream));
+    return FH_OK;
+}
+
+static int check_xforms(fh_reconstructor *h, const float *xforms, int n, int need_identity_2d, std::vector<float> &rot9)
+{
+    rot9.resize((size_t)9 * n);
+    for (int i = 0; i < n; i++) {
+        const float *m = xforms + 12 * (size_t)i;
+        if (need_identity_2d && !pure_rotation(m))
+            return fail(h, FH_ERR_INVALID_PARAM,
+                        "InvalidParameterException: particle " + std::to_string(i) +
+                            ": translation/scale/mirror in preprocess_slice is not supported by fourier_hip");
+        rotation_only(m, &rot9[9 * (size_t)i]);
+    }
+    return FH_OK;
+}
+
+int fh_preprocess_slice(fh_reconstructor *h, const float *img, float *slice_out)
+{
+    if (!h) return FH_ERR_NULL;
+    if (!img || !slice_out) return fail(h, FH_ERR_NULL, "NullPointerException: image is NULL");
+    std::lock_guard<std::mutex> lk(h->mu);
+    FH_TRY(h, hipSetDevice(h->device));
+    const int P = h->P, H = h->H;
+    int rc = ensure_chunk(h, std::max(1, h->cap_n), std::max(P, h->cap_B));
+    if (rc) return rc;
+    FH_TRY(h, hipMemcpyAsync(h->imgs, img, sizeof(float) * P * P, hipMemcpyHostToDevice, h->stream));
+    // preprocess_slice has no edge-mean step: subtract a zero mean
+    FH_TRY(h, launch_fill_f32(h->means, 0.f, 1, h->stream));
+    FH_TRY(h, launch_prep_rows(h->imgs, 1, P, P, h->means, h->slices, h->plan, h->tw, h->stream));
+    FH_TRY(h, launch_prep_cols(1, P, P, h->slices, nullptr, CtfConst{}, h->scale, h->plan, h->tw, h->stream));
+    FH_TRY(h, hipMemcpyAsync(slice_out, h->slices, sizeof(float2) * P * (H + 1), hipMemcpyDeviceToHost, h->stream));
+    FH_TRY(h, hipStreamSynchronize(h->stream));
+    return FH_OK;
+}
+
+// plain insertion of one prepared slice (h->slices + idx) with rotation rot9
+static int insert_slice_plain(fh_reconstructor *h, const float2 *slice_d, const float *rot9, float weight, bool sync_each)
+{
+    const InsertConst c = iconst(h);
+    for (int s = 0; s < h->nsym; s++) {
+        float t[9];
+        compose9(rot9, &h->syms[9 * s], t);
+        FH_TRY(h, launch_insert_plain(slice_d, c, t, weight, h->data, h->norm, h->exptab, h->rtab, h->stream));
+        if (sync_each) FH_TRY(h, hipStreamSynchronize(h->stream));
+    }
+    return FH_OK;
+}
+
+int fh_insert_slice(fh_reconstructor *h, const float *data, int is_preprocessed, const float xform[12], float weight)
+{
+    if (!h) return FH_ERR_NULL;
+    if (!data || !xform) return fail(h, FH_ERR_NULL, "NullPointerException: EMData pointer (input image) is NULL");
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (!h->ready) return fail(h, FH_ERR_STATE, "insert_slice() before setup()");
+    if (weight == 0) return FH_SKIPPED;
+    FH_TRY(h, hipSetDevice(h->device));
+    const int P = h->P, H = h->H;
+    std::vector<float> rot9;
+    int rc = check_xforms(h, xform, 1, !is_preprocessed, rot9);
+    if (rc) return rc;
+    rc = ensure_chunk(h, std::max(1, h->cap_n), std::max(P, h->cap_B));
+    if (rc) return rc;
+    if (is_preprocessed) {
+        FH_TRY(h, hipMemcpyAsync(h->slices, data, sizeof(float2) * P * (H + 1), hipMemcpyHostToDevice, h->stream));
+    } else {
+        FH_TRY(h, hipMemcpyAsync(h->imgs, data, sizeof(float) * P * P, hipMemcpyHostToDevice, h->stream));
+        FH_TRY(h, launch_fill_f32(h->means, 0.f, 1, h->stream));
+        FH_TRY(h, launch_prep_rows(h->imgs, 1, P, P, h->means, h->slices, h->plan, h->tw, h->stream));
+        FH_TRY(h, launch_prep_cols(1, P, P, h->slices, nullptr, CtfConst{}, h->scale, h->plan, h->tw, h->stream));
+    }
+    rc = insert_slice_plain(h, h->slices, rot9.data(), weight, false);
+    if (rc) return rc;
+    FH_TRY(h, hipStreamSynchronize(h->stream));
+    return FH_OK;
+}
+
+static int check_batch_args(fh_reconstructor *h, const float *imgs, int n, int box, const float *xforms,
+                            const float *weights)
+{
+    if (!imgs || !xforms || !weights) return fail(h, FH_ERR_NULL, "NullPointerException: batch input is NULL");
+    if (n < 0) return fail(h, FH_ERR_INVALID_PARAM, "InvalidParameterException: negative particle count");
+    if (box < 2 || box > h->P)
+        return fail(h, FH_ERR_IMAGE_DIM, "ImageDimensionException: particle box " + std::to_string(box) +
+                                             " must be in [2, size=" + std::to_string(h->P) + "]");
+    if (!h->ready) return fail(h, FH_ERR_STATE, "insert before setup()");
+    return FH_OK;
+}
+
+int fh_insert_batch(fh_reconstructor *h, const float *imgs, int n, int box, const float *xforms, const float *weights,
+                    const float *defocus, const fh_ctf *ctf, int where)
+{
+    if (!h) return FH_ERR_NULL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    int rc = check_batch_args(h, imgs, n, box, xforms, weights);
+    if (rc) return rc;
+    FH_TRY(h, hipSetDevice(h->device));
+    std::vector<float> rot9;
+    rc = check_xforms(h, xforms, n, 1, rot9);
+    if (rc) return rc;
+    const int chunk = std::max(1, std::min(h->tune.chunk, std::max(n, 1)));
+    rc = ensure_chunk(h, chunk, box);
+    if (rc) return rc;
+    rc = ensure_bins(h, (long long)chunk * h->nsym);
+    if (rc) return rc;
+    const InsertConst c = iconst(h);
+    fh_stats st{};
+    float ms;
+    const int grid = h->cus * std::max(1, h->tune.wgs_per_cu);
+    for (int c0 = 0; c0 < n; c0 += chunk) {
+        const int m = std::min(chunk, n - c0);
+        const float *src = imgs + (size_t)c0 * box * box;
+        const float *dimgs;
+        FH_TRY(h, hipEventRecord(h->ev[0], h->stream));
+        if (where == FH_DEVICE) {
+            dimgs = src;
+        } else {
+            FH_TRY(h, hipMemcpyAsync(h->imgs, src, sizeof(float) * (size_t)m * box * box, hipMemcpyHostToDevice, h->stream));
+            dimgs = h->imgs;
+        }
+        rc = prep_chunk(h, m, box, dimgs, defocus ? defocus + c0 : nullptr, ctf);
+        if (rc) return rc;
+        FH_TRY(h, hipMemcpyAsync(h->rots, &rot9[9 * (size_t)c0], sizeof(float) * 9 * m, hipMemcpyHostToDevice, h->stream));
+        FH_TRY(h, hipMemcpyAsync(h->wts, weights + c0, sizeof(float) * m, hipMemcpyHostToDevice, h->stream));
+        FH_TRY(h, launch_compose(h->rots, h->wts, m, h->syms_d, h->nsym, h->t3d, h->wq, h->stream));
+        FH_TRY(h, hipEventRecord(h->ev[1], h->stream));
+        FH_TRY(h, launch_bin(h->t3d, h->wq, m * h->nsym, c, h->geom, h->bins, h->tune.slices_per_item, h->stream));
+        FH_TRY(h, hipEventRecord(h->ev[2], h->stream));
+        FH_TRY(h, launch_brick_insert(h->slices, m * h->nsym, h->nsym, h->t3d, h->wq, c, h->geom, h->bins, h->data, h->norm,
+                                      h->exptab, h->rtab, grid, h->stream));
+        FH_TRY(h, hipEventRecord(h->ev[3], h->stream));
+        FH_TRY(h, hipEventSynchronize(h->ev[3]));
+        int ctl[4];
+        FH_TRY(h, hipMemcpy(ctl, h->bins.ctl, sizeof(ctl), hipMemcpyDeviceToHost));
+        if (ctl[3]) return fail(h, FH_ERR_DEVICE, "internal error: brick list capacity exceeded");
+        FH_TRY(h, hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
+        st.prep_ms += ms;
+        FH_TRY(h, hipEventElapsedTime(&ms, h->ev[1], h->ev[2]));
+        st.bin_ms += ms;
+        FH_TRY(h, hipEventElapsedTime(&ms, h->ev[2], h->ev[3]));
+        st.insert_ms += ms;
+        st.insert_kernel_ms += ms;
+        FH_TRY(h, hipEventElapsedTime(&ms, h->ev[0], h->ev[3]));
+        st.total_ms += ms;
+        st.slices += (long long)m * h->nsym;
+        st.pairs += ctl[2];
+        st.items += ctl[1];
+        st.chunks += 1;
+    }
+    h->stats = st;
+    return FH_OK;
+}
+
+int fh_prep_batch(fh_reconstructor *h, const float *imgs, int n, int box, const float *defocus, const fh_ctf *ctf,
+                  float *slices_out, int where)
+{
+    if (!h) return FH_ERR_NULL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (!imgs || !slices_out) return fail(h, FH_ERR_NULL, "NullPointerException: batch input is NULL");
+    if (box < 2 || box > h->P)
+        return fail(h, FH_ERR_IMAGE_DIM, "ImageDimensionException: particle box must be in [2, size]");
+    FH_TRY(h, hipSetDevice(h->device));
+    const int P = h->P, H = h->H;
+    const int chunk = std::max(1, std::min(h->tune.chunk, std::max(n, 1)));
+    int rc = ensure_chunk(h, chunk, box);
+    if (rc) return rc;
+    for (int c0 = 0; c0 < n; c0 += chunk) {
+        const int m = std::min(chunk, n - c0);
+        const float *src = imgs + (size_t)c0 * box * box;
+        const float *dimgs = src;
+        if (where != FH_DEVICE) {
+            FH_TRY(h, hipMemcpyAsync(h->imgs, src, sizeof(float) * (size_t)m * box * box, hipMemcpyHostToDevice, h->stream));
+            dimgs = h->imgs;
+        }
+        rc = prep_chunk(h, m, box, dimgs, defocus ? defocus + c0 : nullptr, ctf);
+        if (rc) return rc;
+        FH_TRY(h, hipMemcpyAsync(slices_out + (size_t)c0 * P * (P + 2), h->slices, sizeof(float2) * (size_t)m * P * (H + 1),
+                                 where == FH_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, h->stream));
+        FH_TRY(h, hipStreamSynchronize(h->stream));
+    }
+    return FH_OK;
+}
+
+int fh_insert_plain(fh_reconstructor *h, const float *imgs, int n, int box, const float *xforms, const float *weights,
+                    const float *defocus, const fh_ctf *ctf)
+{
+    if (!h) return FH_ERR_NULL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    int rc = check_batch_args(h, imgs, n, box, xforms, weights);
+    if (rc) return rc;
+    FH_TRY(h, hipSetDevice(h->device));
+    std::vector<float> rot9;
+    rc = check_xforms(h, xforms, n, 1, rot9);
+    if (rc) return rc;
+    rc = ensure_chunk(h, std::max(1, h->cap_n), std::max(box, h->cap_B));
+    if (rc) return rc;
+    for (int i = 0; i < n; i++) {
+        if (weights[i] == 0) continue;
+        // per-slice host->device copy (reconstructor.cpp:1074-1078)
+        FH_TRY(h, hipMemcpy(h->imgs, imgs + (size_t)i * box * box, sizeof(float) * box * box, hipMemcpyHostToDevice));
+        rc = prep_chunk(h, 1, box, h->imgs, defocus ? defocus + i : nullptr, ctf);
+        if (rc) return rc;
+        // one launch per symmetry op, synchronised (cuda_reconstructor.cu:171-193)
+        rc = insert_slice_plain(h, h->slices, &rot9[9 * (size_t)i], weights[i], true);
+        if (rc) return rc;
+    }
+    return FH_OK;
+}
+
+int fh_determine_slice_agreement(fh_reconstructor *h, const float *data, int is_preprocessed, const float xform[12],
+                                 float weight, int sub, float apix, float out[5])
+{
+    (void)data; (void)is_preprocessed; (void)xform; (void)weight; (void)sub; (void)apix; (void)out;
+    if (!h) return FH_ERR_NULL;
+    return fail(h, FH_ERR_STATE, "determine_slice_agreement is not built yet");
+}
+
+int fh_agreement_batch(fh_reconstructor *h, const float *imgs, int n, int box, const float *xforms, const float *weights,
+                       const float *defocus, const fh_ctf *ctf, int sub, float apix, float *out, int where)
+{
+    (void)imgs; (void)n; (void)box; (void)xforms; (void)weights; (void)defocus; (void)ctf; (void)sub; (void)apix;
+    (void)out; (void)where;
+    if (!h) return FH_ERR_NULL;
+    return fail(h, FH_ERR_STATE, "agreement_batch is not built yet");
+}
+
+int fh_finish(fh_reconstructor *h, int doift, float *out, float *norm_out)
+{
+    if (!h) return FH_ERR_NULL;
+    if (!out) return fail(h, FH_ERR_NULL, "NullPointerException: output buffer is NULL");
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (!h->ready) return fail(h, FH_ERR_STATE, "finish() before setup()");
+    FH_TRY(h, hipSetDevice(h->device));
+    const int P = h->P;
+    const size_t n = vol_elems(P);
+    float *planes = nullptr;
+    FH_TRY(h, hipMalloc(&planes, sizeof(float) * 2 * P * P));
+    hipError_t e = launch_normalize(h->data, h->norm, P, h->sqrtnorm, planes, h->stream);
+    if (e == hipSuccess && doift) {
+        e = launch_fft3d_inverse_zy(h->data, P, h->plan, h->tw, h->stream);
+        float *real = nullptr;
+        if (e == hipSuccess) e = hipMalloc(&real, sizeof(float) * (size_t)P * P * P);
+        if (e == hipSuccess) {
+            const float sc = 1.0f / (float)((size_t)P * P * P);
+            e = launch_c2r_x(h->data, P, real, sc, h->plan, h->tw, h->stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(out, real, sizeof(float) * (size_t)P * P * P, hipMemcpyDeviceToHost, h->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+            (void)hipFree(real);
+        }
+    } else if (e == hipSuccess) {
+        e = hipMemcpyAsync(out, h->data, sizeof(float2) * n, hipMemcpyDeviceToHost, h->stream);
+    }
+    if (e == hipSuccess && norm_out) e = hipMemcpyAsync(norm_out, h->norm, sizeof(float) * n, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(planes);
+    if (e != hipSuccess) return fail(h, FH_ERR_DEVICE, std::string("finish: ") + hipGetErrorString(e));
+    free_volume(h);  // ownership of the map passes to the caller (reconstructor.cpp:1633-1640)
+    return FH_OK;
+}
+
+int fh_get_state(fh_reconstructor *h, float *data, float *weights)
+{
+    if (!h) return FH_ERR_NULL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (!h->ready) return fail(h, FH_ERR_STATE, "get_state() before setup()");
+    FH_TRY(h, hipSetDevice(h->device));
+    const size_t n = vol_elems(h->P);
+    if (data) FH_TRY(h, hipMemcpyAsync(data, h->data, sizeof(float2) * n, hipMemcpyDeviceToHost, h->stream));
+    if (weights) FH_TRY(h, hipMemcpyAsync(weights, h->norm, sizeof(float) * n, hipMemcpyDeviceToHost, h->stream));
+    FH_TRY(h, hipStreamSynchronize(h->stream));
+    return FH_OK;
+}
+
+int fh_set_tuning(fh_reconstructor *h, const fh_tuning *t)
+{
+    if (!h || !t) return FH_ERR_NULL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (t->brick != 8 && t->brick != 16) return fail(h, FH_ERR_INVALID_PARAM, "brick must be 8 or 16");
+    if (t->slices_per_item < 1 || t->chunk < 1 || t->wgs_per_cu < 1)
+        return fail(h, FH_ERR_INVALID_PARAM, "tuning values must be positive");
+    if (t->brick != h->tune.brick || t->slices_per_item != h->tune.slices_per_item) free_bins(h);
+    h->tune = *t;
+    return FH_OK;
+}
+
+int fh_get_tuning(const fh_reconstructor *h, fh_tuning *t)
+{
+    if (!h || !t) return FH_ERR_NULL;
+    *t = h->tune;
+    return FH_OK;
+}
+
+int fh_get_stats(const fh_reconstructor *h, fh_stats *s)
+{
+    if (!h || !s) return FH_ERR_NULL;
+    *s = h->stats;
+    return FH_OK;
+}
+
+int fh_sync(fh_reconstructor *h)
+{
+    if (!h) return FH_ERR_NULL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    FH_TRY(h, hipSetDevice(h->device));
+    FH_TRY(h, hipStreamSynchronize(h->stream));
+    return FH_OK;
+}
+
+long long fh_footprint_bytes(int P, int nsym, int chunk)
+{
+    const long long v = 12LL * P * P * (P / 2 + 1);
+    const long long sl = 8LL * chunk * P * (P / 2 + 1) + 4LL * chunk * P * P;
+    const int nb = (P + 1 + 15) / 16;
+    const long long bins = 4LL * 5 * nb * nb * chunk * nsym * 2 + 16LL * nb * nb * nb;
+    return v + sl + bins;
+}
+
+}  // extern "C"

@@ -1,0 +1,135 @@
+// Device helpers shared by the fourier_hip kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fh_launch.hpp"
+
+namespace fh {
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+// multiply by -i
+__device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }
+
+// ---------------------------------------------------------------------------
+// Radix-R forward DFT on registers (exp(-2 pi i / R) kernel).  The constants
+// are cos/sin of 2 pi k / R.
+// ---------------------------------------------------------------------------
+template <int R>
+__device__ __forceinline__ void dft(float2 *v);
+
+template <>
+__device__ __forceinline__ void dft<2>(float2 *v)
+{
+    float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+
+template <>
+__device__ __forceinline__ void dft<3>(float2 *v)
+{
+    const float s1 = 0.86602540378443864676f;  // sin(2 pi/3)
+    float2 t1 = cadd(v[1], v[2]);
+    float2 t2 = make_float2(v[0].x - 0.5f * t1.x, v[0].y - 0.5f * t1.y);
+    float2 d = csub(v[1], v[2]);
+    float2 t3 = make_float2(s1 * d.x, s1 * d.y);
+    v[0] = cadd(v[0], t1);
+    v[1] = cadd(t2, mul_mi(t3));
+    v[2] = csub(t2, mul_mi(t3));
+}
+
+template <>
+__device__ __forceinline__ void dft<4>(float2 *v)
+{
+    float2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+    float2 b0 = cadd(v[1], v[3]), b1 = csub(v[1], v[3]);
+    v[0] = cadd(a0, b0);
+    v[2] = csub(a0, b0);
+    v[1] = cadd(a1, mul_mi(b1));
+    v[3] = csub(a1, mul_mi(b1));
+}
+
+template <>
+__device__ __forceinline__ void dft<5>(float2 *v)
+{
+    const float c1 = 0.30901699437494742410f;   // cos(2 pi/5)
+    const float c2 = -0.80901699437494742410f;  // cos(4 pi/5)
+    const float s1 = 0.95105651629515357212f;   // sin(2 pi/5)
+    const float s2 = 0.58778525229247312917f;   // sin(4 pi/5)
+    float2 a = v[0];
+    float2 p14 = cadd(v[1], v[4]), m14 = csub(v[1], v[4]);
+    float2 p23 = cadd(v[2], v[3]), m23 = csub(v[2], v[3]);
+    float2 r1 = make_float2(a.x + c1 * p14.x + c2 * p23.x, a.y + c1 * p14.y + c2 * p23.y);
+    float2 r2 = make_float2(a.x + c2 * p14.x + c1 * p23.x, a.y + c2 * p14.y + c1 * p23.y);
+    float2 i1 = make_float2(s1 * m14.x + s2 * m23.x, s1 * m14.y + s2 * m23.y);
+    float2 i2 = make_float2(s2 * m14.x - s1 * m23.x, s2 * m14.y - s1 * m23.y);
+    v[0] = make_float2(a.x + p14.x + p23.x, a.y + p14.y + p23.y);
+    v[1] = cadd(r1, mul_mi(i1));
+    v[4] = csub(r1, mul_mi(i1));
+    v[2] = cadd(r2, mul_mi(i2));
+    v[3] = csub(r2, mul_mi(i2));
+}
+
+// One Stockham autosort pass over T interleaved transforms of length N:
+// element (t, n) lives at buf[n*T + t].  tw[m] = exp(-2 pi i m / N).
+template <int R, int T>
+__device__ __forceinline__ void stockham_pass(const float2 *__restrict__ src, float2 *__restrict__ dst,
+                                              int N, int Ns, const float2 *__restrict__ tw)
+{
+    const int NR = N / R;
+    const int total = NR * T;
+    const int twmul = N / (Ns * R);
+    for (int b = threadIdx.x; b < total; b += blockDim.x) {
+        const int t = b % T;
+        const int j = b / T;
+        const int k = j % Ns;
+        float2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            float2 a = src[(j + r * NR) * T + t];
+            if (r > 0) a = cmul(a, tw[r * k * twmul]);
+            v[r] = a;
+        }
+        dft<R>(v);
+        const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+        for (int r = 0; r < R; r++) dst[(base + r * Ns) * T + t] = v[r];
+    }
+}
+
+// Full forward transform (ping-pong between a and b); returns the buffer that
+// holds the result.  Ends with a barrier.
+template <int T>
+__device__ float2 *fft_lds(float2 *a, float2 *b, const FftPlan &plan, const float2 *tw)
+{
+    int Ns = 1;
+    for (int p = 0; p < plan.npass; p++) {
+        const int R = plan.radix[p];
+        switch (R) {
+        case 2: stockham_pass<2, T>(a, b, plan.N, Ns, tw); break;
+        case 3: stockham_pass<3, T>(a, b, plan.N, Ns, tw); break;
+        case 4: stockham_pass<4, T>(a, b, plan.N, Ns, tw); break;
+        default: stockham_pass<5, T>(a, b, plan.N, Ns, tw); break;
+        }
+        __syncthreads();
+        float2 *t = a;
+        a = b;
+        b = t;
+        Ns *= R;
+    }
+    return a;
+}
+
+__device__ __forceinline__ void load_twiddles(float2 *tw, const float2 *__restrict__ g, int N)
+{
+    for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = g[i];
+}
+
+}  // namespace fh

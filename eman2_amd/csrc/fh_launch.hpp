@@ -1,0 +1,100 @@
+// Host-side declarations of the fourier_hip kernel launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fh {
+
+// Mixed-radix plan for the LDS Stockham FFT: N = prod(radix[i]), radices
+// in {2,3,4,5}.
+struct FftPlan {
+    int N;
+    int npass;
+    int radix[16];
+};
+bool make_plan(int N, FftPlan *p);
+
+// CTF_SIGN constants (libEM/ctf.cpp:985-992) computed on the host.
+struct CtfConst {
+    float g1, g2, acac, ds;
+};
+
+// Per-volume constants of the inserters.
+struct InsertConst {
+    int P;       // padded box
+    int H;       // P/2
+    float Pf;    // (float)P
+    float h;     // 1.0f / EMConsts::I2G
+    int mode;    // 0 gauss_2, 1 trilinear
+    int corners;
+};
+
+// Row-major 3x3 rotation passed by value.
+struct Mat9 {
+    float m[9];
+};
+
+// Work item of the batched insertion: slices list[start, start+count) of
+// brick `brick`.
+struct WorkItem {
+    int brick;
+    int start;
+    int count;
+    int pad;
+};
+
+// Device buffers of the binning stage.
+struct BinBuffers {
+    int *counts;      // nbricks
+    int *offsets;     // nbricks + 1
+    int *cursors;     // nbricks
+    int *item_off;    // nbricks + 1
+    int *list;        // capacity
+    WorkItem *items;  // item capacity
+    int *ctl;         // [0] work counter, [1] total items, [2] total pairs, [3] overflow flag
+    long long list_cap;
+    long long item_cap;
+};
+
+struct BrickGeom {
+    int Bs;     // brick edge (voxels)
+    int nb;     // bricks per axis over the unfolded grid [-H, H]
+    int nbricks;
+};
+
+// ---- prep (prep_kernels.hip) ----
+hipError_t launch_edge_mean(const float *imgs, int n, int B, float *means, hipStream_t s);
+hipError_t launch_prep_rows(const float *imgs, int n, int B, int P, const float *means, float2 *slices,
+                            const FftPlan &plan, const float2 *tw, hipStream_t s);
+hipError_t launch_prep_cols(int n, int B, int P, float2 *slices, const float *defocus, CtfConst cc,
+                            float scale, const FftPlan &plan, const float2 *tw, hipStream_t s);
+// ---- finish (prep_kernels.hip) ----
+hipError_t launch_normalize(float2 *data, float *norm, int P, int sqrtnorm, float *plane_tmp,
+                            hipStream_t s);
+hipError_t launch_fft3d_inverse_zy(float2 *data, int P, const FftPlan &plan, const float2 *tw,
+                                   hipStream_t s);
+hipError_t launch_c2r_x(const float2 *data, int P, float *out, float scale, const FftPlan &plan,
+                        const float2 *tw, hipStream_t s);
+hipError_t launch_fill_f32(float *p, float v, size_t n, hipStream_t s);
+hipError_t launch_fill_i32(int *p, int v, size_t n, hipStream_t s);
+hipError_t launch_seed_weights(float *norm, int P, float w, hipStream_t s);
+hipError_t launch_scale_f32(float *p, float v, size_t n, hipStream_t s);
+
+// ---- insertion (insert_kernels.hip) ----
+hipError_t launch_compose(const float *rots, const float *weights, int n, const float *syms, int nsym,
+                          float *t3d, float *wq, hipStream_t s);
+hipError_t launch_insert_plain(const float2 *slice, InsertConst c, const float *t3d9, float weight,
+                               float2 *data, float *norm, const float *exptab, const float *rtab,
+                               hipStream_t s);
+hipError_t launch_bin(const float *t3d, const float *wq, int nslices, InsertConst c, BrickGeom g,
+                      BinBuffers b, int slices_per_item, hipStream_t s);
+hipError_t launch_brick_insert(const float2 *slices, int nslices, int nsym,
+                               const float *t3d, const float *wq, InsertConst c, BrickGeom g,
+                               BinBuffers b, float2 *data, float *norm, const float *exptab,
+                               const float *rtab, int grid, hipStream_t s);
+hipError_t launch_agreement(const float2 *slices, int nsym, const float *t3d, const float *wq, int n,
+                            InsertConst c, float apix, int sub, const float2 *data, const float *norm,
+                            const float *exptab, const float *rtab, double *partial, float *out,
+                            hipStream_t s);
+
+}  // namespace fh

@@ -1,0 +1,461 @@
+// Particle prep (edge mean, pad, phase origin, r2c FFT, CTF sign, scale) and
+// finish (weight normalisation, 3-D c2r FFT, depad, phase origin) kernels.
+//
+// Prep follows programs/e2make3dpar.py:630-654,704-716 and
+// FourierReconstructor::preprocess_slice (libEM/reconstructor.cpp:1022-1061);
+// finish follows libEM/reconstructor.cpp:315-388,1552-1641.  All FFTs are
+// LDS Stockham transforms (fh_device.hpp) over tiles of T transforms.
+#include "fh_device.hpp"
+
+namespace fh {
+
+// ---------------------------------------------------------------- plans
+bool make_plan(int N, FftPlan *p)
+{
+    p->N = N;
+    p->npass = 0;
+    int m = N;
+    while (m % 4 == 0 && p->npass < 16) { p->radix[p->npass++] = 4; m /= 4; }
+    while (m % 2 == 0 && p->npass < 16) { p->radix[p->npass++] = 2; m /= 2; }
+    while (m % 3 == 0 && p->npass < 16) { p->radix[p->npass++] = 3; m /= 3; }
+    while (m % 5 == 0 && p->npass < 16) { p->radix[p->npass++] = 5; m /= 5; }
+    return m == 1 && N > 1;
+}
+
+static int pick_T(int P)
+{
+    // two ping-pong tiles of T x P float2 plus the twiddles must fit 64 KiB
+    return (P <= 400) ? 8 : 4;
+}
+
+static size_t fft_lds_bytes(int P, int T) { return sizeof(float2) * (size_t)P * (2 * T + 1); }
+
+// ---------------------------------------------------------------- edge mean
+// EMData::get_edge_mean, 2-D branch (libEM/emdata_metadata.cpp:713-736):
+// float pair sums of opposite edge pixels accumulated in double.
+__global__ void __launch_bounds__(256) k_edge_mean(const float *__restrict__ imgs, int B, float *__restrict__ means)
+{
+    const float *d = imgs + (size_t)blockIdx.x * B * B;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < 2 * B; i += blockDim.x) {
+        float pair;
+        if (i < B) pair = d[i] + d[(size_t)(B - 1) * B + i];
+        else {
+            int r = i - B;
+            pair = d[(size_t)r * B] + d[(size_t)r * B + B - 1];
+        }
+        acc += (double)pair;
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) means[blockIdx.x] = (float)red[0] / (float)(B * 2 + B * 2);
+}
+
+hipError_t launch_edge_mean(const float *imgs, int n, int B, float *means, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_edge_mean, dim3(n), dim3(256), 0, s, imgs, B, means);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- row pass
+// Each block transforms 2T padded rows of one particle: the rows are packed
+// pairwise into T complex transforms z = a + i c (two-for-one real FFT).
+// Input element xs of padded row yp is the phase-origin shifted
+// (ifftshift) padded image; output row ys = (yp + P/2) mod P.
+template <int T>
+__global__ void __launch_bounds__(256) k_prep_rows(const float *__restrict__ imgs, int B, int P, int off,
+                                                   const float *__restrict__ means, float2 *__restrict__ slices,
+                                                   FftPlan plan, const float2 *__restrict__ twg)
+{
+    extern __shared__ float2 smem[];
+    float2 *tw = smem;
+    float2 *A = tw + P;
+    float2 *Bf = A + T * P;
+    const int H = P / 2;
+    const int p = blockIdx.y;
+    const int b0 = blockIdx.x * 2 * T;
+    load_twiddles(tw, twg, P);
+    const float mean = means[p];
+    const float *img = imgs + (size_t)p * B * B;
+    for (int idx = threadIdx.x; idx < T * P; idx += blockDim.x) {
+        const int t = idx / P;
+        const int xs = idx - t * P;
+        const int xp = xs + H < P ? xs + H : xs + H - P;
+        const int bx = xp - off;
+        float re = 0.f, im = 0.f;
+        if (bx >= 0 && bx < B) {
+            const int r0 = b0 + 2 * t;
+            if (r0 < B) re = img[(size_t)r0 * B + bx] - mean;
+            if (r0 + 1 < B) im = img[(size_t)(r0 + 1) * B + bx] - mean;
+        }
+        A[xs * T + t] = make_float2(re, im);
+    }
+    __syncthreads();
+    float2 *R = fft_lds<T>(A, Bf, plan, tw);
+    float2 *out = slices + (size_t)p * P * (H + 1);
+    for (int idx = threadIdx.x; idx < T * (H + 1); idx += blockDim.x) {
+        const int t = idx / (H + 1);
+        const int k = idx - t * (H + 1);
+        const int r0 = b0 + 2 * t;
+        if (r0 >= B) continue;
+        const float2 z1 = R[k * T + t];
+        const float2 z2 = R[(k == 0 ? 0 : P - k) * T + t];
+        const float2 a = make_float2(0.5f * (z1.x + z2.x), 0.5f * (z1.y - z2.y));
+        const float2 c = make_float2(0.5f * (z1.y + z2.y), -0.5f * (z1.x - z2.x));
+        int ys0 = r0 + off + H;
+        ys0 = ys0 % P;
+        out[(size_t)ys0 * (H + 1) + k] = a;
+        if (r0 + 1 < B) {
+            const int ys1 = (ys0 + 1) % P;
+            out[(size_t)ys1 * (H + 1) + k] = c;
+        }
+    }
+}
+
+hipError_t launch_prep_rows(const float *imgs, int n, int B, int P, const float *means, float2 *slices,
+                            const FftPlan &plan, const float2 *tw, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    // centre zero-pad (clip_inplace Region((B-P)//2, ...), e2make3dpar.py:651-652)
+    const int d = B - P;
+    const int fl = d >= 0 ? d / 2 : -((-d + 1) / 2);
+    const int off = -fl;
+    const int T = pick_T(P);
+    const size_t lds = fft_lds_bytes(P, T);
+    dim3 grid((B + 2 * T - 1) / (2 * T), n);
+    if (T == 8)
+        hipLaunchKernelGGL(k_prep_rows<8>, grid, dim3(256), lds, s, imgs, B, P, off, means, slices, plan, tw);
+    else
+        hipLaunchKernelGGL(k_prep_rows<4>, grid, dim3(256), lds, s, imgs, B, P, off, means, slices, plan, tw);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- column pass
+// CTF_SIGN of EMAN2Ctf::compute_2d_complex (libEM/ctf.cpp:1076-1090) at
+// Fourier pixel (x, y), y signed.  Evaluated with the reference's float
+// operation order (no contraction); the sign of cos is taken in double.
+__device__ __forceinline__ float ctf_sign(int x, int y, float defocus, const CtfConst &cc)
+{
+#pragma clang fp contract(off)
+    const float hyp = __fsqrt_rn((float)(x * x + y * y));
+    const float s = hyp * cc.ds;
+    const float gam = -cc.g1 * s * s * s * s + cc.g2 * defocus * s * s;
+    const double v = cos((double)(gam - cc.acac));
+    return v < 0 ? -1.0f : 1.0f;
+}
+
+template <int T>
+__global__ void __launch_bounds__(256) k_prep_cols(int B, int P, int off, float2 *__restrict__ slices,
+                                                   const float *__restrict__ defocus, CtfConst cc, float scale,
+                                                   FftPlan plan, const float2 *__restrict__ twg)
+{
+    extern __shared__ float2 smem[];
+    float2 *tw = smem;
+    float2 *A = tw + P;
+    float2 *Bf = A + T * P;
+    const int H = P / 2;
+    const int p = blockIdx.y;
+    const int x0 = blockIdx.x * T;
+    load_twiddles(tw, twg, P);
+    float2 *sl = slices + (size_t)p * P * (H + 1);
+    for (int idx = threadIdx.x; idx < T * P; idx += blockDim.x) {
+        const int y = idx / T;
+        const int t = idx - y * T;
+        const int x = x0 + t;
+        int b = y - off - H;
+        b %= P;
+        if (b < 0) b += P;
+        float2 v = make_float2(0.f, 0.f);
+        if (x <= H && b < B) v = sl[(size_t)y * (H + 1) + x];
+        A[y * T + t] = v;
+    }
+    __syncthreads();
+    float2 *R = fft_lds<T>(A, Bf, plan, tw);
+    const float df = defocus ? defocus[p] : 0.f;
+    for (int idx = threadIdx.x; idx < T * P; idx += blockDim.x) {
+        const int y = idx / T;
+        const int t = idx - y * T;
+        const int x = x0 + t;
+        if (x > H) continue;
+        float2 v = R[y * T + t];
+        if (defocus) {
+            const float sg = ctf_sign(x, y < H ? y : y - P, df, cc);
+            v = make_float2(v.x * sg - v.y * 0.0f, v.x * 0.0f + v.y * sg);
+        }
+        v.x = v.x * scale;
+        v.y = v.y * scale;
+        sl[(size_t)y * (H + 1) + x] = v;
+    }
+}
+
+hipError_t launch_prep_cols(int n, int B, int P, float2 *slices, const float *defocus, CtfConst cc,
+                            float scale, const FftPlan &plan, const float2 *tw, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const int d = B - P;
+    const int fl = d >= 0 ? d / 2 : -((-d + 1) / 2);
+    const int off = -fl;
+    const int T = pick_T(P);
+    const int H = P / 2;
+    const size_t lds = fft_lds_bytes(P, T);
+    dim3 grid((H + 1 + T - 1) / T, n);
+    if (T == 8)
+        hipLaunchKernelGGL(k_prep_cols<8>, grid, dim3(256), lds, s, B, P, off, slices, defocus, cc, scale, plan, tw);
+    else
+        hipLaunchKernelGGL(k_prep_cols<4>, grid, dim3(256), lds, s, B, P, off, slices, defocus, cc, scale, plan, tw);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- normalise
+// ReconstructorVolumeData::normalize_threed (libEM/reconstructor.cpp:315-388).
+// The conjugate-plane loop pairs each weight on the x=0 and x=P/2 planes
+// with its partner at (-y,-z); the pairing is an involution, so each fixed
+// weight is norm[self] + norm[partner] except for self-paired cells, of which
+// five are doubled.  Step 1 writes the fixed planes to plane_tmp [2][P][P].
+__global__ void k_norm_planes(const float *__restrict__ norm, int P, float *__restrict__ plane_tmp)
+{
+    const int H = P / 2;
+    const size_t nnx = H + 1;
+    const int ys = blockIdx.x * blockDim.x + threadIdx.x;
+    const int zs = blockIdx.y;
+    const int pl = blockIdx.z;  // 0: x=0, 1: x=H
+    if (ys >= P) return;
+    const int xs = pl ? H : 0;
+    const int yp = (P - ys) % P, zp = (P - zs) % P;
+    const float self = norm[((size_t)zs * P + ys) * nnx + xs];
+    float v;
+    if (yp == ys && zp == zs) {
+        bool dbl;
+        if (pl == 0) dbl = (ys == 0 && zs == H) || (ys == H && zs == 0) || (ys == H && zs == H);
+        else dbl = (ys == 0 && zs == H) || (ys == H && zs == 0);
+        v = dbl ? self * 2 : self;
+    } else {
+        v = self + norm[((size_t)zp * P + yp) * nnx + xs];
+    }
+    plane_tmp[((size_t)pl * P + zs) * P + ys] = v;
+}
+
+__global__ void k_normalize(float2 *__restrict__ data, float *__restrict__ norm, int P, int sqrtnorm,
+                            const float *__restrict__ plane_tmp)
+{
+    const int H = P / 2;
+    const size_t nnx = H + 1;
+    const size_t total = nnx * P * P;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const int xs = (int)(i % nnx);
+        const size_t yz = i / nnx;
+        float d;
+        if (xs == 0 || xs == H) {
+            const int ys = (int)(yz % P), zs = (int)(yz / P);
+            d = plane_tmp[((size_t)(xs == 0 ? 0 : 1) * P + zs) * P + ys];
+            norm[i] = d;
+        } else {
+            d = norm[i];
+        }
+        if (sqrtnorm) d = __fsqrt_rn(d);
+        float2 v = data[i];
+        if (d == 0) v = make_float2(0.f, 0.f);
+        else v = make_float2(__fdiv_rn(v.x, d), __fdiv_rn(v.y, d));
+        data[i] = v;
+    }
+}
+
+hipError_t launch_normalize(float2 *data, float *norm, int P, int sqrtnorm, float *plane_tmp, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_norm_planes, dim3((P + 127) / 128, P, 2), dim3(128), 0, s, norm, P, plane_tmp);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_normalize, dim3(4096), dim3(256), 0, s, data, norm, P, sqrtnorm, plane_tmp);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- 3-D inverse
+// In-place inverse complex transform of length P along z (pass 0) or y
+// (pass 1) for tiles of T consecutive x; inverse = conj o forward o conj.
+template <int T>
+__global__ void __launch_bounds__(256) k_fft_axis_inv(float2 *__restrict__ data, int P, size_t axis_stride,
+                                                      size_t outer_stride, FftPlan plan,
+                                                      const float2 *__restrict__ twg)
+{
+    extern __shared__ float2 smem[];
+    float2 *tw = smem;
+    float2 *A = tw + P;
+    float2 *Bf = A + T * P;
+    const int H = P / 2;
+    const int x0 = blockIdx.x * T;
+    float2 *base = data + (size_t)blockIdx.y * outer_stride;
+    load_twiddles(tw, twg, P);
+    for (int idx = threadIdx.x; idx < T * P; idx += blockDim.x) {
+        const int n = idx / T;
+        const int t = idx - n * T;
+        const int x = x0 + t;
+        float2 v = make_float2(0.f, 0.f);
+        if (x <= H) v = conjf2(base[(size_t)n * axis_stride + x]);
+        A[n * T + t] = v;
+    }
+    __syncthreads();
+    float2 *R = fft_lds<T>(A, Bf, plan, tw);
+    for (int idx = threadIdx.x; idx < T * P; idx += blockDim.x) {
+        const int n = idx / T;
+        const int t = idx - n * T;
+        const int x = x0 + t;
+        if (x <= H) base[(size_t)n * axis_stride + x] = conjf2(R[n * T + t]);
+    }
+}
+
+hipError_t launch_fft3d_inverse_zy(float2 *data, int P, const FftPlan &plan, const float2 *tw, hipStream_t s)
+{
+    const int H = P / 2;
+    const int T = pick_T(P);
+    const size_t lds = fft_lds_bytes(P, T);
+    const size_t row = H + 1;
+    dim3 grid((H + 1 + T - 1) / T, P);
+    // z pass: tiles at fixed y
+    if (T == 8)
+        hipLaunchKernelGGL(k_fft_axis_inv<8>, grid, dim3(256), lds, s, data, P, (size_t)P * row, row, plan, tw);
+    else
+        hipLaunchKernelGGL(k_fft_axis_inv<4>, grid, dim3(256), lds, s, data, P, (size_t)P * row, row, plan, tw);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // y pass: tiles at fixed z
+    if (T == 8)
+        hipLaunchKernelGGL(k_fft_axis_inv<8>, grid, dim3(256), lds, s, data, P, row, (size_t)P * row, plan, tw);
+    else
+        hipLaunchKernelGGL(k_fft_axis_inv<4>, grid, dim3(256), lds, s, data, P, row, (size_t)P * row, plan, tw);
+    return hipGetLastError();
+}
+
+// c2r along x for row pairs, x 1/P^3 (do_ift_inplace, emdata_transform.cpp:392-425),
+// depad, and phase origin to centre (fftshift) on the write.
+template <int T>
+__global__ void __launch_bounds__(256) k_c2r_x(const float2 *__restrict__ data, int P, float *__restrict__ out,
+                                               float scale, FftPlan plan, const float2 *__restrict__ twg)
+{
+    extern __shared__ float2 smem[];
+    float2 *tw = smem;
+    float2 *A = tw + P;
+    float2 *Bf = A + T * P;
+    const int H = P / 2;
+    const size_t nrows = (size_t)P * P;
+    const size_t row0 = (size_t)blockIdx.x * 2 * T;
+    load_twiddles(tw, twg, P);
+    for (int idx = threadIdx.x; idx < T * P; idx += blockDim.x) {
+        const int t = idx / P;
+        const int k = idx - t * P;
+        const size_t ra = row0 + 2 * t, rc = ra + 1;
+        float2 a = make_float2(0.f, 0.f), c = make_float2(0.f, 0.f);
+        if (ra < nrows) {
+            a = k <= H ? data[ra * (H + 1) + k] : conjf2(data[ra * (H + 1) + (P - k)]);
+            if (k == 0 || k == H) a.y = 0.f;
+        }
+        if (rc < nrows) {
+            c = k <= H ? data[rc * (H + 1) + k] : conjf2(data[rc * (H + 1) + (P - k)]);
+            if (k == 0 || k == H) c.y = 0.f;
+        }
+        // Z = A + iC, inverse via conj o forward o conj
+        A[k * T + t] = make_float2(a.x - c.y, -(a.y + c.x));
+    }
+    __syncthreads();
+    float2 *R = fft_lds<T>(A, Bf, plan, tw);
+    for (int idx = threadIdx.x; idx < T * P; idx += blockDim.x) {
+        const int t = idx / P;
+        const int x = idx - t * P;
+        const size_t ra = row0 + 2 * t;
+        const float2 z = conjf2(R[x * T + t]);
+        const int xc = x + H < P ? x + H : x + H - P;
+        for (int q = 0; q < 2; q++) {
+            const size_t r = ra + q;
+            if (r >= nrows) break;
+            const int y = (int)(r % P), zz = (int)(r / P);
+            const int yc = y + H < P ? y + H : y + H - P;
+            const int zc = zz + H < P ? zz + H : zz + H - P;
+            out[((size_t)zc * P + yc) * P + xc] = (q == 0 ? z.x : z.y) * scale;
+        }
+    }
+}
+
+hipError_t launch_c2r_x(const float2 *data, int P, float *out, float scale, const FftPlan &plan, const float2 *tw,
+                        hipStream_t s)
+{
+    const int T = pick_T(P);
+    const size_t lds = fft_lds_bytes(P, T);
+    const size_t nrows = (size_t)P * P;
+    dim3 grid((unsigned)((nrows + 2 * T - 1) / (2 * T)));
+    if (T == 8)
+        hipLaunchKernelGGL(k_c2r_x<8>, grid, dim3(256), lds, s, data, P, out, scale, plan, tw);
+    else
+        hipLaunchKernelGGL(k_c2r_x<4>, grid, dim3(256), lds, s, data, P, out, scale, plan, tw);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- fills
+// Fill kernels instead of hipMemsetAsync so the sequences stay capturable.
+__global__ void k_fill_f32(float *p, float v, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_i32(int *p, int v, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_scale_f32(float *p, float v, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = p[i] * v;
+}
+
+static unsigned fill_grid(size_t n)
+{
+    size_t g = (n + 255) / 256;
+    return (unsigned)(g < 8192 ? (g ? g : 1) : 8192);
+}
+
+hipError_t launch_fill_f32(float *p, float v, size_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_f32, dim3(fill_grid(n)), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+hipError_t launch_fill_i32(int *p, int v, size_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_i32, dim3(fill_grid(n)), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+hipError_t launch_scale_f32(float *p, float v, size_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_scale_f32, dim3(fill_grid(n)), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+// setup_seed weights (libEM/reconstructor.cpp:870-904): seed weight
+// everywhere, halved on the x=0 and x=P/2 planes except four cells.
+__global__ void k_seed_weights(float *norm, int P, float w)
+{
+    const int H = P / 2;
+    const size_t nnx = H + 1;
+    const size_t total = nnx * P * P;
+    const float half = (float)(w / 2.0);
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const int xs = (int)(i % nnx);
+        const size_t yz = i / nnx;
+        const int ys = (int)(yz % P), zs = (int)(yz / P);
+        float v = w;
+        if (xs == 0 || xs == H) {
+            v = half;
+            if ((ys == H && zs == 0) || (ys == 0 && zs == H)) v = w;
+        }
+        norm[i] = v;
+    }
+}
+
+hipError_t launch_seed_weights(float *norm, int P, float w, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_seed_weights, dim3(4096), dim3(256), 0, s, norm, P, w);
+    return hipGetLastError();
+}
+
+}  // namespace fh

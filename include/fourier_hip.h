@@ -1,0 +1,179 @@
+/*
+ * fourier_hip.h -- C ABI of the MI355X direct Fourier inversion
+ * reconstructor ("fourier_hip").
+ *
+ * The entry points are the calls libEM's Reconstructor surface makes on the
+ * "fourier" Reconstructor (libEM/reconstructor.h:108-206,
+ * libEM/reconstructor.cpp:686-1641), flattened to plain pointers and sizes
+ * so a libEM-side `FourierHipReconstructor : Reconstructor`, a Boost.Python /
+ * pybind11 module or ctypes can bind them.  No C++ exceptions cross this
+ * boundary: every call returns an int status (FH_OK = 0), and the message of
+ * the last failure on a handle is fh_last_error(h).
+ *
+ * Array layouts are the reference's:
+ *   image   : P x P float32, row-major (y, x)
+ *   slice   : P rows x (P+2) floats, interleaved re/im, rows in wrap order
+ *             (libEM/emdata_transform.cpp:118-163 do_fft_inplace layout)
+ *   volume  : P x P x (P+2) floats [z][y][x re/im]  (reconstructor.cpp:751-787)
+ *   weights : P x P x (P/2+1) floats                  (reconstructor.cpp:798-802)
+ *   xform   : 12 floats, the 3x4 row-major Transform::matrix
+ *             (libEM/transform.h:503)
+ * where P is the padded box ("size" parameter).
+ *
+ * Thread safety: one handle may be called from several host threads at
+ * once (e2make3dpar.py:355-381 does this); calls on a handle are
+ * serialised internally.  Different handles run on their own HIP streams
+ * and overlap on the device.
+ */
+#ifndef FOURIER_HIP_H
+#define FOURIER_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes: the reference's exception types (libEM/exception.h:67-380) */
+#define FH_OK 0
+#define FH_SKIPPED (-1)            /* insert_slice with weight 0 (reconstructor.cpp:1088) */
+#define FH_ERR_INVALID_PARAM 1     /* InvalidParameterException */
+#define FH_ERR_NOT_EXISTING 2      /* NotExistingObjectException (unknown sym/mode) */
+#define FH_ERR_IMAGE_DIM 3         /* ImageDimensionException */
+#define FH_ERR_NULL 4              /* NullPointerException */
+#define FH_ERR_DEVICE 5            /* HIP runtime error */
+#define FH_ERR_STATE 6             /* call before setup() or after finish() */
+#define FH_ERR_NOMEM 7             /* device allocation refused */
+
+/* memory kinds for batch inputs */
+#define FH_HOST 0
+#define FH_DEVICE 1
+
+typedef struct fh_reconstructor fh_reconstructor;
+
+/* Construction parameters: the "fourier" Reconstructor's TypeDict
+ * (libEM/reconstructor.h:501-517) restricted to what this path implements. */
+typedef struct {
+    int nx, ny, nz;        /* "size": padded real-space box, nx==ny==nz, even */
+    const char *sym;       /* "sym": c<n> or d<n> (symmetry.cpp:75-142) */
+    const char *mode;      /* "mode": "gauss_2" or "trilinear" */
+    int corners;           /* "corners" (reconstructor.cpp:1175-1181) */
+    int sqrtnorm;          /* "sqrtnorm" (reconstructor.cpp:1564) */
+    int usessnr;           /* "usessnr": must be 0 (no class_ssnr support) */
+    int verbose;           /* "verbose" */
+    int device;            /* HIP device ordinal */
+} fh_params;
+
+/* CTF constants shared by a batch; defocus comes per particle.  Used for the
+ * CTF_SIGN phase flip of e2make3dpar.py:704-714 (ctf.cpp:1076-1090). */
+typedef struct {
+    float voltage;  /* kV */
+    float cs;       /* mm */
+    float ampcont;  /* % amplitude contrast */
+    float apix;     /* A/pixel */
+} fh_ctf;
+
+/* Tuning knobs of the batched insertion (picked by the autotuner). */
+typedef struct {
+    int brick;          /* brick edge in voxels: 8 or 16 */
+    int slices_per_item;/* slices a work item accumulates before flushing */
+    int chunk;          /* particles per batched launch */
+    int wgs_per_cu;     /* persistent grid occupancy */
+} fh_tuning;
+
+/* Timings and counters of the last batched call (device time, ms). */
+typedef struct {
+    double prep_ms, bin_ms, insert_ms, total_ms;
+    double insert_kernel_ms;     /* the brick insertion kernel alone */
+    long long slices, pairs, items, chunks;
+} fh_stats;
+
+/* Reconstructors.get("fourier_hip", {...}) -- Factory<Reconstructor>::get
+ * (libEM/emobject.h:801-834): validates the parameters. */
+int fh_create(const fh_params *p, fh_reconstructor **out);
+int fh_destroy(fh_reconstructor *h);
+/* message of the last failure on h (h == NULL: last failure of fh_create
+ * on this thread) */
+const char *fh_last_error(const fh_reconstructor *h);
+
+/* FourierReconstructor::setup (reconstructor.cpp:730-818) */
+int fh_setup(fh_reconstructor *h);
+/* FourierReconstructor::setup_seed (reconstructor.cpp:820-918); seed is a
+ * host volume in the reference layout */
+int fh_setup_seed(fh_reconstructor *h, const float *seed, float seed_weight);
+/* FourierReconstructor::setup_seedandweights (reconstructor.cpp:920-996) */
+int fh_setup_seedandweights(fh_reconstructor *h, const float *seed, const float *weights);
+/* FourierReconstructor::clear (reconstructor.cpp:999-1020) */
+int fh_clear(fh_reconstructor *h);
+
+/* FourierReconstructor::preprocess_slice (reconstructor.cpp:1022-1061) for
+ * a pure rotation: host P x P image -> host slice. */
+int fh_preprocess_slice(fh_reconstructor *h, const float *img, float *slice_out);
+
+/* FourierReconstructor::insert_slice (reconstructor.cpp:1064-1119).
+ * data is a host P x P image (is_preprocessed = 0) or a host slice
+ * (is_preprocessed = 1, the "reconstruct_preproc" attribute).
+ * Returns FH_SKIPPED for weight 0. */
+int fh_insert_slice(fh_reconstructor *h, const float *data, int is_preprocessed,
+                    const float xform[12], float weight);
+
+/* Batched insertion of n raw particles (box x box, box <= P) with the
+ * caller prep of e2make3dpar.py:630-654,704-716 fused in: edge-mean
+ * subtract, centre zero-pad to P, optional CTF_SIGN flip (defocus != NULL
+ * and ctf != NULL), phase-origin shift, r2c FFT, scale, then gridding of all
+ * n x nsym slices.  imgs/xforms/weights/defocus live in `where` memory. */
+int fh_insert_batch(fh_reconstructor *h, const float *imgs, int n, int box,
+                    const float *xforms, const float *weights, const float *defocus,
+                    const fh_ctf *ctf, int where);
+
+/* The prep stage of fh_insert_batch alone: n raw particles -> n slices
+ * (host, n x P x (P+2) floats).  The caller prep of e2make3dpar.py:630-654,
+ * 704-714 followed by preprocess_slice (reconstructor.cpp:1022-1061). */
+int fh_prep_batch(fh_reconstructor *h, const float *imgs, int n, int box, const float *defocus,
+                  const fh_ctf *ctf, float *slices_out, int where);
+
+/* Same contract, structured like the reference's CUDA path
+ * (reconstructor.cpp:1074-1078,1141-1157; cuda/cuda_reconstructor.cu:171-193):
+ * per particle a host->device copy, one insertion launch per symmetry op,
+ * a synchronise after every launch.  The "plain" GPU form. */
+int fh_insert_plain(fh_reconstructor *h, const float *imgs, int n, int box,
+                    const float *xforms, const float *weights, const float *defocus,
+                    const fh_ctf *ctf);
+
+/* FourierReconstructor::determine_slice_agreement (reconstructor.cpp:1212-1257)
+ * for one slice: out = {reconstruct_norm, reconstruct_absqual,
+ * reconstruct_absqual_lowres, reconstruct_qual, reconstruct_weight}. */
+int fh_determine_slice_agreement(fh_reconstructor *h, const float *data, int is_preprocessed,
+                                 const float xform[12], float weight, int sub, float apix,
+                                 float out[5]);
+
+/* Batched agreement of n raw particles against the current volume, each
+ * compared as if its own contribution were removed (sub) -- the
+ * e2make3d.py:447-491 reweighting pass.  out is n x 5 (host or device per
+ * `where`). */
+int fh_agreement_batch(fh_reconstructor *h, const float *imgs, int n, int box,
+                       const float *xforms, const float *weights, const float *defocus,
+                       const fh_ctf *ctf, int sub, float apix, float *out, int where);
+
+/* FourierReconstructor::finish (reconstructor.cpp:1552-1641):
+ * normalize_threed, then (doift) 3-D c2r, x 1/P^3, depad, phase origin to
+ * centre.  out (host): P^3 floats if doift, else the normalised complex
+ * volume.  norm_out (optional, host): the fixed-up weights ("normout").
+ * The volume is released afterwards, as the reference does. */
+int fh_finish(fh_reconstructor *h, int doift, float *out, float *norm_out);
+
+/* Copy the accumulating (complex, weight) state to host buffers
+ * (the normout/savenorm export of reconstructor.cpp:1621-1631 plus the
+ * complex sums, for checkpointing with fh_setup_seedandweights). */
+int fh_get_state(fh_reconstructor *h, float *data, float *weights);
+
+int fh_set_tuning(fh_reconstructor *h, const fh_tuning *t);
+int fh_get_tuning(const fh_reconstructor *h, fh_tuning *t);
+int fh_get_stats(const fh_reconstructor *h, fh_stats *s);
+/* wait for all queued device work of this handle */
+int fh_sync(fh_reconstructor *h);
+/* device bytes this handle holds / would hold after setup for size P */
+long long fh_footprint_bytes(int P, int nsym, int chunk);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,79 @@
+"""The C-ABI library loads and exports every entry point include/fourier_hip.h
+declares; parameter validation that needs no device behaves like the
+reference's Factory::get.  (CPU only: no compute calls.)"""
+import ctypes
+import os
+import re
+
+import pytest
+
+from eman2_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    src = open(os.path.join(ROOT, "include", "fourier_hip.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(fh_[a-z_0-9]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.load()
+    names = declared()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(L, n), n
+        assert n in _lib.EXPORTS, "ctypes binding lacks %s" % n
+
+
+def _create(**kw):
+    p = dict(nx=96, ny=96, nz=96, sym=b"c1", mode=b"gauss_2", corners=0, sqrtnorm=0, usessnr=0, verbose=0, device=0)
+    p.update(kw)
+    prm = _lib.Params(**p)
+    h = ctypes.c_void_p()
+    rc = _lib.load().fh_create(ctypes.byref(prm), ctypes.byref(h))
+    return rc, _lib.load().fh_last_error(None).decode()
+
+
+@pytest.mark.parametrize("kw,code,word", [
+    (dict(nx=97, ny=97, nz=97), 3, "ImageDimension"),
+    (dict(nx=96, ny=96, nz=64), 3, "cubic"),
+    (dict(nx=98, ny=98, nz=98), 3, "2^a 3^b 5^c"),
+    (dict(sym=b"x5"), 2, "symmetry"),
+    (dict(sym=b"icos"), 2, "symmetry"),
+    (dict(mode=b"gauss_5"), 2, "mode"),
+    (dict(usessnr=1), 1, "usessnr"),
+])
+def test_create_validation(kw, code, word):
+    rc, msg = _create(**kw)
+    assert rc == code
+    assert word in msg
+
+
+def test_null_arguments():
+    L = _lib.load()
+    assert L.fh_create(None, None) == 4
+    assert L.fh_setup(None) == 4
+    assert L.fh_insert_batch(None, None, 0, 0, None, None, None, None, 0) == 4
+    assert L.fh_destroy(None) == 0
+
+
+def test_python_factory_validation():
+    from eman2_amd.reconstructor import Reconstructors, FourierHipReconstructor
+    assert "fourier_hip" in Reconstructors.get_list()
+    with pytest.raises(RuntimeError, match="NotExistingObject"):
+        Reconstructors.get("fourier_nope", {"size": (96, 96, 96)})
+    with pytest.raises(RuntimeError, match="InvalidParameter"):
+        Reconstructors.get("fourier_hip", {"size": (96, 96, 96), "bogus": 1})
+    with pytest.raises(RuntimeError, match="InvalidParameter"):
+        FourierHipReconstructor({"subvolume": (0, 0, 0, 10, 10, 10)})
+    with pytest.raises(RuntimeError, match="ImageDimension"):
+        Reconstructors.get("fourier_hip", {"size": (95, 95, 95)})
+
+
+def test_footprint_grows_with_size():
+    L = _lib.load()
+    a, b = L.fh_footprint_bytes(96, 1, 256), L.fh_footprint_bytes(384, 1, 256)
+    assert 0 < a < b
+    assert b >= 12 * 384 * 384 * 193

@@ -1,0 +1,280 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle and the
+committed golden fixtures.  Bars (tolerances written here):
+  prep / slices        max|a-b| <= 2e-5 * max|b|
+  weights              per-voxel relative error <= 1e-4
+  complex sums, maps   max|a-b| <= 1e-4 * max|b|   and FSC >= 0.9999 to Nyquist
+(FP32 atomics make the summation order differ from the oracle's.)"""
+import os
+import threading
+
+import numpy as np
+import pytest
+
+from eman2_amd import synthetic as S
+from eman2_amd.metrics import fsc, max_rel_err, weight_rel_err
+from eman2_amd.transform import Transform, euler_eman_matrix
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+CTF = dict(voltage=300.0, cs=2.7, ampcont=10.0, apix=1.0)
+
+
+def recon(P, sym="c1", mode="gauss_2", **kw):
+    from eman2_amd import Reconstructors
+    print("recon P=%d sym=%s mode=%s %s" % (P, sym, mode, kw), flush=True)
+    r = Reconstructors.get("fourier_hip", dict(size=(P, P, P), sym=sym, mode=mode, **kw))
+    r.setup()
+    return r
+
+
+def oracle_volume(O, P, sym, mode, slices, mats, weights):
+    v = O.Volume(P, sym, mode)
+    for s, m, w in zip(slices, mats, weights):
+        v.insert(s, m, w)
+    return v
+
+
+def assert_state_close(gd, gn, od, on, tol=1e-4):
+    assert weight_rel_err(gn, on) <= tol, weight_rel_err(gn, on)
+    assert max_rel_err(gd, od) <= tol, max_rel_err(gd, od)
+
+
+def assert_map_close(a, b, tol=1e-4, fmin=0.9999):
+    assert max_rel_err(a, b) <= tol, max_rel_err(a, b)
+    f = fsc(a, b)
+    assert f.min() >= fmin, f.min()
+
+
+# ------------------------------------------------------------------ prep
+@pytest.mark.parametrize("P", [16, 96, 384])
+def test_preprocess_slice(oracle, P):
+    img = np.random.default_rng(P).standard_normal((P, P)).astype(np.float32)
+    r = recon(P)
+    g = r.preprocess_slice(img).numpy()
+    o = oracle.preprocess_slice(img)
+    assert max_rel_err(g, o) <= 2e-5
+
+
+@pytest.mark.parametrize("B,P,ctf", [(64, 96, False), (64, 96, True), (63, 96, True), (96, 96, False),
+                                     (256, 384, True), (192, 288, True), (400, 600, False)])
+def test_prep_batch(oracle, B, P, ctf):
+    rng = np.random.default_rng(B + P)
+    n = 5
+    imgs = (rng.standard_normal((n, B, B)) + 3.0).astype(np.float32)
+    dfs = rng.uniform(0.8, 2.5, n).astype(np.float32) if ctf else None
+    r = recon(P)
+    g = r.prep_batch(imgs, defocus=dfs, ctf=CTF if ctf else None)
+    for i in range(n):
+        c = dict(CTF, defocus=float(dfs[i])) if ctf else None
+        o = oracle.prep_particle(imgs[i], P, c)
+        assert max_rel_err(g[i], o) <= 2e-5, (i, max_rel_err(g[i], o))
+
+
+# ------------------------------------------------------------------ golden fixture
+@pytest.mark.parametrize("sym", ["c1", "d2"])
+@pytest.mark.parametrize("mode", ["gauss_2", "trilinear"])
+def test_golden_insert_slice(sym, mode):
+    from eman2_amd.reconstructor import EMData
+    d = np.load(os.path.join(GOLD, "recon_p16.npz"))
+    P = 16
+    r = recon(P, sym, mode)
+    for s, m, w in zip(d["slices"], d["mats"], d["weights"]):
+        r.insert_slice(EMData(s, True, {"reconstruct_preproc": 1}), Transform(m), float(w))
+    gd, gn = r.get_state()
+    key = "%s_%s" % (sym, mode)
+    assert_state_close(gd, gn, d[key + "_data"], d[key + "_norm"], tol=1e-5)
+    assert_map_close(r.finish(True, return_numpy=True), d[key + "_map"], tol=1e-5)
+
+
+@pytest.mark.parametrize("sym", ["c1", "d2"])
+@pytest.mark.parametrize("mode", ["gauss_2", "trilinear"])
+@pytest.mark.parametrize("brick", [8, 16])
+def test_golden_batch(sym, mode, brick):
+    d = np.load(os.path.join(GOLD, "recon_p16.npz"))
+    P = 16
+    r = recon(P, sym, mode, brick=brick, slices_per_item=2)
+    r.insert_batch(d["imgs"], list(d["mats"]), d["weights"])
+    gd, gn = r.get_state()
+    key = "%s_%s" % (sym, mode)
+    assert_state_close(gd, gn, d[key + "_data"], d[key + "_norm"], tol=1e-5)
+    assert_map_close(r.finish(True, return_numpy=True), d[key + "_map"], tol=1e-5)
+
+
+# ------------------------------------------------------------------ config-1 shaped parity
+def config1_inputs(n, ctf=False, B=64, seed=1):
+    ph = S.phantom(50, 24.0, seed=seed)
+    imgs, mats, dfs = S.particles(ph, n, B, seed=seed + 1, apix=1.5, snr=0.1,
+                                  ctf=dict(CTF) if ctf else None)
+    return imgs, mats, dfs
+
+
+@pytest.mark.parametrize("brick,spi", [(16, 64), (8, 16), (16, 5)])
+def test_batch_config1(oracle, brick, spi):
+    P = 96
+    imgs, mats, _ = config1_inputs(300)
+    w = np.ones(len(imgs), np.float32)
+    r = recon(P, brick=brick, slices_per_item=spi, chunk=128)
+    r.insert_batch(imgs, list(mats), w)
+    gd, gn = r.get_state()
+    slices = [oracle.prep_particle(im, P) for im in imgs]
+    v = oracle_volume(oracle, P, "c1", "gauss_2", slices, mats, w)
+    assert_state_close(gd, gn, v.data, v.norm)
+    assert_map_close(r.finish(True, return_numpy=True), v.finish(True))
+
+
+def test_batch_ctf(oracle):
+    P = 96
+    imgs, mats, dfs = config1_inputs(120, ctf=True)
+    w = np.ones(len(imgs), np.float32)
+    r = recon(P)
+    r.insert_batch(imgs, list(mats), w, defocus=dfs, ctf=dict(CTF, apix=1.5))
+    gd, gn = r.get_state()
+    slices = [oracle.prep_particle(im, P, dict(CTF, apix=1.5, defocus=float(df))) for im, df in zip(imgs, dfs)]
+    v = oracle_volume(oracle, P, "c1", "gauss_2", slices, mats, w)
+    assert_state_close(gd, gn, v.data, v.norm)
+    assert_map_close(r.finish(True, return_numpy=True), v.finish(True))
+
+
+@pytest.mark.parametrize("sym,mode", [("c4", "gauss_2"), ("d2", "trilinear"), ("d7", "gauss_2"), ("d7", "trilinear"),
+                                      ("c1", "trilinear")])
+def test_batch_symmetry_modes(oracle, sym, mode):
+    P, B = 48, 32
+    ph = S.phantom(10, 10.0, seed=5)
+    imgs, mats, _ = S.particles(ph, 40, B, seed=6, snr=1.0)
+    w = np.linspace(0.5, 1.5, 40).astype(np.float32)
+    r = recon(P, sym, mode, slices_per_item=8)
+    r.insert_batch(imgs, list(mats), w)
+    gd, gn = r.get_state()
+    v = oracle_volume(oracle, P, sym, mode, [oracle.prep_particle(im, P) for im in imgs], mats, w)
+    assert_state_close(gd, gn, v.data, v.norm)
+    assert_map_close(r.finish(True, return_numpy=True), v.finish(True))
+
+
+def test_plain_matches_batch():
+    P = 96
+    imgs, mats, _ = config1_inputs(60)
+    w = np.ones(60, np.float32)
+    a = recon(P)
+    a.insert_batch(imgs, list(mats), w)
+    b = recon(P)
+    b.insert_plain(imgs, list(mats), w)
+    ad, an = a.get_state()
+    bd, bn = b.get_state()
+    assert_state_close(ad, an, bd, bn, tol=1e-5)
+
+
+def test_insert_remove_and_zero_weight():
+    P = 96
+    imgs, mats, _ = config1_inputs(30)
+    r = recon(P)
+    r.insert_batch(imgs, list(mats), np.ones(30, np.float32))
+    r.insert_batch(imgs, list(mats), -np.ones(30, np.float32))
+    r.insert_batch(imgs, list(mats), np.zeros(30, np.float32))  # skipped
+    r.insert_batch(imgs[:0], [], np.zeros(0, np.float32))       # empty batch
+    d, n = r.get_state()
+    ref = recon(P)
+    ref.insert_batch(imgs, list(mats), np.ones(30, np.float32))
+    rd, rn = ref.get_state()
+    # rt/pyem/test_reconstructor.py:48-88 tolerance (1e-3), relative to the peak
+    assert np.abs(d).max() <= 1e-3 * np.abs(rd).max()
+    assert np.abs(n).max() <= 1e-3 * np.abs(rn).max()
+
+
+def test_corners_and_sqrtnorm(oracle):
+    P = 32
+    ph = S.phantom(6, 6.0, seed=9)
+    imgs, mats, _ = S.particles(ph, 30, 24, seed=10, snr=2.0)
+    w = np.ones(30, np.float32)
+    r = recon(P, corners=True, sqrtnorm=True)
+    r.insert_batch(imgs, list(mats), w)
+    gd, gn = r.get_state()
+    v = oracle.Volume(P)
+    for im, m in zip(imgs, mats):
+        v.insert(oracle.prep_particle(im, P), m, 1.0, corners=True)
+    assert_state_close(gd, gn, v.data, v.norm)
+    assert_map_close(r.finish(True, return_numpy=True), v.finish(True, sqrtnorm=True))
+
+
+def test_finish_complex_and_normout(oracle):
+    P = 32
+    ph = S.phantom(6, 6.0, seed=11)
+    imgs, mats, _ = S.particles(ph, 20, 24, seed=12, snr=2.0)
+    w = np.ones(20, np.float32)
+    r = recon(P)
+    r.insert_batch(imgs, list(mats), w)
+    normout = np.zeros((P, P, P // 2 + 1), np.float32)
+    r.set_params({"normout": normout})
+    g = r.finish(False, return_numpy=True)
+    assert np.array_equal(normout, r.last_norm)
+    v = oracle_volume(oracle, P, "c1", "gauss_2", [oracle.prep_particle(im, P) for im in imgs], mats, w)
+    o = v.finish(False)
+    assert max_rel_err(g, o) <= 1e-4
+    assert weight_rel_err(r.last_norm, v.norm_fixed) <= 1e-4
+    with pytest.raises(RuntimeError, match="StateError"):
+        r.insert_batch(imgs, list(mats), w)  # finish() released the volume
+
+
+def test_setup_seed(oracle):
+    P = 16
+    rng = np.random.default_rng(13)
+    seed = rng.standard_normal((P, P, P + 2)).astype(np.float32)
+    r = recon(P)
+    r.setup_seed(seed, 0.5)
+    d = np.load(os.path.join(GOLD, "recon_p16.npz"))
+    r.insert_batch(d["imgs"], list(d["mats"]), d["weights"])
+    gd, gn = r.get_state()
+    v = oracle.Volume(P)
+    v.setup_seed(seed, 0.5)
+    for s, m, w in zip(d["slices"], d["mats"], d["weights"]):
+        v.insert(s, m, w)
+    assert_state_close(gd, gn, v.data, v.norm, tol=1e-5)
+
+
+def test_seed_and_weights_round_trip():
+    P = 32
+    imgs, mats, _ = config1_inputs(20, B=24)
+    w = np.ones(20, np.float32)
+    a = recon(P)
+    a.insert_batch(imgs[:10], list(mats[:10]), w[:10])
+    data, norm = a.get_state()
+    b = recon(P)
+    b.setup_seedandweights(data, norm)
+    b.insert_batch(imgs[10:], list(mats[10:]), w[10:])
+    c = recon(P)
+    c.insert_batch(imgs, list(mats), w)
+    bd, bn = b.get_state()
+    cd, cn = c.get_state()
+    assert_state_close(bd, bn, cd, cn, tol=1e-5)
+
+
+def test_threads_share_one_reconstructor():
+    """e2make3dpar.py:355-381 calls insert_slice from several threads on one
+    reconstructor; the handle serialises them."""
+    P = 32
+    imgs, mats, _ = config1_inputs(24, B=24)
+    pad = np.zeros((24, P, P), np.float32)
+    pad[:, 4:28, 4:28] = imgs - np.array([S_edge(i) for i in imgs])[:, None, None]
+    a = recon(P)
+    ths = [threading.Thread(target=lambda k=k: [a.insert_slice(pad[i], Transform(mats[i]), 1.0)
+                                               for i in range(k, 24, 4)]) for k in range(4)]
+    [t.start() for t in ths]
+    [t.join() for t in ths]
+    b = recon(P)
+    for i in range(24):
+        b.insert_slice(pad[i], Transform(mats[i]), 1.0)
+    ad, an = a.get_state()
+    bd, bn = b.get_state()
+    assert_state_close(ad, an, bd, bn, tol=1e-5)
+
+
+def S_edge(img):
+    b = np.concatenate([img[0], img[-1], img[:, 0], img[:, -1]]).astype(np.float64)
+    return np.float32(b.mean())
+
+
+def test_transform_with_translation_rejected():
+    P = 16
+    r = recon(P)
+    img = np.zeros((P, P), np.float32)
+    with pytest.raises(RuntimeError, match="InvalidParameter"):
+        r.insert_slice(img, Transform({"type": "eman", "az": 10, "tx": 2.0}), 1.0)
