@@ -1,0 +1,237 @@
+#!/usr/bin/env python3
+"""Benchmark: particles inserted/s at 256^3 box (pad 384), c1, gauss_2.
+
+Workload (one step = one reconstruction pass over the whole batch):
+  50,000 float32 projections, 256 px box, 1.0 A/px, of a seeded 200-Gaussian
+  phantom at uniform SO(3) orientations, CTF (300 kV, Cs 2.7 mm, 10% amplitude
+  contrast, defocus U(0.8,2.5) um, B 80 A^2), SNR 0.05 noise; CTF phase flip,
+  edge-mean subtraction, zero-pad to 384, r2c FFT and gauss_2 gridding into a
+  384^3 (complex, weight) volume -- fh_insert_batch with the particles already
+  resident in HBM.  Each step re-runs setup() and inserts all particles.
+
+Prints one JSON line with the throughput, the brick-insertion kernel's
+roofline fraction (HBM-bound scatter model), the plain one-particle-per-launch
+GPU form, and the CPU oracle timed on this host's cores on a bounded sample.
+"""
+import argparse
+import json
+import math
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+CTF = dict(voltage=300.0, cs=2.7, ampcont=10.0, bfactor=80.0)
+
+
+def npix(P):
+    h = P // 2
+    y = np.arange(-h, h)[:, None]
+    x = np.arange(0, h + 1)[None, :]
+    return int(((x * x + y * y) <= h * h + h).sum())
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--particles", type=int, default=50000)
+    ap.add_argument("--box", type=int, default=256)
+    ap.add_argument("--pad", type=int, default=384)
+    ap.add_argument("--sym", default="c1")
+    ap.add_argument("--mode", default="gauss_2")
+    ap.add_argument("--apix", type=float, default=1.0)
+    ap.add_argument("--snr", type=float, default=0.05)
+    ap.add_argument("--brick", type=int, default=None)
+    ap.add_argument("--spi", type=int, default=None, help="slices per work item")
+    ap.add_argument("--chunk", type=int, default=None)
+    ap.add_argument("--wgs", type=int, default=None, help="persistent workgroups per CU")
+    ap.add_argument("--plain-sample", type=int, default=200)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a 1 GiB elementwise copy (PMC byte-count calibration)")
+    return ap.parse_args()
+
+
+def cpu_baseline(imgs_host, mats, dfs, P, sym, mode, apix, budget_s):
+    """Oracle (-O3 -ffast-math build) on this host's cores: prep (edge mean,
+    pad, CTF flip, r2c) in a thread pool + multi-threaded insertion with
+    private volumes.  Sample sized for ~budget_s of CPU work."""
+    import ctypes
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle import oracle as O
+
+    L = O.lib(fast=True)
+    T = max(1, min(16, len(os.sched_getaffinity(0))))
+    B = imgs_host.shape[-1]
+
+    def prep(i):
+        out = np.zeros((P, P + 2), np.float32)
+        c = O.ctf_struct(dict(CTF, apix=apix, defocus=float(dfs[i]))) if dfs is not None else None
+        raw = np.ascontiguousarray(imgs_host[i])
+        L.or_prep_particle(O._f(raw), B, P, ctypes.byref(c) if c is not None else None, O._f(out))
+        return out
+
+    # size the sample from a one-particle probe
+    t0 = time.perf_counter()
+    s0 = prep(0)
+    _, _, ti = O.insert_batch_mt(P, s0[None], mats[:1], np.ones(1, np.float32), sym, mode, nthreads=1)
+    one = time.perf_counter() - t0
+    n = int(max(T, min(len(imgs_host), budget_s * T / max(one, 1e-4))))
+    n = (n // T) * T or T
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(T) as ex:
+        slices = np.stack(list(ex.map(prep, range(n))))
+    t_prep = time.perf_counter() - t0
+    _, _, t_ins = O.insert_batch_mt(P, slices, mats[:n], np.ones(n, np.float32), sym, mode, nthreads=T)
+    return {
+        "value": n / (t_prep + t_ins), "unit": "particles/s", "cores": T, "kind": "port",
+        "sample": "%d particles of the same workload (prep %.2fs + insertion %.2fs, %d threads, "
+                  "oracle -O3 -ffast-math, private volumes summed)" % (n, t_prep, t_ins, T),
+        "prep_s": t_prep, "insert_s": t_ins, "insert_only_particles_per_s": n / t_ins,
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        if int(os.environ.get("RANK", "0")) == 0:
+            print("bench.py: this build measures one GPU only (the path shards by particle; see the design notes); "
+                  "--gpus %d is not supported" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from eman2_amd import Reconstructors
+    from eman2_amd import synthetic as S
+
+    dev = "cuda:0"
+    P, B, n = a.pad, a.box, a.particles
+    nsym = {"c": 1, "d": 2}[a.sym[0].lower()] * int(a.sym[1:])
+    print("[bench] generating %d particles %dpx on %s" % (n, B, dev), file=sys.stderr, flush=True)
+    ph = S.phantom(200, 96.0 * B / 256.0, seed=2)
+    imgs, mats, dfs = S.particles_device(ph, n, B, seed=3, apix=a.apix, snr=a.snr, ctf=CTF, device=dev)
+    torch.cuda.synchronize()
+    weights = np.ones(n, np.float32)
+    ctf = dict(CTF, apix=a.apix)
+
+    r = Reconstructors.get("fourier_hip", {"size": (P, P, P), "sym": a.sym, "mode": a.mode})
+    tune = r.get_tuning()
+    for k, v in (("brick", a.brick), ("slices_per_item", a.spi), ("chunk", a.chunk), ("wgs_per_cu", a.wgs)):
+        if v is not None:
+            tune[k] = v
+    r.set_tuning(**tune)
+    ptr = int(imgs.data_ptr())
+
+    def step():
+        r.setup()
+        r.insert_batch(ptr, mats, weights, defocus=dfs, ctf=ctf, box=B)
+
+    for i in range(a.warmup):
+        print("[bench] warmup %d" % i, file=sys.stderr, flush=True)
+        step()
+    torch.cuda.synchronize()
+    r.sync()
+    lat, kern_ms, stats = [], 0.0, []
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        ts = time.perf_counter()
+        step()
+        lat.append((time.perf_counter() - ts) * 1e3)
+        st = r.get_stats()
+        stats.append(st)
+        kern_ms += st["insert_kernel_ms"]
+        print("[bench] step %d %.1f ms (prep %.1f bin %.1f insert %.1f)" % (
+            i, lat[-1], st["prep_ms"], st["bin_ms"], st["insert_ms"]), file=sys.stderr, flush=True)
+    r.sync()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    value = a.steps * n / elapsed
+
+    # roofline of the dominant kernel (k_brick_insert): scatter model, one
+    # 12-byte (complex + weight) update per gridding contribution, 8 per
+    # pixel per symmetry copy
+    bytes_per_particle = nsym * npix(P) * 8 * 12
+    launches = sum(s["chunks"] for s in stats)
+    avg_ms = kern_ms / max(1, launches)
+    per_launch_bytes = bytes_per_particle * (a.steps * n) / max(1, launches)
+    achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "brick_insert_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            t = json.load(f)
+        if t.get("P") == P and t.get("sym") == a.sym:
+            traffic = t.get("hbm_bytes_per_particle", 0) * (a.steps * n) / max(1, launches)
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": "k_brick_insert", "avg_launch_ms": round(avg_ms, 3),
+            "algorithmic_bytes_per_launch": int(per_launch_bytes)}
+    # whole-pipeline effective bandwidth with the 4*B^2 raw read (BASELINE.md formula)
+    pipe_bytes = (4 * B * B + bytes_per_particle) * a.steps * n
+    pipeline = {"effective_GBs": round(pipe_bytes / elapsed / 1e9, 1),
+                "frac": round(pipe_bytes / elapsed / 1e9 / HBM_PEAK_GBS, 4)}
+
+    out = {
+        "metric": "particles inserted/s @256³ c1 gauss_2 (batched vs plain); p50/p99 job latency",
+        "value": round(value, 1), "unit": "particles/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(elapsed / a.steps * 1e3, 2), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "%d x %dpx particles, pad %d, %s, %s, CTF phase flip, SNR %g, HBM-resident" % (
+            n, B, P, a.sym, a.mode, a.snr), "box": B, "pad": P, "particles": n, "sym": a.sym, "mode": a.mode,
+            "tuning": r.get_tuning()},
+        "roofline": roof,
+        "pipeline_roofline": pipeline,
+        "job_latency_ms": {"p50": round(statistics.median(lat), 2),
+                           "p99": round(sorted(lat)[min(len(lat) - 1, int(math.ceil(0.99 * len(lat))) - 1)], 2),
+                           "what": "setup + insert_batch of all particles, per step"},
+        "stage_ms_per_step": {k: round(sum(s[k] for s in stats) / a.steps, 2)
+                              for k in ("prep_ms", "bin_ms", "insert_ms", "total_ms")},
+        "pairs_per_step": int(sum(s["pairs"] for s in stats) / a.steps),
+    }
+
+    # plain one-particle-per-launch form on a host-side sample
+    host = None
+    if not a.no_plain or not a.no_cpu:
+        m = max(a.plain_sample, 256)
+        host = imgs[:m].cpu().numpy()
+    if not a.no_plain:
+        print("[bench] plain form on %d particles" % a.plain_sample, file=sys.stderr, flush=True)
+        rp = Reconstructors.get("fourier_hip", {"size": (P, P, P), "sym": a.sym, "mode": a.mode})
+        rp.setup()
+        k = a.plain_sample
+        rp.insert_plain(host[:2], mats[:2], weights[:2], defocus=dfs[:2], ctf=ctf)  # warm
+        rp.setup()
+        t0 = time.perf_counter()
+        rp.insert_plain(host[:k], mats[:k], weights[:k], defocus=dfs[:k], ctf=ctf)
+        tp = time.perf_counter() - t0
+        pv = k / tp
+        out["plain"] = {"value": round(pv, 1), "unit": "particles/s", "sample": k,
+                        "form": "per particle: host->device copy, prep launches, one insertion launch per "
+                                "symmetry op with global atomics, synchronise after each"}
+        out["batched_vs_plain"] = round(value / pv, 2)
+        del rp
+    if not a.no_cpu:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(host, mats, dfs, P, a.sym, a.mode, a.apix, a.cpu_seconds)
+    if a.calibrate_copy:
+        x = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device=dev).fill_(1.0)
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        out["copy_calibration_bytes"] = {"read": x.numel() * 4, "write": y.numel() * 4}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

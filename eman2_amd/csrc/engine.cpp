@@ -205,6 +205,7 @@ struct fh_reconstructor {
     float *imgs = nullptr, *means = nullptr, *rots = nullptr, *wts = nullptr, *dfs = nullptr;
     float2 *slices = nullptr;
     float *t3d = nullptr, *wq = nullptr;
+    double *partial = nullptr;  // agreement partial sums
     // binning
     BrickGeom geom{};
     BinBuffers bins{};
@@ -379,9 +380,9 @@ int fh_create(const fh_params *p, fh_reconstructor **out)
     for (auto &ev : h->ev) (void)hipEventCreate(&ev);
     // constant tables: fast_exp (util.cpp:792-794), fl(i/P), twiddles, syms
     const int P = h->P, H = h->H;
-    std::vector<float> et(400), rt(P + 1);
+    std::vector<float> et(400), rt(P + 2);
     for (int i = 0; i < 400; i++) et[i] = (float)exp(-i / 50.0);
-    for (int i = 0; i <= P; i++) rt[i] = (float)(i - H) / (float)P;
+    for (int i = 0; i <= P + 1; i++) rt[i] = (float)(i - H) / (float)P;  // fl(i/P), i in [-H, H+1]
     std::vector<float2> tw(P);
     for (int m = 0; m < P; m++) {
         double a = -2.0 * M_PI * (double)m / P;
@@ -390,11 +391,11 @@ int fh_create(const fh_params *p, fh_reconstructor **out)
     make_plan(P, &h->plan);
     h->scale = (float)sqrt((double)(1.0f / (float)P * (float)(P + 2)));
     if ((e = hipMalloc(&h->exptab, sizeof(float) * 400)) != hipSuccess ||
-        (e = hipMalloc(&h->rtab, sizeof(float) * (P + 1))) != hipSuccess ||
+        (e = hipMalloc(&h->rtab, sizeof(float) * (P + 2))) != hipSuccess ||
         (e = hipMalloc(&h->tw, sizeof(float2) * P)) != hipSuccess ||
         (e = hipMalloc(&h->syms_d, sizeof(float) * ops.size())) != hipSuccess ||
         (e = hipMemcpy(h->exptab, et.data(), sizeof(float) * 400, hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = hipMemcpy(h->rtab, rt.data(), sizeof(float) * (P + 1), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(h->rtab, rt.data(), sizeof(float) * (P + 2), hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(h->tw, tw.data(), sizeof(float2) * P, hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(h->syms_d, ops.data(), sizeof(float) * ops.size(), hipMemcpyHostToDevice)) != hipSuccess) {
         g_create_error = std::string("device error: ") + hipGetErrorString(e);
@@ -415,6 +416,7 @@ int fh_destroy(fh_reconstructor *h)
         free_chunk(h);
         free_bins(h);
         free_volume(h);
+        if (h->partial) (void)hipFree(h->partial);
         for (void *p : {(void *)h->exptab, (void *)h->rtab, (void *)h->tw, (void *)h->syms_d})
             if (p) (void)hipFree(p);
         for (auto &ev : h->ev)
@@ -719,21 +721,128 @@ int fh_insert_plain(fh_reconstructor *h, const float *imgs, int n, int box, cons
     return FH_OK;
 }
 
+// rmax of do_compare_slice_work (libEM/reconstructor.cpp:1272-1274)
+static float agree_rmax(int P, float apix)
+{
+    const float iny = (float)P;
+    float rmax = (float)(iny * apix / 12.0);
+    if (rmax > iny / 2.0 - 1.0) rmax = (float)(iny / 2.0 - 1.0);
+    if (rmax < 5.0) rmax = 5.0f;
+    return rmax;
+}
+
+// One slice already in slice_d: optional subtraction, compare, re-add
+// (libEM/reconstructor.cpp:1212-1257) -> out[5] (reconstructor.cpp:1365-1378).
+static int agree_one(fh_reconstructor *h, const float2 *slice_d, const float *rot9, float weight, int sub, float apix,
+                     float out[5])
+{
+    InsertConst c = iconst(h);
+    c.corners = 0;  // do_insert_slice_work default (reconstructor.h:540)
+    const int nb = agree_blocks(h->P);
+    std::vector<float> t(9 * (size_t)h->nsym);
+    for (int s = 0; s < h->nsym; s++) compose9(rot9, &h->syms[9 * s], &t[9 * (size_t)s]);
+    FH_TRY(h, hipMemcpyAsync(h->t3d, t.data(), sizeof(float) * t.size(), hipMemcpyHostToDevice, h->stream));
+    if (sub)
+        for (int s = 0; s < h->nsym; s++)
+            FH_TRY(h, launch_insert_plain(slice_d, c, &t[9 * (size_t)s], -weight, h->data, h->norm, h->exptab, h->rtab,
+                                          h->stream));
+    FH_TRY(h, launch_agree(slice_d, h->nsym, h->t3d, h->nsym, c, agree_rmax(h->P, apix), h->data, h->norm, h->rtab,
+                           h->partial, h->stream));
+    std::vector<double> part((size_t)h->nsym * nb * 8);
+    FH_TRY(h, hipMemcpyAsync(part.data(), h->partial, sizeof(double) * part.size(), hipMemcpyDeviceToHost, h->stream));
+    if (sub)
+        for (int s = 0; s < h->nsym; s++)
+            FH_TRY(h, launch_insert_plain(slice_d, c, &t[9 * (size_t)s], weight, h->data, h->norm, h->exptab, h->rtab,
+                                          h->stream));
+    FH_TRY(h, hipStreamSynchronize(h->stream));
+    double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (size_t i = 0; i < part.size(); i++) a[i % 8] += part[i];
+    double dot = a[0], vweight = a[1], power = a[2], power2 = a[3], dotlow = a[4], dlpow = a[5], dlpow2 = a[6];
+    const int nval = (int)a[7];
+    dot /= sqrt(power * power2);
+    if (dlpow * dlpow2 > 0) dotlow /= sqrt(dlpow * dlpow2);
+    out[0] = (float)(power2 <= 0 ? 1.0 : sqrt(power / power2));
+    out[1] = (float)dot;
+    out[2] = (float)dotlow;
+    const float rw = weight <= 0 ? 1.0f : 1.0f / weight;
+    out[3] = (float)(dot * rw / ((rw - 1.0) * dot + 1.0));
+    out[4] = (float)(vweight / (float)(nval));
+    return FH_OK;
+}
+
+static int ensure_partial(fh_reconstructor *h)
+{
+    if (h->partial) return FH_OK;
+    FH_TRY(h, hipMalloc(&h->partial, sizeof(double) * 8 * (size_t)agree_blocks(h->P) * h->nsym));
+    return FH_OK;
+}
+
 int fh_determine_slice_agreement(fh_reconstructor *h, const float *data, int is_preprocessed, const float xform[12],
                                  float weight, int sub, float apix, float out[5])
 {
-    (void)data; (void)is_preprocessed; (void)xform; (void)weight; (void)sub; (void)apix; (void)out;
     if (!h) return FH_ERR_NULL;
-    return fail(h, FH_ERR_STATE, "determine_slice_agreement is not built yet");
+    if (!data || !xform || !out) return fail(h, FH_ERR_NULL, "NullPointerException: EMData pointer (input image) is NULL");
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (!h->ready) return fail(h, FH_ERR_STATE, "determine_slice_agreement() before setup()");
+    FH_TRY(h, hipSetDevice(h->device));
+    const int P = h->P, H = h->H;
+    std::vector<float> rot9;
+    int rc = check_xforms(h, xform, 1, !is_preprocessed, rot9);
+    if (rc) return rc;
+    if ((rc = ensure_chunk(h, std::max(1, h->cap_n), std::max(P, h->cap_B)))) return rc;
+    if ((rc = ensure_partial(h))) return rc;
+    if (is_preprocessed) {
+        FH_TRY(h, hipMemcpyAsync(h->slices, data, sizeof(float2) * P * (H + 1), hipMemcpyHostToDevice, h->stream));
+    } else {
+        FH_TRY(h, hipMemcpyAsync(h->imgs, data, sizeof(float) * P * P, hipMemcpyHostToDevice, h->stream));
+        FH_TRY(h, launch_fill_f32(h->means, 0.f, 1, h->stream));
+        FH_TRY(h, launch_prep_rows(h->imgs, 1, P, P, h->means, h->slices, h->plan, h->tw, h->stream));
+        FH_TRY(h, launch_prep_cols(1, P, P, h->slices, nullptr, CtfConst{}, h->scale, h->plan, h->tw, h->stream));
+    }
+    return agree_one(h, h->slices, rot9.data(), weight, sub, apix, out);
 }
 
 int fh_agreement_batch(fh_reconstructor *h, const float *imgs, int n, int box, const float *xforms, const float *weights,
                        const float *defocus, const fh_ctf *ctf, int sub, float apix, float *out, int where)
 {
-    (void)imgs; (void)n; (void)box; (void)xforms; (void)weights; (void)defocus; (void)ctf; (void)sub; (void)apix;
-    (void)out; (void)where;
     if (!h) return FH_ERR_NULL;
-    return fail(h, FH_ERR_STATE, "agreement_batch is not built yet");
+    std::lock_guard<std::mutex> lk(h->mu);
+    int rc = check_batch_args(h, imgs, n, box, xforms, weights);
+    if (rc) return rc;
+    if (!out) return fail(h, FH_ERR_NULL, "NullPointerException: output is NULL");
+    FH_TRY(h, hipSetDevice(h->device));
+    std::vector<float> rot9;
+    if ((rc = check_xforms(h, xforms, n, 1, rot9))) return rc;
+    const int chunk = std::max(1, std::min(h->tune.chunk, std::max(n, 1)));
+    if ((rc = ensure_chunk(h, chunk, box))) return rc;
+    if ((rc = ensure_partial(h))) return rc;
+    const size_t se = (size_t)h->P * (h->H + 1);
+    std::vector<float> res(5 * (size_t)std::max(n, 1));
+    for (int c0 = 0; c0 < n; c0 += chunk) {
+        const int m = std::min(chunk, n - c0);
+        const float *src = imgs + (size_t)c0 * box * box;
+        const float *dimgs = src;
+        if (where != FH_DEVICE) {
+            FH_TRY(h, hipMemcpyAsync(h->imgs, src, sizeof(float) * (size_t)m * box * box, hipMemcpyHostToDevice, h->stream));
+            dimgs = h->imgs;
+        }
+        if ((rc = prep_chunk(h, m, box, dimgs, defocus ? defocus + c0 : nullptr, ctf))) return rc;
+        // particles in order, each against the volume without its own
+        // contribution (e2make3d.py:447-491 calls determine_slice_agreement
+        // per particle)
+        for (int i = 0; i < m; i++) {
+            if ((rc = agree_one(h, h->slices + se * i, &rot9[9 * (size_t)(c0 + i)], weights[c0 + i], sub, apix,
+                                &res[5 * (size_t)(c0 + i)])))
+                return rc;
+        }
+    }
+    if (n > 0) {
+        if (where == FH_DEVICE)
+            FH_TRY(h, hipMemcpy(out, res.data(), sizeof(float) * 5 * n, hipMemcpyHostToDevice));
+        else
+            memcpy(out, res.data(), sizeof(float) * 5 * n);
+    }
+    return FH_OK;
 }
 
 int fh_finish(fh_reconstructor *h, int doift, float *out, float *norm_out)

@@ -173,6 +173,93 @@ hipError_t launch_insert_plain(const float2 *slice, InsertConst c, const float *
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------ slice agreement
+// FourierReconstructor::do_compare_slice_work (libEM/reconstructor.cpp:1259-1381)
+// with pixel_at (:1448-1461): nearest-voxel value/weight at each slice pixel
+// x in [0, P/2+1] (the x = P/2+1 column reads the next row's x = 0 element,
+// and past the last row 0), eight double sums per (slice, block).
+__device__ __forceinline__ int round_ref(float v) { return v < 0 ? (int)(v - 0.5f) : (int)(v + 0.5f); }
+
+__global__ void __launch_bounds__(256) k_agree(const float2 *__restrict__ slices, int nsym, const float *__restrict__ t3d,
+                                               InsertConst c, float rmax, const float2 *__restrict__ data,
+                                               const float *__restrict__ norm, const float *__restrict__ rtab,
+                                               double *__restrict__ partial)
+{
+    const int H = c.H, P = c.P;
+    const int q = blockIdx.y;
+    const int pidx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int W = H + 2;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // dot vweight power power2 dotlow dlpow dlpow2 nval
+    if (pidx < W * P) {
+        const int x = pidx % W, y = pidx / W - H;
+        const float *m = t3d + 9 * (size_t)q;
+        if (!(x == 0 && y == 0)) {
+            const float rx = rtab[x + H], ry = rtab[y + H];
+            const float c0 = rx * m[0] + ry * m[3];
+            const float c1 = rx * m[1] + ry * m[4];
+            const float c2 = rx * m[2] + ry * m[5];
+            if (!(fabsf(c0) > 0.5f || fabsf(c1) >= 0.5f || fabsf(c2) >= 0.5f)) {
+                const float xx = c0 * c.Pf, yy = c1 * c.Pf, zz = c2 * c.Pf;
+                const float *Sf = reinterpret_cast<const float *>(slices + (size_t)(q / nsym) * P * (H + 1));
+                const int slen = (P + 2) * P;
+                const int fidx = 2 * x + (P + 2) * (y < 0 ? P + y : y);
+                const float d20 = fidx < slen ? Sf[fidx] : 0.f;
+                const float d21 = fidx + 1 < slen ? Sf[fidx + 1] : 0.f;
+                const int x0 = round_ref(xx), y0 = round_ref(yy), z0 = round_ref(zz);
+                float vr = 0.f, vi = 0.f;
+                size_t ci;
+                if (x0 < 0) ci = cidx(-x0, y0 <= 0 ? -y0 : P - y0, z0 <= 0 ? -z0 : P - z0, P, H);
+                else ci = cidx(x0, y0 < 0 ? P + y0 : y0, z0 < 0 ? P + z0 : z0, P, H);
+                if (!(abs(x0) >= H + 1 || abs(y0) > H || abs(z0) > H)) {
+                    const float2 v = data[ci];
+                    vr = v.x;
+                    vi = x0 < 0 ? -v.y : v.y;
+                }
+                const float nv = norm[ci];
+                if (nv != 0.f) {
+                    const float dt0 = __fdiv_rn(vr, nv), dt1 = __fdiv_rn(vi, nv), dt2 = nv;
+                    const float r = __fsqrt_rn((float)(x * x + y * y));
+                    const float dotv = (dt0 * d20 + dt1 * d21) * dt2;
+                    const float pw = (dt0 * dt0 + dt1 * dt1) * dt2;
+                    const float pw2 = (d20 * d20 + d21 * d21) * dt2;
+                    if (r > 3 && r < rmax) {
+                        acc[4] += dotv;
+                        acc[5] += pw;
+                        acc[6] += pw2;
+                    }
+                    acc[1] += dt2;
+                    acc[0] += dotv;
+                    acc[2] += pw;
+                    acc[3] += pw2;
+                    acc[7] += 1.0;
+                }
+            }
+        }
+    }
+    __shared__ double red[8][256];
+#pragma unroll
+    for (int k = 0; k < 8; k++) red[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+#pragma unroll
+            for (int k = 0; k < 8; k++) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < 8) partial[((size_t)q * gridDim.x + blockIdx.x) * 8 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+int agree_blocks(int P) { return ((P / 2 + 2) * P + 255) / 256; }
+
+hipError_t launch_agree(const float2 *slices, int nsym, const float *t3d, int nslices, InsertConst c, float rmax,
+                        const float2 *data, const float *norm, const float *rtab, double *partial, hipStream_t s)
+{
+    if (nslices <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_agree, dim3(agree_blocks(c.P), nslices), dim3(256), 0, s, slices, nsym, t3d, c, rmax, data,
+                       norm, rtab, partial);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------ binning
 // Brick b = (bx, by, bz) covers unfolded voxels [-H + BS*b, -H + BS*b + BS);
 // a pixel reaches it when its stencil origin floor(p) lies in
@@ -392,20 +479,37 @@ __device__ __forceinline__ void fold_cell(float2 *data, float *norm, int P, int 
     gadd(&norm[a], w);
 }
 
+// LDS accumulator index of brick cell (i,j,k): the row offset is skewed by
+// (9j + 11k) (16^3) or (j + 3k) (8^3) so that the cells hit by one pixel
+// row -- a discretised 3-D line in any direction -- spread over the banks.
 template <int BS>
-__global__ void __launch_bounds__(256) k_brick_insert(const float2 *__restrict__ slices, int nslices, int nsym,
+__device__ __forceinline__ int acc_index(int i, int j, int k)
+{
+    if (BS == 16) return ((k * 16 + j) << 4) + ((i + 9 * j + 11 * k) & 15);
+    return ((k * 8 + j) << 3) + ((i + j + 3 * k) & 7);
+}
+template <int BS>
+__device__ __forceinline__ void acc_cell(int a, int &i, int &j, int &k)
+{
+    k = a / (BS * BS);
+    j = (a / BS) % BS;
+    if (BS == 16) i = ((a & 15) - 9 * j - 11 * k) & 15;
+    else i = ((a & 7) - j - 3 * k) & 7;
+}
+
+template <int BS>
+__global__ void __launch_bounds__(512) k_brick_insert(const float2 *__restrict__ slices, int nslices, int nsym,
                                                       const float *__restrict__ t3d, const float *__restrict__ wq,
                                                       InsertConst c, BrickGeom g, BinBuffers bb, float2 *data,
                                                       float *norm, const float *__restrict__ exptab_g,
                                                       const float *__restrict__ rtab_g)
 {
     constexpr int NC = BS * BS * BS;
-    constexpr int NW = 4;  // waves per block
+    constexpr int NW = 8;  // waves per block
     __shared__ float acc_re[NC], acc_im[NC], acc_w[NC];
     __shared__ float etab[EXPN];
-    __shared__ int s_excl[NW][64], s_xa[NW][64];
     __shared__ int s_item;
-    extern __shared__ float rtab[];  // P+1 entries
+    extern __shared__ float rtab[];  // P+1 entries: fl(i/P), i in [-H, H]
     const int H = c.H, P = c.P, nb = g.nb;
     const float Pf = c.Pf;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -426,7 +530,7 @@ __global__ void __launch_bounds__(256) k_brick_insert(const float2 *__restrict__
         if (it >= total_items) break;
         const WorkItem w = bb.items[it];
         const int bx = w.brick % nb, by = (w.brick / nb) % nb, bz = w.brick / (nb * nb);
-        const int lo[3] = {-H + BS * bx, -H + BS * by, -H + BS * bz};
+        const int lo0 = -H + BS * bx, lo1 = -H + BS * by, lo2 = -H + BS * bz;
 
         for (int si = wave; si < w.count; si += NW) {
             const int q = __builtin_amdgcn_readfirstlane(bb.list[w.start + si]);
@@ -437,16 +541,16 @@ __global__ void __launch_bounds__(256) k_brick_insert(const float2 *__restrict__
             for (int d = 0; d < 6; d++) mm[d] = m[d];
             const float weight = wq[q];
             const float2 *S = slices + (size_t)(q / nsym) * slice_elems;
-            // rows and x intervals of pixels whose position can fall in
-            // [lo-1, lo+BS) (+/- 1e-4 covers the FP32 coordinate error)
-            double L[3], U[3], uu[3], vv[3];
-            double ymin = 0, ymax = 0, xmin = 0, xmax = 0;
+            // Rows and x intervals of the pixels whose position can fall in
+            // [lo-1, lo+BS) (+/- 1e-4 covers the FP32 coordinate error).
+            const double L[3] = {lo0 - 1 - 1e-4, lo1 - 1 - 1e-4, lo2 - 1 - 1e-4};
+            const double U[3] = {lo0 + BS + 1e-4, lo1 + BS + 1e-4, lo2 + BS + 1e-4};
+            double ymin = 0, ymax = 0, xmin = 0, xmax = 0, uu[3], vv[3], iu[3];
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                L[d] = lo[d] - 1 - 1e-4;
-                U[d] = lo[d] + BS + 1e-4;
                 uu[d] = mm[d];
                 vv[d] = mm[3 + d];
+                iu[d] = fabs(uu[d]) > 1e-9 ? 1.0 / uu[d] : 0.0;
                 ymin += vv[d] >= 0 ? L[d] * vv[d] : U[d] * vv[d];
                 ymax += vv[d] >= 0 ? U[d] * vv[d] : L[d] * vv[d];
                 xmin += uu[d] >= 0 ? L[d] * uu[d] : U[d] * uu[d];
@@ -454,7 +558,7 @@ __global__ void __launch_bounds__(256) k_brick_insert(const float2 *__restrict__
             }
             const int ya = max(-H, (int)ceil(ymin - 1e-3));
             const int yb = min(H - 1, (int)floor(ymax + 1e-3));
-            // (BS+1)*sum|v| + 1 <= 31 rows for BS <= 16; clamp to the 64-lane table
+            // (BS+1)*sum|v| + 1 <= 31 rows for BS <= 16; clamp to the wave
             const int nrows = min(64, yb - ya + 1);
             int cnt = 0, xa = 0;
             if (lane < nrows) {
@@ -463,8 +567,8 @@ __global__ void __launch_bounds__(256) k_brick_insert(const float2 *__restrict__
 #pragma unroll
                 for (int d = 0; d < 3; d++) {
                     const double yv = y * vv[d];
-                    if (fabs(uu[d]) > 1e-9) {
-                        const double t1 = (L[d] - yv) / uu[d], t2 = (U[d] - yv) / uu[d];
+                    if (iu[d] != 0.0) {
+                        const double t1 = (L[d] - yv) * iu[d], t2 = (U[d] - yv) * iu[d];
                         xl = fmax(xl, fmin(t1, t2) - 1e-3);
                         xh = fmin(xh, fmax(t1, t2) + 1e-3);
                     } else if (yv < L[d] || yv > U[d]) {
@@ -484,60 +588,61 @@ __global__ void __launch_bounds__(256) k_brick_insert(const float2 *__restrict__
                 if (lane >= off) incl += v;
             }
             const int total = __shfl(incl, 63, 64);
-            s_excl[wave][lane] = incl - cnt;
-            s_xa[wave][lane] = xa;
-            __builtin_amdgcn_wave_barrier();
+            const int excl = incl - cnt;
             for (int base = 0; base < total; base += 64) {
                 const int gidx = base + lane;
-                if (gidx < total) {
-                    // largest row r with excl[r] <= gidx
-                    int lo_r = 0, hi_r = nrows - 1;
-                    while (lo_r < hi_r) {
-                        const int mid = (lo_r + hi_r + 1) >> 1;
-                        if (s_excl[wave][mid] <= gidx) lo_r = mid;
-                        else hi_r = mid - 1;
-                    }
-                    const int y = ya + lo_r;
-                    const int x = s_xa[wave][lo_r] + (gidx - s_excl[wave][lo_r]);
-                    bool ok = c.corners || (x * x + y * y <= H * H + H);
-                    float xx, yy, zz;
-                    pixel_coords(x, y, mm, rtab, H, Pf, xx, yy, zz);
-                    const int x0 = (int)floorf(xx), y0 = (int)floorf(yy), z0 = (int)floorf(zz);
-                    ok = ok && !(x0 < -H - 1 || y0 < -H - 1 || z0 < -H - 1 || x0 > H || y0 > H || z0 > H);
-                    ok = ok && x0 >= lo[0] - 1 && x0 <= lo[0] + BS - 1 && y0 >= lo[1] - 1 &&
-                         y0 <= lo[1] + BS - 1 && z0 >= lo[2] - 1 && z0 <= lo[2] + BS - 1;
-                    if (ok) {
-                        const float2 dt = slice_value(S, x, y, P, H);
+                // row of slot gidx: the last row r with excl_r <= gidx
+                // (empty rows share the next row's excl and are skipped)
+                int rrow = 0, rex = 0, rxa = 0;
+                for (int r = 0; r < nrows; r++) {
+                    const int er = __builtin_amdgcn_readlane(excl, r);
+                    const int xr = __builtin_amdgcn_readlane(xa, r);
+                    const bool sel = er <= gidx;
+                    rrow = sel ? r : rrow;
+                    rex = sel ? er : rex;
+                    rxa = sel ? xr : rxa;
+                }
+                if (gidx >= total) continue;
+                const int y = ya + rrow;
+                const int x = rxa + (gidx - rex);
+                const float2 dt = slice_value(S, x, y, P, H);  // issued before the coordinate math
+                bool ok = c.corners || (x * x + y * y <= H * H + H);
+                float xx, yy, zz;
+                pixel_coords(x, y, mm, rtab, H, Pf, xx, yy, zz);
+                const int x0 = (int)floorf(xx), y0 = (int)floorf(yy), z0 = (int)floorf(zz);
+                ok = ok && !(x0 < -H - 1 || y0 < -H - 1 || z0 < -H - 1 || x0 > H || y0 > H || z0 > H);
+                ok = ok && x0 >= lo0 - 1 && x0 <= lo0 + BS - 1 && y0 >= lo1 - 1 && y0 <= lo1 + BS - 1 &&
+                     z0 >= lo2 - 1 && z0 <= lo2 + BS - 1;
+                if (!ok) continue;
 #pragma unroll
-                        for (int dk = 0; dk < 2; dk++) {
-                            const int k = z0 + dk;
-                            if (k < lo[2] || k >= lo[2] + BS || k > H) continue;
+                for (int dk = 0; dk < 2; dk++) {
+                    const int k = z0 + dk;
+                    if (k < lo2 || k >= lo2 + BS || k > H) continue;
 #pragma unroll
-                            for (int dj = 0; dj < 2; dj++) {
-                                const int j = y0 + dj;
-                                if (j < lo[1] || j >= lo[1] + BS || j > H) continue;
+                    for (int dj = 0; dj < 2; dj++) {
+                        const int j = y0 + dj;
+                        if (j < lo1 || j >= lo1 + BS || j > H) continue;
 #pragma unroll
-                                for (int di = 0; di < 2; di++) {
-                                    const int i = x0 + di;
-                                    if (i < lo[0] || i >= lo[0] + BS || i > H) continue;
-                                    const float gg = corner_weight(c.mode, i, j, k, xx, yy, zz, c.h, weight, etab);
-                                    const int cell = ((k - lo[2]) * BS + (j - lo[1])) * BS + (i - lo[0]);
-                                    atomicAdd(&acc_re[cell], dt.x * gg);
-                                    atomicAdd(&acc_im[cell], dt.y * gg);
-                                    atomicAdd(&acc_w[cell], gg);
-                                }
-                            }
+                        for (int di = 0; di < 2; di++) {
+                            const int i = x0 + di;
+                            if (i < lo0 || i >= lo0 + BS || i > H) continue;
+                            const float gg = corner_weight(c.mode, i, j, k, xx, yy, zz, c.h, weight, etab);
+                            const int a = acc_index<BS>(i - lo0, j - lo1, k - lo2);
+                            atomicAdd(&acc_re[a], dt.x * gg);
+                            atomicAdd(&acc_im[a], dt.y * gg);
+                            atomicAdd(&acc_w[a], gg);
                         }
                     }
                 }
             }
-            __builtin_amdgcn_wave_barrier();
         }
         __syncthreads();
-        for (int cell = tid; cell < NC; cell += blockDim.x) {
-            const float re = acc_re[cell], im = acc_im[cell], wv = acc_w[cell];
+        for (int a = tid; a < NC; a += blockDim.x) {
+            const float re = acc_re[a], im = acc_im[a], wv = acc_w[a];
             if (re == 0.f && im == 0.f && wv == 0.f) continue;
-            const int i = lo[0] + cell % BS, j = lo[1] + (cell / BS) % BS, k = lo[2] + cell / (BS * BS);
+            int ci, cj, ck;
+            acc_cell<BS>(a, ci, cj, ck);
+            const int i = lo0 + ci, j = lo1 + cj, k = lo2 + ck;
             if (i > H || j > H || k > H) continue;
             fold_cell(data, norm, P, H, i, j, k, re, im, wv);
         }
@@ -551,10 +656,10 @@ hipError_t launch_brick_insert(const float2 *slices, int nslices, int nsym, cons
 {
     const size_t lds = sizeof(float) * (c.P + 1);
     if (g.Bs == 16)
-        hipLaunchKernelGGL(k_brick_insert<16>, dim3(grid), dim3(256), lds, s, slices, nslices, nsym, t3d, wq, c, g,
+        hipLaunchKernelGGL(k_brick_insert<16>, dim3(grid), dim3(512), lds, s, slices, nslices, nsym, t3d, wq, c, g,
                            b, data, norm, exptab, rtab);
     else
-        hipLaunchKernelGGL(k_brick_insert<8>, dim3(grid), dim3(256), lds, s, slices, nslices, nsym, t3d, wq, c, g,
+        hipLaunchKernelGGL(k_brick_insert<8>, dim3(grid), dim3(512), lds, s, slices, nslices, nsym, t3d, wq, c, g,
                            b, data, norm, exptab, rtab);
     return hipGetLastError();
 }

@@ -237,7 +237,12 @@ class FourierHipReconstructor:
 
     def _batch_args(self, imgs, xforms, weights, defocus, ctf):
         n = len(xforms)
-        m = np.ascontiguousarray(np.stack([_xform12(t) for t in xforms]) if n else np.zeros((0, 12)), np.float32)
+        if isinstance(xforms, np.ndarray) and xforms.ndim == 3 and xforms.shape[1:] == (3, 4):
+            m = np.ascontiguousarray(xforms, np.float32).reshape(n, 12)
+        elif isinstance(xforms, np.ndarray) and xforms.ndim == 2 and xforms.shape[1] == 12:
+            m = np.ascontiguousarray(xforms, np.float32)
+        else:
+            m = np.ascontiguousarray(np.stack([_xform12(t) for t in xforms]) if n else np.zeros((0, 12)), np.float32)
         w = np.ascontiguousarray(np.ones(n) if weights is None else weights, np.float32)
         d = None if defocus is None else np.ascontiguousarray(defocus, np.float32)
         c = None

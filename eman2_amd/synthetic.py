@@ -124,6 +124,64 @@ def particles(ph, n, box, seed, start=0, apix=1.0, snr=None, ctf=None, defocus_r
     return imgs.astype(np.float32), mats, dfs
 
 
+def particles_device(ph, n, box, seed, apix=1.0, snr=None, ctf=None, defocus_range=(0.8, 2.5),
+                     device="cuda", chunk=1024):
+    """Device-side generator with the same definition as ``particles`` (not
+    bit-identical: torch RNG and FP32 on the GPU).  Returns
+    (images torch.float32 (n,box,box) on ``device``, matrices (n,3,4) numpy,
+    defocus (n,) numpy or None).  Used to make benchmark-sized inputs that
+    already live in HBM."""
+    import torch
+
+    az, alt, phi = orientations_fast(n, seed)
+    mats = euler_eman_matrices(az, alt, phi)
+    rng = np.random.default_rng([seed, 7])
+    dfs = rng.uniform(*defocus_range, n).astype(np.float32) if ctf is not None else None
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed))
+    C = torch.tensor(ph["centres"], dtype=torch.float32, device=device)            # (G,3)
+    sig = torch.tensor(ph["sigmas"], dtype=torch.float32, device=device)          # (G,)
+    amp = torch.tensor(ph["amps"] * ph["sigmas"] * math.sqrt(2 * math.pi), dtype=torch.float32, device=device)
+    px = torch.arange(box, dtype=torch.float32, device=device) - box // 2
+    out = torch.empty((n, box, box), dtype=torch.float32, device=device)
+    ctfk = None
+    if ctf is not None:
+        lam = 12.2639 / math.sqrt(ctf.get("voltage", 300.0) * 1000.0 + 0.97845 * ctf.get("voltage", 300.0) ** 2)
+        g1 = math.pi / 2.0 * ctf.get("cs", 2.7) * 1.0e7 * lam ** 3
+        g2 = math.pi * lam * 10000.0
+        acac = math.pi / 2.0 - math.asin(ctf.get("ampcont", 10.0) / 100.0)
+        ky = torch.fft.fftfreq(box, device=device)[:, None]
+        kx = torch.fft.rfftfreq(box, device=device)[None, :]
+        s2 = (kx * kx + ky * ky) / (apix * apix)
+        ctfk = (g1, g2, acac, s2, ctf.get("bfactor", 80.0))
+    for c0 in range(0, n, chunk):
+        m = min(chunk, n - c0)
+        R = torch.tensor(mats[c0:c0 + m, :2, :3], dtype=torch.float32, device=device)   # (m,2,3)
+        cxy = torch.matmul(R, C.t().contiguous())                                        # (m,2,G)
+        gx = torch.exp(-((px[None, None, :] - cxy[:, 0, :, None]) ** 2) / (2 * sig[None, :, None] ** 2))
+        gy = torch.exp(-((px[None, None, :] - cxy[:, 1, :, None]) ** 2) / (2 * sig[None, :, None] ** 2))
+        img = torch.bmm((gy * amp[None, :, None]).transpose(1, 2).contiguous(), gx.contiguous())
+        if ctfk is not None:
+            g1, g2, acac, s2, bf = ctfk
+            df = torch.tensor(dfs[c0:c0 + m], dtype=torch.float32, device=device)[:, None, None]
+            gam = -g1 * s2[None] ** 2 + g2 * df * s2[None]
+            k = torch.cos(gam - acac) * torch.exp(-(bf / 4.0) * s2[None])
+            img = torch.fft.irfft2(torch.fft.rfft2(img) * k, s=(box, box))
+        if snr is not None:
+            sd = torch.sqrt(img.var(dim=(1, 2), keepdim=True) / snr)
+            img = img + torch.randn(img.shape, generator=g, device=device) * sd
+        out[c0:c0 + m] = img
+    return out, mats, dfs
+
+
+def orientations_fast(n, seed):
+    """Uniform SO(3) eman Euler angles from one seeded stream (fast path for
+    large n; same distribution as ``orientations``)."""
+    rng = np.random.default_rng([seed, 11])
+    u = rng.random((n, 3))
+    return 360.0 * u[:, 0], np.degrees(np.arccos(2.0 * u[:, 1] - 1.0)), 360.0 * u[:, 2]
+
+
 def good_size(n):
     """FFT-friendly box sizes (libpyEM/EMAN2.py:660-677 good_size)."""
     k = int(n)

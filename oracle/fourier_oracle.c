@@ -910,15 +910,16 @@ static inline size_t vol_index_fast(int P, int x, int y, int z)
     return x * 2 + (y < 0 ? ny + y : y) * (size_t)nx + (z < 0 ? nz + z : z) * nxy;
 }
 
-void or_determine_slice_agreement(int P, float *data, float *norm, float *slice, const float rot_in[12],
+void or_determine_slice_agreement(int P, float *data, float *norm, float *slice, int mode, const float rot_in[12],
                                   const float *syms, int nsym, float weight, int sub, float apix,
                                   float out[5])
 {
-    /* libEM/reconstructor.cpp:1212-1257 */
+    /* libEM/reconstructor.cpp:1212-1257; the sub/re-add use the reconstructor's
+     * inserter mode with corners=false (reconstructor.h:540 default) */
     float rot[12];
     memcpy(rot, rot_in, sizeof(rot));
     or_rotation_only(rot);
-    if (sub) insert_work_f32(P, data, norm, slice, rot, syms, nsym, -weight, OR_MODE_GAUSS2, 0);
+    if (sub) insert_work_f32(P, data, norm, slice, rot, syms, nsym, -weight, mode, 0);
 
     /* do_compare_slice_work, libEM/reconstructor.cpp:1259-1381 */
     const float inx = (float)(P + 2), iny = (float)P;
@@ -982,7 +983,7 @@ void or_determine_slice_agreement(int P, float *data, float *norm, float *slice,
     out[3] = (float)(dot * rw / ((rw - 1.0) * dot + 1.0));
     out[4] = (float)(vweight / (float)(nval));
 
-    if (sub) insert_work_f32(P, data, norm, slice, rot, syms, nsym, weight, OR_MODE_GAUSS2, 0);
+    if (sub) insert_work_f32(P, data, norm, slice, rot, syms, nsym, weight, mode, 0);
 }
 
 /* ======================================================================== */

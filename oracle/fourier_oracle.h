@@ -96,7 +96,7 @@ void or_finish(int P, float *data, float *norm, int sqrtnorm, int doift,
                float *out_real /* P^3, only if doift */);
 
 /* ---- slice agreement (libEM/reconstructor.cpp:1212-1381,1448-1461) ------ */
-void or_determine_slice_agreement(int P, float *data, float *norm, float *slice,
+void or_determine_slice_agreement(int P, float *data, float *norm, float *slice, int mode,
                                   const float rot[12], const float *syms, int nsym,
                                   float weight, int sub, float apix, float out[5]);
 

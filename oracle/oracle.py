@@ -79,7 +79,7 @@ def lib(fast=False):
         "or_insert_batch_mt": (d, [i, _fp, _fp, _fp, _fp, _fp, i, _fp, i, i, i]),
         "or_normalize_threed": (v, [i, _fp, _fp, i]),
         "or_finish": (v, [i, _fp, _fp, i, i, _fp]),
-        "or_determine_slice_agreement": (v, [i, _fp, _fp, _fp, _fp, _fp, i, f, i, f, _fp]),
+        "or_determine_slice_agreement": (v, [i, _fp, _fp, _fp, i, _fp, _fp, i, f, i, f, _fp]),
         "or_setup_seed": (v, [i, _fp, _fp, _fp, f]),
     }
     for k, (r, a) in sig.items():
@@ -299,11 +299,11 @@ class Volume:
         lib().or_setup_seed(self.P, _f(self.data), _f(self.norm), _f(f32(seed)), float(seed_weight))
 
     def agreement(self, slice_, rot, weight=1.0, sub=True, apix=1.0):
-        assert not self.fp64 and self.mode == GAUSS2
+        assert not self.fp64
         s = f32(slice_).copy()
         r = f32(rot).reshape(12)
         out = np.zeros(5, np.float32)
-        lib().or_determine_slice_agreement(self.P, _f(self.data), _f(self.norm), _f(s), _f(r),
+        lib().or_determine_slice_agreement(self.P, _f(self.data), _f(self.norm), _f(s), self.mode, _f(r),
                                           _f(self.syms), len(self.syms), float(weight),
                                           int(sub), float(apix), _f(out))
         return dict(reconstruct_norm=float(out[0]), reconstruct_absqual=float(out[1]),

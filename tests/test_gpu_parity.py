@@ -272,6 +272,52 @@ def S_edge(img):
     return np.float32(b.mean())
 
 
+AGREE_KEYS = ("reconstruct_norm", "reconstruct_absqual", "reconstruct_absqual_lowres", "reconstruct_qual",
+              "reconstruct_weight")
+
+
+@pytest.mark.parametrize("sym,mode", [("c1", "gauss_2"), ("d2", "trilinear")])
+def test_slice_agreement(oracle, sym, mode):
+    """determine_slice_agreement (reconstructor.cpp:1212-1381) vs the oracle:
+    the five outputs agree to 1e-4 relative.  Enough particles that no voxel
+    is reached by the subtracted slice alone: such a voxel keeps only the
+    rounding residue of +w then -w, which is summation-order dependent in the
+    reference as well (60 particles at P=48 leave 28 of them)."""
+    from eman2_amd.reconstructor import EMData
+    P, B, n = 32, 24, 600
+    ph = S.phantom(10, 7.0, seed=41)
+    imgs, mats, _ = S.particles(ph, n, B, seed=42, snr=0.5)
+    w = np.ones(n, np.float32)
+    r = recon(P, sym, mode)
+    r.insert_batch(imgs, list(mats), w)
+    slices = [oracle.prep_particle(im, P) for im in imgs]
+    v = oracle_volume(oracle, P, sym, mode, slices, mats, w)
+    for i, sub in ((0, True), (7, False), (13, True)):
+        g = r.determine_slice_agreement(EMData(slices[i], True, {"reconstruct_preproc": 1, "apix_x": 1.5}),
+                                        Transform(mats[i]), 1.0, sub)
+        o = v.agreement(slices[i], mats[i], 1.0, sub=sub, apix=1.5)
+        for k in AGREE_KEYS:
+            assert abs(g[k] - o[k]) <= 1e-4 * max(1.0, abs(o[k])), (i, k, g[k], o[k])
+    # the volume is unchanged by sub/compare/re-add up to rounding
+    gd, gn = r.get_state()
+    assert_state_close(gd, gn, v.data, v.norm)
+
+
+def test_agreement_batch_matches_sequential(oracle):
+    P, B, n = 32, 24, 500
+    ph = S.phantom(10, 7.0, seed=43)
+    imgs, mats, _ = S.particles(ph, n, B, seed=44, snr=0.5)
+    w = np.ones(n, np.float32)
+    r = recon(P)
+    r.insert_batch(imgs, list(mats), w)
+    got = r.agreement_batch(imgs[:12], mats[:12], w[:12], sub=True, apix=1.0)
+    v = oracle_volume(oracle, P, "c1", "gauss_2", [oracle.prep_particle(im, P) for im in imgs], mats, w)
+    for i in range(12):
+        o = v.agreement(oracle.prep_particle(imgs[i], P), mats[i], 1.0, sub=True, apix=1.0)
+        for j, k in enumerate(AGREE_KEYS):
+            assert abs(got[i, j] - o[k]) <= 1e-4 * max(1.0, abs(o[k])), (i, k, got[i, j], o[k])
+
+
 def test_transform_with_translation_rejected():
     P = 16
     r = recon(P)
