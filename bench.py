@@ -59,7 +59,45 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB elementwise copy (PMC byte-count calibration)")
+    ap.add_argument("--serve-jobs", type=int, default=40,
+                    help="jobs in the serving measurement (0 = skip); Poisson arrivals at --serve-rate")
+    ap.add_argument("--serve-rate", type=float, default=4.0)
     return ap.parse_args()
+
+
+def serving(a, torch, dev):
+    """Mixed-size job stream through the native serving loop: box in
+    {64,128,192,256,320} (pad 1.5x to a good size), 500-20,000 particles
+    (log-uniform), sym in {c1,c4,d2}, CTF, SNR 0.1, Poisson arrivals.
+    Particles come from per-box pools generated in HBM beforehand; each job
+    takes a random contiguous run of its box's pool."""
+    from eman2_amd import serving as SV
+    from eman2_amd import synthetic as S
+
+    pools = {}
+    for i, B in enumerate((64, 128, 192, 256, 320)):
+        cap = 20000
+        ph = S.phantom(60, 0.3 * B, seed=40 + i)
+        imgs, mats, dfs = S.particles_device(ph, cap, B, seed=50 + i, apix=1.0, snr=0.1, ctf=CTF, device=dev)
+        base = int(imgs.data_ptr())
+        pools[B] = (imgs, mats, dfs, cap, 1.0, (lambda o, base=base, B=B: base + 4 * B * B * o))
+    torch.cuda.synchronize()
+    jobs = SV.config4_jobs(a.serve_jobs, seed=9, pools=pools)
+    srv = SV.Server(device=0, hbm_budget_bytes=160 << 30, max_active=4, chunk=1024)
+    # warm the code paths once per size
+    for B in sorted(pools):
+        j = dict(SV.config4_jobs(1, seed=B, pools={B: pools[B]})[0])
+        j["xforms"], j["defocus"] = j["xforms"][:64], j["defocus"][:64]
+        srv.submit(**j)
+    srv.drain()
+    recs = SV.poisson_stream(srv, jobs, a.serve_rate, seed=11)
+    srv.close()
+    out = SV.latency_summary(recs)
+    out.update({"rate_jobs_per_s": a.serve_rate, "max_active": 4,
+                "what": "latency = submit -> finished map on the host (setup, insertion, finish, D2H)"})
+    del pools
+    torch.cuda.empty_cache()
+    return out
 
 
 def cpu_baseline(imgs_host, mats, dfs, P, sym, mode, apix, budget_s):
@@ -220,6 +258,13 @@ def main():
                                 "symmetry op with global atomics, synchronise after each"}
         out["batched_vs_plain"] = round(value / pv, 2)
         del rp
+    if a.serve_jobs > 0:
+        print("[bench] serving %d jobs" % a.serve_jobs, file=sys.stderr, flush=True)
+        del imgs
+        torch.cuda.empty_cache()
+        out["serving"] = serving(a, torch, dev)
+        out["job_latency_ms"] = {"p50": out["serving"].get("p50_ms"), "p99": out["serving"].get("p99_ms"),
+                                 "what": "serving stream of mixed-size jobs (see serving)"}
     if not a.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(host, mats, dfs, P, a.sym, a.mode, a.apix, a.cpu_seconds)

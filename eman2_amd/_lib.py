@@ -42,7 +42,28 @@ class Stats(ctypes.Structure):
                 ("pairs", _c_ll), ("items", _c_ll), ("chunks", _c_ll)]
 
 
+class Job(ctypes.Structure):
+    _fields_ = [("params", Params), ("imgs", _vp), ("n", _c_int), ("box", _c_int), ("where", _c_int),
+                ("xforms", _vp), ("weights", _vp), ("defocus", _vp), ("ctf", Ctf), ("use_ctf", _c_int),
+                ("out_map", _vp)]
+
+
+class JobStatus(ctypes.Structure):
+    _fields_ = [("state", _c_int), ("error", _c_int), ("t_submit", _c_double), ("t_admit", _c_double),
+                ("t_done", _c_double), ("particles_done", _c_ll), ("footprint_bytes", _c_ll)]
+
+
+JOB_STATES = {0: "queued", 1: "running", 2: "done", 3: "failed", 4: "rejected", 5: "cancelled"}
+
 EXPORTS = {
+    "fh_server_create": (_c_int, [_c_int, _c_ll, _c_int, _c_int, ctypes.POINTER(_vp)]),
+    "fh_server_submit": (_c_int, [_vp, ctypes.POINTER(Job), ctypes.POINTER(_c_int)]),
+    "fh_server_poll": (_c_int, [_vp, _c_int, ctypes.POINTER(JobStatus)]),
+    "fh_server_cancel": (_c_int, [_vp, _c_int]),
+    "fh_server_drain": (_c_int, [_vp]),
+    "fh_server_clock": (_c_double, [_vp]),
+    "fh_server_destroy": (_c_int, [_vp]),
+    "fh_server_last_error": (ctypes.c_char_p, [_vp]),
     "fh_create": (_c_int, [ctypes.POINTER(Params), ctypes.POINTER(_vp)]),
     "fh_destroy": (_c_int, [_vp]),
     "fh_last_error": (ctypes.c_char_p, [_vp]),

@@ -1,0 +1,105 @@
+"""Autotuner for the batched insertion: sweeps brick edge, slices per work
+item, persistent workgroups per CU and particles per launch on a short
+HBM-resident synthetic workload and records the fastest setting per
+(pad, symmetry-op count) in tuning_table.json next to this file.
+
+    python -m eman2_amd.autotune --box 256 --pad 384 --sym c1 --particles 8192
+
+``FourierHipReconstructor`` applies the table entry whose key matches, or the
+nearest pad with the same symmetry-op count, when it is created.
+"""
+import argparse
+import itertools
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TABLE = os.path.join(HERE, "tuning_table.json")
+
+
+def nsym_of(sym):
+    s = sym.lower()
+    return int(s[1:]) * (2 if s[0] == "d" else 1)
+
+
+def load_table():
+    if not os.path.exists(TABLE):
+        return {}
+    with open(TABLE) as f:
+        return json.load(f)
+
+
+def lookup(P, nsym):
+    """Best known tuning for pad P and nsym symmetry ops (or None)."""
+    t = load_table()
+    key = "%d,%d" % (P, nsym)
+    if key in t:
+        return t[key]["tuning"]
+    cands = [(abs(int(k.split(",")[0]) - P), v["tuning"]) for k, v in t.items() if int(k.split(",")[1]) == nsym]
+    if not cands:
+        cands = [(abs(int(k.split(",")[0]) - P), v["tuning"]) for k, v in t.items()]
+    return min(cands, key=lambda c: c[0])[1] if cands else None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--box", type=int, default=256)
+    ap.add_argument("--pad", type=int, default=384)
+    ap.add_argument("--sym", default="c1")
+    ap.add_argument("--particles", type=int, default=8192)
+    ap.add_argument("--bricks", default="8,16")
+    ap.add_argument("--spi", default="16,32,64,128,256")
+    ap.add_argument("--wgs", default="1,2,3")
+    ap.add_argument("--chunks", default="2048,4096")
+    ap.add_argument("--write", action="store_true", help="update tuning_table.json")
+    a = ap.parse_args()
+    import torch
+
+    from . import synthetic as S
+    from .reconstructor import Reconstructors
+
+    dev = "cuda:0"
+    ph = S.phantom(200, 0.375 * a.box, seed=2)
+    imgs, mats, dfs = S.particles_device(ph, a.particles, a.box, seed=3, snr=0.05,
+                                         ctf={"voltage": 300.0, "cs": 2.7, "ampcont": 10.0}, device=dev)
+    torch.cuda.synchronize()
+    ctf = {"voltage": 300.0, "cs": 2.7, "ampcont": 10.0, "apix": 1.0}
+    w = np.ones(a.particles, np.float32)
+    r = Reconstructors.get("fourier_hip", {"size": (a.pad,) * 3, "sym": a.sym})
+    ptr = int(imgs.data_ptr())
+    results = []
+    grid = itertools.product([int(x) for x in a.bricks.split(",")], [int(x) for x in a.spi.split(",")],
+                             [int(x) for x in a.wgs.split(",")], [int(x) for x in a.chunks.split(",")])
+    for brick, spi, wgs, chunk in grid:
+        if brick == 8 and wgs > 3:
+            continue
+        r.set_tuning(brick=brick, slices_per_item=spi, chunk=chunk, wgs_per_cu=wgs)
+        r.setup()
+        r.insert_batch(ptr, mats, w, defocus=dfs, ctf=ctf, box=a.box)  # warm
+        r.setup()
+        t0 = time.perf_counter()
+        r.insert_batch(ptr, mats, w, defocus=dfs, ctf=ctf, box=a.box)
+        r.sync()
+        dt = time.perf_counter() - t0
+        st = r.get_stats()
+        rec = {"brick": brick, "slices_per_item": spi, "wgs_per_cu": wgs, "chunk": chunk,
+               "particles_per_s": a.particles / dt, "insert_kernel_ms_per_particle": st["insert_kernel_ms"] / a.particles}
+        results.append(rec)
+        print(json.dumps(rec), flush=True)
+    best = max(results, key=lambda x: x["particles_per_s"])
+    print("best", json.dumps(best), flush=True)
+    if a.write:
+        t = load_table()
+        t["%d,%d" % (a.pad, nsym_of(a.sym))] = {
+            "tuning": {k: best[k] for k in ("brick", "slices_per_item", "chunk", "wgs_per_cu")},
+            "particles_per_s": round(best["particles_per_s"], 1), "box": a.box, "particles": a.particles}
+        with open(TABLE, "w") as f:
+            json.dump(t, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -30,6 +30,7 @@ UNITS = [
     ("prep_kernels.hip", []),
     ("insert_kernels.hip", ["-ffp-contract=off"]),
     ("engine.cpp", ["-x", "hip", "-ffp-contract=off"]),
+    ("server.cpp", ["-x", "hip"]),
 ]
 
 

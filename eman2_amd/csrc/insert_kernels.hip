@@ -591,10 +591,14 @@ __global__ void __launch_bounds__(512) k_brick_insert(const float2 *__restrict__
             const int excl = incl - cnt;
             for (int base = 0; base < total; base += 64) {
                 const int gidx = base + lane;
-                // row of slot gidx: the last row r with excl_r <= gidx
-                // (empty rows share the next row's excl and are skipped)
+                // rows overlapping the window [base, base+64) (a contiguous run)
+                const unsigned long long ov =
+                    __ballot(lane < nrows && cnt > 0 && excl < base + 64 && excl + cnt > base);
+                const int r_lo = __ffsll((long long)ov) - 1;
+                const int r_hi = 63 - __clzll((long long)ov);
+                // row of slot gidx: the last overlapping row r with excl_r <= gidx
                 int rrow = 0, rex = 0, rxa = 0;
-                for (int r = 0; r < nrows; r++) {
+                for (int r = r_lo; r <= r_hi; r++) {
                     const int er = __builtin_amdgcn_readlane(excl, r);
                     const int xr = __builtin_amdgcn_readlane(xa, r);
                     const bool sel = er <= gidx;

@@ -171,6 +171,13 @@ class FourierHipReconstructor:
         _lib.check(rc, None)
         self._h = h
         self.P = nx
+        from .autotune import lookup, nsym_of
+        try:
+            best = lookup(nx, nsym_of(str(p.get("sym", "c1"))))
+        except (ValueError, IndexError):
+            best = None
+        if best:
+            self.set_tuning(**best)
         if any(k in p for k in ("brick", "slices_per_item", "chunk", "wgs_per_cu")):
             t = self.get_tuning()
             t.update({k: int(p[k]) for k in ("brick", "slices_per_item", "chunk", "wgs_per_cu") if k in p})

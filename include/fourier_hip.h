@@ -173,6 +173,57 @@ int fh_sync(fh_reconstructor *h);
 /* device bytes this handle holds / would hold after setup for size P */
 long long fh_footprint_bytes(int P, int nsym, int chunk);
 
+/* ------------------------------------------------------------------------
+ * Serving: a request loop that accepts reconstruction jobs of mixed box
+ * sizes as they arrive, admits them by HBM footprint, keeps each job's
+ * volume resident on the device, and interleaves the jobs' chunks on the
+ * GPU (continuous batching: a job admitted while others run starts
+ * immediately, its work overlapping theirs on separate streams).  No
+ * counterpart in the reference, whose task queue (libpyEM/EMAN2PAR.py)
+ * runs each make3d job in its own process; this is that queue for one GPU.
+ * ---------------------------------------------------------------------- */
+typedef struct fh_server fh_server;
+
+/* A job: n particles (box x box, in `where` memory; must stay valid until
+ * the job completes), host xforms (n x 12), weights (n), optional defocus
+ * (n) + ctf, the reconstructor parameters, and where to put the result. */
+typedef struct {
+    fh_params params;
+    const float *imgs;
+    int n, box, where;
+    const float *xforms, *weights, *defocus;
+    fh_ctf ctf;
+    int use_ctf;
+    float *out_map;    /* host P^3 floats, filled by finish(doift=1) */
+} fh_job;
+
+#define FH_JOB_QUEUED 0
+#define FH_JOB_RUNNING 1
+#define FH_JOB_DONE 2
+#define FH_JOB_FAILED 3
+#define FH_JOB_REJECTED 4
+#define FH_JOB_CANCELLED 5
+
+typedef struct {
+    int state;
+    int error;                  /* status code when FAILED / REJECTED */
+    double t_submit, t_admit, t_done;  /* seconds on the server's clock */
+    long long particles_done;
+    long long footprint_bytes;
+} fh_job_status;
+
+/* hbm_budget_bytes: device memory the server may commit to admitted jobs;
+ * max_active: jobs running at once (each has its own stream and worker). */
+int fh_server_create(int device, long long hbm_budget_bytes, int max_active, int chunk, fh_server **out);
+int fh_server_submit(fh_server *s, const fh_job *job, int *job_id);
+int fh_server_poll(fh_server *s, int job_id, fh_job_status *st);
+int fh_server_cancel(fh_server *s, int job_id);
+/* block until every submitted job has left the queue and finished */
+int fh_server_drain(fh_server *s);
+double fh_server_clock(fh_server *s);
+int fh_server_destroy(fh_server *s);
+const char *fh_server_last_error(const fh_server *s);
+
 #ifdef __cplusplus
 }
 #endif
