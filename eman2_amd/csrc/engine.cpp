@@ -6,7 +6,7 @@
 // reference layout), the particle staging buffers of one batched chunk and
 // the binning buffers.  Every call on a handle is serialised by its mutex and
 // issued on the handle's own HIP stream (default priority).
-#include "../../include/fourier_hip.h"
+#include "fourier_hip.h"
 
 #include <hip/hip_runtime.h>
 
@@ -206,6 +206,8 @@ struct fh_reconstructor {
     float2 *slices = nullptr;
     float *t3d = nullptr, *wq = nullptr;
     double *partial = nullptr;  // agreement partial sums
+    double *acc8 = nullptr;     // batched agreement sums, 8 per particle
+    int acc8_cap = 0;
     // binning
     BrickGeom geom{};
     BinBuffers bins{};
@@ -417,6 +419,7 @@ int fh_destroy(fh_reconstructor *h)
         free_bins(h);
         free_volume(h);
         if (h->partial) (void)hipFree(h->partial);
+        if (h->acc8) (void)hipFree(h->acc8);
         for (void *p : {(void *)h->exptab, (void *)h->rtab, (void *)h->tw, (void *)h->syms_d})
             if (p) (void)hipFree(p);
         for (auto &ev : h->ev)
@@ -731,6 +734,22 @@ static float agree_rmax(int P, float apix)
     return rmax;
 }
 
+// The five attributes from the eight sums {dot, vweight, power, power2,
+// dotlow, dlpow, dlpow2, nval} (libEM/reconstructor.cpp:1365-1378).
+static void agree_outputs(const double *a, float weight, float out[5])
+{
+    double dot = a[0], vweight = a[1], power = a[2], power2 = a[3], dotlow = a[4], dlpow = a[5], dlpow2 = a[6];
+    const int nval = (int)a[7];
+    dot /= sqrt(power * power2);
+    if (dlpow * dlpow2 > 0) dotlow /= sqrt(dlpow * dlpow2);
+    out[0] = (float)(power2 <= 0 ? 1.0 : sqrt(power / power2));
+    out[1] = (float)dot;
+    out[2] = (float)dotlow;
+    const float rw = weight <= 0 ? 1.0f : 1.0f / weight;
+    out[3] = (float)(dot * rw / ((rw - 1.0) * dot + 1.0));
+    out[4] = (float)(vweight / (float)(nval));
+}
+
 // One slice already in slice_d: optional subtraction, compare, re-add
 // (libEM/reconstructor.cpp:1212-1257) -> out[5] (reconstructor.cpp:1365-1378).
 static int agree_one(fh_reconstructor *h, const float2 *slice_d, const float *rot9, float weight, int sub, float apix,
@@ -757,16 +776,7 @@ static int agree_one(fh_reconstructor *h, const float2 *slice_d, const float *ro
     FH_TRY(h, hipStreamSynchronize(h->stream));
     double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (size_t i = 0; i < part.size(); i++) a[i % 8] += part[i];
-    double dot = a[0], vweight = a[1], power = a[2], power2 = a[3], dotlow = a[4], dlpow = a[5], dlpow2 = a[6];
-    const int nval = (int)a[7];
-    dot /= sqrt(power * power2);
-    if (dlpow * dlpow2 > 0) dotlow /= sqrt(dlpow * dlpow2);
-    out[0] = (float)(power2 <= 0 ? 1.0 : sqrt(power / power2));
-    out[1] = (float)dot;
-    out[2] = (float)dotlow;
-    const float rw = weight <= 0 ? 1.0f : 1.0f / weight;
-    out[3] = (float)(dot * rw / ((rw - 1.0) * dot + 1.0));
-    out[4] = (float)(vweight / (float)(nval));
+    agree_outputs(a, weight, out);
     return FH_OK;
 }
 
@@ -815,9 +825,16 @@ int fh_agreement_batch(fh_reconstructor *h, const float *imgs, int n, int box, c
     if ((rc = check_xforms(h, xforms, n, 1, rot9))) return rc;
     const int chunk = std::max(1, std::min(h->tune.chunk, std::max(n, 1)));
     if ((rc = ensure_chunk(h, chunk, box))) return rc;
-    if ((rc = ensure_partial(h))) return rc;
-    const size_t se = (size_t)h->P * (h->H + 1);
+    if (!h->acc8 || h->acc8_cap < chunk) {
+        if (h->acc8) (void)hipFree(h->acc8);
+        h->acc8 = nullptr;
+        FH_TRY(h, hipMalloc(&h->acc8, sizeof(double) * 8 * (size_t)chunk));
+        h->acc8_cap = chunk;
+    }
+    const InsertConst c = iconst(h);
+    const float rmax = agree_rmax(h->P, apix);
     std::vector<float> res(5 * (size_t)std::max(n, 1));
+    std::vector<double> acc(8 * (size_t)chunk);
     for (int c0 = 0; c0 < n; c0 += chunk) {
         const int m = std::min(chunk, n - c0);
         const float *src = imgs + (size_t)c0 * box * box;
@@ -827,14 +844,19 @@ int fh_agreement_batch(fh_reconstructor *h, const float *imgs, int n, int box, c
             dimgs = h->imgs;
         }
         if ((rc = prep_chunk(h, m, box, dimgs, defocus ? defocus + c0 : nullptr, ctf))) return rc;
-        // particles in order, each against the volume without its own
-        // contribution (e2make3d.py:447-491 calls determine_slice_agreement
-        // per particle)
-        for (int i = 0; i < m; i++) {
-            if ((rc = agree_one(h, h->slices + se * i, &rot9[9 * (size_t)(c0 + i)], weights[c0 + i], sub, apix,
-                                &res[5 * (size_t)(c0 + i)])))
-                return rc;
-        }
+        FH_TRY(h, hipMemcpyAsync(h->rots, &rot9[9 * (size_t)c0], sizeof(float) * 9 * m, hipMemcpyHostToDevice, h->stream));
+        FH_TRY(h, hipMemcpyAsync(h->wts, weights + c0, sizeof(float) * m, hipMemcpyHostToDevice, h->stream));
+        FH_TRY(h, launch_compose(h->rots, h->wts, m, h->syms_d, h->nsym, h->t3d, h->wq, h->stream));
+        FH_TRY(h, launch_fill_i32((int *)h->acc8, 0, 16 * (size_t)m, h->stream));
+        // every particle is compared against V minus its own contribution
+        // (what e2make3d.py:447-491 gets from determine_slice_agreement called
+        // particle by particle), recomputed by a gather instead of mutating V
+        FH_TRY(h, launch_agree_batch(h->slices, h->nsym, h->t3d, h->wq, m * h->nsym, c, rmax, sub, h->data, h->norm,
+                                     h->exptab, h->rtab, h->acc8, h->stream));
+        FH_TRY(h, hipMemcpyAsync(acc.data(), h->acc8, sizeof(double) * 8 * m, hipMemcpyDeviceToHost, h->stream));
+        FH_TRY(h, hipStreamSynchronize(h->stream));
+        for (int i = 0; i < m; i++)
+            agree_outputs(&acc[8 * (size_t)i], weights[c0 + i], &res[5 * (size_t)(c0 + i)]);
     }
     if (n > 0) {
         if (where == FH_DEVICE)

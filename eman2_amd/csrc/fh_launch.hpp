@@ -93,6 +93,9 @@ hipError_t launch_brick_insert(const float2 *slices, int nslices, int nsym,
                                BinBuffers b, float2 *data, float *norm, const float *exptab,
                                const float *rtab, int grid, hipStream_t s);
 int agree_blocks(int P);
+hipError_t launch_agree_batch(const float2 *slices, int nsym, const float *t3d, const float *wq, int nslices,
+                              InsertConst c, float rmax, int sub, const float2 *data, const float *norm,
+                              const float *exptab, const float *rtab, double *acc, hipStream_t s);
 hipError_t launch_agree(const float2 *slices, int nsym, const float *t3d, int nslices, InsertConst c, float rmax,
                         const float2 *data, const float *norm, const float *rtab, double *partial, hipStream_t s);
 

@@ -251,6 +251,214 @@ __global__ void __launch_bounds__(256) k_agree(const float2 *__restrict__ slices
 
 int agree_blocks(int P) { return ((P / 2 + 2) * P + 255) / 256; }
 
+// ---- batched agreement: compare against V - own(p) without mutating V ----
+// own(p) at a storage voxel is what determine_slice_agreement's
+// do_insert_slice_work(slice, R, -w) would subtract there
+// (libEM/reconstructor.cpp:1236): the gauss_2 / trilinear contributions of
+// all symmetry copies of particle p whose stencil cell folds onto the voxel.
+// Unfolded sources of storage voxel (xs, ys, zs), following
+// add_complex_at_fast: direct cells (xs, j, k) with wrap(j)=ys, wrap(k)=zs,
+// conjugate cells (-xs, j, k) with wrapneg(j)=ys, wrapneg(k)=zs, and on the
+// x=0 / x=P/2 planes the conjugate copies of the double insertion.
+
+// contributions of slice copy m (t3d rows) of particle S to unfolded cell c
+__device__ __forceinline__ void gather_cell(const float2 *__restrict__ S, const float *m, int ci, int cj, int ck,
+                                            const InsertConst &c, float weight, const float *etab,
+                                            const float *rtab, float &re, float &im, float &w)
+{
+    const int H = c.H, P = c.P;
+    const float n0 = m[1] * m[5] - m[2] * m[4], n1 = m[2] * m[3] - m[0] * m[5], n2 = m[0] * m[4] - m[1] * m[3];
+    const float dist = n0 * ci + n1 * cj + n2 * ck;
+    if (fabsf(dist) > 1.7330f) return;  // |p - c|_inf < 1 on the plane => |n.c| < sqrt(3)
+    const float s = m[0] * ci + m[1] * cj + m[2] * ck;
+    const float t = m[3] * ci + m[4] * cj + m[5] * ck;
+    const int xa = max(0, (int)ceilf(s - 1.7330f)), xb = min(H, (int)floorf(s + 1.7330f));
+    const int ya = max(-H, (int)ceilf(t - 1.7330f)), yb = min(H - 1, (int)floorf(t + 1.7330f));
+    for (int y = ya; y <= yb; y++)
+        for (int x = xa; x <= xb; x++) {
+            if (x * x + y * y > H * H + H) continue;  // radius cut (corners=false)
+            float xx, yy, zz;
+            pixel_coords(x, y, m, rtab, H, c.Pf, xx, yy, zz);
+            const int x0 = (int)floorf(xx), y0 = (int)floorf(yy), z0 = (int)floorf(zz);
+            if (x0 < -H - 1 || y0 < -H - 1 || z0 < -H - 1 || x0 > H || y0 > H || z0 > H) continue;
+            if ((unsigned)(ci - x0) > 1u || (unsigned)(cj - y0) > 1u || (unsigned)(ck - z0) > 1u) continue;
+            const float gg = corner_weight(c.mode, ci, cj, ck, xx, yy, zz, c.h, weight, etab);
+            const float2 dt = slice_value(S, x, y, P, H);
+            re += dt.x * gg;
+            im += dt.y * gg;
+            w += gg;
+        }
+}
+
+__device__ __forceinline__ void gather_cell_all(const float2 *S, const float *t3dp, int nsym, int ci, int cj, int ck,
+                                                const InsertConst &c, float weight, const float *etab,
+                                                const float *rtab, float &re, float &im, float &w)
+{
+    for (int s = 0; s < nsym; s++) gather_cell(S, t3dp + 9 * s, ci, cj, ck, c, weight, etab, rtab, re, im, w);
+}
+
+// own(p) at storage voxel (xs, ys, zs): value (ore, oim) and weight ow
+__device__ void own_contribution(const float2 *S, const float *t3dp, int nsym, int xs, int ys, int zs,
+                                 const InsertConst &c, float weight, const float *etab, const float *rtab,
+                                 float &ore, float &oim, float &ow)
+{
+    const int H = c.H, P = c.P;
+    ore = oim = ow = 0.f;
+    int jd[2], njd = 0, jn[2], njn = 0, kd[2], nkd = 0, kn[2], nkn = 0;
+    if (ys <= H) jd[njd++] = ys;
+    if (ys >= H) jd[njd++] = ys - P;
+    if (ys <= H) jn[njn++] = -ys;
+    if (ys >= H && P - ys <= H && P - ys != -ys) jn[njn++] = P - ys;
+    if (zs <= H) kd[nkd++] = zs;
+    if (zs >= H) kd[nkd++] = zs - P;
+    if (zs <= H) kn[nkn++] = -zs;
+    if (zs >= H && P - zs <= H && P - zs != -zs) kn[nkn++] = P - zs;
+    if (ys == 0 && zs == 0 && (xs == 0 || xs == H)) {
+        float r = 0, i = 0, w = 0;
+        gather_cell_all(S, t3dp, nsym, xs, 0, 0, c, weight, etab, rtab, r, i, w);
+        if (xs == H) gather_cell_all(S, t3dp, nsym, -H, 0, 0, c, weight, etab, rtab, r, i, w);
+        ore = r;
+        ow = w;
+        return;
+    }
+    for (int a = 0; a < njd; a++)
+        for (int b = 0; b < nkd; b++) {  // direct: value and weight
+            float r = 0, i = 0, w = 0;
+            gather_cell_all(S, t3dp, nsym, xs, jd[a], kd[b], c, weight, etab, rtab, r, i, w);
+            ore += r;
+            oim += i;
+            ow += w;
+        }
+    for (int a = 0; a < njn; a++)
+        for (int b = 0; b < nkn; b++) {
+            float r = 0, i = 0, w = 0;
+            if (xs == 0 || xs == H) {
+                // conjugate copy of the double insertion: value only
+                gather_cell_all(S, t3dp, nsym, xs, jn[a], kn[b], c, weight, etab, rtab, r, i, w);
+                ore += r;
+                oim -= i;
+                if (xs == H) {  // x = -P/2 cells: conjugate added twice, weight once
+                    float r2 = 0, i2 = 0, w2 = 0;
+                    gather_cell_all(S, t3dp, nsym, -H, jn[a], kn[b], c, weight, etab, rtab, r2, i2, w2);
+                    ore += 2.f * r2;
+                    oim -= 2.f * i2;
+                    ow += w2;
+                }
+            } else {
+                gather_cell_all(S, t3dp, nsym, -xs, jn[a], kn[b], c, weight, etab, rtab, r, i, w);
+                ore += r;
+                oim -= i;
+                ow += w;
+            }
+        }
+}
+
+// One block row per (particle, symmetry op) slice; eight double sums per
+// particle accumulated with float64 atomics into acc[p][8].
+__global__ void __launch_bounds__(256) k_agree_batch(const float2 *__restrict__ slices, int nsym,
+                                                     const float *__restrict__ t3d, const float *__restrict__ wq,
+                                                     InsertConst c, float rmax, int sub,
+                                                     const float2 *__restrict__ data, const float *__restrict__ norm,
+                                                     const float *__restrict__ exptab, const float *__restrict__ rtab,
+                                                     double *__restrict__ acc_out)
+{
+    const int H = c.H, P = c.P;
+    const int q = blockIdx.y, p = q / nsym;
+    const int pidx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int W = H + 2;
+    const float2 *S = slices + (size_t)p * P * (H + 1);
+    const float *t3dp = t3d + 9 * (size_t)p * nsym;
+    const float weight = wq[q];
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (pidx < W * P) {
+        const int x = pidx % W, y = pidx / W - H;
+        const float *m = t3d + 9 * (size_t)q;
+        if (!(x == 0 && y == 0)) {
+            const float rx = rtab[x + H], ry = rtab[y + H];
+            const float c0 = rx * m[0] + ry * m[3];
+            const float c1 = rx * m[1] + ry * m[4];
+            const float c2 = rx * m[2] + ry * m[5];
+            if (!(fabsf(c0) > 0.5f || fabsf(c1) >= 0.5f || fabsf(c2) >= 0.5f)) {
+                const float xx = c0 * c.Pf, yy = c1 * c.Pf, zz = c2 * c.Pf;
+                const float *Sf = reinterpret_cast<const float *>(S);
+                const int slen = (P + 2) * P;
+                const int fidx = 2 * x + (P + 2) * (y < 0 ? P + y : y);
+                const float d20 = fidx < slen ? Sf[fidx] : 0.f;
+                const float d21 = fidx + 1 < slen ? Sf[fidx + 1] : 0.f;
+                const int x0 = round_ref(xx), y0 = round_ref(yy), z0 = round_ref(zz);
+                int xs, ysn, zsn;
+                if (x0 < 0) {
+                    xs = -x0;
+                    ysn = y0 <= 0 ? -y0 : P - y0;
+                    zsn = z0 <= 0 ? -z0 : P - z0;
+                } else {
+                    xs = x0;
+                    ysn = y0 < 0 ? P + y0 : y0;
+                    zsn = z0 < 0 ? P + z0 : z0;
+                }
+                const size_t ci = cidx(xs, ysn, zsn, P, H);
+                float vr = 0.f, vi = 0.f;
+                const bool inb = !(abs(x0) >= H + 1 || abs(y0) > H || abs(z0) > H);
+                if (inb) {
+                    const float2 v = data[ci];
+                    vr = v.x;
+                    vi = v.y;
+                }
+                float nv = norm[ci];
+                if (sub) {
+                    float ore, oim, ow;
+                    own_contribution(S, t3dp, nsym, xs, ysn, zsn, c, weight, exptab, rtab, ore, oim, ow);
+                    if (inb) {
+                        vr = vr - ore;
+                        vi = vi - oim;
+                    }
+                    nv = nv - ow;
+                }
+                if (x0 < 0) vi = -vi;  // get_complex_at conjugate for x<0
+                if (nv != 0.f) {
+                    const float dt0 = __fdiv_rn(vr, nv), dt1 = __fdiv_rn(vi, nv), dt2 = nv;
+                    const float r = __fsqrt_rn((float)(x * x + y * y));
+                    const float dotv = (dt0 * d20 + dt1 * d21) * dt2;
+                    const float pw = (dt0 * dt0 + dt1 * dt1) * dt2;
+                    const float pw2 = (d20 * d20 + d21 * d21) * dt2;
+                    if (r > 3 && r < rmax) {
+                        acc[4] += dotv;
+                        acc[5] += pw;
+                        acc[6] += pw2;
+                    }
+                    acc[1] += dt2;
+                    acc[0] += dotv;
+                    acc[2] += pw;
+                    acc[3] += pw2;
+                    acc[7] += 1.0;
+                }
+            }
+        }
+    }
+    __shared__ double red[8][256];
+#pragma unroll
+    for (int k = 0; k < 8; k++) red[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+#pragma unroll
+            for (int k = 0; k < 8; k++) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < 8) unsafeAtomicAdd(&acc_out[(size_t)p * 8 + threadIdx.x], red[threadIdx.x][0]);
+}
+
+hipError_t launch_agree_batch(const float2 *slices, int nsym, const float *t3d, const float *wq, int nslices,
+                              InsertConst c, float rmax, int sub, const float2 *data, const float *norm,
+                              const float *exptab, const float *rtab, double *acc, hipStream_t s)
+{
+    if (nslices <= 0) return hipSuccess;
+    c.corners = 0;
+    hipLaunchKernelGGL(k_agree_batch, dim3(agree_blocks(c.P), nslices), dim3(256), 0, s, slices, nsym, t3d, wq, c, rmax,
+                       sub, data, norm, exptab, rtab, acc);
+    return hipGetLastError();
+}
+
 hipError_t launch_agree(const float2 *slices, int nsym, const float *t3d, int nslices, InsertConst c, float rmax,
                         const float2 *data, const float *norm, const float *rtab, double *partial, hipStream_t s)
 {

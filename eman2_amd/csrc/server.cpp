@@ -1,7 +1,7 @@
 // fourier_hip serving loop: FIFO admission by HBM footprint, one worker per
 // active job, each job's volume resident on the device on its own stream,
 // chunks of concurrently running jobs overlapping on the GPU.
-#include "../../include/fourier_hip.h"
+#include "fourier_hip.h"
 
 #include <hip/hip_runtime.h>
 

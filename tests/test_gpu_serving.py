@@ -64,7 +64,7 @@ def test_matches_direct_api():
 def test_admission_by_footprint_and_rejection():
     from eman2_amd.serving import Server
     jobs = make_jobs()
-    probe = Server(max_active=1)
+    probe = Server(max_active=1, chunk=64)  # same chunk as below: same footprint
     fp = [probe.poll(probe.submit(**j))["footprint_bytes"] for j in jobs[:1]]
     probe.drain()
     probe.close()
