@@ -9,8 +9,8 @@ Workload (one step = one reconstruction pass over the whole batch):
   384^3 (complex, weight) volume -- fh_insert_batch with the particles already
   resident in HBM.  Each step re-runs setup() and inserts all particles.
 
-Prints one JSON line with the throughput, the brick-insertion kernel's
-roofline fraction (HBM-bound scatter model), the plain one-particle-per-launch
+Prints one JSON line with the throughput, the column-gather insertion
+kernel's roofline fraction (HBM-bound scatter model), the plain one-particle-per-launch
 GPU form, and the CPU oracle timed on this host's cores on a bounded sample.
 """
 import argparse
@@ -49,10 +49,10 @@ def parse():
     ap.add_argument("--mode", default="gauss_2")
     ap.add_argument("--apix", type=float, default=1.0)
     ap.add_argument("--snr", type=float, default=0.05)
-    ap.add_argument("--brick", type=int, default=None)
+    ap.add_argument("--wpb", type=int, default=None, help="waves (item workers) per workgroup")
     ap.add_argument("--spi", type=int, default=None, help="slices per work item")
     ap.add_argument("--chunk", type=int, default=None)
-    ap.add_argument("--wgs", type=int, default=None, help="persistent workgroups per CU")
+    ap.add_argument("--bpc", type=int, default=None, help="persistent workgroups per CU")
     ap.add_argument("--plain-sample", type=int, default=200)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -164,7 +164,7 @@ def main():
 
     r = Reconstructors.get("fourier_hip", {"size": (P, P, P), "sym": a.sym, "mode": a.mode})
     tune = r.get_tuning()
-    for k, v in (("brick", a.brick), ("slices_per_item", a.spi), ("chunk", a.chunk), ("wgs_per_cu", a.wgs)):
+    for k, v in (("waves_per_block", a.wpb), ("slices_per_item", a.spi), ("chunk", a.chunk), ("blocks_per_cu", a.bpc)):
         if v is not None:
             tune[k] = v
     r.set_tuning(**tune)
@@ -188,14 +188,14 @@ def main():
         st = r.get_stats()
         stats.append(st)
         kern_ms += st["insert_kernel_ms"]
-        print("[bench] step %d %.1f ms (prep %.1f bin %.1f insert %.1f)" % (
-            i, lat[-1], st["prep_ms"], st["bin_ms"], st["insert_ms"]), file=sys.stderr, flush=True)
+        print("[bench] step %d %.1f ms (prep %.1f bin %.1f insert %.1f merge %.1f)" % (
+            i, lat[-1], st["prep_ms"], st["bin_ms"], st["insert_kernel_ms"], st["merge_ms"]), file=sys.stderr, flush=True)
     r.sync()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     value = a.steps * n / elapsed
 
-    # roofline of the dominant kernel (k_brick_insert): scatter model, one
+    # roofline of the dominant kernel (k_column_insert): scatter model, one
     # 12-byte (complex + weight) update per gridding contribution, 8 per
     # pixel per symmetry copy
     bytes_per_particle = nsym * npix(P) * 8 * 12
@@ -204,7 +204,7 @@ def main():
     per_launch_bytes = bytes_per_particle * (a.steps * n) / max(1, launches)
     achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "brick_insert_traffic.json")
+    tf = os.path.join(ROOT, "profiles", "column_insert_traffic.json")
     if os.path.exists(tf):
         with open(tf) as f:
             t = json.load(f)
@@ -212,7 +212,7 @@ def main():
             traffic = t.get("hbm_bytes_per_particle", 0) * (a.steps * n) / max(1, launches)
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": "k_brick_insert", "avg_launch_ms": round(avg_ms, 3),
+            "kernel": "k_column_insert", "avg_launch_ms": round(avg_ms, 3),
             "algorithmic_bytes_per_launch": int(per_launch_bytes)}
     # whole-pipeline effective bandwidth with the 4*B^2 raw read (BASELINE.md formula)
     pipe_bytes = (4 * B * B + bytes_per_particle) * a.steps * n
@@ -233,7 +233,7 @@ def main():
                            "p99": round(sorted(lat)[min(len(lat) - 1, int(math.ceil(0.99 * len(lat))) - 1)], 2),
                            "what": "setup + insert_batch of all particles, per step"},
         "stage_ms_per_step": {k: round(sum(s[k] for s in stats) / a.steps, 2)
-                              for k in ("prep_ms", "bin_ms", "insert_ms", "total_ms")},
+                              for k in ("prep_ms", "bin_ms", "insert_kernel_ms", "merge_ms", "insert_ms", "total_ms")},
         "pairs_per_step": int(sum(s["pairs"] for s in stats) / a.steps),
     }
 

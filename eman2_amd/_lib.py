@@ -33,12 +33,12 @@ class Ctf(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("brick", _c_int), ("slices_per_item", _c_int), ("chunk", _c_int), ("wgs_per_cu", _c_int)]
+    _fields_ = [("waves_per_block", _c_int), ("slices_per_item", _c_int), ("chunk", _c_int), ("blocks_per_cu", _c_int)]
 
 
 class Stats(ctypes.Structure):
     _fields_ = [("prep_ms", _c_double), ("bin_ms", _c_double), ("insert_ms", _c_double),
-                ("total_ms", _c_double), ("insert_kernel_ms", _c_double), ("slices", _c_ll),
+                ("total_ms", _c_double), ("insert_kernel_ms", _c_double), ("merge_ms", _c_double), ("slices", _c_ll),
                 ("pairs", _c_ll), ("items", _c_ll), ("chunks", _c_ll)]
 
 

@@ -1,5 +1,5 @@
-"""Autotuner for the batched insertion: sweeps brick edge, slices per work
-item, persistent workgroups per CU and particles per launch on a short
+"""Autotuner for the batched insertion: sweeps waves per workgroup, slices
+per work item, persistent workgroups per CU and particles per launch on a short
 HBM-resident synthetic workload and records the fastest setting per
 (pad, symmetry-op count) in tuning_table.json next to this file.
 
@@ -51,16 +51,16 @@ def main():
     ap.add_argument("--pad", type=int, default=384)
     ap.add_argument("--sym", default="c1")
     ap.add_argument("--particles", type=int, default=8192)
-    ap.add_argument("--bricks", default="8,16")
+    ap.add_argument("--wpb", default="1,2,4")
     ap.add_argument("--spi", default="16,32,64,128,256")
-    ap.add_argument("--wgs", default="1,2,3")
+    ap.add_argument("--bpc", default="2,4,8")
     ap.add_argument("--chunks", default="2048,4096")
     ap.add_argument("--write", action="store_true", help="update tuning_table.json")
     a = ap.parse_args()
     import torch
 
     from . import synthetic as S
-    from .reconstructor import Reconstructors
+    from .reconstructor import TUNING_KEYS, Reconstructors
 
     dev = "cuda:0"
     ph = S.phantom(200, 0.375 * a.box, seed=2)
@@ -72,12 +72,10 @@ def main():
     r = Reconstructors.get("fourier_hip", {"size": (a.pad,) * 3, "sym": a.sym})
     ptr = int(imgs.data_ptr())
     results = []
-    grid = itertools.product([int(x) for x in a.bricks.split(",")], [int(x) for x in a.spi.split(",")],
-                             [int(x) for x in a.wgs.split(",")], [int(x) for x in a.chunks.split(",")])
-    for brick, spi, wgs, chunk in grid:
-        if brick == 8 and wgs > 3:
-            continue
-        r.set_tuning(brick=brick, slices_per_item=spi, chunk=chunk, wgs_per_cu=wgs)
+    grid = itertools.product([int(x) for x in a.wpb.split(",")], [int(x) for x in a.spi.split(",")],
+                             [int(x) for x in a.bpc.split(",")], [int(x) for x in a.chunks.split(",")])
+    for wpb, spi, bpc, chunk in grid:
+        r.set_tuning(waves_per_block=wpb, slices_per_item=spi, chunk=chunk, blocks_per_cu=bpc)
         r.setup()
         r.insert_batch(ptr, mats, w, defocus=dfs, ctf=ctf, box=a.box)  # warm
         r.setup()
@@ -86,7 +84,7 @@ def main():
         r.sync()
         dt = time.perf_counter() - t0
         st = r.get_stats()
-        rec = {"brick": brick, "slices_per_item": spi, "wgs_per_cu": wgs, "chunk": chunk,
+        rec = {"waves_per_block": wpb, "slices_per_item": spi, "blocks_per_cu": bpc, "chunk": chunk,
                "particles_per_s": a.particles / dt, "insert_kernel_ms_per_particle": st["insert_kernel_ms"] / a.particles}
         results.append(rec)
         print(json.dumps(rec), flush=True)
@@ -95,7 +93,7 @@ def main():
     if a.write:
         t = load_table()
         t["%d,%d" % (a.pad, nsym_of(a.sym))] = {
-            "tuning": {k: best[k] for k in ("brick", "slices_per_item", "chunk", "wgs_per_cu")},
+            "tuning": {k: best[k] for k in TUNING_KEYS},
             "particles_per_s": round(best["particles_per_s"], 1), "box": a.box, "particles": a.particles}
         with open(TABLE, "w") as f:
             json.dump(t, f, indent=1, sort_keys=True)

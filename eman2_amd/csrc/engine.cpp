@@ -212,8 +212,7 @@ struct fh_reconstructor {
     BrickGeom geom{};
     BinBuffers bins{};
     long long bins_cap_slices = 0;
-    int bins_brick = 0;
-    fh_tuning tune{16, 64, 2048, 3};
+    fh_tuning tune{4, 64, 2048, 4};
     fh_stats stats{};
     hipEvent_t ev[6] = {};
 };
@@ -249,6 +248,8 @@ static void free_bins(fh_reconstructor *h)
     for (int **p : {&b.counts, &b.offsets, &b.cursors, &b.item_off, &b.list, &b.ctl})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
     if (b.items) { (void)hipFree(b.items); b.items = nullptr; }
+    if (b.part) { (void)hipFree(b.part); b.part = nullptr; }
+    b.part_cap = 0;
     h->bins_cap_slices = 0;
 }
 
@@ -278,29 +279,46 @@ static int ensure_chunk(fh_reconstructor *h, int n, int B)
     return FH_OK;
 }
 
+static BrickGeom brick_geom(int P)
+{
+    BrickGeom g;
+    g.nb = (P + 1 + BRICK - 1) / BRICK;
+    g.nbricks = g.nb * g.nb * g.nb;
+    g.nkeys = 3 * g.nbricks;
+    return g;
+}
+
 static int ensure_bins(fh_reconstructor *h, long long nslices)
 {
-    const int Bs = h->tune.brick;
-    if (nslices <= h->bins_cap_slices && Bs == h->bins_brick) return FH_OK;
+    if (nslices <= h->bins_cap_slices) return FH_OK;
     free_bins(h);
-    BrickGeom g;
-    g.Bs = Bs;
-    g.nb = (h->P + 1 + Bs - 1) / Bs;
-    g.nbricks = g.nb * g.nb * g.nb;
+    const BrickGeom g = brick_geom(h->P);
     h->geom = g;
     BinBuffers &b = h->bins;
     // at most 5 bricks per brick column per slice (see k_bin)
     b.list_cap = 5LL * g.nb * g.nb * nslices;
-    b.item_cap = b.list_cap / std::max(1, h->tune.slices_per_item) + g.nbricks + 1;
-    FH_TRY(h, hipMalloc(&b.counts, sizeof(int) * g.nbricks));
-    FH_TRY(h, hipMalloc(&b.offsets, sizeof(int) * (g.nbricks + 1)));
-    FH_TRY(h, hipMalloc(&b.cursors, sizeof(int) * g.nbricks));
-    FH_TRY(h, hipMalloc(&b.item_off, sizeof(int) * (g.nbricks + 1)));
+    b.item_cap = b.list_cap / std::max(1, h->tune.slices_per_item) + g.nkeys + 1;
+    FH_TRY(h, hipMalloc(&b.counts, sizeof(int) * g.nkeys));
+    FH_TRY(h, hipMalloc(&b.offsets, sizeof(int) * (g.nkeys + 1)));
+    FH_TRY(h, hipMalloc(&b.cursors, sizeof(int) * g.nkeys));
+    FH_TRY(h, hipMalloc(&b.item_off, sizeof(int) * (g.nkeys + 1)));
     FH_TRY(h, hipMalloc(&b.list, sizeof(int) * b.list_cap));
     FH_TRY(h, hipMalloc(&b.items, sizeof(WorkItem) * b.item_cap));
     FH_TRY(h, hipMalloc(&b.ctl, sizeof(int) * 4));
     h->bins_cap_slices = nslices;
-    h->bins_brick = Bs;
+    return FH_OK;
+}
+
+// Partial bricks for `items` work items (grown on demand, 6 KiB each).
+static int ensure_partials(fh_reconstructor *h, long long items)
+{
+    BinBuffers &b = h->bins;
+    if (items <= b.part_cap) return FH_OK;
+    if (b.part) { (void)hipFree(b.part); b.part = nullptr; }
+    b.part_cap = 0;
+    const long long cap = items + items / 4 + 1024;
+    FH_TRY(h, hipMalloc(&b.part, sizeof(float) * PART_FLOATS * (size_t)cap));
+    b.part_cap = cap;
     return FH_OK;
 }
 
@@ -623,7 +641,8 @@ int fh_insert_batch(fh_reconstructor *h, const float *imgs, int n, int box, cons
     const InsertConst c = iconst(h);
     fh_stats st{};
     float ms;
-    const int grid = h->cus * std::max(1, h->tune.wgs_per_cu);
+    const int wpb = h->tune.waves_per_block;
+    const int grid = h->cus * h->tune.blocks_per_cu;
     for (int c0 = 0; c0 < n; c0 += chunk) {
         const int m = std::min(chunk, n - c0);
         const float *src = imgs + (size_t)c0 * box * box;
@@ -643,21 +662,34 @@ int fh_insert_batch(fh_reconstructor *h, const float *imgs, int n, int box, cons
         FH_TRY(h, hipEventRecord(h->ev[1], h->stream));
         FH_TRY(h, launch_bin(h->t3d, h->wq, m * h->nsym, c, h->geom, h->bins, h->tune.slices_per_item, h->stream));
         FH_TRY(h, hipEventRecord(h->ev[2], h->stream));
-        FH_TRY(h, launch_brick_insert(h->slices, m * h->nsym, h->nsym, h->t3d, h->wq, c, h->geom, h->bins, h->data, h->norm,
-                                      h->exptab, h->rtab, grid, h->stream));
-        FH_TRY(h, hipEventRecord(h->ev[3], h->stream));
-        FH_TRY(h, hipEventSynchronize(h->ev[3]));
+        // the work item count sizes the partial-brick buffer
         int ctl[4];
-        FH_TRY(h, hipMemcpy(ctl, h->bins.ctl, sizeof(ctl), hipMemcpyDeviceToHost));
+        FH_TRY(h, hipMemcpyAsync(ctl, h->bins.ctl, sizeof(ctl), hipMemcpyDeviceToHost, h->stream));
+        FH_TRY(h, hipStreamSynchronize(h->stream));
         if (ctl[3]) return fail(h, FH_ERR_DEVICE, "internal error: brick list capacity exceeded");
+        rc = ensure_partials(h, ctl[1]);
+        if (rc) return rc;
+        FH_TRY(h, hipEventRecord(h->ev[3], h->stream));
+        if (ctl[1] > 0) {
+            FH_TRY(h, launch_column_insert(h->slices, m * h->nsym, h->nsym, h->t3d, h->wq, c, h->geom, h->bins,
+                                           h->exptab, h->rtab, std::min(grid, (ctl[1] + wpb - 1) / wpb), wpb,
+                                           h->stream));
+        }
+        FH_TRY(h, hipEventRecord(h->ev[4], h->stream));
+        if (ctl[1] > 0) FH_TRY(h, launch_merge_partials(h->geom, h->bins, c, h->data, h->norm, h->stream));
+        FH_TRY(h, hipEventRecord(h->ev[5], h->stream));
+        FH_TRY(h, hipEventSynchronize(h->ev[5]));
         FH_TRY(h, hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
         st.prep_ms += ms;
         FH_TRY(h, hipEventElapsedTime(&ms, h->ev[1], h->ev[2]));
         st.bin_ms += ms;
-        FH_TRY(h, hipEventElapsedTime(&ms, h->ev[2], h->ev[3]));
-        st.insert_ms += ms;
+        FH_TRY(h, hipEventElapsedTime(&ms, h->ev[3], h->ev[4]));
         st.insert_kernel_ms += ms;
-        FH_TRY(h, hipEventElapsedTime(&ms, h->ev[0], h->ev[3]));
+        st.insert_ms += ms;
+        FH_TRY(h, hipEventElapsedTime(&ms, h->ev[4], h->ev[5]));
+        st.merge_ms += ms;
+        st.insert_ms += ms;
+        FH_TRY(h, hipEventElapsedTime(&ms, h->ev[0], h->ev[5]));
         st.total_ms += ms;
         st.slices += (long long)m * h->nsym;
         st.pairs += ctl[2];
@@ -919,10 +951,11 @@ int fh_set_tuning(fh_reconstructor *h, const fh_tuning *t)
 {
     if (!h || !t) return FH_ERR_NULL;
     std::lock_guard<std::mutex> lk(h->mu);
-    if (t->brick != 8 && t->brick != 16) return fail(h, FH_ERR_INVALID_PARAM, "brick must be 8 or 16");
-    if (t->slices_per_item < 1 || t->chunk < 1 || t->wgs_per_cu < 1)
+    if (t->waves_per_block < 1 || t->waves_per_block > 4)
+        return fail(h, FH_ERR_INVALID_PARAM, "waves_per_block must be in [1, 4]");
+    if (t->slices_per_item < 1 || t->chunk < 1 || t->blocks_per_cu < 1)
         return fail(h, FH_ERR_INVALID_PARAM, "tuning values must be positive");
-    if (t->brick != h->tune.brick || t->slices_per_item != h->tune.slices_per_item) free_bins(h);
+    if (t->slices_per_item != h->tune.slices_per_item) free_bins(h);
     h->tune = *t;
     return FH_OK;
 }
@@ -954,9 +987,15 @@ long long fh_footprint_bytes(int P, int nsym, int chunk)
 {
     const long long v = 12LL * P * P * (P / 2 + 1);
     const long long sl = 8LL * chunk * P * (P / 2 + 1) + 4LL * chunk * P * P;
-    const int nb = (P + 1 + 15) / 16;
-    const long long bins = 4LL * 5 * nb * nb * chunk * nsym * 2 + 16LL * nb * nb * nb;
-    return v + sl + bins;
+    const BrickGeom g = brick_geom(P);
+    const long long nslices = (long long)chunk * nsym;
+    const long long list = 5LL * g.nb * g.nb * nslices;
+    const long long items = list / 64 + g.nkeys + 1;
+    const long long bins = 4LL * list + 16LL * items + 16LL * g.nkeys;
+    // partial bricks: about 1.1 (P/8)^2 (slice, brick) pairs per slice, one
+    // item per 64 pairs of a key plus one per key the slices reach
+    const long long part_items = (long long)(1.1 * (P / 8.0) * (P / 8.0) * nslices / 64) + g.nkeys / 2;
+    return v + sl + bins + 4LL * PART_FLOATS * part_items;
 }
 
 }  // extern "C"

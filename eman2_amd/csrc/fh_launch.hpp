@@ -35,31 +35,36 @@ struct Mat9 {
 };
 
 // Work item of the batched insertion: slices list[start, start+count) of
-// brick `brick`.
+// bin key `key` = 3 * brick + dominant normal axis of the slices.
 struct WorkItem {
-    int brick;
+    int key;
     int start;
     int count;
     int pad;
 };
 
-// Device buffers of the binning stage.
+// Device buffers of the binning stage and the partial bricks.
 struct BinBuffers {
-    int *counts;      // nbricks
-    int *offsets;     // nbricks + 1
-    int *cursors;     // nbricks
-    int *item_off;    // nbricks + 1
+    int *counts;      // nkeys
+    int *offsets;     // nkeys + 1
+    int *cursors;     // nkeys
+    int *item_off;    // nkeys + 1: first work item of each key
     int *list;        // capacity
     WorkItem *items;  // item capacity
     int *ctl;         // [0] work counter, [1] total items, [2] total pairs, [3] overflow flag
+    float *part;      // partial bricks, PART_FLOATS per work item
     long long list_cap;
     long long item_cap;
+    long long part_cap;  // work items the partial buffer holds
 };
 
+constexpr int BRICK = 8;                             // brick edge (voxels)
+constexpr int PART_FLOATS = 3 * BRICK * BRICK * BRICK;  // re, im, weight of one brick
+
 struct BrickGeom {
-    int Bs;     // brick edge (voxels)
-    int nb;     // bricks per axis over the unfolded grid [-H, H]
-    int nbricks;
+    int nb;       // bricks per axis over the unfolded grid [-H, H]
+    int nbricks;  // nb^3
+    int nkeys;    // 3 * nbricks
 };
 
 // ---- prep (prep_kernels.hip) ----
@@ -88,10 +93,12 @@ hipError_t launch_insert_plain(const float2 *slice, InsertConst c, const float *
                                hipStream_t s);
 hipError_t launch_bin(const float *t3d, const float *wq, int nslices, InsertConst c, BrickGeom g,
                       BinBuffers b, int slices_per_item, hipStream_t s);
-hipError_t launch_brick_insert(const float2 *slices, int nslices, int nsym,
-                               const float *t3d, const float *wq, InsertConst c, BrickGeom g,
-                               BinBuffers b, float2 *data, float *norm, const float *exptab,
-                               const float *rtab, int grid, hipStream_t s);
+hipError_t launch_column_insert(const float2 *slices, int nslices, int nsym, const float *t3d,
+                                const float *wq, InsertConst c, BrickGeom g, BinBuffers b,
+                                const float *exptab, const float *rtab, int grid, int waves_per_block,
+                                hipStream_t s);
+hipError_t launch_merge_partials(BrickGeom g, BinBuffers b, InsertConst c, float2 *data, float *norm,
+                                 hipStream_t s);
 int agree_blocks(int P);
 hipError_t launch_agree_batch(const float2 *slices, int nsym, const float *t3d, const float *wq, int nslices,
                               InsertConst c, float rmax, int sub, const float2 *data, const float *norm,

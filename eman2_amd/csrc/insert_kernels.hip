@@ -9,14 +9,20 @@
 // order with contraction off, so every contribution is bit-identical to the
 // reference's; only the order of the FP32 sums differs.
 //
-//  k_insert_plain  one launch per (slice, symmetry op), one thread per pixel,
-//                  global atomics per contribution (the plain form).
-//  k_bin_*         bins (slice, brick) pairs over the unfolded frequency grid
-//                  [-P/2, P/2]^3 cut into BS^3 bricks.
-//  k_brick_insert  persistent brick-stationary kernel: each work item holds a
-//                  brick's accumulator in LDS, pulls the slice pixels whose
-//                  2x2x2 stencil reaches the brick, and flushes once with the
-//                  Hermitian fold of add_complex_at_fast as global atomics.
+//  k_insert_plain    one launch per (slice, symmetry op), one thread per pixel,
+//                    global atomics per contribution (the plain form).
+//  k_bin_*           bins (slice, brick) pairs over the unfolded frequency
+//                    grid [-P/2, P/2]^3 cut into 8^3 bricks, keyed by brick
+//                    and the slice's dominant normal axis.
+//  k_column_insert   one wave per work item: each lane owns one of the
+//                    brick's 8x8 voxel columns along the item's axis and
+//                    gathers the pixels whose stencil reaches that column;
+//                    accumulation is lane-private (no atomics).  Each item
+//                    writes its partial brick to HBM.
+//  k_merge_partials  one thread per half-volume voxel sums the partial
+//                    bricks of its unfolded source cells under the Hermitian
+//                    fold of add_complex_at_fast and adds them with plain
+//                    stores (each voxel has one owner: no atomics).
 #pragma clang fp contract(off)
 #include "fh_device.hpp"
 
@@ -473,7 +479,9 @@ hipError_t launch_agree(const float2 *slices, int nsym, const float *t3d, int ns
 // a pixel reaches it when its stencil origin floor(p) lies in
 // [lo-1, lo+BS-1], i.e. p in [lo-1, lo+BS) on every axis.  Pixel positions
 // lie on the plane n.p = 0 (n = u x v of the slice's t3d rows), so per brick
-// column along the dominant normal axis at most 5 bricks qualify.
+// column along the dominant normal axis at most 5 bricks qualify.  A slice
+// is binned under key 3 * brick + (its dominant normal axis), so that every
+// work item's slices share one column direction.
 struct SliceFrame {
     double u[3], v[3], n[3];
     int a, b1, b2;
@@ -513,7 +521,7 @@ template <int MODE>  // 0 count, 1 fill
 __global__ void __launch_bounds__(256) k_bin(const float *__restrict__ t3d, const float *__restrict__ wq, int nslices,
                                              InsertConst c, BrickGeom g, BinBuffers bb)
 {
-    const int nb = g.nb, BS = g.Bs, H = c.H;
+    const int nb = g.nb, BS = BRICK, H = c.H;
     const int col = blockIdx.x * blockDim.x + threadIdx.x;
     const int q = blockIdx.y;
     if (col >= nb * nb || q >= nslices) return;
@@ -543,11 +551,11 @@ __global__ void __launch_bounds__(256) k_bin(const float *__restrict__ t3d, cons
         bc[f.b1] = c1;
         bc[f.b2] = c2;
         if (!brick_in_domain(f, bc, H, BS, c.corners)) continue;
-        const int brick = (bc[2] * nb + bc[1]) * nb + bc[0];
+        const int key = 3 * ((bc[2] * nb + bc[1]) * nb + bc[0]) + f.a;
         if (MODE == 0) {
-            atomicAdd(&bb.counts[brick], 1);
+            atomicAdd(&bb.counts[key], 1);
         } else {
-            const long long pos = (long long)bb.offsets[brick] + atomicAdd(&bb.cursors[brick], 1);
+            const long long pos = (long long)bb.offsets[key] + atomicAdd(&bb.cursors[key], 1);
             if (pos < bb.list_cap) bb.list[pos] = q;
             else bb.ctl[3] = 1;
         }
@@ -557,7 +565,7 @@ __global__ void __launch_bounds__(256) k_bin(const float *__restrict__ t3d, cons
 // Single-block scan: offsets / item offsets, totals, counter reset.
 __global__ void __launch_bounds__(1024) k_bin_scan(BrickGeom g, BinBuffers bb, int K)
 {
-    const int nbk = g.nbricks;
+    const int nbk = g.nkeys;
     const int tid = threadIdx.x;
     const int per = (nbk + 1023) / 1024;
     const int b0 = tid * per, b1 = min(nbk, b0 + per);
@@ -602,14 +610,14 @@ __global__ void __launch_bounds__(1024) k_bin_scan(BrickGeom g, BinBuffers bb, i
 __global__ void k_bin_items(BrickGeom g, BinBuffers bb, int K)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= g.nbricks) return;
+    if (b >= g.nkeys) return;
     const int cnt = bb.counts[b];
     const int base = bb.item_off[b];
     const int off = bb.offsets[b];
     for (int i = 0; i * K < cnt; i++) {
         if ((long long)base + i >= bb.item_cap) break;
         WorkItem w;
-        w.brick = b;
+        w.key = b;
         w.start = off + i * K;
         w.count = min(K, cnt - i * K);
         if ((long long)w.start + w.count > bb.list_cap) w.count = (int)max(0LL, bb.list_cap - w.start);
@@ -622,7 +630,7 @@ hipError_t launch_bin(const float *t3d, const float *wq, int nslices, InsertCons
                       int K, hipStream_t s)
 {
     hipError_t e;
-    if ((e = launch_fill_i32(b.counts, 0, g.nbricks, s)) != hipSuccess) return e;
+    if ((e = launch_fill_i32(b.counts, 0, g.nkeys, s)) != hipSuccess) return e;
     if ((e = launch_fill_i32(b.ctl, 0, 4, s)) != hipSuccess) return e;
     if (nslices > 0) {
         dim3 grid((g.nb * g.nb + 255) / 256, nslices);
@@ -636,243 +644,322 @@ hipError_t launch_bin(const float *t3d, const float *wq, int nslices, InsertCons
         hipLaunchKernelGGL(k_bin<1>, grid, dim3(256), 0, s, t3d, wq, nslices, c, g, b);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_bin_items, dim3((g.nbricks + 255) / 256), dim3(256), 0, s, g, b, K);
+    hipLaunchKernelGGL(k_bin_items, dim3((g.nkeys + 255) / 256), dim3(256), 0, s, g, b, K);
     return hipGetLastError();
 }
 
-// ------------------------------------------------------------ brick insertion
-// Fold one unfolded cell's accumulated (value, weight) into the half-volume
-// storage exactly as add_complex_at_fast places each contribution.
-__device__ __forceinline__ void fold_cell(float2 *data, float *norm, int P, int H, int i, int j, int k, float re,
-                                          float im, float w)
-{
-    float *d = reinterpret_cast<float *>(data);
-    const int wy = j < 0 ? P + j : j, wz = k < 0 ? P + k : k;
-    const int ny = j <= 0 ? -j : P - j, nz = k <= 0 ? -k : P - k;
-    if (j == 0 && k == 0 && (i == 0 || i == H || i == -H)) {
-        const int xs = i < 0 ? -i : i;
-        gadd(&d[2 * xs], re);
-        d[2 * xs + 1] = 0.f;
-        gadd(&norm[xs], w);
-        return;
-    }
-    if (i == 0 || i == H) {
-        const size_t a = cidx(i, ny, nz, P, H), b = cidx(i, wy, wz, P, H);
-        gadd(&d[2 * a], re);
-        gadd(&d[2 * a + 1], -im);
-        gadd(&d[2 * b], re);
-        gadd(&d[2 * b + 1], im);
-        gadd(&norm[b], w);
-        return;
-    }
-    if (i == -H) {
-        const size_t a = cidx(H, ny, nz, P, H);
-        gadd(&d[2 * a], re);
-        gadd(&d[2 * a + 1], -im);
-        gadd(&d[2 * a], re);
-        gadd(&d[2 * a + 1], -im);
-        gadd(&norm[a], w);
-        return;
-    }
-    if (i > 0) {
-        const size_t b = cidx(i, wy, wz, P, H);
-        gadd(&d[2 * b], re);
-        gadd(&d[2 * b + 1], im);
-        gadd(&norm[b], w);
-        return;
-    }
-    const size_t a = cidx(-i, ny, nz, P, H);
-    gadd(&d[2 * a], re);
-    gadd(&d[2 * a + 1], -im);
-    gadd(&norm[a], w);
-}
+// ------------------------------------------------------------ column-gather insertion
+// A work item is (brick, axis A, up to K slices whose dominant normal axis is
+// A).  Lane l of the item's wave owns the brick's voxel column
+// (c1, c2) = (lo[B1] + l%8, lo[B2] + l/8) along A (B1 < B2 the other two
+// axes).  A pixel at p adds to cells floor(p) + {0,1}^3, so it reaches the
+// column iff p[B] lies in [c_B - 1, c_B + 1) on both B axes: per slice the
+// lane visits that parallelogram of slice pixels and adds each pixel's two
+// cells (floor(p[A]) and +1) of its column.  Every (pixel, cell)
+// contribution of the reference loop lands in exactly one (brick, column)
+// -- the accumulation needs no atomics.
+//
+// Per-wave LDS accumulator: [field re/im/w][slot][lane], slot s holding the
+// column cell lo[A] - 1 + s (s = 0 and CSLOT-1 are pads for stencil cells
+// just outside the brick, never merged).  Lanes of one access hit distinct
+// lane columns, so the read-modify-writes are bank-conflict free.
+constexpr int CSLOT = BRICK + 2;
+constexpr int CACC = 3 * CSLOT * 64;  // floats of one wave's accumulator
+#define COL_DLT 2e-3f                 // margin on p[B]: FP32 position error is < 3e-4 for P <= 1024
+#define COL_EPS 1e-3f                 // margin on enumeration bounds
 
-// LDS accumulator index of brick cell (i,j,k): the row offset is skewed by
-// (9j + 11k) (16^3) or (j + 3k) (8^3) so that the cells hit by one pixel
-// row -- a discretised 3-D line in any direction -- spread over the banks.
-template <int BS>
-__device__ __forceinline__ int acc_index(int i, int j, int k)
+// Sweep of one slice over the wave's 64 columns.  The parallelogram is
+// enumerated in rows of the slice coordinate (x or y) with fewer rows,
+// relative to an integer pixel (xc0, yc0) near the brick's preimage so that
+// the bounds are small numbers; the bounds carry margins and every pixel is
+// then checked with the reference's own FP32 coordinates.
+template <int A>
+__device__ __forceinline__ void column_slice(const float2 *__restrict__ S, const float mm[6], float weight,
+                                             const InsertConst &c, const int lo[3], int lane, float *acc,
+                                             const float *etab, const float *rtab)
 {
-    if (BS == 16) return ((k * 16 + j) << 4) + ((i + 9 * j + 11 * k) & 15);
-    return ((k * 8 + j) << 3) + ((i + j + 3 * k) & 7);
-}
-template <int BS>
-__device__ __forceinline__ void acc_cell(int a, int &i, int &j, int &k)
-{
-    k = a / (BS * BS);
-    j = (a / BS) % BS;
-    if (BS == 16) i = ((a & 15) - 9 * j - 11 * k) & 15;
-    else i = ((a & 7) - j - 3 * k) & 7;
-}
-
-template <int BS>
-__global__ void __launch_bounds__(512) k_brick_insert(const float2 *__restrict__ slices, int nslices, int nsym,
-                                                      const float *__restrict__ t3d, const float *__restrict__ wq,
-                                                      InsertConst c, BrickGeom g, BinBuffers bb, float2 *data,
-                                                      float *norm, const float *__restrict__ exptab_g,
-                                                      const float *__restrict__ rtab_g)
-{
-    constexpr int NC = BS * BS * BS;
-    constexpr int NW = 8;  // waves per block
-    __shared__ float acc_re[NC], acc_im[NC], acc_w[NC];
-    __shared__ float etab[EXPN];
-    __shared__ int s_item;
-    extern __shared__ float rtab[];  // P+1 entries: fl(i/P), i in [-H, H]
-    const int H = c.H, P = c.P, nb = g.nb;
-    const float Pf = c.Pf;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int i = tid; i < EXPN; i += blockDim.x) etab[i] = exptab_g[i];
-    for (int i = tid; i <= P; i += blockDim.x) rtab[i] = rtab_g[i];
-    const int total_items = bb.ctl[1];
-    const size_t slice_elems = (size_t)P * (H + 1);
-
+    constexpr int B1 = A == 0 ? 1 : 0, B2 = A == 2 ? 1 : 2;
+    const int H = c.H, P = c.P;
+    const int c1 = lo[B1] + (lane & 7), c2 = lo[B2] + (lane >> 3);
+    // slice set-up (wave-uniform)
+    const float u1 = mm[B1], u2 = mm[B2], v1 = mm[3 + B1], v2 = mm[3 + B2];
+    const float D = u1 * v2 - u2 * v1;  // = +-n[A], |n[A]| >= 1/sqrt(3)
+    if (!(fabsf(D) >= 1e-3f)) return;   // not a rotation (or NaN): keeps every bound finite
+    const float iD = 1.0f / D;
+    const float C1 = (float)lo[B1] + 3.5f, C2 = (float)lo[B2] + 3.5f;
+    const int xc0 = (int)rintf((v2 * C1 - v1 * C2) * iD), yc0 = (int)rintf((u1 * C2 - u2 * C1) * iD);
+    const float P01 = (float)xc0 * u1 + (float)yc0 * v1, P02 = (float)xc0 * u2 + (float)yc0 * v2;
+    const bool rows_y = fabsf(u1) + fabsf(u2) <= fabsf(v1) + fabsf(v2);
+    // p[B] = S * a_B + R * b_B (+ P0_B): R the row coordinate, S the in-row one
+    const float a1 = rows_y ? u1 : v1, a2 = rows_y ? u2 : v2, b1 = rows_y ? v1 : u1, b2 = rows_y ? v2 : u2;
+    const float iDp = rows_y ? iD : -iD;  // 1 / (a1 b2 - a2 b1)
+    const float rr = (fabsf(a1) + fabsf(a2)) * (1.0f + COL_DLT) * fabsf(iDp);
+    // in-row bounds from each B axis whose coefficient is not small; one of
+    // |a1|, |a2| is >= 0.408 because |det| >= 0.577 and |(b1, b2)| <= 1
+    const bool use1 = fabsf(a1) >= 0.25f, use2 = fabsf(a2) >= 0.25f;
+    const float ia1 = use1 ? 1.0f / a1 : 0.f, ia2 = use2 ? 1.0f / a2 : 0.f;
+    const float s1 = b1 * ia1, s2 = b2 * ia2;
+    // lane set-up
+    const float e1 = (float)c1 - P01, e2 = (float)c2 - P02;
+    const float rmid = (a1 * e2 - a2 * e1) * iDp;
+    int Ra = (int)ceilf(rmid - rr - COL_EPS), Rb = (int)floorf(rmid + rr + COL_EPS);
+    int Smin, Smax;
+    if (rows_y) {
+        Ra = max(Ra, -H - yc0);
+        Rb = min(Rb, H - 1 - yc0);
+        Smin = -xc0;
+        Smax = H - xc0;
+    } else {
+        Ra = max(Ra, -xc0);
+        Rb = min(Rb, H - xc0);
+        Smin = -H - yc0;
+        Smax = H - 1 - yc0;
+    }
+    if (c1 > H || c2 > H) Rb = Ra - 1;  // column outside the unfolded grid
+    const float k10 = (e1 - (1.0f + COL_DLT)) * ia1, k11 = (e1 + (1.0f + COL_DLT)) * ia1;
+    const float k20 = (e2 - (1.0f + COL_DLT)) * ia2, k21 = (e2 + (1.0f + COL_DLT)) * ia2;
+    int R = Ra - 1, Sc = 1, Sb = 0;
     for (;;) {
-        if (tid == 0) s_item = atomicAdd(&bb.ctl[0], 1);
-        for (int i = tid; i < NC; i += blockDim.x) {
-            acc_re[i] = 0.f;
-            acc_im[i] = 0.f;
-            acc_w[i] = 0.f;
+        while (Sc > Sb) {  // next non-empty row
+            if (++R > Rb) break;
+            const float Rf = (float)R;
+            float lo_s = (float)Smin, hi_s = (float)Smax;
+            if (use1) {
+                const float t0 = k10 - Rf * s1, t1 = k11 - Rf * s1;
+                lo_s = fmaxf(lo_s, fminf(t0, t1));
+                hi_s = fminf(hi_s, fmaxf(t0, t1));
+            }
+            if (use2) {
+                const float t0 = k20 - Rf * s2, t1 = k21 - Rf * s2;
+                lo_s = fmaxf(lo_s, fminf(t0, t1));
+                hi_s = fminf(hi_s, fmaxf(t0, t1));
+            }
+            Sc = max(Smin, (int)ceilf(lo_s - COL_EPS));
+            Sb = min(Smax, (int)floorf(hi_s + COL_EPS));
         }
-        __syncthreads();
-        const int it = s_item;
-        if (it >= total_items) break;
-        const WorkItem w = bb.items[it];
-        const int bx = w.brick % nb, by = (w.brick / nb) % nb, bz = w.brick / (nb * nb);
-        const int lo0 = -H + BS * bx, lo1 = -H + BS * by, lo2 = -H + BS * bz;
-
-        for (int si = wave; si < w.count; si += NW) {
-            const int q = __builtin_amdgcn_readfirstlane(bb.list[w.start + si]);
-            if (q < 0 || q >= nslices) continue;  // never written list entry
-            const float *m = t3d + 9 * (size_t)q;
-            float mm[6];
+        if (R > Rb) break;
+        const int x = rows_y ? xc0 + Sc : xc0 + R;
+        const int y = rows_y ? yc0 + R : yc0 + Sc;
+        Sc++;
+        if (!c.corners && x * x + y * y > H * H + H) continue;  // hypot_fast_int(x,y) > P/2
+        const float2 dt = slice_value(S, x, y, P, H);
+        float xx, yy, zz;
+        pixel_coords(x, y, mm, rtab, H, c.Pf, xx, yy, zz);
+        const int f0 = (int)floorf(xx), f1 = (int)floorf(yy), f2 = (int)floorf(zz);
+        if (f0 < -H - 1 || f1 < -H - 1 || f2 < -H - 1 || f0 > H || f1 > H || f2 > H) continue;
+        const int fa = A == 0 ? f0 : (A == 1 ? f1 : f2);
+        const int fb1 = B1 == 0 ? f0 : f1;
+        const int fb2 = B2 == 1 ? f1 : f2;
+        if ((unsigned)(c1 - fb1) > 1u || (unsigned)(c2 - fb2) > 1u) continue;
+        const int t = fa - lo[A];
+        if (t < -1 || t > BRICK - 1) continue;
 #pragma unroll
-            for (int d = 0; d < 6; d++) mm[d] = m[d];
-            const float weight = wq[q];
-            const float2 *S = slices + (size_t)(q / nsym) * slice_elems;
-            // Rows and x intervals of the pixels whose position can fall in
-            // [lo-1, lo+BS) (+/- 1e-4 covers the FP32 coordinate error).
-            const double L[3] = {lo0 - 1 - 1e-4, lo1 - 1 - 1e-4, lo2 - 1 - 1e-4};
-            const double U[3] = {lo0 + BS + 1e-4, lo1 + BS + 1e-4, lo2 + BS + 1e-4};
-            double ymin = 0, ymax = 0, xmin = 0, xmax = 0, uu[3], vv[3], iu[3];
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                uu[d] = mm[d];
-                vv[d] = mm[3 + d];
-                iu[d] = fabs(uu[d]) > 1e-9 ? 1.0 / uu[d] : 0.0;
-                ymin += vv[d] >= 0 ? L[d] * vv[d] : U[d] * vv[d];
-                ymax += vv[d] >= 0 ? U[d] * vv[d] : L[d] * vv[d];
-                xmin += uu[d] >= 0 ? L[d] * uu[d] : U[d] * uu[d];
-                xmax += uu[d] >= 0 ? U[d] * uu[d] : L[d] * uu[d];
-            }
-            const int ya = max(-H, (int)ceil(ymin - 1e-3));
-            const int yb = min(H - 1, (int)floor(ymax + 1e-3));
-            // (BS+1)*sum|v| + 1 <= 31 rows for BS <= 16; clamp to the wave
-            const int nrows = min(64, yb - ya + 1);
-            int cnt = 0, xa = 0;
-            if (lane < nrows) {
-                const int y = ya + lane;
-                double xl = fmax(0.0, xmin - 1e-3), xh = fmin((double)H, xmax + 1e-3);
-#pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    const double yv = y * vv[d];
-                    if (iu[d] != 0.0) {
-                        const double t1 = (L[d] - yv) * iu[d], t2 = (U[d] - yv) * iu[d];
-                        xl = fmax(xl, fmin(t1, t2) - 1e-3);
-                        xh = fmin(xh, fmax(t1, t2) + 1e-3);
-                    } else if (yv < L[d] || yv > U[d]) {
-                        xh = -1.0;
-                    }
-                }
-                if (!c.corners) xh = fmin(xh, floor(sqrt((double)(H * H + H - y * y))));
-                xa = (int)ceil(xl);
-                const int xb = (int)floor(xh);
-                cnt = xb >= xa ? xb - xa + 1 : 0;
-            }
-            // inclusive wave scan of cnt
-            int incl = cnt;
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const int v = __shfl_up(incl, off, 64);
-                if (lane >= off) incl += v;
-            }
-            const int total = __shfl(incl, 63, 64);
-            const int excl = incl - cnt;
-            for (int base = 0; base < total; base += 64) {
-                const int gidx = base + lane;
-                // rows overlapping the window [base, base+64) (a contiguous run)
-                const unsigned long long ov =
-                    __ballot(lane < nrows && cnt > 0 && excl < base + 64 && excl + cnt > base);
-                const int r_lo = __ffsll((long long)ov) - 1;
-                const int r_hi = 63 - __clzll((long long)ov);
-                // row of slot gidx: the last overlapping row r with excl_r <= gidx
-                int rrow = 0, rex = 0, rxa = 0;
-                for (int r = r_lo; r <= r_hi; r++) {
-                    const int er = __builtin_amdgcn_readlane(excl, r);
-                    const int xr = __builtin_amdgcn_readlane(xa, r);
-                    const bool sel = er <= gidx;
-                    rrow = sel ? r : rrow;
-                    rex = sel ? er : rex;
-                    rxa = sel ? xr : rxa;
-                }
-                if (gidx >= total) continue;
-                const int y = ya + rrow;
-                const int x = rxa + (gidx - rex);
-                const float2 dt = slice_value(S, x, y, P, H);  // issued before the coordinate math
-                bool ok = c.corners || (x * x + y * y <= H * H + H);
-                float xx, yy, zz;
-                pixel_coords(x, y, mm, rtab, H, Pf, xx, yy, zz);
-                const int x0 = (int)floorf(xx), y0 = (int)floorf(yy), z0 = (int)floorf(zz);
-                ok = ok && !(x0 < -H - 1 || y0 < -H - 1 || z0 < -H - 1 || x0 > H || y0 > H || z0 > H);
-                ok = ok && x0 >= lo0 - 1 && x0 <= lo0 + BS - 1 && y0 >= lo1 - 1 && y0 <= lo1 + BS - 1 &&
-                     z0 >= lo2 - 1 && z0 <= lo2 + BS - 1;
-                if (!ok) continue;
-#pragma unroll
-                for (int dk = 0; dk < 2; dk++) {
-                    const int k = z0 + dk;
-                    if (k < lo2 || k >= lo2 + BS || k > H) continue;
-#pragma unroll
-                    for (int dj = 0; dj < 2; dj++) {
-                        const int j = y0 + dj;
-                        if (j < lo1 || j >= lo1 + BS || j > H) continue;
-#pragma unroll
-                        for (int di = 0; di < 2; di++) {
-                            const int i = x0 + di;
-                            if (i < lo0 || i >= lo0 + BS || i > H) continue;
-                            const float gg = corner_weight(c.mode, i, j, k, xx, yy, zz, c.h, weight, etab);
-                            const int a = acc_index<BS>(i - lo0, j - lo1, k - lo2);
-                            atomicAdd(&acc_re[a], dt.x * gg);
-                            atomicAdd(&acc_im[a], dt.y * gg);
-                            atomicAdd(&acc_w[a], gg);
-                        }
-                    }
-                }
-            }
+        for (int dk = 0; dk < 2; dk++) {
+            const int ca = fa + dk;
+            const int i = A == 0 ? ca : c1;
+            const int j = A == 0 ? c1 : (A == 1 ? ca : c2);
+            const int k = A == 2 ? ca : c2;
+            const float gg = corner_weight(c.mode, i, j, k, xx, yy, zz, c.h, weight, etab);
+            float *a = acc + (t + 1 + dk) * 64 + lane;
+            a[0] += dt.x * gg;
+            a[CSLOT * 64] += dt.y * gg;
+            a[2 * CSLOT * 64] += gg;
         }
-        __syncthreads();
-        for (int a = tid; a < NC; a += blockDim.x) {
-            const float re = acc_re[a], im = acc_im[a], wv = acc_w[a];
-            if (re == 0.f && im == 0.f && wv == 0.f) continue;
-            int ci, cj, ck;
-            acc_cell<BS>(a, ci, cj, ck);
-            const int i = lo0 + ci, j = lo1 + cj, k = lo2 + ck;
-            if (i > H || j > H || k > H) continue;
-            fold_cell(data, norm, P, H, i, j, k, re, im, wv);
-        }
-        __syncthreads();
     }
 }
 
-hipError_t launch_brick_insert(const float2 *slices, int nslices, int nsym, const float *t3d,
-                               const float *wq, InsertConst c, BrickGeom g, BinBuffers b, float2 *data, float *norm,
-                               const float *exptab, const float *rtab, int grid, hipStream_t s)
+__device__ __forceinline__ void wave_sync_lds()
 {
-    const size_t lds = sizeof(float) * (c.P + 1);
-    if (g.Bs == 16)
-        hipLaunchKernelGGL(k_brick_insert<16>, dim3(grid), dim3(512), lds, s, slices, nslices, nsym, t3d, wq, c, g,
-                           b, data, norm, exptab, rtab);
-    else
-        hipLaunchKernelGGL(k_brick_insert<8>, dim3(grid), dim3(512), lds, s, slices, nslices, nsym, t3d, wq, c, g,
-                           b, data, norm, exptab, rtab);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Partial brick layout: [field][z][y][x] of the brick's local cells.
+template <int A>
+__device__ __forceinline__ void column_flush(const float *acc, float *out, int lane)
+{
+#pragma unroll
+    for (int w = 0; w < BRICK; w++) {
+        const int cc = lane + 64 * w;
+        const int lx = cc & 7, ly = (cc >> 3) & 7, lz = cc >> 6;
+        const int slot = (A == 0 ? lx : (A == 1 ? ly : lz)) + 1;
+        const int ln = A == 0 ? ly + 8 * lz : (A == 1 ? lx + 8 * lz : lx + 8 * ly);
+        const float *a = acc + slot * 64 + ln;
+        out[cc] = a[0];
+        out[512 + cc] = a[CSLOT * 64];
+        out[1024 + cc] = a[2 * CSLOT * 64];
+    }
+}
+
+template <int A>
+__device__ __forceinline__ void column_item(const float2 *__restrict__ slices, int nslices, int nsym,
+                                            const float *__restrict__ t3d, const float *__restrict__ wq,
+                                            const InsertConst &c, const int *__restrict__ list, const WorkItem &w,
+                                            const int lo[3], int lane, float *acc, const float *etab,
+                                            const float *rtab, float *out)
+{
+    const size_t slice_elems = (size_t)c.P * (c.H + 1);
+    for (int si = 0; si < w.count; si++) {
+        const int q = __builtin_amdgcn_readfirstlane(list[w.start + si]);
+        if (q < 0 || q >= nslices) continue;  // never written list entry
+        const float *m = t3d + 9 * (size_t)q;
+        float mm[6];
+#pragma unroll
+        for (int d = 0; d < 6; d++) mm[d] = m[d];
+        column_slice<A>(slices + (size_t)(q / nsym) * slice_elems, mm, wq[q], c, lo, lane, acc, etab, rtab);
+    }
+    wave_sync_lds();
+    column_flush<A>(acc, out, lane);
+    wave_sync_lds();
+}
+
+__global__ void __launch_bounds__(256) k_column_insert(const float2 *__restrict__ slices, int nslices, int nsym,
+                                                       const float *__restrict__ t3d, const float *__restrict__ wq,
+                                                       InsertConst c, BrickGeom g, BinBuffers bb,
+                                                       const float *__restrict__ exptab_g,
+                                                       const float *__restrict__ rtab_g)
+{
+    extern __shared__ float lds[];
+    const int rlen = (c.P + 2 + 3) & ~3;
+    float *etab = lds;
+    float *rtab = lds + EXPN;  // P+2 entries: fl(i/P), i in [-H, H+1]
+    const int lane = threadIdx.x & 63;
+    float *acc = lds + EXPN + rlen + (threadIdx.x >> 6) * CACC;
+    for (int i = threadIdx.x; i < EXPN; i += blockDim.x) etab[i] = exptab_g[i];
+    for (int i = threadIdx.x; i < c.P + 2; i += blockDim.x) rtab[i] = rtab_g[i];
+    __syncthreads();
+    const int H = c.H, nb = g.nb;
+    const int total = bb.ctl[1];
+    for (;;) {
+        int it = 0;
+        if (lane == 0) it = atomicAdd(&bb.ctl[0], 1);
+        it = __shfl(it, 0, 64);
+        if (it >= total) break;
+        const WorkItem w = bb.items[it];
+        const int brick = w.key / 3, A = w.key - 3 * brick;
+        const int lo[3] = {-H + BRICK * (brick % nb), -H + BRICK * ((brick / nb) % nb), -H + BRICK * (brick / (nb * nb))};
+        for (int s = 0; s < 3 * CSLOT; s++) acc[s * 64 + lane] = 0.f;
+        float *out = bb.part + (size_t)it * PART_FLOATS;
+        if (A == 0) column_item<0>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, etab, rtab, out);
+        else if (A == 1) column_item<1>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, etab, rtab, out);
+        else column_item<2>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, etab, rtab, out);
+    }
+}
+
+hipError_t launch_column_insert(const float2 *slices, int nslices, int nsym, const float *t3d, const float *wq,
+                                InsertConst c, BrickGeom g, BinBuffers b, const float *exptab, const float *rtab,
+                                int grid, int waves_per_block, hipStream_t s)
+{
+    if (waves_per_block < 1 || waves_per_block > 4) return hipErrorInvalidValue;
+    const size_t lds = sizeof(float) * (EXPN + ((c.P + 2 + 3) & ~3) + (size_t)waves_per_block * CACC);
+    hipLaunchKernelGGL(k_column_insert, dim3(grid), dim3(64 * waves_per_block), lds, s, slices, nslices, nsym, t3d,
+                       wq, c, g, b, exptab, rtab);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ merge of partial bricks
+// Sum over the work items of the brick holding unfolded cell (i, j, k).
+__device__ __forceinline__ bool partial_sum(const float *__restrict__ part, const int *__restrict__ item_off, int nb,
+                                            int H, int i, int j, int k, float &re, float &im, float &w)
+{
+    const int ui = i + H, uj = j + H, uk = k + H;
+    const int b = ((uk >> 3) * nb + (uj >> 3)) * nb + (ui >> 3);
+    const int cc = ((uk & 7) << 6) | ((uj & 7) << 3) | (ui & 7);
+    const int i0 = item_off[3 * b], i1 = item_off[3 * b + 3];
+    re = im = w = 0.f;
+    for (int it = i0; it < i1; it++) {
+        const float *p = part + (size_t)it * PART_FLOATS + cc;
+        re += p[0];
+        im += p[512];
+        w += p[1024];
+    }
+    return i1 > i0;
+}
+
+// Storage voxel (xs, ys, zs) of the half volume receives, per
+// add_complex_at_fast (libEM/emdata_core.h:368-413): the direct cells
+// (xs, j, k) with wrap(j) = ys, wrap(k) = zs; the conjugate cells (-xs, j, k)
+// with wrapneg(j) = ys, wrapneg(k) = zs; on the x = 0 and x = P/2 planes the
+// conjugate copies of the double insertion (value only) and, at x = P/2,
+// the x = -P/2 cells added twice with their weight once; at (0,0,0) and
+// (P/2,0,0) the real part and weight with the imaginary part set to 0.
+__global__ void __launch_bounds__(256) k_merge_partials(const float *__restrict__ part, const int *__restrict__ item_off,
+                                                        BrickGeom g, InsertConst c, float2 *data, float *norm)
+{
+    const int H = c.H, P = c.P, nb = g.nb;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)P * P * (H + 1)) return;
+    const int xs = (int)(idx % (H + 1));
+    const long long rest = idx / (H + 1);
+    const int ys = (int)(rest % P), zs = (int)(rest / P);
+    float r, i, w;
+    if (ys == 0 && zs == 0 && (xs == 0 || xs == H)) {
+        partial_sum(part, item_off, nb, H, xs, 0, 0, r, i, w);
+        if (xs == H) {
+            float r2, i2, w2;
+            partial_sum(part, item_off, nb, H, -H, 0, 0, r2, i2, w2);
+            r += r2;
+            i += i2;
+            w += w2;
+        }
+        if (r == 0.f && i == 0.f && w == 0.f) return;
+        float2 d = data[idx];
+        d.x += r;
+        d.y = 0.f;
+        data[idx] = d;
+        norm[idx] += w;
+        return;
+    }
+    int jd[2], njd = 0, jn[2], njn = 0, kd[2], nkd = 0, kn[2], nkn = 0;
+    if (ys <= H) jd[njd++] = ys;
+    if (ys >= H) jd[njd++] = ys - P;
+    if (ys <= H) jn[njn++] = -ys;
+    if (ys >= H && P - ys <= H && P - ys != -ys) jn[njn++] = P - ys;
+    if (zs <= H) kd[nkd++] = zs;
+    if (zs >= H) kd[nkd++] = zs - P;
+    if (zs <= H) kn[nkn++] = -zs;
+    if (zs >= H && P - zs <= H && P - zs != -zs) kn[nkn++] = P - zs;
+    float dre = 0.f, dim = 0.f, dw = 0.f;
+    for (int a = 0; a < njd; a++)
+        for (int b = 0; b < nkd; b++) {
+            partial_sum(part, item_off, nb, H, xs, jd[a], kd[b], r, i, w);
+            dre += r;
+            dim += i;
+            dw += w;
+        }
+    for (int a = 0; a < njn; a++)
+        for (int b = 0; b < nkn; b++) {
+            if (xs == 0 || xs == H) {
+                partial_sum(part, item_off, nb, H, xs, jn[a], kn[b], r, i, w);
+                dre += r;
+                dim -= i;
+                if (xs == H) {
+                    partial_sum(part, item_off, nb, H, -H, jn[a], kn[b], r, i, w);
+                    dre += 2.f * r;
+                    dim -= 2.f * i;
+                    dw += w;
+                }
+            } else {
+                partial_sum(part, item_off, nb, H, -xs, jn[a], kn[b], r, i, w);
+                dre += r;
+                dim -= i;
+                dw += w;
+            }
+        }
+    if (dre == 0.f && dim == 0.f && dw == 0.f) return;
+    float2 d = data[idx];
+    d.x += dre;
+    d.y += dim;
+    data[idx] = d;
+    norm[idx] += dw;
+}
+
+hipError_t launch_merge_partials(BrickGeom g, BinBuffers b, InsertConst c, float2 *data, float *norm, hipStream_t s)
+{
+    const long long n = (long long)c.P * c.P * (c.H + 1);
+    hipLaunchKernelGGL(k_merge_partials, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, b.part, b.item_off, g, c,
+                       data, norm);
     return hipGetLastError();
 }
 

@@ -22,6 +22,8 @@ import numpy as np
 from . import _lib
 from .transform import Transform
 
+TUNING_KEYS = ("waves_per_block", "slices_per_item", "chunk", "blocks_per_cu")
+
 
 class EMData:
     """Numpy-backed stand-in for the EMData objects the reconstructor API
@@ -106,8 +108,8 @@ class FourierHipReconstructor:
         "size": "INTARRAY", "sym": "STRING", "mode": "STRING", "corners": "BOOL",
         "sqrtnorm": "BOOL", "usessnr": "BOOL", "verbose": "BOOL", "quiet": "BOOL",
         "subvolume": "INTARRAY", "savenorm": "STRING", "normout": "EMDATA",
-        "pad": "INT", "device": "INT", "brick": "INT", "slices_per_item": "INT",
-        "chunk": "INT", "wgs_per_cu": "INT",
+        "pad": "INT", "device": "INT", "waves_per_block": "INT", "slices_per_item": "INT",
+        "chunk": "INT", "blocks_per_cu": "INT",
     }
 
     def __init__(self, params=None):
@@ -140,9 +142,9 @@ class FourierHipReconstructor:
         self.params.update(params)
         if changed:
             self._create()
-        elif any(k in params for k in ("brick", "slices_per_item", "chunk", "wgs_per_cu")):
+        elif any(k in params for k in TUNING_KEYS):
             t = self.get_tuning()
-            t.update({k: int(params[k]) for k in ("brick", "slices_per_item", "chunk", "wgs_per_cu") if k in params})
+            t.update({k: int(params[k]) for k in TUNING_KEYS if k in params})
             self.set_tuning(**t)
 
     def set_param(self, k, v):
@@ -178,9 +180,9 @@ class FourierHipReconstructor:
             best = None
         if best:
             self.set_tuning(**best)
-        if any(k in p for k in ("brick", "slices_per_item", "chunk", "wgs_per_cu")):
+        if any(k in p for k in TUNING_KEYS):
             t = self.get_tuning()
-            t.update({k: int(p[k]) for k in ("brick", "slices_per_item", "chunk", "wgs_per_cu") if k in p})
+            t.update({k: int(p[k]) for k in TUNING_KEYS if k in p})
             self.set_tuning(**t)
 
     def __del__(self):
@@ -344,8 +346,8 @@ class FourierHipReconstructor:
         return data, norm
 
     # ---------------------------------------------------------------- tuning
-    def set_tuning(self, brick=16, slices_per_item=64, chunk=2048, wgs_per_cu=3):
-        t = _lib.Tuning(int(brick), int(slices_per_item), int(chunk), int(wgs_per_cu))
+    def set_tuning(self, waves_per_block=4, slices_per_item=64, chunk=2048, blocks_per_cu=4):
+        t = _lib.Tuning(int(waves_per_block), int(slices_per_item), int(chunk), int(blocks_per_cu))
         self._check(self._lib.fh_set_tuning(self._need(), ctypes.byref(t)))
 
     def get_tuning(self):

@@ -71,18 +71,21 @@ typedef struct {
     float apix;     /* A/pixel */
 } fh_ctf;
 
-/* Tuning knobs of the batched insertion (picked by the autotuner). */
+/* Tuning knobs of the batched insertion (picked by the autotuner).  The
+ * unfolded frequency grid is cut into 8^3 bricks; a work item is one brick
+ * and up to slices_per_item slices sharing a dominant normal axis. */
 typedef struct {
-    int brick;          /* brick edge in voxels: 8 or 16 */
-    int slices_per_item;/* slices a work item accumulates before flushing */
-    int chunk;          /* particles per batched launch */
-    int wgs_per_cu;     /* persistent grid occupancy */
+    int waves_per_block; /* independent item workers per workgroup: 1, 2, 4 or 8 */
+    int slices_per_item; /* slices a work item gathers before writing its partial brick */
+    int chunk;           /* particles per batched launch */
+    int blocks_per_cu;   /* persistent grid: workgroups per compute unit */
 } fh_tuning;
 
 /* Timings and counters of the last batched call (device time, ms). */
 typedef struct {
     double prep_ms, bin_ms, insert_ms, total_ms;
-    double insert_kernel_ms;     /* the brick insertion kernel alone */
+    double insert_kernel_ms;     /* the column-gather kernel alone */
+    double merge_ms;             /* the partial-brick merge into the volume */
     long long slices, pairs, items, chunks;
 } fh_stats;
 

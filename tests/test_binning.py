@@ -50,75 +50,6 @@ def bricks_for_slice(m, H, BS, nb, corners=False):
     return out
 
 
-def enum_pixels(m, H, BS, bc, corners=False):
-    """k_brick_insert: candidate pixels of one (brick, slice) pair."""
-    lo = [-H + BS * b for b in bc]
-    uu = m[0, :3].astype(np.float64)
-    vv = m[1, :3].astype(np.float64)
-    L = [lo[d] - 1 - 1e-4 for d in range(3)]
-    U = [lo[d] + BS + 1e-4 for d in range(3)]
-    ymin = sum(L[d] * vv[d] if vv[d] >= 0 else U[d] * vv[d] for d in range(3))
-    ymax = sum(U[d] * vv[d] if vv[d] >= 0 else L[d] * vv[d] for d in range(3))
-    xmin = sum(L[d] * uu[d] if uu[d] >= 0 else U[d] * uu[d] for d in range(3))
-    xmax = sum(U[d] * uu[d] if uu[d] >= 0 else L[d] * uu[d] for d in range(3))
-    ya, yb = max(-H, math.ceil(ymin - 1e-3)), min(H - 1, math.floor(ymax + 1e-3))
-    assert yb - ya + 1 <= 64
-    pix = []
-    for y in range(ya, yb + 1):
-        xl, xh = max(0.0, xmin - 1e-3), min(float(H), xmax + 1e-3)
-        for d in range(3):
-            yv = y * vv[d]
-            if abs(uu[d]) > 1e-9:
-                iu = 1.0 / uu[d]
-                t1, t2 = (L[d] - yv) * iu, (U[d] - yv) * iu
-                xl, xh = max(xl, min(t1, t2) - 1e-3), min(xh, max(t1, t2) + 1e-3)
-            elif yv < L[d] or yv > U[d]:
-                xh = -1.0
-        if not corners:
-            xh = min(xh, math.floor(math.sqrt(H * H + H - y * y)))
-        pix.extend((x, y) for x in range(math.ceil(xl), math.floor(xh) + 1))
-    return pix
-
-
-def coverage(P, BS, m, corners):
-    H = P // 2
-    nb = (P + 1 + BS - 1) // BS
-    covered = {bc: set(enum_pixels(m, H, BS, bc, corners)) for bc in bricks_for_slice(m, H, BS, nb, corners)}
-    f = np.float32
-    missing = total = 0
-    for y in range(-H, H):
-        for x in range(0, H + 1):
-            if not corners and x * x + y * y > H * H + H:
-                continue
-            rx, ry = f(x) / f(P), f(y) / f(P)
-            c = [f(f(f(rx * m[0, d]) + f(ry * m[1, d])) * f(P)) for d in range(3)]
-            x0, y0, z0 = (int(math.floor(q)) for q in c)
-            if min(x0, y0, z0) < -H - 1 or max(x0, y0, z0) > H:
-                continue
-            for k in (z0, z0 + 1):
-                for j in (y0, y0 + 1):
-                    for i in (x0, x0 + 1):
-                        if abs(i) > H or abs(j) > H or abs(k) > H:
-                            continue
-                        total += 1
-                        bc = ((i + H) // BS, (j + H) // BS, (k + H) // BS)
-                        if (x, y) not in covered.get(bc, ()):
-                            missing += 1
-    return missing, total
-
-
-@pytest.mark.parametrize("P,BS", [(16, 8), (16, 16), (32, 16), (48, 8)])
-def test_every_contribution_is_covered(P, BS):
-    rng = np.random.default_rng(P * 100 + BS)
-    angles = [(0.0, 0.0, 0.0), (90.0, 90.0, 0.0), (0.0, 90.0, 90.0), (45.0, 45.0, 45.0)]
-    angles += [tuple(rng.random(3) * (360, 180, 360)) for _ in range(4)]
-    for ang in angles:
-        m = euler_eman_matrix(*ang)
-        for corners in (False, True):
-            miss, tot = coverage(P, BS, m, corners)
-            assert tot > 0 and miss == 0, (ang, corners, miss, tot)
-
-
 def test_bricks_per_column_bound():
     """At most 5 bricks per brick column along the dominant normal axis, the
     bound that sizes the brick list (engine.cpp ensure_bins)."""

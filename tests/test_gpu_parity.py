@@ -88,11 +88,11 @@ def test_golden_insert_slice(sym, mode):
 
 @pytest.mark.parametrize("sym", ["c1", "d2"])
 @pytest.mark.parametrize("mode", ["gauss_2", "trilinear"])
-@pytest.mark.parametrize("brick", [8, 16])
-def test_golden_batch(sym, mode, brick):
+@pytest.mark.parametrize("wpb", [1, 4])
+def test_golden_batch(sym, mode, wpb):
     d = np.load(os.path.join(GOLD, "recon_p16.npz"))
     P = 16
-    r = recon(P, sym, mode, brick=brick, slices_per_item=2)
+    r = recon(P, sym, mode, waves_per_block=wpb, slices_per_item=2)
     r.insert_batch(d["imgs"], list(d["mats"]), d["weights"])
     gd, gn = r.get_state()
     key = "%s_%s" % (sym, mode)
@@ -108,12 +108,12 @@ def config1_inputs(n, ctf=False, B=64, seed=1):
     return imgs, mats, dfs
 
 
-@pytest.mark.parametrize("brick,spi", [(16, 64), (8, 16), (16, 5)])
-def test_batch_config1(oracle, brick, spi):
+@pytest.mark.parametrize("wpb,spi", [(4, 64), (1, 16), (2, 5)])
+def test_batch_config1(oracle, wpb, spi):
     P = 96
     imgs, mats, _ = config1_inputs(300)
     w = np.ones(len(imgs), np.float32)
-    r = recon(P, brick=brick, slices_per_item=spi, chunk=128)
+    r = recon(P, waves_per_block=wpb, slices_per_item=spi, chunk=128)
     r.insert_batch(imgs, list(mats), w)
     gd, gn = r.get_state()
     slices = [oracle.prep_particle(im, P) for im in imgs]
