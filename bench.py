@@ -57,6 +57,7 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-max-sample", type=int, default=4096, help="particles copied to the host for the CPU leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB elementwise copy (PMC byte-count calibration)")
     ap.add_argument("--serve-jobs", type=int, default=40,
@@ -120,18 +121,24 @@ def cpu_baseline(imgs_host, mats, dfs, P, sym, mode, apix, budget_s):
         L.or_prep_particle(O._f(raw), B, P, ctypes.byref(c) if c is not None else None, O._f(out))
         return out
 
-    # size the sample from a one-particle probe
-    t0 = time.perf_counter()
-    s0 = prep(0)
-    _, _, ti = O.insert_batch_mt(P, s0[None], mats[:1], np.ones(1, np.float32), sym, mode, nthreads=1)
-    one = time.perf_counter() - t0
-    n = int(max(T, min(len(imgs_host), budget_s * T / max(one, 1e-4))))
-    n = (n // T) * T or T
-    t0 = time.perf_counter()
-    with ThreadPoolExecutor(T) as ex:
-        slices = np.stack(list(ex.map(prep, range(n))))
-    t_prep = time.perf_counter() - t0
-    _, _, t_ins = O.insert_batch_mt(P, slices, mats[:n], np.ones(n, np.float32), sym, mode, nthreads=T)
+    def run(k):
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(T) as ex:
+            sl = np.stack(list(ex.map(prep, range(k))))
+        tp = time.perf_counter() - t0
+        _, _, ti = O.insert_batch_mt(P, sl, mats[:k], np.ones(k, np.float32), sym, mode, nthreads=T)
+        return tp, ti
+
+    # size the sample: two probes separate the per-particle cost from the
+    # fixed cost of the private volumes (allocation and the final sum)
+    k1 = min(len(imgs_host), 2 * T)
+    k2 = min(len(imgs_host), 4 * T)
+    p1, i1 = run(k1)
+    p2, i2 = run(k2)
+    per = max((p2 + i2 - p1 - i1) / max(1, k2 - k1), (p2 + i2) / k2 * 0.05)
+    n = int(min(len(imgs_host), max(k2, budget_s / per)))
+    n = max(T, (n // T) * T)
+    t_prep, t_ins = run(n)
     return {
         "value": n / (t_prep + t_ins), "unit": "particles/s", "cores": T, "kind": "port",
         "sample": "%d particles of the same workload (prep %.2fs + insertion %.2fs, %d threads, "
@@ -240,7 +247,7 @@ def main():
     # plain one-particle-per-launch form on a host-side sample
     host = None
     if not a.no_plain or not a.no_cpu:
-        m = max(a.plain_sample, 256)
+        m = min(n, max(a.plain_sample, 0 if a.no_cpu else a.cpu_max_sample))
         host = imgs[:m].cpu().numpy()
     if not a.no_plain:
         print("[bench] plain form on %d particles" % a.plain_sample, file=sys.stderr, flush=True)
