@@ -9,7 +9,7 @@ Workload (one step = one reconstruction pass over the whole batch):
   384^3 (complex, weight) volume -- fh_insert_batch with the particles already
   resident in HBM.  Each step re-runs setup() and inserts all particles.
 
-Prints one JSON line with the throughput, the column-gather insertion
+Prints one JSON line with the throughput, the brick scatter insertion
 kernel's roofline fraction (HBM-bound scatter model), the plain one-particle-per-launch
 GPU form, and the CPU oracle timed on this host's cores on a bounded sample.
 """
@@ -202,7 +202,7 @@ def main():
     elapsed = time.perf_counter() - t0
     value = a.steps * n / elapsed
 
-    # roofline of the dominant kernel (k_column_insert): scatter model, one
+    # roofline of the dominant kernel (k_scatter_insert): scatter model, one
     # 12-byte (complex + weight) update per gridding contribution, 8 per
     # pixel per symmetry copy
     bytes_per_particle = nsym * npix(P) * 8 * 12
@@ -219,7 +219,7 @@ def main():
             traffic = t.get("hbm_bytes_per_particle", 0) * (a.steps * n) / max(1, launches)
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": "k_column_insert", "avg_launch_ms": round(avg_ms, 3),
+            "kernel": "k_scatter_insert", "avg_launch_ms": round(avg_ms, 3),
             "algorithmic_bytes_per_launch": int(per_launch_bytes)}
     # whole-pipeline effective bandwidth with the 4*B^2 raw read (BASELINE.md formula)
     pipe_bytes = (4 * B * B + bytes_per_particle) * a.steps * n

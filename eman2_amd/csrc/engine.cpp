@@ -212,7 +212,7 @@ struct fh_reconstructor {
     BrickGeom geom{};
     BinBuffers bins{};
     long long bins_cap_slices = 0;
-    fh_tuning tune{4, 64, 2048, 4};
+    fh_tuning tune{4, 64, 8192, 4};
     fh_stats stats{};
     hipEvent_t ev[6] = {};
 };
@@ -245,7 +245,7 @@ static void free_chunk(fh_reconstructor *h)
 static void free_bins(fh_reconstructor *h)
 {
     BinBuffers &b = h->bins;
-    for (int **p : {&b.counts, &b.offsets, &b.cursors, &b.item_off, &b.list, &b.ctl})
+    for (int **p : {&b.counts, &b.offsets, &b.cursors, &b.item_off, &b.tile, &b.list, &b.ctl})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
     if (b.items) { (void)hipFree(b.items); b.items = nullptr; }
     if (b.part) { (void)hipFree(b.part); b.part = nullptr; }
@@ -302,6 +302,7 @@ static int ensure_bins(fh_reconstructor *h, long long nslices)
     FH_TRY(h, hipMalloc(&b.offsets, sizeof(int) * (g.nkeys + 1)));
     FH_TRY(h, hipMalloc(&b.cursors, sizeof(int) * g.nkeys));
     FH_TRY(h, hipMalloc(&b.item_off, sizeof(int) * (g.nkeys + 1)));
+    FH_TRY(h, hipMalloc(&b.tile, sizeof(int) * 2 * scan_tiles(g)));
     FH_TRY(h, hipMalloc(&b.list, sizeof(int) * b.list_cap));
     FH_TRY(h, hipMalloc(&b.items, sizeof(WorkItem) * b.item_cap));
     FH_TRY(h, hipMalloc(&b.ctl, sizeof(int) * 4));
@@ -633,7 +634,11 @@ int fh_insert_batch(fh_reconstructor *h, const float *imgs, int n, int box, cons
     std::vector<float> rot9;
     rc = check_xforms(h, xforms, n, 1, rot9);
     if (rc) return rc;
-    const int chunk = std::max(1, std::min(h->tune.chunk, std::max(n, 1)));
+    // the brick list (at most 5 nb^2 entries per slice) is indexed with int
+    const BrickGeom bg = brick_geom(h->P);
+    const long long max_slices = std::max(1LL, (long long)INT32_MAX / (5LL * bg.nb * bg.nb) - 1);
+    const int chunk = (int)std::max(1LL, std::min({(long long)h->tune.chunk, (long long)std::max(n, 1),
+                                                   max_slices / h->nsym}));
     rc = ensure_chunk(h, chunk, box);
     if (rc) return rc;
     rc = ensure_bins(h, (long long)chunk * h->nsym);
@@ -671,9 +676,9 @@ int fh_insert_batch(fh_reconstructor *h, const float *imgs, int n, int box, cons
         if (rc) return rc;
         FH_TRY(h, hipEventRecord(h->ev[3], h->stream));
         if (ctl[1] > 0) {
-            FH_TRY(h, launch_column_insert(h->slices, m * h->nsym, h->nsym, h->t3d, h->wq, c, h->geom, h->bins,
-                                           h->exptab, h->rtab, std::min(grid, (ctl[1] + wpb - 1) / wpb), wpb,
-                                           h->stream));
+            FH_TRY(h, launch_scatter_insert(h->slices, m * h->nsym, h->nsym, h->t3d, h->wq, c, h->geom, h->bins,
+                                            h->exptab, h->rtab, std::min(grid, (ctl[1] + wpb - 1) / wpb), wpb,
+                                            h->stream));
         }
         FH_TRY(h, hipEventRecord(h->ev[4], h->stream));
         if (ctl[1] > 0) FH_TRY(h, launch_merge_partials(h->geom, h->bins, c, h->data, h->norm, h->stream));

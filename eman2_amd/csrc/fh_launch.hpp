@@ -49,6 +49,7 @@ struct BinBuffers {
     int *offsets;     // nkeys + 1
     int *cursors;     // nkeys
     int *item_off;    // nkeys + 1: first work item of each key
+    int *tile;        // 2 per scan tile of 1024 keys: count and item sums / offsets
     int *list;        // capacity
     WorkItem *items;  // item capacity
     int *ctl;         // [0] work counter, [1] total items, [2] total pairs, [3] overflow flag
@@ -93,10 +94,12 @@ hipError_t launch_insert_plain(const float2 *slice, InsertConst c, const float *
                                hipStream_t s);
 hipError_t launch_bin(const float *t3d, const float *wq, int nslices, InsertConst c, BrickGeom g,
                       BinBuffers b, int slices_per_item, hipStream_t s);
-hipError_t launch_column_insert(const float2 *slices, int nslices, int nsym, const float *t3d,
-                                const float *wq, InsertConst c, BrickGeom g, BinBuffers b,
-                                const float *exptab, const float *rtab, int grid, int waves_per_block,
-                                hipStream_t s);
+int scan_tiles(const BrickGeom &g);
+size_t scatter_lds_bytes(int P, int waves_per_block);
+hipError_t launch_scatter_insert(const float2 *slices, int nslices, int nsym, const float *t3d,
+                                 const float *wq, InsertConst c, BrickGeom g, BinBuffers b,
+                                 const float *exptab, const float *rtab, int grid, int waves_per_block,
+                                 hipStream_t s);
 hipError_t launch_merge_partials(BrickGeom g, BinBuffers b, InsertConst c, float2 *data, float *norm,
                                  hipStream_t s);
 int agree_blocks(int P);

@@ -14,11 +14,11 @@
 //  k_bin_*           bins (slice, brick) pairs over the unfolded frequency
 //                    grid [-P/2, P/2]^3 cut into 8^3 bricks, keyed by brick
 //                    and the slice's dominant normal axis.
-//  k_column_insert   one wave per work item: each lane owns one of the
-//                    brick's 8x8 voxel columns along the item's axis and
-//                    gathers the pixels whose stencil reaches that column;
-//                    accumulation is lane-private (no atomics).  Each item
-//                    writes its partial brick to HBM.
+//  k_scatter_insert  one wave per work item holds the brick's accumulator in
+//                    LDS and adds the stencils of the slice pixels reaching
+//                    the brick with plain read-modify-writes (lanes sharing
+//                    a stencil origin take turns, elected by an LDS tag).
+//                    Each item writes its partial brick to HBM.
 //  k_merge_partials  one thread per half-volume voxel sums the partial
 //                    bricks of its unfolded source cells under the Hermitian
 //                    fold of add_complex_at_fast and adds them with plain
@@ -562,50 +562,131 @@ __global__ void __launch_bounds__(256) k_bin(const float *__restrict__ t3d, cons
     }
 }
 
-// Single-block scan: offsets / item offsets, totals, counter reset.
-__global__ void __launch_bounds__(1024) k_bin_scan(BrickGeom g, BinBuffers bb, int K)
+// Exclusive scans of the key counts (-> list offsets) and of the work items
+// per key (ceil(count / K) -> item offsets) in three passes: per-tile sums,
+// a scan of the tile sums with the totals, and the per-key offsets.
+constexpr int SCAN_TILE = 1024;  // keys per tile: 256 threads x 4
+
+__device__ __forceinline__ int4 load_counts4(const int *counts, int k, int n)
 {
-    const int nbk = g.nkeys;
+    if (k + 3 < n) return *reinterpret_cast<const int4 *>(counts + k);
+    int4 v;
+    v.x = k < n ? counts[k] : 0;
+    v.y = k + 1 < n ? counts[k + 1] : 0;
+    v.z = k + 2 < n ? counts[k + 2] : 0;
+    v.w = 0;
+    return v;
+}
+
+// 256-thread exclusive scan of (a, b); returns the block totals
+__device__ __forceinline__ void block_scan2(int &a, int &b, int &ta, int &tb)
+{
+    __shared__ int sa[4], sb[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int ia = a, ib = b;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int va = __shfl_up(ia, off, 64), vb = __shfl_up(ib, off, 64);
+        if (lane >= off) {
+            ia += va;
+            ib += vb;
+        }
+    }
+    if (lane == 63) {
+        sa[wave] = ia;
+        sb[wave] = ib;
+    }
+    __syncthreads();
+    int pa = 0, pb = 0;
+    for (int w = 0; w < wave; w++) {
+        pa += sa[w];
+        pb += sb[w];
+    }
+    ta = sa[0] + sa[1] + sa[2] + sa[3];
+    tb = sb[0] + sb[1] + sb[2] + sb[3];
+    a = pa + ia - a;
+    b = pb + ib - b;
+}
+
+__global__ void __launch_bounds__(256) k_scan_tiles(BrickGeom g, BinBuffers bb, int K)
+{
+    const int k = blockIdx.x * SCAN_TILE + threadIdx.x * 4;
+    const int4 c = load_counts4(bb.counts, k, g.nkeys);
+    int a = c.x + c.y + c.z + c.w;
+    int b = (c.x + K - 1) / K + (c.y + K - 1) / K + (c.z + K - 1) / K + (c.w + K - 1) / K;
+    int ta, tb;
+    block_scan2(a, b, ta, tb);
+    if (threadIdx.x == 0) {
+        bb.tile[2 * blockIdx.x] = ta;
+        bb.tile[2 * blockIdx.x + 1] = tb;
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_scan_top(BrickGeom g, BinBuffers bb, int ntiles)
+{
     const int tid = threadIdx.x;
-    const int per = (nbk + 1023) / 1024;
-    const int b0 = tid * per, b1 = min(nbk, b0 + per);
+    const int per = (ntiles + 1023) / 1024;
+    const int t0 = tid * per, t1 = min(ntiles, t0 + per);
     long long sc = 0, si = 0;
-    for (int b = b0; b < b1; b++) {
-        const int cnt = bb.counts[b];
-        sc += cnt;
-        si += (cnt + K - 1) / K;
+    for (int t = t0; t < t1; t++) {
+        sc += bb.tile[2 * t];
+        si += bb.tile[2 * t + 1];
     }
     __shared__ long long s_c[1024], s_i[1024];
     s_c[tid] = sc;
     s_i[tid] = si;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
-        long long vc = tid >= off ? s_c[tid - off] : 0;
-        long long vi = tid >= off ? s_i[tid - off] : 0;
+        const long long vc = tid >= off ? s_c[tid - off] : 0;
+        const long long vi = tid >= off ? s_i[tid - off] : 0;
         __syncthreads();
         s_c[tid] += vc;
         s_i[tid] += vi;
         __syncthreads();
     }
     long long ec = s_c[tid] - sc, ei = s_i[tid] - si;
-    for (int b = b0; b < b1; b++) {
-        const int cnt = bb.counts[b];
-        bb.offsets[b] = (int)ec;
-        bb.item_off[b] = (int)ei;
-        bb.cursors[b] = 0;
-        ec += cnt;
-        ei += (cnt + K - 1) / K;
+    for (int t = t0; t < t1; t++) {
+        const int a = bb.tile[2 * t], b = bb.tile[2 * t + 1];
+        bb.tile[2 * t] = (int)ec;
+        bb.tile[2 * t + 1] = (int)ei;
+        ec += a;
+        ei += b;
     }
     if (tid == 1023) {
-        bb.offsets[nbk] = (int)s_c[1023];
-        bb.item_off[nbk] = (int)s_i[1023];
-        const long long items = s_i[1023] < bb.item_cap ? s_i[1023] : bb.item_cap;
+        const long long tc = s_c[1023], ti = s_i[1023];
+        bb.offsets[g.nkeys] = (int)tc;
+        bb.item_off[g.nkeys] = (int)ti;
         bb.ctl[0] = 0;
-        bb.ctl[1] = (int)items;
-        bb.ctl[2] = (int)(s_c[1023] < bb.list_cap ? s_c[1023] : bb.list_cap);
-        if (s_c[1023] > bb.list_cap || s_i[1023] > bb.item_cap) bb.ctl[3] = 1;
+        bb.ctl[1] = (int)(ti < bb.item_cap ? ti : bb.item_cap);
+        bb.ctl[2] = (int)(tc < bb.list_cap ? tc : bb.list_cap);
+        if (tc > bb.list_cap || ti > bb.item_cap) bb.ctl[3] = 1;
     }
 }
+
+__global__ void __launch_bounds__(256) k_scan_apply(BrickGeom g, BinBuffers bb, int K)
+{
+    const int k = blockIdx.x * SCAN_TILE + threadIdx.x * 4;
+    const int4 c = load_counts4(bb.counts, k, g.nkeys);
+    const int cv[4] = {c.x, c.y, c.z, c.w};
+    int a = c.x + c.y + c.z + c.w;
+    int b = (c.x + K - 1) / K + (c.y + K - 1) / K + (c.z + K - 1) / K + (c.w + K - 1) / K;
+    int ta, tb;
+    block_scan2(a, b, ta, tb);
+    a += bb.tile[2 * blockIdx.x];
+    b += bb.tile[2 * blockIdx.x + 1];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        if (k + t < g.nkeys) {
+            bb.offsets[k + t] = a;
+            bb.item_off[k + t] = b;
+            bb.cursors[k + t] = 0;
+        }
+        a += cv[t];
+        b += (cv[t] + K - 1) / K;
+    }
+}
+
+int scan_tiles(const BrickGeom &g) { return (g.nkeys + SCAN_TILE - 1) / SCAN_TILE; }
 
 __global__ void k_bin_items(BrickGeom g, BinBuffers bb, int K)
 {
@@ -637,7 +718,10 @@ hipError_t launch_bin(const float *t3d, const float *wq, int nslices, InsertCons
         hipLaunchKernelGGL(k_bin<0>, grid, dim3(256), 0, s, t3d, wq, nslices, c, g, b);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, g, b, K);
+    const int nt = scan_tiles(g);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(256), 0, s, g, b, K);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, g, b, nt);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(256), 0, s, g, b, K);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (nslices > 0) {
         dim3 grid((g.nb * g.nb + 255) / 256, nslices);
@@ -648,124 +732,33 @@ hipError_t launch_bin(const float *t3d, const float *wq, int nslices, InsertCons
     return hipGetLastError();
 }
 
-// ------------------------------------------------------------ column-gather insertion
-// A work item is (brick, axis A, up to K slices whose dominant normal axis is
-// A).  Lane l of the item's wave owns the brick's voxel column
-// (c1, c2) = (lo[B1] + l%8, lo[B2] + l/8) along A (B1 < B2 the other two
-// axes).  A pixel at p adds to cells floor(p) + {0,1}^3, so it reaches the
-// column iff p[B] lies in [c_B - 1, c_B + 1) on both B axes: per slice the
-// lane visits that parallelogram of slice pixels and adds each pixel's two
-// cells (floor(p[A]) and +1) of its column.  Every (pixel, cell)
-// contribution of the reference loop lands in exactly one (brick, column)
-// -- the accumulation needs no atomics.
-//
-// Per-wave LDS accumulator: [field re/im/w][slot][lane], slot s holding the
-// column cell lo[A] - 1 + s (s = 0 and CSLOT-1 are pads for stencil cells
-// just outside the brick, never merged).  Lanes of one access hit distinct
-// lane columns, so the read-modify-writes are bank-conflict free.
-constexpr int CSLOT = BRICK + 2;
-constexpr int CACC = 3 * CSLOT * 64;  // floats of one wave's accumulator
-#define COL_DLT 2e-3f                 // margin on p[B]: FP32 position error is < 3e-4 for P <= 1024
-#define COL_EPS 1e-3f                 // margin on enumeration bounds
+// ------------------------------------------------------------ brick scatter insertion
+// A work item is (8^3 brick, axis A, up to K slices whose dominant normal
+// axis is A).  Its wave owns a private accumulator of the brick in LDS and,
+// per slice, walks the slice pixels whose 2x2x2 stencil reaches the brick in
+// windows of 64 (one pixel per lane), adding each pixel's in-brick corners
+// with plain LDS read-modify-writes:
+//  - two lanes whose pixels share a stencil origin cell would race on all
+//    eight corners; an LDS tag per origin cell elects one lane per cell and
+//    round, the others retry in the next round (no atomics);
+//  - two lanes with different origins can still meet on a cell through
+//    different corners, so the corners are processed one after another with
+//    a wave-level ordering point between them.
+// The accumulator is three float arrays (re, im, w) over the brick's cells
+// in [A][B2][B1] order (B1 < B2 the other axes): one corner's cells of a
+// slice with dominant axis A fall in distinct (B1, B2) columns, so the
+// lanes of an access spread over the 32 banks.
+constexpr int ACC_CELLS = BRICK * BRICK * BRICK;
+constexpr int TAG_N = (BRICK + 1) * (BRICK + 1) * (BRICK + 1);  // stencil origins in [lo-1, lo+7]^3
+constexpr int WAVE_LDS_BYTES = 12 * ACC_CELLS + ((TAG_N + 15) & ~15) + 4 * 64;
+#define SC_DLT 2e-3f   // margin on p: the FP32 position error is < 3e-4 for P <= 1024
+#define SC_EPS 1e-3f   // margin on enumeration bounds
+#define SC_RAD 7.81f   // 4.5 sqrt(3) + margin: |p - brick centre| for an origin in [lo-1, lo+7]^3
 
-// Sweep of one slice over the wave's 64 columns.  The parallelogram is
-// enumerated in rows of the slice coordinate (x or y) with fewer rows,
-// relative to an integer pixel (xc0, yc0) near the brick's preimage so that
-// the bounds are small numbers; the bounds carry margins and every pixel is
-// then checked with the reference's own FP32 coordinates.
 template <int A>
-__device__ __forceinline__ void column_slice(const float2 *__restrict__ S, const float mm[6], float weight,
-                                             const InsertConst &c, const int lo[3], int lane, float *acc,
-                                             const float *etab, const float *rtab)
+__device__ __forceinline__ int acc_slot(int i, int j, int k)
 {
-    constexpr int B1 = A == 0 ? 1 : 0, B2 = A == 2 ? 1 : 2;
-    const int H = c.H, P = c.P;
-    const int c1 = lo[B1] + (lane & 7), c2 = lo[B2] + (lane >> 3);
-    // slice set-up (wave-uniform)
-    const float u1 = mm[B1], u2 = mm[B2], v1 = mm[3 + B1], v2 = mm[3 + B2];
-    const float D = u1 * v2 - u2 * v1;  // = +-n[A], |n[A]| >= 1/sqrt(3)
-    if (!(fabsf(D) >= 1e-3f)) return;   // not a rotation (or NaN): keeps every bound finite
-    const float iD = 1.0f / D;
-    const float C1 = (float)lo[B1] + 3.5f, C2 = (float)lo[B2] + 3.5f;
-    const int xc0 = (int)rintf((v2 * C1 - v1 * C2) * iD), yc0 = (int)rintf((u1 * C2 - u2 * C1) * iD);
-    const float P01 = (float)xc0 * u1 + (float)yc0 * v1, P02 = (float)xc0 * u2 + (float)yc0 * v2;
-    const bool rows_y = fabsf(u1) + fabsf(u2) <= fabsf(v1) + fabsf(v2);
-    // p[B] = S * a_B + R * b_B (+ P0_B): R the row coordinate, S the in-row one
-    const float a1 = rows_y ? u1 : v1, a2 = rows_y ? u2 : v2, b1 = rows_y ? v1 : u1, b2 = rows_y ? v2 : u2;
-    const float iDp = rows_y ? iD : -iD;  // 1 / (a1 b2 - a2 b1)
-    const float rr = (fabsf(a1) + fabsf(a2)) * (1.0f + COL_DLT) * fabsf(iDp);
-    // in-row bounds from each B axis whose coefficient is not small; one of
-    // |a1|, |a2| is >= 0.408 because |det| >= 0.577 and |(b1, b2)| <= 1
-    const bool use1 = fabsf(a1) >= 0.25f, use2 = fabsf(a2) >= 0.25f;
-    const float ia1 = use1 ? 1.0f / a1 : 0.f, ia2 = use2 ? 1.0f / a2 : 0.f;
-    const float s1 = b1 * ia1, s2 = b2 * ia2;
-    // lane set-up
-    const float e1 = (float)c1 - P01, e2 = (float)c2 - P02;
-    const float rmid = (a1 * e2 - a2 * e1) * iDp;
-    int Ra = (int)ceilf(rmid - rr - COL_EPS), Rb = (int)floorf(rmid + rr + COL_EPS);
-    int Smin, Smax;
-    if (rows_y) {
-        Ra = max(Ra, -H - yc0);
-        Rb = min(Rb, H - 1 - yc0);
-        Smin = -xc0;
-        Smax = H - xc0;
-    } else {
-        Ra = max(Ra, -xc0);
-        Rb = min(Rb, H - xc0);
-        Smin = -H - yc0;
-        Smax = H - 1 - yc0;
-    }
-    if (c1 > H || c2 > H) Rb = Ra - 1;  // column outside the unfolded grid
-    const float k10 = (e1 - (1.0f + COL_DLT)) * ia1, k11 = (e1 + (1.0f + COL_DLT)) * ia1;
-    const float k20 = (e2 - (1.0f + COL_DLT)) * ia2, k21 = (e2 + (1.0f + COL_DLT)) * ia2;
-    int R = Ra - 1, Sc = 1, Sb = 0;
-    for (;;) {
-        while (Sc > Sb) {  // next non-empty row
-            if (++R > Rb) break;
-            const float Rf = (float)R;
-            float lo_s = (float)Smin, hi_s = (float)Smax;
-            if (use1) {
-                const float t0 = k10 - Rf * s1, t1 = k11 - Rf * s1;
-                lo_s = fmaxf(lo_s, fminf(t0, t1));
-                hi_s = fminf(hi_s, fmaxf(t0, t1));
-            }
-            if (use2) {
-                const float t0 = k20 - Rf * s2, t1 = k21 - Rf * s2;
-                lo_s = fmaxf(lo_s, fminf(t0, t1));
-                hi_s = fminf(hi_s, fmaxf(t0, t1));
-            }
-            Sc = max(Smin, (int)ceilf(lo_s - COL_EPS));
-            Sb = min(Smax, (int)floorf(hi_s + COL_EPS));
-        }
-        if (R > Rb) break;
-        const int x = rows_y ? xc0 + Sc : xc0 + R;
-        const int y = rows_y ? yc0 + R : yc0 + Sc;
-        Sc++;
-        if (!c.corners && x * x + y * y > H * H + H) continue;  // hypot_fast_int(x,y) > P/2
-        const float2 dt = slice_value(S, x, y, P, H);
-        float xx, yy, zz;
-        pixel_coords(x, y, mm, rtab, H, c.Pf, xx, yy, zz);
-        const int f0 = (int)floorf(xx), f1 = (int)floorf(yy), f2 = (int)floorf(zz);
-        if (f0 < -H - 1 || f1 < -H - 1 || f2 < -H - 1 || f0 > H || f1 > H || f2 > H) continue;
-        const int fa = A == 0 ? f0 : (A == 1 ? f1 : f2);
-        const int fb1 = B1 == 0 ? f0 : f1;
-        const int fb2 = B2 == 1 ? f1 : f2;
-        if ((unsigned)(c1 - fb1) > 1u || (unsigned)(c2 - fb2) > 1u) continue;
-        const int t = fa - lo[A];
-        if (t < -1 || t > BRICK - 1) continue;
-#pragma unroll
-        for (int dk = 0; dk < 2; dk++) {
-            const int ca = fa + dk;
-            const int i = A == 0 ? ca : c1;
-            const int j = A == 0 ? c1 : (A == 1 ? ca : c2);
-            const int k = A == 2 ? ca : c2;
-            const float gg = corner_weight(c.mode, i, j, k, xx, yy, zz, c.h, weight, etab);
-            float *a = acc + (t + 1 + dk) * 64 + lane;
-            a[0] += dt.x * gg;
-            a[CSLOT * 64] += dt.y * gg;
-            a[2 * CSLOT * 64] += gg;
-        }
-    }
+    return A == 0 ? i * 64 + k * 8 + j : (A == 1 ? j * 64 + k * 8 + i : k * 64 + j * 8 + i);
 }
 
 __device__ __forceinline__ void wave_sync_lds()
@@ -775,29 +768,163 @@ __device__ __forceinline__ void wave_sync_lds()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Partial brick layout: [field][z][y][x] of the brick's local cells.
-template <int A>
-__device__ __forceinline__ void column_flush(const float *acc, float *out, int lane)
+// floor(sqrt(n)) for 0 <= n < 2^24
+__device__ __forceinline__ int isqrt24(int n)
 {
+    int r = (int)__builtin_amdgcn_sqrtf((float)n);
+    r -= (r * r > n);
+    r += ((r + 1) * (r + 1) <= n);
+    return r;
+}
+
+// gauss_2 / trilinear weights of the 8 corners (i0+di, j0+dj, k0+dk),
+// corner index di + 2 dj + 4 dk; each equals corner_weight() bit for bit.
+__device__ __forceinline__ void corner_weights8(int mode, int i0, int j0, int k0, float xx, float yy, float zz,
+                                                float h, float weight, const float *etab, float wg[8])
+{
+    const float dx[2] = {(float)i0 - xx, (float)(i0 + 1) - xx};
+    const float dy[2] = {(float)j0 - yy, (float)(j0 + 1) - yy};
+    const float dz[2] = {(float)k0 - zz, (float)(k0 + 1) - zz};
+    if (mode == 0) {
 #pragma unroll
-    for (int w = 0; w < BRICK; w++) {
-        const int cc = lane + 64 * w;
-        const int lx = cc & 7, ly = (cc >> 3) & 7, lz = cc >> 6;
-        const int slot = (A == 0 ? lx : (A == 1 ? ly : lz)) + 1;
-        const int ln = A == 0 ? ly + 8 * lz : (A == 1 ? lx + 8 * lz : lx + 8 * ly);
-        const float *a = acc + slot * 64 + ln;
-        out[cc] = a[0];
-        out[512 + cc] = a[CSLOT * 64];
-        out[1024 + cc] = a[2 * CSLOT * 64];
+        for (int b = 0; b < 4; b++) {
+            const float sxy = dx[b & 1] * dx[b & 1] + dy[b >> 1] * dy[b >> 1];
+#pragma unroll
+            for (int dk = 0; dk < 2; dk++) {
+                const float r = sxy + dz[dk] * dz[dk];       // Util::hypot3sq
+                const float f = r * h;
+                const int g = (int)((double)f * 50.0 + 0.5);  // Util::fast_exp index
+                wg[b + 4 * dk] = etab[g] * weight;
+            }
+        }
+    } else {
+        double ax[2], ay[2], az[2];
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            ax[t] = 1.0 - (double)fabsf(dx[t]);
+            ay[t] = 1.0 - (double)fabsf(dy[t]);
+            az[t] = 1.0 - (double)fabsf(dz[t]);
+        }
+#pragma unroll
+        for (int cI = 0; cI < 8; cI++)
+            wg[cI] = (float)(ax[cI & 1] * ay[(cI >> 1) & 1] * az[cI >> 2] * (double)weight);
     }
 }
 
 template <int A>
-__device__ __forceinline__ void column_item(const float2 *__restrict__ slices, int nslices, int nsym,
-                                            const float *__restrict__ t3d, const float *__restrict__ wq,
-                                            const InsertConst &c, const int *__restrict__ list, const WorkItem &w,
-                                            const int lo[3], int lane, float *acc, const float *etab,
-                                            const float *rtab, float *out)
+__device__ __forceinline__ void scatter_slice(const float2 *__restrict__ S, const float mm[6], float weight,
+                                              const InsertConst &c, const int lo[3], int lane, float *acc,
+                                              unsigned char *tag, int *mark, const float *etab, const float *rtab)
+{
+    const int H = c.H, P = c.P;
+    float *acc_re = acc, *acc_im = acc + ACC_CELLS, *acc_w = acc + 2 * ACC_CELLS;
+    // the pixel nearest the brick centre's projection; bounds are relative to it
+    const float cx = (float)lo[0] + 3.5f, cy = (float)lo[1] + 3.5f, cz = (float)lo[2] + 3.5f;
+    const float sc = cx * mm[0] + cy * mm[1] + cz * mm[2];
+    const float tc = cx * mm[3] + cy * mm[4] + cz * mm[5];
+    const int xc0 = (int)rintf(sc), yc0 = (int)rintf(tc);
+    const int dya = (int)ceilf(tc - (float)yc0 - SC_RAD), dyb = (int)floorf(tc - (float)yc0 + SC_RAD);
+    const int dxa = (int)ceilf(sc - (float)xc0 - SC_RAD), dxb = (int)floorf(sc - (float)xc0 + SC_RAD);
+    // row dy (one per lane, <= 16 rows): x range where p[d] in [lo-1, lo+8) (+- margin) on every axis
+    int cnt = 0, xa = 0;
+    const int y = yc0 + dya + lane;
+    if (dya + lane <= dyb && y >= -H && y <= H - 1) {
+        const float dyf = (float)(dya + lane);
+        float xl = (float)dxa, xh = (float)dxb;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const float ud = mm[d], vd = mm[3 + d];
+            if (fabsf(ud) >= 0.05f) {
+                const float P0 = (float)xc0 * ud + (float)yc0 * vd;
+                const float iu = __builtin_amdgcn_rcpf(ud);
+                const float t0 = ((float)(lo[d] - 1) - P0 - SC_DLT - dyf * vd) * iu;
+                const float t1 = ((float)(lo[d] + BRICK) - P0 + SC_DLT - dyf * vd) * iu;
+                xl = fmaxf(xl, fminf(t0, t1));
+                xh = fminf(xh, fmaxf(t0, t1));
+            }
+        }
+        int a = max((int)ceilf(xl - SC_EPS), -xc0);
+        int b = min((int)floorf(xh + SC_EPS), H - xc0);
+        if (!c.corners) b = min(b, isqrt24(H * H + H - y * y) - xc0);  // hypot_fast_int(x,y) <= P/2
+        xa = a;
+        cnt = max(0, b - a + 1);
+    }
+    int incl = cnt;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int v = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += v;
+    }
+    const int total = __shfl(incl, 63, 64);
+    const int excl = incl - cnt;
+    for (int base = 0; base < total; base += 64) {
+        // row of slot base+lane: the last non-empty row starting at or before it
+        mark[lane] = -1;
+        wave_sync_lds();
+        if (cnt > 0 && excl >= base && excl < base + 64) mark[excl - base] = lane;
+        if (cnt > 0 && excl < base && excl + cnt > base) mark[0] = lane;
+        wave_sync_lds();
+        int r = mark[lane];
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int v = __shfl_up(r, off, 64);
+            if (lane >= off) r = max(r, v);
+        }
+        wave_sync_lds();
+        const int s = base + lane;
+        const int rex = __shfl(excl, r & 63, 64), rxa = __shfl(xa, r & 63, 64);
+        bool ok = s < total;
+        const int x = xc0 + rxa + (s - rex), yy_ = yc0 + dya + r;
+        float2 dt = make_float2(0.f, 0.f);
+        float px = 0.f, py = 0.f, pz = 0.f;
+        int f0 = 0, f1 = 0, f2 = 0;
+        if (ok) {
+            dt = slice_value(S, x, yy_, P, H);
+            pixel_coords(x, yy_, mm, rtab, H, c.Pf, px, py, pz);
+            f0 = (int)floorf(px);
+            f1 = (int)floorf(py);
+            f2 = (int)floorf(pz);
+            ok = !(f0 < -H - 1 || f1 < -H - 1 || f2 < -H - 1 || f0 > H || f1 > H || f2 > H);
+            ok = ok && (unsigned)(f0 - lo[0] + 1) < (unsigned)(BRICK + 1) &&
+                 (unsigned)(f1 - lo[1] + 1) < (unsigned)(BRICK + 1) && (unsigned)(f2 - lo[2] + 1) < (unsigned)(BRICK + 1);
+        }
+        float wg[8];
+        corner_weights8(c.mode, f0, f1, f2, px, py, pz, c.h, weight, etab, wg);
+        const int i0 = f0 - lo[0], j0 = f1 - lo[1], k0 = f2 - lo[2];  // in [-1, BRICK-1]
+        // in-brick corners: bit di + 2 dj + 4 dk
+        const unsigned mi = (i0 >= 0 ? 1u : 0u) | (i0 < BRICK - 1 ? 2u : 0u);
+        const unsigned mj = (j0 >= 0 ? 1u : 0u) | (j0 < BRICK - 1 ? 2u : 0u);
+        const unsigned mk = (k0 >= 0 ? 1u : 0u) | (k0 < BRICK - 1 ? 2u : 0u);
+        const int cbase = acc_slot<A>(i0, j0, k0);
+        const int ti = ok ? (i0 + 1) + (BRICK + 1) * ((j0 + 1) + (BRICK + 1) * (k0 + 1)) : 0;
+        bool pend = ok;
+        while (__ballot(pend)) {  // rounds: one lane per stencil origin each
+            if (pend) tag[ti] = (unsigned char)lane;
+            wave_sync_lds();
+            const bool win = pend && tag[ti] == (unsigned char)lane;
+#pragma unroll
+            for (int cI = 0; cI < 8; cI++) {
+                const int di = cI & 1, dj = (cI >> 1) & 1, dk = cI >> 2;
+                if (win && ((mi >> di) & (mj >> dj) & (mk >> dk) & 1u)) {
+                    const int a = cbase + acc_slot<A>(di, dj, dk);
+                    acc_re[a] += dt.x * wg[cI];
+                    acc_im[a] += dt.y * wg[cI];
+                    acc_w[a] += wg[cI];
+                }
+                wave_sync_lds();
+            }
+            pend = pend && !win;
+        }
+    }
+}
+
+// Partial brick layout in HBM: [field re/im/w][z][y][x] of the brick's cells.
+template <int A>
+__device__ __forceinline__ void scatter_item(const float2 *__restrict__ slices, int nslices, int nsym,
+                                             const float *__restrict__ t3d, const float *__restrict__ wq,
+                                             const InsertConst &c, const int *__restrict__ list, const WorkItem &w,
+                                             const int lo[3], int lane, float *acc, unsigned char *tag, int *mark,
+                                             const float *etab, const float *rtab, float *out)
 {
     const size_t slice_elems = (size_t)c.P * (c.H + 1);
     for (int si = 0; si < w.count; si++) {
@@ -807,25 +934,36 @@ __device__ __forceinline__ void column_item(const float2 *__restrict__ slices, i
         float mm[6];
 #pragma unroll
         for (int d = 0; d < 6; d++) mm[d] = m[d];
-        column_slice<A>(slices + (size_t)(q / nsym) * slice_elems, mm, wq[q], c, lo, lane, acc, etab, rtab);
+        scatter_slice<A>(slices + (size_t)(q / nsym) * slice_elems, mm, wq[q], c, lo, lane, acc, tag, mark, etab,
+                         rtab);
     }
     wave_sync_lds();
-    column_flush<A>(acc, out, lane);
+#pragma unroll
+    for (int wv = 0; wv < BRICK; wv++) {
+        const int cc = lane + 64 * wv;
+        const int a = acc_slot<A>(cc & 7, (cc >> 3) & 7, cc >> 6);
+        out[cc] = acc[a];
+        out[ACC_CELLS + cc] = acc[ACC_CELLS + a];
+        out[2 * ACC_CELLS + cc] = acc[2 * ACC_CELLS + a];
+    }
     wave_sync_lds();
 }
 
-__global__ void __launch_bounds__(256) k_column_insert(const float2 *__restrict__ slices, int nslices, int nsym,
-                                                       const float *__restrict__ t3d, const float *__restrict__ wq,
-                                                       InsertConst c, BrickGeom g, BinBuffers bb,
-                                                       const float *__restrict__ exptab_g,
-                                                       const float *__restrict__ rtab_g)
+__global__ void __launch_bounds__(256) k_scatter_insert(const float2 *__restrict__ slices, int nslices, int nsym,
+                                                        const float *__restrict__ t3d, const float *__restrict__ wq,
+                                                        InsertConst c, BrickGeom g, BinBuffers bb,
+                                                        const float *__restrict__ exptab_g,
+                                                        const float *__restrict__ rtab_g)
 {
-    extern __shared__ float lds[];
-    const int rlen = (c.P + 2 + 3) & ~3;
-    float *etab = lds;
-    float *rtab = lds + EXPN;  // P+2 entries: fl(i/P), i in [-H, H+1]
+    extern __shared__ float4 lds4[];
+    char *base = reinterpret_cast<char *>(lds4);
     const int lane = threadIdx.x & 63;
-    float *acc = lds + EXPN + rlen + (threadIdx.x >> 6) * CACC;
+    char *mine = base + (size_t)(threadIdx.x >> 6) * WAVE_LDS_BYTES;
+    float *acc = reinterpret_cast<float *>(mine);
+    unsigned char *tag = reinterpret_cast<unsigned char *>(mine + 12 * ACC_CELLS);
+    int *mark = reinterpret_cast<int *>(mine + 12 * ACC_CELLS + ((TAG_N + 15) & ~15));
+    float *etab = reinterpret_cast<float *>(base + (size_t)(blockDim.x >> 6) * WAVE_LDS_BYTES);
+    float *rtab = etab + EXPN;  // P+2 entries: fl(i/P), i in [-H, H+1]
     for (int i = threadIdx.x; i < EXPN; i += blockDim.x) etab[i] = exptab_g[i];
     for (int i = threadIdx.x; i < c.P + 2; i += blockDim.x) rtab[i] = rtab_g[i];
     __syncthreads();
@@ -839,22 +977,28 @@ __global__ void __launch_bounds__(256) k_column_insert(const float2 *__restrict_
         const WorkItem w = bb.items[it];
         const int brick = w.key / 3, A = w.key - 3 * brick;
         const int lo[3] = {-H + BRICK * (brick % nb), -H + BRICK * ((brick / nb) % nb), -H + BRICK * (brick / (nb * nb))};
-        for (int s = 0; s < 3 * CSLOT; s++) acc[s * 64 + lane] = 0.f;
+#pragma unroll
+        for (int wv = 0; wv < 3 * BRICK; wv++) acc[lane + 64 * wv] = 0.f;
+        wave_sync_lds();
         float *out = bb.part + (size_t)it * PART_FLOATS;
-        if (A == 0) column_item<0>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, etab, rtab, out);
-        else if (A == 1) column_item<1>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, etab, rtab, out);
-        else column_item<2>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, etab, rtab, out);
+        if (A == 0) scatter_item<0>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, tag, mark, etab, rtab, out);
+        else if (A == 1) scatter_item<1>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, tag, mark, etab, rtab, out);
+        else scatter_item<2>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, tag, mark, etab, rtab, out);
     }
 }
 
-hipError_t launch_column_insert(const float2 *slices, int nslices, int nsym, const float *t3d, const float *wq,
-                                InsertConst c, BrickGeom g, BinBuffers b, const float *exptab, const float *rtab,
-                                int grid, int waves_per_block, hipStream_t s)
+size_t scatter_lds_bytes(int P, int waves_per_block)
+{
+    return (size_t)waves_per_block * WAVE_LDS_BYTES + sizeof(float) * (EXPN + P + 2);
+}
+
+hipError_t launch_scatter_insert(const float2 *slices, int nslices, int nsym, const float *t3d, const float *wq,
+                                 InsertConst c, BrickGeom g, BinBuffers b, const float *exptab, const float *rtab,
+                                 int grid, int waves_per_block, hipStream_t s)
 {
     if (waves_per_block < 1 || waves_per_block > 4) return hipErrorInvalidValue;
-    const size_t lds = sizeof(float) * (EXPN + ((c.P + 2 + 3) & ~3) + (size_t)waves_per_block * CACC);
-    hipLaunchKernelGGL(k_column_insert, dim3(grid), dim3(64 * waves_per_block), lds, s, slices, nslices, nsym, t3d,
-                       wq, c, g, b, exptab, rtab);
+    hipLaunchKernelGGL(k_scatter_insert, dim3(grid), dim3(64 * waves_per_block), scatter_lds_bytes(c.P, waves_per_block),
+                       s, slices, nslices, nsym, t3d, wq, c, g, b, exptab, rtab);
     return hipGetLastError();
 }
 
