@@ -750,7 +750,7 @@ hipError_t launch_bin(const float *t3d, const float *wq, int nslices, InsertCons
 // lanes of an access spread over the 32 banks.
 constexpr int ACC_CELLS = BRICK * BRICK * BRICK;
 constexpr int TAG_N = (BRICK + 1) * (BRICK + 1) * (BRICK + 1);  // stencil origins in [lo-1, lo+7]^3
-constexpr int WAVE_LDS_BYTES = 12 * ACC_CELLS + ((TAG_N + 15) & ~15) + 4 * 64;
+constexpr int WAVE_LDS_BYTES = 12 * ACC_CELLS + ((TAG_N + 15) & ~15) + 4 * 64 + 16 * 8;
 #define SC_DLT 2e-3f   // margin on p: the FP32 position error is < 3e-4 for P <= 1024
 #define SC_EPS 1e-3f   // margin on enumeration bounds
 #define SC_RAD 7.81f   // 4.5 sqrt(3) + margin: |p - brick centre| for an origin in [lo-1, lo+7]^3
@@ -811,13 +811,35 @@ __device__ __forceinline__ void corner_weights8(int mode, int i0, int j0, int k0
     }
 }
 
+// Up to GROUP slices of an item in one stream of 64-pixel windows: lane
+// (g, r) = (lane / 16, lane % 16) sets up row r of slice g (a stencil origin
+// in [lo-1, lo+7]^3 puts p within 4.5 sqrt(3) of the brick centre, i.e. at
+// most 16 rows), the rows' pixel counts are scanned over the wave, and each
+// window takes the next 64 pixels whatever slice they belong to.
+constexpr int GROUP = 4;
+
 template <int A>
-__device__ __forceinline__ void scatter_slice(const float2 *__restrict__ S, const float mm[6], float weight,
-                                              const InsertConst &c, const int lo[3], int lane, float *acc,
-                                              unsigned char *tag, int *mark, const float *etab, const float *rtab)
+__device__ __forceinline__ void scatter_group(const float2 *__restrict__ slices, int nslices, int nsym,
+                                              const float *__restrict__ t3d, const float *__restrict__ wq,
+                                              const int *__restrict__ list, int first, int ng, const InsertConst &c,
+                                              const int lo[3], int lane, float *acc, unsigned char *tag, int *mark,
+                                              float4 *rec, const float *etab, const float *rtab)
 {
     const int H = c.H, P = c.P;
+    const size_t slice_elems = (size_t)P * (H + 1);
     float *acc_re = acc, *acc_im = acc + ACC_CELLS, *acc_w = acc + 2 * ACC_CELLS;
+    const int g = lane >> 4, r = lane & 15;
+    int q = -1;
+    if (g < ng) q = list[first + g];
+    const bool qv = q >= 0 && q < nslices;  // never written list entries are skipped
+    float mm[6];
+#pragma unroll
+    for (int d = 0; d < 6; d++) mm[d] = qv ? t3d[9 * (size_t)q + d] : 0.f;
+    const float wgt = qv ? wq[q] : 0.f;
+    if (r == 0) {
+        rec[2 * g] = make_float4(mm[0], mm[1], mm[2], mm[3]);
+        rec[2 * g + 1] = make_float4(mm[4], mm[5], wgt, __int_as_float(q));
+    }
     // the pixel nearest the brick centre's projection; bounds are relative to it
     const float cx = (float)lo[0] + 3.5f, cy = (float)lo[1] + 3.5f, cz = (float)lo[2] + 3.5f;
     const float sc = cx * mm[0] + cy * mm[1] + cz * mm[2];
@@ -825,11 +847,11 @@ __device__ __forceinline__ void scatter_slice(const float2 *__restrict__ S, cons
     const int xc0 = (int)rintf(sc), yc0 = (int)rintf(tc);
     const int dya = (int)ceilf(tc - (float)yc0 - SC_RAD), dyb = (int)floorf(tc - (float)yc0 + SC_RAD);
     const int dxa = (int)ceilf(sc - (float)xc0 - SC_RAD), dxb = (int)floorf(sc - (float)xc0 + SC_RAD);
-    // row dy (one per lane, <= 16 rows): x range where p[d] in [lo-1, lo+8) (+- margin) on every axis
-    int cnt = 0, xa = 0;
-    const int y = yc0 + dya + lane;
-    if (dya + lane <= dyb && y >= -H && y <= H - 1) {
-        const float dyf = (float)(dya + lane);
+    // row dy: x range where p[d] in [lo-1, lo+8) (+- margin) on every axis
+    int cnt = 0, xs = 0;
+    const int y = yc0 + dya + r;
+    if (qv && dya + r <= dyb && y >= -H && y <= H - 1) {
+        const float dyf = (float)(dya + r);
         float xl = (float)dxa, xh = (float)dxb;
 #pragma unroll
         for (int d = 0; d < 3; d++) {
@@ -846,7 +868,7 @@ __device__ __forceinline__ void scatter_slice(const float2 *__restrict__ S, cons
         int a = max((int)ceilf(xl - SC_EPS), -xc0);
         int b = min((int)floorf(xh + SC_EPS), H - xc0);
         if (!c.corners) b = min(b, isqrt24(H * H + H - y * y) - xc0);  // hypot_fast_int(x,y) <= P/2
-        xa = a;
+        xs = xc0 + a;
         cnt = max(0, b - a + 1);
     }
     int incl = cnt;
@@ -864,23 +886,28 @@ __device__ __forceinline__ void scatter_slice(const float2 *__restrict__ S, cons
         if (cnt > 0 && excl >= base && excl < base + 64) mark[excl - base] = lane;
         if (cnt > 0 && excl < base && excl + cnt > base) mark[0] = lane;
         wave_sync_lds();
-        int r = mark[lane];
+        int rl = mark[lane];
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
-            const int v = __shfl_up(r, off, 64);
-            if (lane >= off) r = max(r, v);
+            const int v = __shfl_up(rl, off, 64);
+            if (lane >= off) rl = max(rl, v);
         }
         wave_sync_lds();
         const int s = base + lane;
-        const int rex = __shfl(excl, r & 63, 64), rxa = __shfl(xa, r & 63, 64);
+        rl &= 63;
+        const int rex = __shfl(excl, rl, 64), rxs = __shfl(xs, rl, 64), ry = __shfl(y, rl, 64);
         bool ok = s < total;
-        const int x = xc0 + rxa + (s - rex), yy_ = yc0 + dya + r;
+        const int x = rxs + (s - rex);
+        const float4 ra = rec[2 * (rl >> 4)], rb = rec[2 * (rl >> 4) + 1];
+        const float pm[6] = {ra.x, ra.y, ra.z, ra.w, rb.x, rb.y};
+        const float pw = rb.z;
+        const int pq = __float_as_int(rb.w);
         float2 dt = make_float2(0.f, 0.f);
         float px = 0.f, py = 0.f, pz = 0.f;
         int f0 = 0, f1 = 0, f2 = 0;
         if (ok) {
-            dt = slice_value(S, x, yy_, P, H);
-            pixel_coords(x, yy_, mm, rtab, H, c.Pf, px, py, pz);
+            dt = slice_value(slices + (size_t)(pq / nsym) * slice_elems, x, ry, P, H);
+            pixel_coords(x, ry, pm, rtab, H, c.Pf, px, py, pz);
             f0 = (int)floorf(px);
             f1 = (int)floorf(py);
             f2 = (int)floorf(pz);
@@ -889,7 +916,7 @@ __device__ __forceinline__ void scatter_slice(const float2 *__restrict__ S, cons
                  (unsigned)(f1 - lo[1] + 1) < (unsigned)(BRICK + 1) && (unsigned)(f2 - lo[2] + 1) < (unsigned)(BRICK + 1);
         }
         float wg[8];
-        corner_weights8(c.mode, f0, f1, f2, px, py, pz, c.h, weight, etab, wg);
+        corner_weights8(c.mode, f0, f1, f2, px, py, pz, c.h, pw, etab, wg);
         const int i0 = f0 - lo[0], j0 = f1 - lo[1], k0 = f2 - lo[2];  // in [-1, BRICK-1]
         // in-brick corners: bit di + 2 dj + 4 dk
         const unsigned mi = (i0 >= 0 ? 1u : 0u) | (i0 < BRICK - 1 ? 2u : 0u);
@@ -916,6 +943,7 @@ __device__ __forceinline__ void scatter_slice(const float2 *__restrict__ S, cons
             pend = pend && !win;
         }
     }
+    wave_sync_lds();  // the next group rewrites the slice records
 }
 
 // Partial brick layout in HBM: [field re/im/w][z][y][x] of the brick's cells.
@@ -924,19 +952,11 @@ __device__ __forceinline__ void scatter_item(const float2 *__restrict__ slices, 
                                              const float *__restrict__ t3d, const float *__restrict__ wq,
                                              const InsertConst &c, const int *__restrict__ list, const WorkItem &w,
                                              const int lo[3], int lane, float *acc, unsigned char *tag, int *mark,
-                                             const float *etab, const float *rtab, float *out)
+                                             float4 *rec, const float *etab, const float *rtab, float *out)
 {
-    const size_t slice_elems = (size_t)c.P * (c.H + 1);
-    for (int si = 0; si < w.count; si++) {
-        const int q = __builtin_amdgcn_readfirstlane(list[w.start + si]);
-        if (q < 0 || q >= nslices) continue;  // never written list entry
-        const float *m = t3d + 9 * (size_t)q;
-        float mm[6];
-#pragma unroll
-        for (int d = 0; d < 6; d++) mm[d] = m[d];
-        scatter_slice<A>(slices + (size_t)(q / nsym) * slice_elems, mm, wq[q], c, lo, lane, acc, tag, mark, etab,
-                         rtab);
-    }
+    for (int s0 = 0; s0 < w.count; s0 += GROUP)
+        scatter_group<A>(slices, nslices, nsym, t3d, wq, list, w.start + s0, min(GROUP, w.count - s0), c, lo, lane,
+                         acc, tag, mark, rec, etab, rtab);
     wave_sync_lds();
 #pragma unroll
     for (int wv = 0; wv < BRICK; wv++) {
@@ -962,6 +982,7 @@ __global__ void __launch_bounds__(256) k_scatter_insert(const float2 *__restrict
     float *acc = reinterpret_cast<float *>(mine);
     unsigned char *tag = reinterpret_cast<unsigned char *>(mine + 12 * ACC_CELLS);
     int *mark = reinterpret_cast<int *>(mine + 12 * ACC_CELLS + ((TAG_N + 15) & ~15));
+    float4 *rec = reinterpret_cast<float4 *>(mine + 12 * ACC_CELLS + ((TAG_N + 15) & ~15) + 4 * 64);
     float *etab = reinterpret_cast<float *>(base + (size_t)(blockDim.x >> 6) * WAVE_LDS_BYTES);
     float *rtab = etab + EXPN;  // P+2 entries: fl(i/P), i in [-H, H+1]
     for (int i = threadIdx.x; i < EXPN; i += blockDim.x) etab[i] = exptab_g[i];
@@ -981,9 +1002,9 @@ __global__ void __launch_bounds__(256) k_scatter_insert(const float2 *__restrict
         for (int wv = 0; wv < 3 * BRICK; wv++) acc[lane + 64 * wv] = 0.f;
         wave_sync_lds();
         float *out = bb.part + (size_t)it * PART_FLOATS;
-        if (A == 0) scatter_item<0>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, tag, mark, etab, rtab, out);
-        else if (A == 1) scatter_item<1>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, tag, mark, etab, rtab, out);
-        else scatter_item<2>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, tag, mark, etab, rtab, out);
+        if (A == 0) scatter_item<0>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, tag, mark, rec, etab, rtab, out);
+        else if (A == 1) scatter_item<1>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, tag, mark, rec, etab, rtab, out);
+        else scatter_item<2>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, tag, mark, rec, etab, rtab, out);
     }
 }
 
