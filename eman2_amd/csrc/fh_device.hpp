@@ -77,28 +77,71 @@ __device__ __forceinline__ void dft<5>(float2 *v)
     v[3] = csub(r2, mul_mi(i2));
 }
 
+// 6 = 2 x 3: DFT3 of the even and odd halves, X[k] = E[k] + W6^k O[k],
+// X[k+3] = E[k] - W6^k O[k].
+template <>
+__device__ __forceinline__ void dft<6>(float2 *v)
+{
+    const float h = 0.86602540378443864676f;  // sin(2 pi/6)
+    float2 e[3] = {v[0], v[2], v[4]}, o[3] = {v[1], v[3], v[5]};
+    dft<3>(e);
+    dft<3>(o);
+    const float2 o1 = make_float2(0.5f * o[1].x + h * o[1].y, 0.5f * o[1].y - h * o[1].x);   // W6^1 = 1/2 - i h
+    const float2 o2 = make_float2(-0.5f * o[2].x + h * o[2].y, -0.5f * o[2].y - h * o[2].x); // W6^2 = -1/2 - i h
+    v[0] = cadd(e[0], o[0]);
+    v[3] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], o1);
+    v[4] = csub(e[1], o1);
+    v[2] = cadd(e[2], o2);
+    v[5] = csub(e[2], o2);
+}
+
+// 8 = 2 x 4: DFT4 of the even and odd halves, X[k] = E[k] + W8^k O[k],
+// X[k+4] = E[k] - W8^k O[k].
+template <>
+__device__ __forceinline__ void dft<8>(float2 *v)
+{
+    const float r = 0.70710678118654752440f;  // 1/sqrt(2)
+    float2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+    dft<4>(e);
+    dft<4>(o);
+    const float2 o1 = make_float2(r * (o[1].x + o[1].y), r * (o[1].y - o[1].x));    // W8^1 = (1 - i)/sqrt 2
+    const float2 o2 = mul_mi(o[2]);                                                // W8^2 = -i
+    const float2 o3 = make_float2(r * (o[3].y - o[3].x), -r * (o[3].x + o[3].y));   // W8^3 = -(1 + i)/sqrt 2
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], o1);
+    v[5] = csub(e[1], o1);
+    v[2] = cadd(e[2], o2);
+    v[6] = csub(e[2], o2);
+    v[3] = cadd(e[3], o3);
+    v[7] = csub(e[3], o3);
+}
+
 // One Stockham autosort pass over T interleaved transforms of length N:
-// element (t, n) lives at buf[n*T + t].  tw[m] = exp(-2 pi i m / N).
+// element (t, n) lives at buf[n*T + t].  tw[m] = exp(-2 pi i m / N).  Pass
+// p has stride Ns = prod(radix[<p]); j / Ns is taken with the plan's
+// multiply-high constant.
 template <int R, int T>
 __device__ __forceinline__ void stockham_pass(const float2 *__restrict__ src, float2 *__restrict__ dst,
-                                              int N, int Ns, const float2 *__restrict__ tw)
+                                              int N, int Ns, unsigned mag, int twm, const float2 *__restrict__ tw)
 {
     const int NR = N / R;
     const int total = NR * T;
-    const int twmul = N / (Ns * R);
     for (int b = threadIdx.x; b < total; b += blockDim.x) {
         const int t = b % T;
         const int j = b / T;
-        const int k = j % Ns;
+        const int q = Ns == 1 ? j : (int)__umulhi((unsigned)j, mag);
+        const int k = j - q * Ns;
         float2 v[R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
             float2 a = src[(j + r * NR) * T + t];
-            if (r > 0) a = cmul(a, tw[r * k * twmul]);
+            if (r > 0) a = cmul(a, tw[r * k * twm]);
             v[r] = a;
         }
         dft<R>(v);
-        const int base = (j / Ns) * Ns * R + k;
+        const int base = q * Ns * R + k;
 #pragma unroll
         for (int r = 0; r < R; r++) dst[(base + r * Ns) * T + t] = v[r];
     }
@@ -109,20 +152,22 @@ __device__ __forceinline__ void stockham_pass(const float2 *__restrict__ src, fl
 template <int T>
 __device__ float2 *fft_lds(float2 *a, float2 *b, const FftPlan &plan, const float2 *tw)
 {
-    int Ns = 1;
     for (int p = 0; p < plan.npass; p++) {
-        const int R = plan.radix[p];
+        const int R = plan.radix[p], Ns = plan.ns[p];
+        const unsigned mag = plan.mag[p];
+        const int twm = plan.twm[p];
         switch (R) {
-        case 2: stockham_pass<2, T>(a, b, plan.N, Ns, tw); break;
-        case 3: stockham_pass<3, T>(a, b, plan.N, Ns, tw); break;
-        case 4: stockham_pass<4, T>(a, b, plan.N, Ns, tw); break;
-        default: stockham_pass<5, T>(a, b, plan.N, Ns, tw); break;
+        case 2: stockham_pass<2, T>(a, b, plan.N, Ns, mag, twm, tw); break;
+        case 3: stockham_pass<3, T>(a, b, plan.N, Ns, mag, twm, tw); break;
+        case 4: stockham_pass<4, T>(a, b, plan.N, Ns, mag, twm, tw); break;
+        case 5: stockham_pass<5, T>(a, b, plan.N, Ns, mag, twm, tw); break;
+        case 6: stockham_pass<6, T>(a, b, plan.N, Ns, mag, twm, tw); break;
+        default: stockham_pass<8, T>(a, b, plan.N, Ns, mag, twm, tw); break;
         }
         __syncthreads();
         float2 *t = a;
         a = b;
         b = t;
-        Ns *= R;
     }
     return a;
 }

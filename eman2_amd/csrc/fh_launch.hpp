@@ -6,11 +6,16 @@
 namespace fh {
 
 // Mixed-radix plan for the LDS Stockham FFT: N = prod(radix[i]), radices
-// in {2,3,4,5}.
+// in {2,3,4,5,6,8}; pass p has stride ns[p] = prod(radix[<p]), mag[p] =
+// ceil(2^32 / ns[p]) (exact j / ns for j < 2^16) and twiddle step
+// twm[p] = N / (ns[p] * radix[p]).
 struct FftPlan {
     int N;
     int npass;
     int radix[16];
+    int ns[16];
+    unsigned mag[16];
+    int twm[16];
 };
 bool make_plan(int N, FftPlan *p);
 

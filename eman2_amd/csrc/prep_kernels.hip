@@ -15,11 +15,21 @@ bool make_plan(int N, FftPlan *p)
     p->N = N;
     p->npass = 0;
     int m = N;
+    while (m % 8 == 0 && p->npass < 16) { p->radix[p->npass++] = 8; m /= 8; }
+    while (m % 6 == 0 && p->npass < 16) { p->radix[p->npass++] = 6; m /= 6; }
     while (m % 4 == 0 && p->npass < 16) { p->radix[p->npass++] = 4; m /= 4; }
     while (m % 2 == 0 && p->npass < 16) { p->radix[p->npass++] = 2; m /= 2; }
     while (m % 3 == 0 && p->npass < 16) { p->radix[p->npass++] = 3; m /= 3; }
     while (m % 5 == 0 && p->npass < 16) { p->radix[p->npass++] = 5; m /= 5; }
-    return m == 1 && N > 1;
+    if (m != 1 || N <= 1 || N >= 65536) return false;
+    int ns = 1;
+    for (int i = 0; i < p->npass; i++) {
+        p->ns[i] = ns;
+        p->mag[i] = ns == 1 ? 0u : (unsigned)((((uint64_t)1 << 32) + ns - 1) / ns);
+        p->twm[i] = N / (ns * p->radix[i]);
+        ns *= p->radix[i];
+    }
+    return true;
 }
 
 static int pick_T(int P)
@@ -84,8 +94,8 @@ __global__ void __launch_bounds__(256) k_prep_rows(const float *__restrict__ img
     const float mean = means[p];
     const float *img = imgs + (size_t)p * B * B;
     for (int idx = threadIdx.x; idx < T * P; idx += blockDim.x) {
-        const int t = idx / P;
-        const int xs = idx - t * P;
+        const int t = idx % T;  // transform fastest: consecutive LDS slots
+        const int xs = idx / T;
         const int xp = xs + H < P ? xs + H : xs + H - P;
         const int bx = xp - off;
         float re = 0.f, im = 0.f;
@@ -100,8 +110,8 @@ __global__ void __launch_bounds__(256) k_prep_rows(const float *__restrict__ img
     float2 *R = fft_lds<T>(A, Bf, plan, tw);
     float2 *out = slices + (size_t)p * P * (H + 1);
     for (int idx = threadIdx.x; idx < T * (H + 1); idx += blockDim.x) {
-        const int t = idx / (H + 1);
-        const int k = idx - t * (H + 1);
+        const int t = idx % T;
+        const int k = idx / T;
         const int r0 = b0 + 2 * t;
         if (r0 >= B) continue;
         const float2 z1 = R[k * T + t];
@@ -140,14 +150,21 @@ hipError_t launch_prep_rows(const float *imgs, int n, int B, int P, const float 
 // CTF_SIGN of EMAN2Ctf::compute_2d_complex (libEM/ctf.cpp:1076-1090) at
 // Fourier pixel (x, y), y signed.  Evaluated with the reference's float
 // operation order (no contraction); the sign of cos is taken in double.
+// cos(t) < 0 exactly when floor(t / pi - 1/2) is even; away from the
+// zeros of cos (|frac| > 1e-9, far above the ~1e-13 error of u) the parity
+// decides, next to them cos itself is evaluated.
 __device__ __forceinline__ float ctf_sign(int x, int y, float defocus, const CtfConst &cc)
 {
 #pragma clang fp contract(off)
     const float hyp = __fsqrt_rn((float)(x * x + y * y));
     const float s = hyp * cc.ds;
     const float gam = -cc.g1 * s * s * s * s + cc.g2 * defocus * s * s;
-    const double v = cos((double)(gam - cc.acac));
-    return v < 0 ? -1.0f : 1.0f;
+    const double t = (double)(gam - cc.acac);
+    const double u = t * 0.31830988618379067154 - 0.5;
+    const double fl = floor(u);
+    const double fr = u - fl;
+    if (fr > 1e-9 && fr < 1.0 - 1e-9) return (((long long)fl) & 1) == 0 ? -1.0f : 1.0f;
+    return cos(t) < 0 ? -1.0f : 1.0f;
 }
 
 template <int T>
