@@ -147,6 +147,69 @@ __device__ __forceinline__ void stockham_pass(const float2 *__restrict__ src, fl
     }
 }
 
+// In-place pass: every thread first reads and transforms all of its
+// butterflies (at most G per thread, G * R <= 18 registers of float2), then,
+// after a barrier, writes them back to their Stockham output positions in
+// the same buffer.
+template <int R, int T, int G>
+__device__ __forceinline__ void stockham_pass_inplace(float2 *buf, int N, int Ns, unsigned mag, int twm,
+                                                      const float2 *__restrict__ tw)
+{
+    const int NR = N / R;
+    const int total = NR * T;
+    float2 v[G][R];
+    int dsti[G];
+#pragma unroll
+    for (int gi = 0; gi < G; gi++) {
+        const int b = threadIdx.x + gi * blockDim.x;
+        dsti[gi] = -1;
+        if (b < total) {
+            const int t = b % T;
+            const int j = b / T;
+            const int q = Ns == 1 ? j : (int)__umulhi((unsigned)j, mag);
+            const int k = j - q * Ns;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                float2 a = buf[(j + r * NR) * T + t];
+                if (r > 0) a = cmul(a, tw[r * k * twm]);
+                v[gi][r] = a;
+            }
+            dft<R>(v[gi]);
+            dsti[gi] = (q * Ns * R + k) * T + t;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int gi = 0; gi < G; gi++) {
+        if (dsti[gi] >= 0) {
+#pragma unroll
+            for (int r = 0; r < R; r++) buf[dsti[gi] + r * Ns * T] = v[gi][r];
+        }
+    }
+    __syncthreads();
+}
+
+// Full forward transform of T interleaved length-N transforms in place in
+// buf (T * N float2), 256 threads; requires N * T <= 3200.  Ends with a
+// barrier.
+template <int T>
+__device__ void fft_lds_inplace(float2 *buf, const FftPlan &plan, const float2 *tw)
+{
+    for (int p = 0; p < plan.npass; p++) {
+        const int R = plan.radix[p], Ns = plan.ns[p];
+        const unsigned mag = plan.mag[p];
+        const int twm = plan.twm[p];
+        switch (R) {
+        case 2: stockham_pass_inplace<2, T, 7>(buf, plan.N, Ns, mag, twm, tw); break;
+        case 3: stockham_pass_inplace<3, T, 5>(buf, plan.N, Ns, mag, twm, tw); break;
+        case 4: stockham_pass_inplace<4, T, 4>(buf, plan.N, Ns, mag, twm, tw); break;
+        case 5: stockham_pass_inplace<5, T, 3>(buf, plan.N, Ns, mag, twm, tw); break;
+        case 6: stockham_pass_inplace<6, T, 3>(buf, plan.N, Ns, mag, twm, tw); break;
+        default: stockham_pass_inplace<8, T, 2>(buf, plan.N, Ns, mag, twm, tw); break;
+        }
+    }
+}
+
 // Full forward transform (ping-pong between a and b); returns the buffer that
 // holds the result.  Ends with a barrier.
 template <int T>
@@ -170,6 +233,18 @@ __device__ float2 *fft_lds(float2 *a, float2 *b, const FftPlan &plan, const floa
         b = t;
     }
     return a;
+}
+
+// In place when the tile fits the register bound of fft_lds_inplace, else
+// ping-pong through b (the launch sizes the LDS accordingly).
+template <int T>
+__device__ __forceinline__ float2 *fft_tile(float2 *a, float2 *b, const FftPlan &plan, const float2 *tw)
+{
+    if (plan.N * T <= 3200) {
+        fft_lds_inplace<T>(a, plan, tw);
+        return a;
+    }
+    return fft_lds<T>(a, b, plan, tw);
 }
 
 __device__ __forceinline__ void load_twiddles(float2 *tw, const float2 *__restrict__ g, int N)

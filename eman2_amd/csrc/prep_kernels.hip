@@ -38,7 +38,11 @@ static int pick_T(int P)
     return (P <= 400) ? 8 : 4;
 }
 
-static size_t fft_lds_bytes(int P, int T) { return sizeof(float2) * (size_t)P * (2 * T + 1); }
+// twiddles + one tile (in place) or two (ping-pong), see fft_tile
+static size_t fft_lds_bytes(int P, int T)
+{
+    return sizeof(float2) * (size_t)P * (P * T <= 3200 ? T + 1 : 2 * T + 1);
+}
 
 // ---------------------------------------------------------------- edge mean
 // EMData::get_edge_mean, 2-D branch (libEM/emdata_metadata.cpp:713-736):
@@ -107,7 +111,7 @@ __global__ void __launch_bounds__(256) k_prep_rows(const float *__restrict__ img
         A[xs * T + t] = make_float2(re, im);
     }
     __syncthreads();
-    float2 *R = fft_lds<T>(A, Bf, plan, tw);
+    float2 *R = fft_tile<T>(A, Bf, plan, tw);
     float2 *out = slices + (size_t)p * P * (H + 1);
     for (int idx = threadIdx.x; idx < T * (H + 1); idx += blockDim.x) {
         const int t = idx % T;
@@ -193,7 +197,7 @@ __global__ void __launch_bounds__(256) k_prep_cols(int B, int P, int off, float2
         A[y * T + t] = v;
     }
     __syncthreads();
-    float2 *R = fft_lds<T>(A, Bf, plan, tw);
+    float2 *R = fft_tile<T>(A, Bf, plan, tw);
     const float df = defocus ? defocus[p] : 0.f;
     for (int idx = threadIdx.x; idx < T * P; idx += blockDim.x) {
         const int y = idx / T;
@@ -317,7 +321,7 @@ __global__ void __launch_bounds__(256) k_fft_axis_inv(float2 *__restrict__ data,
         A[n * T + t] = v;
     }
     __syncthreads();
-    float2 *R = fft_lds<T>(A, Bf, plan, tw);
+    float2 *R = fft_tile<T>(A, Bf, plan, tw);
     for (int idx = threadIdx.x; idx < T * P; idx += blockDim.x) {
         const int n = idx / T;
         const int t = idx - n * T;
@@ -379,7 +383,7 @@ __global__ void __launch_bounds__(256) k_c2r_x(const float2 *__restrict__ data, 
         A[k * T + t] = make_float2(a.x - c.y, -(a.y + c.x));
     }
     __syncthreads();
-    float2 *R = fft_lds<T>(A, Bf, plan, tw);
+    float2 *R = fft_tile<T>(A, Bf, plan, tw);
     for (int idx = threadIdx.x; idx < T * P; idx += blockDim.x) {
         const int t = idx / P;
         const int x = idx - t * P;
