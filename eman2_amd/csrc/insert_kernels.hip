@@ -750,7 +750,7 @@ hipError_t launch_bin(const float *t3d, const float *wq, int nslices, InsertCons
 // lanes of an access spread over the 32 banks.
 constexpr int ACC_CELLS = BRICK * BRICK * BRICK;
 constexpr int TAG_N = (BRICK + 1) * (BRICK + 1) * (BRICK + 1);  // stencil origins in [lo-1, lo+7]^3
-constexpr int WAVE_LDS_BYTES = 12 * ACC_CELLS + ((TAG_N + 15) & ~15) + 4 * 64 + 16 * 8;
+constexpr int WAVE_LDS_BYTES = 12 * ACC_CELLS + ((TAG_N + 15) & ~15) + 4 * 64 + 16 * 8 + 16 * 64;
 #define SC_DLT 2e-3f   // margin on p: the FP32 position error is < 3e-4 for P <= 1024
 #define SC_EPS 1e-3f   // margin on enumeration bounds
 #define SC_RAD 7.81f   // 4.5 sqrt(3) + margin: |p - brick centre| for an origin in [lo-1, lo+7]^3
@@ -766,6 +766,33 @@ __device__ __forceinline__ void wave_sync_lds()
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Wave64 inclusive scans on DPP (GFX9 row shifts and row broadcasts): no
+// LDS traffic.  Lanes without a source keep the identity.
+#define DPP_ROW_SHR(n) (0x110 + (n))
+#define DPP_ROW_BCAST15 0x142
+#define DPP_ROW_BCAST31 0x143
+__device__ __forceinline__ int wave_scan_add(int v)
+{
+    v += __builtin_amdgcn_update_dpp(0, v, DPP_ROW_SHR(1), 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, DPP_ROW_SHR(2), 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, DPP_ROW_SHR(4), 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, DPP_ROW_SHR(8), 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, DPP_ROW_BCAST15, 0xa, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, DPP_ROW_BCAST31, 0xc, 0xf, false);
+    return v;
+}
+__device__ __forceinline__ int wave_scan_max(int v)
+{
+    const int lo = -2147483647 - 1;
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, DPP_ROW_SHR(1), 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, DPP_ROW_SHR(2), 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, DPP_ROW_SHR(4), 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, DPP_ROW_SHR(8), 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, DPP_ROW_BCAST15, 0xa, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, DPP_ROW_BCAST31, 0xc, 0xf, false));
+    return v;
 }
 
 // floor(sqrt(n)) for 0 <= n < 2^24
@@ -823,7 +850,7 @@ __device__ __forceinline__ void scatter_group(const float2 *__restrict__ slices,
                                               const float *__restrict__ t3d, const float *__restrict__ wq,
                                               const int *__restrict__ list, int first, int ng, const InsertConst &c,
                                               const int lo[3], int lane, float *acc, unsigned char *tag, int *mark,
-                                              float4 *rec, const float *etab, const float *rtab)
+                                              float4 *rec, int4 *rowi, const float *etab, const float *rtab)
 {
     const int H = c.H, P = c.P;
     const size_t slice_elems = (size_t)P * (H + 1);
@@ -871,14 +898,10 @@ __device__ __forceinline__ void scatter_group(const float2 *__restrict__ slices,
         xs = xc0 + a;
         cnt = max(0, b - a + 1);
     }
-    int incl = cnt;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int v = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += v;
-    }
-    const int total = __shfl(incl, 63, 64);
+    const int incl = wave_scan_add(cnt);
+    const int total = __builtin_amdgcn_readlane(incl, 63);
     const int excl = incl - cnt;
+    rowi[lane] = make_int4(excl, xs, y, 0);  // read back by the slots of this row
     for (int base = 0; base < total; base += 64) {
         // row of slot base+lane: the last non-empty row starting at or before it
         mark[lane] = -1;
@@ -886,18 +909,14 @@ __device__ __forceinline__ void scatter_group(const float2 *__restrict__ slices,
         if (cnt > 0 && excl >= base && excl < base + 64) mark[excl - base] = lane;
         if (cnt > 0 && excl < base && excl + cnt > base) mark[0] = lane;
         wave_sync_lds();
-        int rl = mark[lane];
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const int v = __shfl_up(rl, off, 64);
-            if (lane >= off) rl = max(rl, v);
-        }
-        wave_sync_lds();
+        int rl = wave_scan_max(mark[lane]);
         const int s = base + lane;
         rl &= 63;
-        const int rex = __shfl(excl, rl, 64), rxs = __shfl(xs, rl, 64), ry = __shfl(y, rl, 64);
-        bool ok = s < total;
+        const int4 ri = rowi[rl];
+        wave_sync_lds();
+        const int rex = ri.x, rxs = ri.y, ry = ri.z;
         const int x = rxs + (s - rex);
+        bool ok = s < total && (unsigned)x <= (unsigned)H;  // x in [0, P/2] holds by construction
         const float4 ra = rec[2 * (rl >> 4)], rb = rec[2 * (rl >> 4) + 1];
         const float pm[6] = {ra.x, ra.y, ra.z, ra.w, rb.x, rb.y};
         const float pw = rb.z;
@@ -952,11 +971,11 @@ __device__ __forceinline__ void scatter_item(const float2 *__restrict__ slices, 
                                              const float *__restrict__ t3d, const float *__restrict__ wq,
                                              const InsertConst &c, const int *__restrict__ list, const WorkItem &w,
                                              const int lo[3], int lane, float *acc, unsigned char *tag, int *mark,
-                                             float4 *rec, const float *etab, const float *rtab, float *out)
+                                             float4 *rec, int4 *rowi, const float *etab, const float *rtab, float *out)
 {
     for (int s0 = 0; s0 < w.count; s0 += GROUP)
         scatter_group<A>(slices, nslices, nsym, t3d, wq, list, w.start + s0, min(GROUP, w.count - s0), c, lo, lane,
-                         acc, tag, mark, rec, etab, rtab);
+                         acc, tag, mark, rec, rowi, etab, rtab);
     wave_sync_lds();
 #pragma unroll
     for (int wv = 0; wv < BRICK; wv++) {
@@ -983,6 +1002,7 @@ __global__ void __launch_bounds__(256) k_scatter_insert(const float2 *__restrict
     unsigned char *tag = reinterpret_cast<unsigned char *>(mine + 12 * ACC_CELLS);
     int *mark = reinterpret_cast<int *>(mine + 12 * ACC_CELLS + ((TAG_N + 15) & ~15));
     float4 *rec = reinterpret_cast<float4 *>(mine + 12 * ACC_CELLS + ((TAG_N + 15) & ~15) + 4 * 64);
+    int4 *rowi = reinterpret_cast<int4 *>(mine + 12 * ACC_CELLS + ((TAG_N + 15) & ~15) + 4 * 64 + 16 * 8);
     float *etab = reinterpret_cast<float *>(base + (size_t)(blockDim.x >> 6) * WAVE_LDS_BYTES);
     float *rtab = etab + EXPN;  // P+2 entries: fl(i/P), i in [-H, H+1]
     for (int i = threadIdx.x; i < EXPN; i += blockDim.x) etab[i] = exptab_g[i];
@@ -1002,9 +1022,9 @@ __global__ void __launch_bounds__(256) k_scatter_insert(const float2 *__restrict
         for (int wv = 0; wv < 3 * BRICK; wv++) acc[lane + 64 * wv] = 0.f;
         wave_sync_lds();
         float *out = bb.part + (size_t)it * PART_FLOATS;
-        if (A == 0) scatter_item<0>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, tag, mark, rec, etab, rtab, out);
-        else if (A == 1) scatter_item<1>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, tag, mark, rec, etab, rtab, out);
-        else scatter_item<2>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, tag, mark, rec, etab, rtab, out);
+        if (A == 0) scatter_item<0>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, tag, mark, rec, rowi, etab, rtab, out);
+        else if (A == 1) scatter_item<1>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, tag, mark, rec, rowi, etab, rtab, out);
+        else scatter_item<2>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, tag, mark, rec, rowi, etab, rtab, out);
     }
 }
 
