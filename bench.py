@@ -211,12 +211,14 @@ def main():
     per_launch_bytes = bytes_per_particle * (a.steps * n) / max(1, launches)
     achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "column_insert_traffic.json")
+    # HBM bytes per particle of the same kernel from the FETCH_SIZE /
+    # WRITE_SIZE runs (scaled by a 1 GiB copy), when measured on this workload
+    tf = os.path.join(ROOT, "profiles", "scatter_insert_traffic.json")
     if os.path.exists(tf):
         with open(tf) as f:
             t = json.load(f)
-        if t.get("P") == P and t.get("sym") == a.sym:
-            traffic = t.get("hbm_bytes_per_particle", 0) * (a.steps * n) / max(1, launches)
+        if t.get("P") == P and t.get("sym") == a.sym and t.get("box") == B:
+            traffic = round(t["hbm_bytes_per_particle"] * (a.steps * n) / max(1, launches))
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": "k_scatter_insert", "avg_launch_ms": round(avg_ms, 3),
@@ -242,6 +244,7 @@ def main():
         "stage_ms_per_step": {k: round(sum(s[k] for s in stats) / a.steps, 2)
                               for k in ("prep_ms", "bin_ms", "insert_kernel_ms", "merge_ms", "insert_ms", "total_ms")},
         "pairs_per_step": int(sum(s["pairs"] for s in stats) / a.steps),
+        "items_per_step": int(sum(s["items"] for s in stats) / a.steps),
     }
 
     # plain one-particle-per-launch form on a host-side sample
