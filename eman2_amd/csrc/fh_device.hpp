@@ -14,6 +14,12 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b)
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 __device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+
+// Correctly rounded float sqrt: sqrt of a float never lies within 2^-53 of a
+// float rounding midpoint, so the double sqrt rounded once to float is exact
+// (the single-precision hardware sqrt is only accurate to 1 ulp).  Equals
+// hypot((float)x, (float)y) of the reference for integer x^2 + y^2 < 2^24.
+__device__ __forceinline__ float sqrt_rn(float v) { return (float)sqrt((double)v); }
 // multiply by -i
 __device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }
 

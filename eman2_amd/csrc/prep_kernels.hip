@@ -160,7 +160,7 @@ hipError_t launch_prep_rows(const float *imgs, int n, int B, int P, const float 
 __device__ __forceinline__ float ctf_sign(int x, int y, float defocus, const CtfConst &cc)
 {
 #pragma clang fp contract(off)
-    const float hyp = __fsqrt_rn((float)(x * x + y * y));
+    const float hyp = sqrt_rn((float)(x * x + y * y));  // Util::hypot_fast
     const float s = hyp * cc.ds;
     const float gam = -cc.g1 * s * s * s * s + cc.g2 * defocus * s * s;
     const double t = (double)(gam - cc.acac);
@@ -279,7 +279,7 @@ __global__ void k_normalize(float2 *__restrict__ data, float *__restrict__ norm,
         } else {
             d = norm[i];
         }
-        if (sqrtnorm) d = __fsqrt_rn(d);
+        if (sqrtnorm) d = sqrt_rn(d);
         float2 v = data[i];
         if (d == 0) v = make_float2(0.f, 0.f);
         else v = make_float2(__fdiv_rn(v.x, d), __fdiv_rn(v.y, d));

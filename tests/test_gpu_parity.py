@@ -55,6 +55,20 @@ def test_preprocess_slice(oracle, P):
     assert max_rel_err(g, o) <= 2e-5
 
 
+def test_prep_ctf_sign_many_particles(oracle):
+    """CTF_SIGN on every pixel of 200 particles at 256 -> 384: the sign
+    follows hypot_fast, a correctly rounded sqrt (a 1-ulp sqrt flipped the
+    sign of whole rings of pixels near zeros of the CTF)."""
+    B, P, n = 256, 384, 200
+    ph = S.phantom(200, 96.0, seed=7)
+    ctf = dict(voltage=300.0, cs=2.7, ampcont=10.0, bfactor=80.0)
+    imgs, _, dfs = S.particles(ph, n, B, seed=8, apix=1.0, snr=0.05, ctf=ctf)
+    g = recon(P).prep_batch(imgs, defocus=dfs, ctf=dict(ctf, apix=1.0))
+    for i in range(n):
+        o = oracle.prep_particle(imgs[i], P, dict(ctf, apix=1.0, defocus=float(dfs[i])))
+        assert max_rel_err(g[i], o) <= 2e-5, i
+
+
 @pytest.mark.parametrize("B,P,ctf", [(64, 96, False), (64, 96, True), (63, 96, True), (96, 96, False),
                                      (256, 384, True), (192, 288, True), (400, 600, False)])
 def test_prep_batch(oracle, B, P, ctf):
