@@ -76,9 +76,11 @@ __device__ __forceinline__ void pixel_coords(int x, int y, const float *m, const
 // the x=0, y<0 pixel is read as the conjugate of (0, -y).
 __device__ __forceinline__ float2 slice_value(const float2 *__restrict__ S, int x, int y, int P, int H)
 {
-    if (x == 0 && y < 0) return conjf2(S[(size_t)(-y) * (H + 1)]);
-    const int row = y < 0 ? P + y : y;
-    return S[(size_t)row * (H + 1) + x];
+    const bool cj = x == 0 && y < 0;
+    const int row = cj ? -y : (y < 0 ? P + y : y);
+    float2 v = S[row * (H + 1) + x];  // P (P/2 + 1) < 2^31
+    if (cj) v.y = -v.y;
+    return v;
 }
 
 // gauss_2 / trilinear weight of corner (i,j,k) for a pixel at (xx,yy,zz).
@@ -89,7 +91,7 @@ __device__ __forceinline__ float corner_weight(int mode, int i, int j, int k, fl
     if (mode == 0) {
         const float r = dx * dx + dy * dy + dz * dz;  // Util::hypot3sq
         const float f = r * h;                          // -(-r*h)
-        const int g = (int)((double)f * 50.0 + 0.5);    // Util::fast_exp index
+        const int g = (int)__builtin_fma((double)f, 50.0, 0.5);  // Util::fast_exp index: f*50 is exact in double
         return etab[g] * weight;
     }
     return (float)((1.0 - (double)fabsf(dx)) * (1.0 - (double)fabsf(dy)) * (1.0 - (double)fabsf(dz)) *
@@ -750,7 +752,7 @@ hipError_t launch_bin(const float *t3d, const float *wq, int nslices, InsertCons
 // lanes of an access spread over the 32 banks.
 constexpr int ACC_CELLS = BRICK * BRICK * BRICK;
 constexpr int TAG_N = (BRICK + 1) * (BRICK + 1) * (BRICK + 1);  // stencil origins in [lo-1, lo+7]^3
-constexpr int WAVE_LDS_BYTES = 12 * ACC_CELLS + ((TAG_N + 15) & ~15) + 4 * 64 + 16 * 8 + 16 * 64;
+constexpr int WAVE_LDS_BYTES = 12 * ACC_CELLS + ((TAG_N + 15) & ~15) + 4 * 64 + 16 * 8;
 #define SC_DLT 2e-3f   // margin on p: the FP32 position error is < 3e-4 for P <= 1024
 #define SC_EPS 1e-3f   // margin on enumeration bounds
 #define SC_RAD 7.81f   // 4.5 sqrt(3) + margin: |p - brick centre| for an origin in [lo-1, lo+7]^3
@@ -820,7 +822,7 @@ __device__ __forceinline__ void corner_weights8(int mode, int i0, int j0, int k0
             for (int dk = 0; dk < 2; dk++) {
                 const float r = sxy + dz[dk] * dz[dk];       // Util::hypot3sq
                 const float f = r * h;
-                const int g = (int)((double)f * 50.0 + 0.5);  // Util::fast_exp index
+                const int g = (int)__builtin_fma((double)f, 50.0, 0.5);  // Util::fast_exp index: f*50 is exact in double
                 wg[b + 4 * dk] = etab[g] * weight;
             }
         }
@@ -838,11 +840,61 @@ __device__ __forceinline__ void corner_weights8(int mode, int i0, int j0, int k0
     }
 }
 
+// A pixel waiting for its turn to add into the brick accumulator: value,
+// position and weight (the corner weights are recomputed from these in the
+// window that adds it).  Non-pending lanes hold a finite position, so the
+// weight computation they run along with the wave stays inside the tables.
+struct PendPix {
+    float re, im, px, py, pz, pw;
+    bool pend;
+};
+
+// One collision-free pass of the pending pixels of a window: one lane per
+// stencil origin adds its 8 corners; pixels carried over from earlier
+// windows take precedence over the window's new ones (`fresh`), so a
+// carried pixel is added in the next window it takes part in unless it
+// collides with another carried pixel.  The losers stay pending.
+template <int A>
+__device__ __forceinline__ void scatter_pass(PendPix &st, bool fresh, const int lo[3], const InsertConst &c, int lane,
+                                             float *acc, unsigned char *tag, const float *etab)
+{
+    float *acc_re = acc, *acc_im = acc + ACC_CELLS, *acc_w = acc + 2 * ACC_CELLS;
+    const int f0 = (int)floorf(st.px), f1 = (int)floorf(st.py), f2 = (int)floorf(st.pz);
+    const int i0 = f0 - lo[0], j0 = f1 - lo[1], k0 = f2 - lo[2];  // in [-1, BRICK-1] while pending
+    const int ti = st.pend ? (i0 + 1) + (BRICK + 1) * ((j0 + 1) + (BRICK + 1) * (k0 + 1)) : 0;
+    if (st.pend && fresh) tag[ti] = (unsigned char)lane;
+    wave_sync_lds();
+    if (st.pend && !fresh) tag[ti] = (unsigned char)lane;
+    wave_sync_lds();
+    const bool win = st.pend && tag[ti] == (unsigned char)lane;
+    wave_sync_lds();
+    float wg[8];
+    corner_weights8(c.mode, f0, f1, f2, st.px, st.py, st.pz, c.h, st.pw, etab, wg);
+    // corner (di, dj, dk) lies in the brick on each axis: lo side for d = 0, hi side for d = 1
+    const bool in_i[2] = {i0 >= 0, i0 < BRICK - 1}, in_j[2] = {j0 >= 0, j0 < BRICK - 1};
+    const bool in_k[2] = {k0 >= 0, k0 < BRICK - 1};
+    const int cbase = acc_slot<A>(i0, j0, k0);
+    float *const p_re = acc_re + cbase, *const p_im = acc_im + cbase, *const p_w = acc_w + cbase;
+#pragma unroll
+    for (int cI = 0; cI < 8; cI++) {
+        const int di = cI & 1, dj = (cI >> 1) & 1, dk = cI >> 2;
+        if (win && in_i[di] && in_j[dj] && in_k[dk]) {
+            const int a = acc_slot<A>(di, dj, dk);
+            p_re[a] += st.re * wg[cI];
+            p_im[a] += st.im * wg[cI];
+            p_w[a] += wg[cI];
+        }
+        wave_sync_lds();
+    }
+    st.pend = st.pend && !win;
+}
+
 // Up to GROUP slices of an item in one stream of 64-pixel windows: lane
 // (g, r) = (lane / 16, lane % 16) sets up row r of slice g (a stencil origin
 // in [lo-1, lo+7]^3 puts p within 4.5 sqrt(3) of the brick centre, i.e. at
 // most 16 rows), the rows' pixel counts are scanned over the wave, and each
-// window takes the next 64 pixels whatever slice they belong to.
+// window gives the next pixels of the stream, whatever slice they belong to,
+// to the lanes that hold no pending pixel.
 constexpr int GROUP = 4;
 
 template <int A>
@@ -850,11 +902,10 @@ __device__ __forceinline__ void scatter_group(const float2 *__restrict__ slices,
                                               const float *__restrict__ t3d, const float *__restrict__ wq,
                                               const int *__restrict__ list, int first, int ng, const InsertConst &c,
                                               const int lo[3], int lane, float *acc, unsigned char *tag, int *mark,
-                                              float4 *rec, int4 *rowi, const float *etab, const float *rtab)
+                                              float4 *rec, const float *etab, const float *rtab, PendPix &st)
 {
     const int H = c.H, P = c.P;
-    const size_t slice_elems = (size_t)P * (H + 1);
-    float *acc_re = acc, *acc_im = acc + ACC_CELLS, *acc_w = acc + 2 * ACC_CELLS;
+    const unsigned slice_elems = (unsigned)P * (unsigned)(H + 1);
     const int g = lane >> 4, r = lane & 15;
     int q = -1;
     if (g < ng) q = list[first + g];
@@ -865,7 +916,7 @@ __device__ __forceinline__ void scatter_group(const float2 *__restrict__ slices,
     const float wgt = qv ? wq[q] : 0.f;
     if (r == 0) {
         rec[2 * g] = make_float4(mm[0], mm[1], mm[2], mm[3]);
-        rec[2 * g + 1] = make_float4(mm[4], mm[5], wgt, __int_as_float(q));
+        rec[2 * g + 1] = make_float4(mm[4], mm[5], wgt, __int_as_float(qv ? q / nsym : 0));  // particle
     }
     // the pixel nearest the brick centre's projection; bounds are relative to it
     const float cx = (float)lo[0] + 3.5f, cy = (float)lo[1] + 3.5f, cz = (float)lo[2] + 3.5f;
@@ -901,68 +952,50 @@ __device__ __forceinline__ void scatter_group(const float2 *__restrict__ slices,
     const int incl = wave_scan_add(cnt);
     const int total = __builtin_amdgcn_readlane(incl, 63);
     const int excl = incl - cnt;
-    rowi[lane] = make_int4(excl, xs, y, 0);  // read back by the slots of this row
-    for (int base = 0; base < total; base += 64) {
-        // row of slot base+lane: the last non-empty row starting at or before it
+    // row record read by the slots of this row: (x - slot, y) as two int16
+    const int rowrec = (int)(((unsigned)(xs - excl) & 0xffffu) | ((unsigned)y << 16));
+    for (int next = 0; next < total;) {
+        // the lanes without a pending pixel take slots next, next+1, ... in lane order
+        const unsigned long long fm = __ballot(!st.pend);
+        const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(fm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)fm, 0u));
+        // row of slot next+lane: the last non-empty row starting at or before it
         mark[lane] = -1;
         wave_sync_lds();
-        if (cnt > 0 && excl >= base && excl < base + 64) mark[excl - base] = lane;
-        if (cnt > 0 && excl < base && excl + cnt > base) mark[0] = lane;
+        if (cnt > 0 && excl >= next && excl < next + 64) mark[excl - next] = lane;
+        if (cnt > 0 && excl < next && excl + cnt > next) mark[0] = lane;
         wave_sync_lds();
-        int rl = wave_scan_max(mark[lane]);
-        const int s = base + lane;
-        rl &= 63;
-        const int4 ri = rowi[rl];
-        wave_sync_lds();
-        const int rex = ri.x, rxs = ri.y, ry = ri.z;
-        const int x = rxs + (s - rex);
-        bool ok = s < total && (unsigned)x <= (unsigned)H;  // x in [0, P/2] holds by construction
-        const float4 ra = rec[2 * (rl >> 4)], rb = rec[2 * (rl >> 4) + 1];
-        const float pm[6] = {ra.x, ra.y, ra.z, ra.w, rb.x, rb.y};
-        const float pw = rb.z;
-        const int pq = __float_as_int(rb.w);
-        float2 dt = make_float2(0.f, 0.f);
-        float px = 0.f, py = 0.f, pz = 0.f;
-        int f0 = 0, f1 = 0, f2 = 0;
-        if (ok) {
-            dt = slice_value(slices + (size_t)(pq / nsym) * slice_elems, x, ry, P, H);
+        const int rk = wave_scan_max(mark[lane]) & 63;
+        const int rl = __builtin_amdgcn_ds_bpermute(rank << 2, rk);  // row of slot next+rank
+        const int rr = __builtin_amdgcn_ds_bpermute(rl << 2, rowrec);
+        const int s = next + rank;
+        const bool fresh = !st.pend && s < total;
+        if (fresh) {
+            const float4 ra = rec[2 * (rl >> 4)], rb = rec[2 * (rl >> 4) + 1];
+            const int x = (int)(short)(rr & 0xffff) + s, ry = rr >> 16;
+            const float pm[6] = {ra.x, ra.y, ra.z, ra.w, rb.x, rb.y};
+            float px, py, pz;
             pixel_coords(x, ry, pm, rtab, H, c.Pf, px, py, pz);
-            f0 = (int)floorf(px);
-            f1 = (int)floorf(py);
-            f2 = (int)floorf(pz);
-            ok = !(f0 < -H - 1 || f1 < -H - 1 || f2 < -H - 1 || f0 > H || f1 > H || f2 > H);
-            ok = ok && (unsigned)(f0 - lo[0] + 1) < (unsigned)(BRICK + 1) &&
-                 (unsigned)(f1 - lo[1] + 1) < (unsigned)(BRICK + 1) && (unsigned)(f2 - lo[2] + 1) < (unsigned)(BRICK + 1);
-        }
-        float wg[8];
-        corner_weights8(c.mode, f0, f1, f2, px, py, pz, c.h, pw, etab, wg);
-        const int i0 = f0 - lo[0], j0 = f1 - lo[1], k0 = f2 - lo[2];  // in [-1, BRICK-1]
-        // in-brick corners: bit di + 2 dj + 4 dk
-        const unsigned mi = (i0 >= 0 ? 1u : 0u) | (i0 < BRICK - 1 ? 2u : 0u);
-        const unsigned mj = (j0 >= 0 ? 1u : 0u) | (j0 < BRICK - 1 ? 2u : 0u);
-        const unsigned mk = (k0 >= 0 ? 1u : 0u) | (k0 < BRICK - 1 ? 2u : 0u);
-        const int cbase = acc_slot<A>(i0, j0, k0);
-        const int ti = ok ? (i0 + 1) + (BRICK + 1) * ((j0 + 1) + (BRICK + 1) * (k0 + 1)) : 0;
-        bool pend = ok;
-        while (__ballot(pend)) {  // rounds: one lane per stencil origin each
-            if (pend) tag[ti] = (unsigned char)lane;
-            wave_sync_lds();
-            const bool win = pend && tag[ti] == (unsigned char)lane;
-#pragma unroll
-            for (int cI = 0; cI < 8; cI++) {
-                const int di = cI & 1, dj = (cI >> 1) & 1, dk = cI >> 2;
-                if (win && ((mi >> di) & (mj >> dj) & (mk >> dk) & 1u)) {
-                    const int a = cbase + acc_slot<A>(di, dj, dk);
-                    acc_re[a] += dt.x * wg[cI];
-                    acc_im[a] += dt.y * wg[cI];
-                    acc_w[a] += wg[cI];
-                }
-                wave_sync_lds();
+            const int f0 = (int)floorf(px), f1 = (int)floorf(py), f2 = (int)floorf(pz);
+            // x in [0, P/2] holds by construction; an origin in [lo-1, lo+7]^3 lies in
+            // [-P/2-1, P/2+...]: origins past P/2 only reach cells that are never merged
+            if ((unsigned)x <= (unsigned)H && (unsigned)(f0 - lo[0] + 1) < (unsigned)(BRICK + 1) &&
+                (unsigned)(f1 - lo[1] + 1) < (unsigned)(BRICK + 1) &&
+                (unsigned)(f2 - lo[2] + 1) < (unsigned)(BRICK + 1)) {
+                const float2 dt =
+                    slice_value(slices + (unsigned long long)(unsigned)__float_as_int(rb.w) * slice_elems, x, ry, P, H);
+                st.re = dt.x;
+                st.im = dt.y;
+                st.px = px;
+                st.py = py;
+                st.pz = pz;
+                st.pw = rb.z;
+                st.pend = true;
             }
-            pend = pend && !win;
         }
+        wave_sync_lds();  // rec reads before the next group rewrites them
+        next += __popcll(fm);
+        scatter_pass<A>(st, fresh, lo, c, lane, acc, tag, etab);
     }
-    wave_sync_lds();  // the next group rewrites the slice records
 }
 
 // Partial brick layout in HBM: [field re/im/w][z][y][x] of the brick's cells.
@@ -971,11 +1004,18 @@ __device__ __forceinline__ void scatter_item(const float2 *__restrict__ slices, 
                                              const float *__restrict__ t3d, const float *__restrict__ wq,
                                              const InsertConst &c, const int *__restrict__ list, const WorkItem &w,
                                              const int lo[3], int lane, float *acc, unsigned char *tag, int *mark,
-                                             float4 *rec, int4 *rowi, const float *etab, const float *rtab, float *out)
+                                             float4 *rec, const float *etab, const float *rtab, float *out)
 {
+    PendPix st;
+    st.re = st.im = st.pw = 0.f;
+    st.px = (float)lo[0];
+    st.py = (float)lo[1];
+    st.pz = (float)lo[2];
+    st.pend = false;
     for (int s0 = 0; s0 < w.count; s0 += GROUP)
         scatter_group<A>(slices, nslices, nsym, t3d, wq, list, w.start + s0, min(GROUP, w.count - s0), c, lo, lane,
-                         acc, tag, mark, rec, rowi, etab, rtab);
+                         acc, tag, mark, rec, etab, rtab, st);
+    while (__ballot(st.pend)) scatter_pass<A>(st, false, lo, c, lane, acc, tag, etab);
     wave_sync_lds();
 #pragma unroll
     for (int wv = 0; wv < BRICK; wv++) {
@@ -1002,7 +1042,6 @@ __global__ void __launch_bounds__(256) k_scatter_insert(const float2 *__restrict
     unsigned char *tag = reinterpret_cast<unsigned char *>(mine + 12 * ACC_CELLS);
     int *mark = reinterpret_cast<int *>(mine + 12 * ACC_CELLS + ((TAG_N + 15) & ~15));
     float4 *rec = reinterpret_cast<float4 *>(mine + 12 * ACC_CELLS + ((TAG_N + 15) & ~15) + 4 * 64);
-    int4 *rowi = reinterpret_cast<int4 *>(mine + 12 * ACC_CELLS + ((TAG_N + 15) & ~15) + 4 * 64 + 16 * 8);
     float *etab = reinterpret_cast<float *>(base + (size_t)(blockDim.x >> 6) * WAVE_LDS_BYTES);
     float *rtab = etab + EXPN;  // P+2 entries: fl(i/P), i in [-H, H+1]
     for (int i = threadIdx.x; i < EXPN; i += blockDim.x) etab[i] = exptab_g[i];
@@ -1022,9 +1061,9 @@ __global__ void __launch_bounds__(256) k_scatter_insert(const float2 *__restrict
         for (int wv = 0; wv < 3 * BRICK; wv++) acc[lane + 64 * wv] = 0.f;
         wave_sync_lds();
         float *out = bb.part + (size_t)it * PART_FLOATS;
-        if (A == 0) scatter_item<0>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, tag, mark, rec, rowi, etab, rtab, out);
-        else if (A == 1) scatter_item<1>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, tag, mark, rec, rowi, etab, rtab, out);
-        else scatter_item<2>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, tag, mark, rec, rowi, etab, rtab, out);
+        if (A == 0) scatter_item<0>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, tag, mark, rec, etab, rtab, out);
+        else if (A == 1) scatter_item<1>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, tag, mark, rec, etab, rtab, out);
+        else scatter_item<2>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, tag, mark, rec, etab, rtab, out);
     }
 }
 
