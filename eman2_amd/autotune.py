@@ -3,7 +3,7 @@ per work item, persistent workgroups per CU and particles per launch on a short
 HBM-resident synthetic workload and records the fastest setting per
 (pad, symmetry-op count) in tuning_table.json next to this file.
 
-    python -m eman2_amd.autotune --box 256 --pad 384 --sym c1 --particles 8192
+    python -m eman2_amd.autotune --box 256 --pad 384 --sym c1 --particles 50000
 
 ``FourierHipReconstructor`` applies the table entry whose key matches, or the
 nearest pad with the same symmetry-op count, when it is created.
@@ -50,11 +50,11 @@ def main():
     ap.add_argument("--box", type=int, default=256)
     ap.add_argument("--pad", type=int, default=384)
     ap.add_argument("--sym", default="c1")
-    ap.add_argument("--particles", type=int, default=8192)
-    ap.add_argument("--wpb", default="1,2,4")
-    ap.add_argument("--spi", default="16,32,64,128,256")
-    ap.add_argument("--bpc", default="2,4,8")
-    ap.add_argument("--chunks", default="2048,4096")
+    ap.add_argument("--particles", type=int, default=50000)
+    ap.add_argument("--wpb", default="1,2,3,4")
+    ap.add_argument("--spi", default="256,512,1024,2048")
+    ap.add_argument("--bpc", default="8")
+    ap.add_argument("--chunks", default="0,25000", help="0 = automatic (48 GiB of staging)")
     ap.add_argument("--write", action="store_true", help="update tuning_table.json")
     a = ap.parse_args()
     import torch

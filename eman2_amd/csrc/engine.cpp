@@ -212,7 +212,7 @@ struct fh_reconstructor {
     BrickGeom geom{};
     BinBuffers bins{};
     long long bins_cap_slices = 0;
-    fh_tuning tune{4, 64, 8192, 4};
+    fh_tuning tune{2, 1024, 0, 8};
     fh_stats stats{};
     hipEvent_t ev[6] = {};
 };
@@ -279,10 +279,22 @@ static int ensure_chunk(fh_reconstructor *h, int n, int B)
     return FH_OK;
 }
 
+// Particles per batched launch: the tuned value, or (chunk 0) as many as
+// fit in AUTO_CHUNK_BYTES of image staging + slices, within [256, 65536].
+// Larger chunks put more slices into each work item, so fewer partial
+// bricks are written and merged per particle.
+constexpr long long AUTO_CHUNK_BYTES = 48LL << 30;
+static int chunk_for(const fh_tuning &t, int P, int box)
+{
+    if (t.chunk > 0) return t.chunk;
+    const long long per = 8LL * P * (P / 2 + 1) + 4LL * box * box;
+    return (int)std::max(256LL, std::min(65536LL, AUTO_CHUNK_BYTES / per));
+}
+
 static BrickGeom brick_geom(int P)
 {
     BrickGeom g;
-    g.nb = (P + 1 + BRICK - 1) / BRICK;
+    g.nb = (P + 2 + BRICK - 1) / BRICK;  // origins -P/2-1 .. P/2
     g.nbricks = g.nb * g.nb * g.nb;
     g.nkeys = 3 * g.nbricks;
     return g;
@@ -637,8 +649,8 @@ int fh_insert_batch(fh_reconstructor *h, const float *imgs, int n, int box, cons
     // the brick list (at most 5 nb^2 entries per slice) is indexed with int
     const BrickGeom bg = brick_geom(h->P);
     const long long max_slices = std::max(1LL, (long long)INT32_MAX / (5LL * bg.nb * bg.nb) - 1);
-    const int chunk = (int)std::max(1LL, std::min({(long long)h->tune.chunk, (long long)std::max(n, 1),
-                                                   max_slices / h->nsym}));
+    const int chunk = (int)std::max(1LL, std::min({(long long)chunk_for(h->tune, h->P, box),
+                                                   (long long)std::max(n, 1), max_slices / h->nsym}));
     rc = ensure_chunk(h, chunk, box);
     if (rc) return rc;
     rc = ensure_bins(h, (long long)chunk * h->nsym);
@@ -715,7 +727,7 @@ int fh_prep_batch(fh_reconstructor *h, const float *imgs, int n, int box, const 
         return fail(h, FH_ERR_IMAGE_DIM, "ImageDimensionException: particle box must be in [2, size]");
     FH_TRY(h, hipSetDevice(h->device));
     const int P = h->P, H = h->H;
-    const int chunk = std::max(1, std::min(h->tune.chunk, std::max(n, 1)));
+    const int chunk = std::max(1, std::min(chunk_for(h->tune, P, box), std::max(n, 1)));
     int rc = ensure_chunk(h, chunk, box);
     if (rc) return rc;
     for (int c0 = 0; c0 < n; c0 += chunk) {
@@ -860,7 +872,7 @@ int fh_agreement_batch(fh_reconstructor *h, const float *imgs, int n, int box, c
     FH_TRY(h, hipSetDevice(h->device));
     std::vector<float> rot9;
     if ((rc = check_xforms(h, xforms, n, 1, rot9))) return rc;
-    const int chunk = std::max(1, std::min(h->tune.chunk, std::max(n, 1)));
+    const int chunk = std::max(1, std::min(chunk_for(h->tune, h->P, box), std::max(n, 1)));
     if ((rc = ensure_chunk(h, chunk, box))) return rc;
     if (!h->acc8 || h->acc8_cap < chunk) {
         if (h->acc8) (void)hipFree(h->acc8);
@@ -958,8 +970,8 @@ int fh_set_tuning(fh_reconstructor *h, const fh_tuning *t)
     std::lock_guard<std::mutex> lk(h->mu);
     if (t->waves_per_block < 1 || t->waves_per_block > 4)
         return fail(h, FH_ERR_INVALID_PARAM, "waves_per_block must be in [1, 4]");
-    if (t->slices_per_item < 1 || t->chunk < 1 || t->blocks_per_cu < 1)
-        return fail(h, FH_ERR_INVALID_PARAM, "tuning values must be positive");
+    if (t->slices_per_item < 1 || t->chunk < 0 || t->blocks_per_cu < 1)
+        return fail(h, FH_ERR_INVALID_PARAM, "tuning values must be positive (chunk 0: automatic)");
     if (t->slices_per_item != h->tune.slices_per_item) free_bins(h);
     h->tune = *t;
     return FH_OK;
@@ -990,6 +1002,7 @@ int fh_sync(fh_reconstructor *h)
 
 long long fh_footprint_bytes(int P, int nsym, int chunk)
 {
+    if (chunk <= 0) chunk = chunk_for(fh_tuning{1, 1, 0, 1}, P, P);
     const long long v = 12LL * P * P * (P / 2 + 1);
     const long long sl = 8LL * chunk * P * (P / 2 + 1) + 4LL * chunk * P * P;
     const BrickGeom g = brick_geom(P);

@@ -64,11 +64,13 @@ struct BinBuffers {
     long long part_cap;  // work items the partial buffer holds
 };
 
-constexpr int BRICK = 8;                             // brick edge (voxels)
-constexpr int PART_FLOATS = 3 * BRICK * BRICK * BRICK;  // re, im, weight of one brick
+constexpr int BRICK = 8;                           // brick edge, in stencil origins
+constexpr int HALO = BRICK + 1;                    // cells a brick's origins reach per axis
+constexpr int HALO_CELLS = HALO * HALO * HALO;     // cells of one partial brick
+constexpr int PART_FLOATS = 3 * HALO_CELLS;        // re, im, weight of one partial brick
 
 struct BrickGeom {
-    int nb;       // bricks per axis over the unfolded grid [-H, H]
+    int nb;       // bricks per axis over the stencil origins [-H-1, H]
     int nbricks;  // nb^3
     int nkeys;    // 3 * nbricks
 };

@@ -477,12 +477,13 @@ hipError_t launch_agree(const float2 *slices, int nsym, const float *t3d, int ns
 }
 
 // ------------------------------------------------------------ binning
-// Brick b = (bx, by, bz) covers unfolded voxels [-H + BS*b, -H + BS*b + BS);
-// a pixel reaches it when its stencil origin floor(p) lies in
-// [lo-1, lo+BS-1], i.e. p in [lo-1, lo+BS) on every axis.  Pixel positions
-// lie on the plane n.p = 0 (n = u x v of the slice's t3d rows), so per brick
-// column along the dominant normal axis at most 5 bricks qualify.  A slice
-// is binned under key 3 * brick + (its dominant normal axis), so that every
+// Brick b = (bx, by, bz) owns the stencil origins [lo, lo + BS) per axis,
+// lo = -H - 1 + BS*b: a pixel is added by the one brick holding its origin
+// floor(p), i.e. p in [lo, lo+BS) on every axis, and its 8 corners land in
+// the brick's (BS+1)^3 cells [lo, lo+BS]^3.  Pixel positions lie on the
+// plane n.p = 0 (n = u x v of the slice's t3d rows), so per brick column
+// along the dominant normal axis at most 4 bricks qualify.  A slice is
+// binned under key 3 * brick + (its dominant normal axis), so that every
 // work item's slices share one column direction.
 struct SliceFrame {
     double u[3], v[3], n[3];
@@ -509,10 +510,10 @@ __device__ __forceinline__ void slice_frame(const float *m, SliceFrame &f)
 __device__ __forceinline__ bool brick_in_domain(const SliceFrame &f, const int bc[3], int H, int BS, int corners)
 {
     double c[3];
-    for (int d = 0; d < 3; d++) c[d] = -H + BS * bc[d] - 1 + 0.5 * (BS + 1);
+    for (int d = 0; d < 3; d++) c[d] = -H - 1 + BS * bc[d] + 0.5 * BS;
     const double s = c[0] * f.u[0] + c[1] * f.u[1] + c[2] * f.u[2];
     const double t = c[0] * f.v[0] + c[1] * f.v[1] + c[2] * f.v[2];
-    const double rh = 0.8660254037844386 * (BS + 1) + 0.01;
+    const double rh = 0.8660254037844386 * BS + 0.01;
     if (s < -rh) return false;
     if (corners) return s <= H + rh && t >= -H - rh && t <= H - 1 + rh;
     const double rr = sqrt((double)H * H + H) + rh;
@@ -532,8 +533,8 @@ __global__ void __launch_bounds__(256) k_bin(const float *__restrict__ t3d, cons
     slice_frame(t3d + 9 * (size_t)q, f);
     const int c1 = col % nb, c2 = col / nb;
     const double dlt = 1e-3;
-    const double L1 = -H + BS * c1 - 1 - dlt, U1 = -H + BS * c1 + BS + dlt;
-    const double L2 = -H + BS * c2 - 1 - dlt, U2 = -H + BS * c2 + BS + dlt;
+    const double L1 = -H - 1 + BS * c1 - dlt, U1 = -H - 1 + BS * c1 + BS + dlt;
+    const double L2 = -H - 1 + BS * c2 - dlt, U2 = -H - 1 + BS * c2 + BS + dlt;
     const double na = f.n[f.a], n1 = f.n[f.b1], n2 = f.n[f.b2];
     double amin = 1e30, amax = -1e30;
     const double p1s[2] = {L1, U1}, p2s[2] = {L2, U2};
@@ -543,7 +544,7 @@ __global__ void __launch_bounds__(256) k_bin(const float *__restrict__ t3d, cons
             amin = fmin(amin, pa);
             amax = fmax(amax, pa);
         }
-    int lo = (int)floor((amin - dlt + H - BS) / BS) + 1;
+    int lo = (int)floor((amin - dlt + H + 1) / BS);
     int hi = (int)floor((amax + dlt + H + 1) / BS);
     if (lo < 0) lo = 0;
     if (hi > nb - 1) hi = nb - 1;
@@ -735,32 +736,38 @@ hipError_t launch_bin(const float *t3d, const float *wq, int nslices, InsertCons
 }
 
 // ------------------------------------------------------------ brick scatter insertion
-// A work item is (8^3 brick, axis A, up to K slices whose dominant normal
-// axis is A).  Its wave owns a private accumulator of the brick in LDS and,
-// per slice, walks the slice pixels whose 2x2x2 stencil reaches the brick in
-// windows of 64 (one pixel per lane), adding each pixel's in-brick corners
-// with plain LDS read-modify-writes:
+// A work item is (brick of 8^3 stencil origins, axis A, up to K slices whose
+// dominant normal axis is A).  Its wave owns a private accumulator of the
+// brick's 9^3 cells in LDS and walks the slice pixels whose stencil origin
+// lies in the brick in windows of 64 (one pixel per lane), adding each
+// pixel's 8 corners with plain LDS read-modify-writes; every pixel is added
+// by exactly one brick, and the cells on a brick's high faces are summed
+// with the next bricks' in the merge:
 //  - two lanes whose pixels share a stencil origin cell would race on all
-//    eight corners; an LDS tag per origin cell elects one lane per cell and
-//    round, the others retry in the next round (no atomics);
+//    eight corners; an LDS tag per origin cell elects one lane per cell, the
+//    others stay pending for the next window (no atomics);
 //  - two lanes with different origins can still meet on a cell through
 //    different corners, so the corners are processed one after another with
 //    a wave-level ordering point between them.
-// The accumulator is three float arrays (re, im, w) over the brick's cells
-// in [A][B2][B1] order (B1 < B2 the other axes): one corner's cells of a
-// slice with dominant axis A fall in distinct (B1, B2) columns, so the
-// lanes of an access spread over the 32 banks.
-constexpr int ACC_CELLS = BRICK * BRICK * BRICK;
-constexpr int TAG_N = (BRICK + 1) * (BRICK + 1) * (BRICK + 1);  // stencil origins in [lo-1, lo+7]^3
-constexpr int WAVE_LDS_BYTES = 12 * ACC_CELLS + ((TAG_N + 15) & ~15) + 4 * 64 + 16 * 8;
+// The accumulator is three float arrays (re, im, w) over the cells in
+// [A][B2][B1] order (B1 < B2 the other axes): one corner's cells of a slice
+// with dominant axis A fall in distinct (B1, B2) columns, so the lanes of an
+// access spread over the 32 banks.
+constexpr int ACC_N = HALO_CELLS;                      // cells per field
+constexpr int TAG_N = BRICK * BRICK * BRICK;           // stencil origins in [lo, lo+7]^3
+constexpr int ACC_BYTES = (12 * ACC_N + 15) & ~15;     // re, im, w
+constexpr int REC_OFF = ACC_BYTES;                     // slice records: 2 float4 per slice of a group
+constexpr int MARK_OFF = REC_OFF + 16 * 8;             // 64 row marks
+constexpr int TAG_OFF = MARK_OFF + 4 * 64;             // one byte per origin
+constexpr int WAVE_LDS_BYTES = TAG_OFF + TAG_N;
 #define SC_DLT 2e-3f   // margin on p: the FP32 position error is < 3e-4 for P <= 1024
 #define SC_EPS 1e-3f   // margin on enumeration bounds
-#define SC_RAD 7.81f   // 4.5 sqrt(3) + margin: |p - brick centre| for an origin in [lo-1, lo+7]^3
+#define SC_RAD 6.95f   // 4 sqrt(3) + margin: |p - brick centre| for an origin in [lo, lo+7]^3
 
 template <int A>
 __device__ __forceinline__ int acc_slot(int i, int j, int k)
 {
-    return A == 0 ? i * 64 + k * 8 + j : (A == 1 ? j * 64 + k * 8 + i : k * 64 + j * 8 + i);
+    return A == 0 ? (i * HALO + k) * HALO + j : (A == 1 ? (j * HALO + k) * HALO + i : (k * HALO + j) * HALO + i);
 }
 
 __device__ __forceinline__ void wave_sync_lds()
@@ -858,10 +865,9 @@ template <int A>
 __device__ __forceinline__ void scatter_pass(PendPix &st, bool fresh, const int lo[3], const InsertConst &c, int lane,
                                              float *acc, unsigned char *tag, const float *etab)
 {
-    float *acc_re = acc, *acc_im = acc + ACC_CELLS, *acc_w = acc + 2 * ACC_CELLS;
     const int f0 = (int)floorf(st.px), f1 = (int)floorf(st.py), f2 = (int)floorf(st.pz);
-    const int i0 = f0 - lo[0], j0 = f1 - lo[1], k0 = f2 - lo[2];  // in [-1, BRICK-1] while pending
-    const int ti = st.pend ? (i0 + 1) + (BRICK + 1) * ((j0 + 1) + (BRICK + 1) * (k0 + 1)) : 0;
+    const int i0 = f0 - lo[0], j0 = f1 - lo[1], k0 = f2 - lo[2];  // in [0, BRICK-1] while pending
+    const int ti = st.pend ? i0 + BRICK * (j0 + BRICK * k0) : 0;
     if (st.pend && fresh) tag[ti] = (unsigned char)lane;
     wave_sync_lds();
     if (st.pend && !fresh) tag[ti] = (unsigned char)lane;
@@ -870,15 +876,12 @@ __device__ __forceinline__ void scatter_pass(PendPix &st, bool fresh, const int 
     wave_sync_lds();
     float wg[8];
     corner_weights8(c.mode, f0, f1, f2, st.px, st.py, st.pz, c.h, st.pw, etab, wg);
-    // corner (di, dj, dk) lies in the brick on each axis: lo side for d = 0, hi side for d = 1
-    const bool in_i[2] = {i0 >= 0, i0 < BRICK - 1}, in_j[2] = {j0 >= 0, j0 < BRICK - 1};
-    const bool in_k[2] = {k0 >= 0, k0 < BRICK - 1};
     const int cbase = acc_slot<A>(i0, j0, k0);
-    float *const p_re = acc_re + cbase, *const p_im = acc_im + cbase, *const p_w = acc_w + cbase;
+    float *const p_re = acc + cbase, *const p_im = acc + ACC_N + cbase, *const p_w = acc + 2 * ACC_N + cbase;
 #pragma unroll
     for (int cI = 0; cI < 8; cI++) {
         const int di = cI & 1, dj = (cI >> 1) & 1, dk = cI >> 2;
-        if (win && in_i[di] && in_j[dj] && in_k[dk]) {
+        if (win) {
             const int a = acc_slot<A>(di, dj, dk);
             p_re[a] += st.re * wg[cI];
             p_im[a] += st.im * wg[cI];
@@ -891,8 +894,8 @@ __device__ __forceinline__ void scatter_pass(PendPix &st, bool fresh, const int 
 
 // Up to GROUP slices of an item in one stream of 64-pixel windows: lane
 // (g, r) = (lane / 16, lane % 16) sets up row r of slice g (a stencil origin
-// in [lo-1, lo+7]^3 puts p within 4.5 sqrt(3) of the brick centre, i.e. at
-// most 16 rows), the rows' pixel counts are scanned over the wave, and each
+// in [lo, lo+7]^3 puts p within 4 sqrt(3) of the brick centre, i.e. at most
+// 14 rows), the rows' pixel counts are scanned over the wave, and each
 // window gives the next pixels of the stream, whatever slice they belong to,
 // to the lanes that hold no pending pixel.
 constexpr int GROUP = 4;
@@ -919,13 +922,13 @@ __device__ __forceinline__ void scatter_group(const float2 *__restrict__ slices,
         rec[2 * g + 1] = make_float4(mm[4], mm[5], wgt, __int_as_float(qv ? q / nsym : 0));  // particle
     }
     // the pixel nearest the brick centre's projection; bounds are relative to it
-    const float cx = (float)lo[0] + 3.5f, cy = (float)lo[1] + 3.5f, cz = (float)lo[2] + 3.5f;
+    const float cx = (float)lo[0] + 4.0f, cy = (float)lo[1] + 4.0f, cz = (float)lo[2] + 4.0f;
     const float sc = cx * mm[0] + cy * mm[1] + cz * mm[2];
     const float tc = cx * mm[3] + cy * mm[4] + cz * mm[5];
     const int xc0 = (int)rintf(sc), yc0 = (int)rintf(tc);
     const int dya = (int)ceilf(tc - (float)yc0 - SC_RAD), dyb = (int)floorf(tc - (float)yc0 + SC_RAD);
     const int dxa = (int)ceilf(sc - (float)xc0 - SC_RAD), dxb = (int)floorf(sc - (float)xc0 + SC_RAD);
-    // row dy: x range where p[d] in [lo-1, lo+8) (+- margin) on every axis
+    // row dy: x range where p[d] in [lo, lo+8) (+- margin) on every axis
     int cnt = 0, xs = 0;
     const int y = yc0 + dya + r;
     if (qv && dya + r <= dyb && y >= -H && y <= H - 1) {
@@ -937,7 +940,7 @@ __device__ __forceinline__ void scatter_group(const float2 *__restrict__ slices,
             if (fabsf(ud) >= 0.05f) {
                 const float P0 = (float)xc0 * ud + (float)yc0 * vd;
                 const float iu = __builtin_amdgcn_rcpf(ud);
-                const float t0 = ((float)(lo[d] - 1) - P0 - SC_DLT - dyf * vd) * iu;
+                const float t0 = ((float)lo[d] - P0 - SC_DLT - dyf * vd) * iu;
                 const float t1 = ((float)(lo[d] + BRICK) - P0 + SC_DLT - dyf * vd) * iu;
                 xl = fmaxf(xl, fminf(t0, t1));
                 xh = fminf(xh, fmaxf(t0, t1));
@@ -976,11 +979,10 @@ __device__ __forceinline__ void scatter_group(const float2 *__restrict__ slices,
             float px, py, pz;
             pixel_coords(x, ry, pm, rtab, H, c.Pf, px, py, pz);
             const int f0 = (int)floorf(px), f1 = (int)floorf(py), f2 = (int)floorf(pz);
-            // x in [0, P/2] holds by construction; an origin in [lo-1, lo+7]^3 lies in
-            // [-P/2-1, P/2+...]: origins past P/2 only reach cells that are never merged
-            if ((unsigned)x <= (unsigned)H && (unsigned)(f0 - lo[0] + 1) < (unsigned)(BRICK + 1) &&
-                (unsigned)(f1 - lo[1] + 1) < (unsigned)(BRICK + 1) &&
-                (unsigned)(f2 - lo[2] + 1) < (unsigned)(BRICK + 1)) {
+            // x in [0, P/2] holds by construction; origins lie in [-P/2-1, ...): the
+            // ones past P/2 only reach cells that are never merged
+            if ((unsigned)x <= (unsigned)H && (unsigned)(f0 - lo[0]) < (unsigned)BRICK &&
+                (unsigned)(f1 - lo[1]) < (unsigned)BRICK && (unsigned)(f2 - lo[2]) < (unsigned)BRICK) {
                 const float2 dt =
                     slice_value(slices + (unsigned long long)(unsigned)__float_as_int(rb.w) * slice_elems, x, ry, P, H);
                 st.re = dt.x;
@@ -998,7 +1000,7 @@ __device__ __forceinline__ void scatter_group(const float2 *__restrict__ slices,
     }
 }
 
-// Partial brick layout in HBM: [field re/im/w][z][y][x] of the brick's cells.
+// Partial brick layout in HBM: [field re/im/w][z][y][x] of the brick's 9^3 cells.
 template <int A>
 __device__ __forceinline__ void scatter_item(const float2 *__restrict__ slices, int nslices, int nsym,
                                              const float *__restrict__ t3d, const float *__restrict__ wq,
@@ -1018,12 +1020,15 @@ __device__ __forceinline__ void scatter_item(const float2 *__restrict__ slices, 
     while (__ballot(st.pend)) scatter_pass<A>(st, false, lo, c, lane, acc, tag, etab);
     wave_sync_lds();
 #pragma unroll
-    for (int wv = 0; wv < BRICK; wv++) {
+    for (int wv = 0; wv < (ACC_N + 63) / 64; wv++) {
         const int cc = lane + 64 * wv;
-        const int a = acc_slot<A>(cc & 7, (cc >> 3) & 7, cc >> 6);
-        out[cc] = acc[a];
-        out[ACC_CELLS + cc] = acc[ACC_CELLS + a];
-        out[2 * ACC_CELLS + cc] = acc[2 * ACC_CELLS + a];
+        if (cc < ACC_N) {
+            const int z = cc / (HALO * HALO), yx = cc - z * HALO * HALO, y = yx / HALO, x = yx - y * HALO;
+            const int a = acc_slot<A>(x, y, z);
+            out[cc] = acc[a];
+            out[ACC_N + cc] = acc[ACC_N + a];
+            out[2 * ACC_N + cc] = acc[2 * ACC_N + a];
+        }
     }
     wave_sync_lds();
 }
@@ -1039,9 +1044,9 @@ __global__ void __launch_bounds__(256) k_scatter_insert(const float2 *__restrict
     const int lane = threadIdx.x & 63;
     char *mine = base + (size_t)(threadIdx.x >> 6) * WAVE_LDS_BYTES;
     float *acc = reinterpret_cast<float *>(mine);
-    unsigned char *tag = reinterpret_cast<unsigned char *>(mine + 12 * ACC_CELLS);
-    int *mark = reinterpret_cast<int *>(mine + 12 * ACC_CELLS + ((TAG_N + 15) & ~15));
-    float4 *rec = reinterpret_cast<float4 *>(mine + 12 * ACC_CELLS + ((TAG_N + 15) & ~15) + 4 * 64);
+    float4 *rec = reinterpret_cast<float4 *>(mine + REC_OFF);
+    int *mark = reinterpret_cast<int *>(mine + MARK_OFF);
+    unsigned char *tag = reinterpret_cast<unsigned char *>(mine + TAG_OFF);
     float *etab = reinterpret_cast<float *>(base + (size_t)(blockDim.x >> 6) * WAVE_LDS_BYTES);
     float *rtab = etab + EXPN;  // P+2 entries: fl(i/P), i in [-H, H+1]
     for (int i = threadIdx.x; i < EXPN; i += blockDim.x) etab[i] = exptab_g[i];
@@ -1056,9 +1061,11 @@ __global__ void __launch_bounds__(256) k_scatter_insert(const float2 *__restrict
         if (it >= total) break;
         const WorkItem w = bb.items[it];
         const int brick = w.key / 3, A = w.key - 3 * brick;
-        const int lo[3] = {-H + BRICK * (brick % nb), -H + BRICK * ((brick / nb) % nb), -H + BRICK * (brick / (nb * nb))};
+        const int lo[3] = {-H - 1 + BRICK * (brick % nb), -H - 1 + BRICK * ((brick / nb) % nb),
+                           -H - 1 + BRICK * (brick / (nb * nb))};
 #pragma unroll
-        for (int wv = 0; wv < 3 * BRICK; wv++) acc[lane + 64 * wv] = 0.f;
+        for (int wv = 0; wv < (3 * ACC_N + 63) / 64; wv++)
+            if (lane + 64 * wv < 3 * ACC_N) acc[lane + 64 * wv] = 0.f;
         wave_sync_lds();
         float *out = bb.part + (size_t)it * PART_FLOATS;
         if (A == 0) scatter_item<0>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, tag, mark, rec, etab, rtab, out);
@@ -1083,22 +1090,29 @@ hipError_t launch_scatter_insert(const float2 *slices, int nslices, int nsym, co
 }
 
 // ------------------------------------------------------------ merge of partial bricks
-// Sum over the work items of the brick holding unfolded cell (i, j, k).
-__device__ __forceinline__ bool partial_sum(const float *__restrict__ part, const int *__restrict__ item_off, int nb,
+// Sum over the work items of the bricks whose cells include unfolded cell
+// (i, j, k): the brick owning origin (i, j, k), and on each axis where the
+// cell is a brick's first, also the previous brick (its cell BRICK there).
+__device__ __forceinline__ void partial_sum(const float *__restrict__ part, const int *__restrict__ item_off, int nb,
                                             int H, int i, int j, int k, float &re, float &im, float &w)
 {
-    const int ui = i + H, uj = j + H, uk = k + H;
-    const int b = ((uk >> 3) * nb + (uj >> 3)) * nb + (ui >> 3);
-    const int cc = ((uk & 7) << 6) | ((uj & 7) << 3) | (ui & 7);
-    const int i0 = item_off[3 * b], i1 = item_off[3 * b + 3];
+    const int ui = i + H + 1, uj = j + H + 1, uk = k + H + 1;  // >= 1 for cells >= -H
+    const int bi = ui / BRICK, bj = uj / BRICK, bk = uk / BRICK;
+    const int li = ui - BRICK * bi, lj = uj - BRICK * bj, lk = uk - BRICK * bk;
     re = im = w = 0.f;
-    for (int it = i0; it < i1; it++) {
-        const float *p = part + (size_t)it * PART_FLOATS + cc;
-        re += p[0];
-        im += p[512];
-        w += p[1024];
-    }
-    return i1 > i0;
+    for (int ek = 0; ek <= (lk == 0 ? 1 : 0); ek++)
+        for (int ej = 0; ej <= (lj == 0 ? 1 : 0); ej++)
+            for (int ei = 0; ei <= (li == 0 ? 1 : 0); ei++) {
+                const int b = ((bk - ek) * nb + (bj - ej)) * nb + (bi - ei);
+                const int cc = ((ek ? BRICK : lk) * HALO + (ej ? BRICK : lj)) * HALO + (ei ? BRICK : li);
+                const int i0 = item_off[3 * b], i1 = item_off[3 * b + 3];
+                for (int it = i0; it < i1; it++) {
+                    const float *p = part + (size_t)it * PART_FLOATS + cc;
+                    re += p[0];
+                    im += p[HALO_CELLS];
+                    w += p[2 * HALO_CELLS];
+                }
+            }
 }
 
 // Storage voxel (xs, ys, zs) of the half volume receives, per

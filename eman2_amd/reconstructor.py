@@ -346,7 +346,7 @@ class FourierHipReconstructor:
         return data, norm
 
     # ---------------------------------------------------------------- tuning
-    def set_tuning(self, waves_per_block=4, slices_per_item=64, chunk=8192, blocks_per_cu=4):
+    def set_tuning(self, waves_per_block=2, slices_per_item=1024, chunk=0, blocks_per_cu=8):
         t = _lib.Tuning(int(waves_per_block), int(slices_per_item), int(chunk), int(blocks_per_cu))
         self._check(self._lib.fh_set_tuning(self._need(), ctypes.byref(t)))
 

@@ -72,19 +72,20 @@ typedef struct {
 } fh_ctf;
 
 /* Tuning knobs of the batched insertion (picked by the autotuner).  The
- * unfolded frequency grid is cut into 8^3 bricks; a work item is one brick
- * and up to slices_per_item slices sharing a dominant normal axis. */
+ * stencil origins of the unfolded frequency grid are cut into 8^3 bricks; a
+ * work item is one brick and up to slices_per_item slices sharing a dominant
+ * normal axis. */
 typedef struct {
-    int waves_per_block; /* independent item workers per workgroup: 1, 2, 4 or 8 */
-    int slices_per_item; /* slices a work item gathers before writing its partial brick */
-    int chunk;           /* particles per batched launch */
+    int waves_per_block; /* independent item workers per workgroup: 1 to 4 */
+    int slices_per_item; /* slices a work item adds before writing its partial brick */
+    int chunk;           /* particles per batched launch; 0 = as many as fit in 48 GiB of staging */
     int blocks_per_cu;   /* persistent grid: workgroups per compute unit */
 } fh_tuning;
 
 /* Timings and counters of the last batched call (device time, ms). */
 typedef struct {
     double prep_ms, bin_ms, insert_ms, total_ms;
-    double insert_kernel_ms;     /* the column-gather kernel alone */
+    double insert_kernel_ms;     /* the brick scatter kernel alone */
     double merge_ms;             /* the partial-brick merge into the volume */
     long long slices, pairs, items, chunks;
 } fh_stats;
@@ -173,7 +174,8 @@ int fh_get_tuning(const fh_reconstructor *h, fh_tuning *t);
 int fh_get_stats(const fh_reconstructor *h, fh_stats *s);
 /* wait for all queued device work of this handle */
 int fh_sync(fh_reconstructor *h);
-/* device bytes this handle holds / would hold after setup for size P */
+/* device bytes this handle holds / would hold after setup for size P
+ * (chunk 0: the automatic chunk for particles of box P) */
 long long fh_footprint_bytes(int P, int nsym, int chunk);
 
 /* ------------------------------------------------------------------------

@@ -1,24 +1,25 @@
 """CPU emulation of the brick scatter insertion (k_scatter_insert in
 insert_kernels.hip): for every (slice, brick) pair the binning lists, the
 kernel's float32 row / x-range enumeration must include every pixel whose
-stencil origin lies in [lo-1, lo+7]^3, so that each (pixel, in-bounds
-stencil cell) contribution of the reference loop
-(libEM/reconstructor.cpp:1166-1209) is added by exactly one brick, and each
-pixel is enumerated at most once per brick.  Restates the kernel's index
-arithmetic in Python.  (CPU only.)"""
+stencil origin lies in the brick's [lo, lo+7]^3, so that each (pixel,
+in-bounds stencil cell) contribution of the reference loop
+(libEM/reconstructor.cpp:1166-1209) is added exactly once, by the brick
+owning the pixel's origin (cells [lo, lo+8]^3, the high faces shared with
+the next bricks in the merge), and each pixel is enumerated at most once per
+brick.  Restates the kernel's index arithmetic in Python.  (CPU only.)"""
 import math
 
 import numpy as np
 import pytest
 
 from eman2_amd.transform import euler_eman_matrix
-from test_binning import bricks_for_slice
+from test_binning import brick_count, bricks_for_slice
 
 f32 = np.float32
 BS = 8
 DLT = f32(2e-3)
 EPS = f32(1e-3)
-RAD = f32(7.81)
+RAD = f32(6.95)
 
 
 ANGLES = [(0.0, 0.0, 0.0), (90.0, 90.0, 0.0), (0.0, 90.0, 90.0), (45.0, 45.0, 45.0), (35.26, 45.0, 0.0),
@@ -56,7 +57,7 @@ def reference_contributions(m, P, corners):
 def brick_pixels(m, lo, H, corners):
     """The pixels k_scatter_insert visits for one (slice, brick) pair."""
     mm = [f32(m[0, 0]), f32(m[0, 1]), f32(m[0, 2]), f32(m[1, 0]), f32(m[1, 1]), f32(m[1, 2])]
-    cx, cy, cz = f32(lo[0] + 3.5), f32(lo[1] + 3.5), f32(lo[2] + 3.5)
+    cx, cy, cz = f32(lo[0] + 4.0), f32(lo[1] + 4.0), f32(lo[2] + 4.0)
     sc = f32(f32(f32(cx * mm[0]) + f32(cy * mm[1])) + f32(cz * mm[2]))
     tc = f32(f32(f32(cx * mm[3]) + f32(cy * mm[4])) + f32(cz * mm[5]))
     xc0, yc0 = int(np.rint(sc)), int(np.rint(tc))
@@ -76,7 +77,7 @@ def brick_pixels(m, lo, H, corners):
             if abs(ud) >= 0.05:
                 P0 = f32(f32(f32(xc0) * ud) + f32(f32(yc0) * vd))
                 iu = f32(f32(1.0) / ud)
-                t0 = f32(f32(f32(f32(f32(lo[d] - 1) - P0) - DLT) - f32(dyf * vd)) * iu)
+                t0 = f32(f32(f32(f32(f32(lo[d]) - P0) - DLT) - f32(dyf * vd)) * iu)
                 t1 = f32(f32(f32(f32(f32(lo[d] + BS) - P0) + DLT) - f32(dyf * vd)) * iu)
                 xl, xh = max(xl, min(t0, t1)), min(xh, max(t0, t1))
         a = max(math.ceil(f32(xl - EPS)), -xc0)
@@ -89,10 +90,10 @@ def brick_pixels(m, lo, H, corners):
 
 def scattered(m, P, corners):
     H = P // 2
-    nb = (P + 1 + BS - 1) // BS
+    nb = brick_count(P, BS)
     got = {}
     for bc in bricks_for_slice(m, H, BS, nb, corners):
-        lo = [-H + BS * b for b in bc]
+        lo = [-H - 1 + BS * b for b in bc]
         pix = brick_pixels(m, lo, H, corners)
         assert len(pix) == len(set(pix))
         for x, y in pix:
@@ -100,14 +101,14 @@ def scattered(m, P, corners):
             f = [int(math.floor(q)) for q in c]
             if min(f) < -H - 1 or max(f) > H:
                 continue
-            if not all(lo[d] - 1 <= f[d] <= lo[d] + BS - 1 for d in range(3)):
-                continue
+            if not all(lo[d] <= f[d] <= lo[d] + BS - 1 for d in range(3)):
+                continue  # origin owned by another brick
             for dk in (0, 1):
                 for dj in (0, 1):
                     for di in (0, 1):
                         cell = (f[0] + di, f[1] + dj, f[2] + dk)
-                        if not all(lo[d] <= cell[d] < lo[d] + BS for d in range(3)) or max(cell) > H:
-                            continue  # other bricks' cells; cells past P/2 are never merged
+                        if max(abs(q) for q in cell) > H:
+                            continue  # cells past P/2 are never merged
                         key = ((x, y), cell)
                         got[key] = got.get(key, 0) + 1
     return got
