@@ -181,8 +181,15 @@ __global__ void __launch_bounds__(256) k_prep_cols(int B, int P, int off, float2
     float2 *A = tw + P;
     float2 *Bf = A + T * P;
     const int H = P / 2;
-    const int p = blockIdx.y;
-    const int x0 = blockIdx.x * T;
+    // XCD-aware order: the hardware deals consecutive blocks round-robin to
+    // the 8 XCDs (separate L2s); blocks k, k+8, k+16, ... of one XCD take
+    // consecutive (column group, particle) tiles, so the column groups that
+    // share a cache line meet in the same L2.
+    unsigned lin = blockIdx.x + gridDim.x * blockIdx.y;
+    const unsigned per = (gridDim.x * gridDim.y) / 8;
+    if (lin < 8 * per) lin = (lin % 8) * per + lin / 8;
+    const int p = (int)(lin / gridDim.x);
+    const int x0 = (int)(lin % gridDim.x) * T;
     load_twiddles(tw, twg, P);
     float2 *sl = slices + (size_t)p * P * (H + 1);
     for (int idx = threadIdx.x; idx < T * P; idx += blockDim.x) {
