@@ -139,11 +139,19 @@ def cpu_baseline(imgs_host, mats, dfs, P, sym, mode, apix, budget_s):
     n = int(min(len(imgs_host), max(k2, budget_s / per)))
     n = max(T, (n // T) * T)
     t_prep, t_ins = run(n)
+    # one thread: prep and insertion of a small sample in this thread
+    k1 = min(len(imgs_host), 16)
+    t0 = time.perf_counter()
+    sl = np.stack([prep(i) for i in range(k1)])
+    tp1 = time.perf_counter() - t0
+    _, _, ti1 = O.insert_batch_mt(P, sl, mats[:k1], np.ones(k1, np.float32), sym, mode, nthreads=1)
     return {
         "value": n / (t_prep + t_ins), "unit": "particles/s", "cores": T, "kind": "port",
         "sample": "%d particles of the same workload (prep %.2fs + insertion %.2fs, %d threads, "
                   "oracle -O3 -ffast-math, private volumes summed)" % (n, t_prep, t_ins, T),
         "prep_s": t_prep, "insert_s": t_ins, "insert_only_particles_per_s": n / t_ins,
+        "single_thread": {"value": k1 / (tp1 + ti1), "unit": "particles/s", "sample": k1,
+                          "prep_s": tp1, "insert_s": ti1},
     }
 
 
@@ -227,6 +235,13 @@ def main():
     pipe_bytes = (4 * B * B + bytes_per_particle) * a.steps * n
     pipeline = {"effective_GBs": round(pipe_bytes / elapsed / 1e9, 1),
                 "frac": round(pipe_bytes / elapsed / 1e9 / HBM_PEAK_GBS, 4)}
+    # strict compulsory traffic: the raw particle read once per particle, and
+    # finish() (normalise, 3-pass 3D FFT, real output) once per job
+    H = P // 2
+    finish_bytes = 20 * (H + 1) * P * P + 3 * 16 * (H + 1) * P * P + 4 * P ** 3
+    bound = HBM_PEAK_GBS * 1e9 / (4 * B * B)
+    compulsory = {"bytes_per_particle": 4 * B * B, "finish_bytes_per_job": finish_bytes,
+                  "particles_per_s_at_peak": round(bound, 1), "frac": round(value / bound, 4)}
 
     out = {
         "metric": "particles inserted/s @256³ c1 gauss_2 (batched vs plain); p50/p99 job latency",
@@ -238,6 +253,7 @@ def main():
             "tuning": r.get_tuning()},
         "roofline": roof,
         "pipeline_roofline": pipeline,
+        "compulsory_bound": compulsory,
         "job_latency_ms": {"p50": round(statistics.median(lat), 2),
                            "p99": round(sorted(lat)[min(len(lat) - 1, int(math.ceil(0.99 * len(lat))) - 1)], 2),
                            "what": "setup + insert_batch of all particles, per step"},
