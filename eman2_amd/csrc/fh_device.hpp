@@ -241,6 +241,80 @@ __device__ float2 *fft_lds(float2 *a, float2 *b, const FftPlan &plan, const floa
     return a;
 }
 
+// ---------------------------------------------------------------------------
+// Compile-time plans for the common pads: the radices make_plan picks (8, 6,
+// 4, 2, 3, 5, greedily), so a fixed-size transform computes exactly what the
+// runtime plan computes, with the strides, divisions and loop bounds folded
+// into constants.
+// ---------------------------------------------------------------------------
+constexpr int plan_radix(int N, int p)
+{
+    const int order[6] = {8, 6, 4, 2, 3, 5};
+    int m = N, k = 0;
+    for (int oi = 0; oi < 6; oi++)
+        while (m % order[oi] == 0) {
+            if (k == p) return order[oi];
+            k++;
+            m /= order[oi];
+        }
+    return 0;
+}
+constexpr int plan_ns(int N, int p)
+{
+    int ns = 1;
+    for (int i = 0; i < p; i++) ns *= plan_radix(N, i);
+    return ns;
+}
+
+// In-place pass of a fixed plan (256 threads): reads and transforms every
+// butterfly of the thread into registers, one barrier, writes them back.
+template <int R, int T, int N, int NS>
+__device__ __forceinline__ void stockham_pass_fixed(float2 *buf, const float2 *__restrict__ tw)
+{
+    constexpr int NR = N / R, TOTAL = NR * T, G = (TOTAL + 255) / 256, TWM = N / (NS * R);
+    float2 v[G][R];
+    int dsti[G];
+#pragma unroll
+    for (int gi = 0; gi < G; gi++) {
+        const int b = threadIdx.x + gi * 256;
+        dsti[gi] = -1;
+        if (TOTAL % 256 == 0 || b < TOTAL) {
+            const int t = b % T;
+            const int j = b / T;
+            const int q = j / NS;
+            const int k = j - q * NS;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                float2 a = buf[(j + r * NR) * T + t];
+                if (r > 0) a = cmul(a, tw[r * k * TWM]);
+                v[gi][r] = a;
+            }
+            dft<R>(v[gi]);
+            dsti[gi] = (q * NS * R + k) * T + t;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int gi = 0; gi < G; gi++) {
+        if (dsti[gi] >= 0) {
+#pragma unroll
+            for (int r = 0; r < R; r++) buf[dsti[gi] + r * NS * T] = v[gi][r];
+        }
+    }
+    __syncthreads();
+}
+
+// Full forward transform of T interleaved length-N transforms in place
+// (fixed plan, 256 threads).  Ends with a barrier.
+template <int T, int N, int P = 0>
+__device__ __forceinline__ void fft_fixed(float2 *buf, const float2 *tw)
+{
+    if constexpr (plan_radix(N, P) != 0) {
+        stockham_pass_fixed<plan_radix(N, P), T, N, plan_ns(N, P)>(buf, tw);
+        fft_fixed<T, N, P + 1>(buf, tw);
+    }
+}
+
 // In place when the tile fits the register bound of fft_lds_inplace, else
 // ping-pong through b (the launch sizes the LDS accordingly).
 template <int T>

@@ -82,11 +82,13 @@ hipError_t launch_edge_mean(const float *imgs, int n, int B, float *means, hipSt
 // pairwise into T complex transforms z = a + i c (two-for-one real FFT).
 // Input element xs of padded row yp is the phase-origin shifted
 // (ifftshift) padded image; output row ys = (yp + P/2) mod P.
-template <int T>
-__global__ void __launch_bounds__(256) k_prep_rows(const float *__restrict__ imgs, int B, int P, int off,
+// NF: the pad as a compile-time constant (0: the runtime plan and P).
+template <int T, int NF>
+__global__ void __launch_bounds__(256) k_prep_rows(const float *__restrict__ imgs, int B, int Pr, int off,
                                                    const float *__restrict__ means, float2 *__restrict__ slices,
                                                    FftPlan plan, const float2 *__restrict__ twg)
 {
+    const int P = NF ? NF : Pr;
     extern __shared__ float2 smem[];
     float2 *tw = smem;
     float2 *A = tw + P;
@@ -111,7 +113,9 @@ __global__ void __launch_bounds__(256) k_prep_rows(const float *__restrict__ img
         A[xs * T + t] = make_float2(re, im);
     }
     __syncthreads();
-    float2 *R = fft_tile<T>(A, Bf, plan, tw);
+    float2 *R = A;
+    if constexpr (NF != 0) fft_fixed<T, NF>(A, tw);
+    else R = fft_tile<T>(A, Bf, plan, tw);
     float2 *out = slices + (size_t)p * P * (H + 1);
     for (int idx = threadIdx.x; idx < T * (H + 1); idx += blockDim.x) {
         const int t = idx % T;
@@ -122,15 +126,18 @@ __global__ void __launch_bounds__(256) k_prep_rows(const float *__restrict__ img
         const float2 z2 = R[(k == 0 ? 0 : P - k) * T + t];
         const float2 a = make_float2(0.5f * (z1.x + z2.x), 0.5f * (z1.y - z2.y));
         const float2 c = make_float2(0.5f * (z1.y + z2.y), -0.5f * (z1.x - z2.x));
-        int ys0 = r0 + off + H;
-        ys0 = ys0 % P;
+        int ys0 = r0 + off + H;  // < 2P
+        if (ys0 >= P) ys0 -= P;
         out[(size_t)ys0 * (H + 1) + k] = a;
         if (r0 + 1 < B) {
-            const int ys1 = (ys0 + 1) % P;
+            const int ys1 = ys0 + 1 == P ? 0 : ys0 + 1;
             out[(size_t)ys1 * (H + 1) + k] = c;
         }
     }
 }
+
+// Pads with a compile-time plan; others take the runtime plan.
+#define FH_FIXED_PADS(X) X(8, 96) X(8, 192) X(8, 288) X(8, 384) X(4, 480) X(4, 600)
 
 hipError_t launch_prep_rows(const float *imgs, int n, int B, int P, const float *means, float2 *slices,
                             const FftPlan &plan, const float2 *tw, hipStream_t s)
@@ -143,10 +150,17 @@ hipError_t launch_prep_rows(const float *imgs, int n, int B, int P, const float 
     const int T = pick_T(P);
     const size_t lds = fft_lds_bytes(P, T);
     dim3 grid((B + 2 * T - 1) / (2 * T), n);
+#define FH_ROWS_CASE(TT, NN)                                                                                    \
+    if (T == TT && P == NN) {                                                                                   \
+        hipLaunchKernelGGL((k_prep_rows<TT, NN>), grid, dim3(256), lds, s, imgs, B, P, off, means, slices, plan, tw); \
+        return hipGetLastError();                                                                               \
+    }
+    FH_FIXED_PADS(FH_ROWS_CASE)
+#undef FH_ROWS_CASE
     if (T == 8)
-        hipLaunchKernelGGL(k_prep_rows<8>, grid, dim3(256), lds, s, imgs, B, P, off, means, slices, plan, tw);
+        hipLaunchKernelGGL((k_prep_rows<8, 0>), grid, dim3(256), lds, s, imgs, B, P, off, means, slices, plan, tw);
     else
-        hipLaunchKernelGGL(k_prep_rows<4>, grid, dim3(256), lds, s, imgs, B, P, off, means, slices, plan, tw);
+        hipLaunchKernelGGL((k_prep_rows<4, 0>), grid, dim3(256), lds, s, imgs, B, P, off, means, slices, plan, tw);
     return hipGetLastError();
 }
 
@@ -171,11 +185,12 @@ __device__ __forceinline__ float ctf_sign(int x, int y, float defocus, const Ctf
     return cos(t) < 0 ? -1.0f : 1.0f;
 }
 
-template <int T>
-__global__ void __launch_bounds__(256) k_prep_cols(int B, int P, int off, float2 *__restrict__ slices,
+template <int T, int NF>
+__global__ void __launch_bounds__(256) k_prep_cols(int B, int Pr, int off, float2 *__restrict__ slices,
                                                    const float *__restrict__ defocus, CtfConst cc, float scale,
                                                    FftPlan plan, const float2 *__restrict__ twg)
 {
+    const int P = NF ? NF : Pr;
     extern __shared__ float2 smem[];
     float2 *tw = smem;
     float2 *A = tw + P;
@@ -196,15 +211,16 @@ __global__ void __launch_bounds__(256) k_prep_cols(int B, int P, int off, float2
         const int y = idx / T;
         const int t = idx - y * T;
         const int x = x0 + t;
-        int b = y - off - H;
-        b %= P;
+        int b = y - off - H;  // in [-P, P/2)
         if (b < 0) b += P;
         float2 v = make_float2(0.f, 0.f);
         if (x <= H && b < B) v = sl[(size_t)y * (H + 1) + x];
         A[y * T + t] = v;
     }
     __syncthreads();
-    float2 *R = fft_tile<T>(A, Bf, plan, tw);
+    float2 *R = A;
+    if constexpr (NF != 0) fft_fixed<T, NF>(A, tw);
+    else R = fft_tile<T>(A, Bf, plan, tw);
     const float df = defocus ? defocus[p] : 0.f;
     for (int idx = threadIdx.x; idx < T * P; idx += blockDim.x) {
         const int y = idx / T;
@@ -233,10 +249,18 @@ hipError_t launch_prep_cols(int n, int B, int P, float2 *slices, const float *de
     const int H = P / 2;
     const size_t lds = fft_lds_bytes(P, T);
     dim3 grid((H + 1 + T - 1) / T, n);
+#define FH_COLS_CASE(TT, NN)                                                                                      \
+    if (T == TT && P == NN) {                                                                                     \
+        hipLaunchKernelGGL((k_prep_cols<TT, NN>), grid, dim3(256), lds, s, B, P, off, slices, defocus, cc, scale, plan, \
+                           tw);                                                                                   \
+        return hipGetLastError();                                                                                 \
+    }
+    FH_FIXED_PADS(FH_COLS_CASE)
+#undef FH_COLS_CASE
     if (T == 8)
-        hipLaunchKernelGGL(k_prep_cols<8>, grid, dim3(256), lds, s, B, P, off, slices, defocus, cc, scale, plan, tw);
+        hipLaunchKernelGGL((k_prep_cols<8, 0>), grid, dim3(256), lds, s, B, P, off, slices, defocus, cc, scale, plan, tw);
     else
-        hipLaunchKernelGGL(k_prep_cols<4>, grid, dim3(256), lds, s, B, P, off, slices, defocus, cc, scale, plan, tw);
+        hipLaunchKernelGGL((k_prep_cols<4, 0>), grid, dim3(256), lds, s, B, P, off, slices, defocus, cc, scale, plan, tw);
     return hipGetLastError();
 }
 
