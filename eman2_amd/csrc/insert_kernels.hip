@@ -226,7 +226,7 @@ __global__ void __launch_bounds__(256) k_agree(const float2 *__restrict__ slices
                 const float nv = norm[ci];
                 if (nv != 0.f) {
                     const float dt0 = __fdiv_rn(vr, nv), dt1 = __fdiv_rn(vi, nv), dt2 = nv;
-                    const float r = sqrt_rn((float)(x * x + y * y));  // Util::hypot_fast
+                    const float r = sqrt_int_rn(x * x + y * y);  // Util::hypot_fast
                     const float dotv = (dt0 * d20 + dt1 * d21) * dt2;
                     const float pw = (dt0 * dt0 + dt1 * dt1) * dt2;
                     const float pw2 = (d20 * d20 + d21 * d21) * dt2;
@@ -425,7 +425,7 @@ __global__ void __launch_bounds__(256) k_agree_batch(const float2 *__restrict__ 
                 if (x0 < 0) vi = -vi;  // get_complex_at conjugate for x<0
                 if (nv != 0.f) {
                     const float dt0 = __fdiv_rn(vr, nv), dt1 = __fdiv_rn(vi, nv), dt2 = nv;
-                    const float r = sqrt_rn((float)(x * x + y * y));  // Util::hypot_fast
+                    const float r = sqrt_int_rn(x * x + y * y);  // Util::hypot_fast
                     const float dotv = (dt0 * d20 + dt1 * d21) * dt2;
                     const float pw = (dt0 * dt0 + dt1 * dt1) * dt2;
                     const float pw2 = (d20 * d20 + d21 * d21) * dt2;

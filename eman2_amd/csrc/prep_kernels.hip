@@ -34,8 +34,11 @@ bool make_plan(int N, FftPlan *p)
 
 static int pick_T(int P)
 {
-    // two ping-pong tiles of T x P float2 plus the twiddles must fit 64 KiB
-    return (P <= 400) ? 8 : 4;
+    // 4 transforms per block: the in-place tile (5 P float2 with the
+    // twiddles) lets 8 blocks share a CU; pads above 3200 / 4 take the
+    // ping-pong form
+    (void)P;
+    return 4;
 }
 
 // twiddles + one tile (in place) or two (ping-pong), see fft_tile
@@ -137,7 +140,7 @@ __global__ void __launch_bounds__(256) k_prep_rows(const float *__restrict__ img
 }
 
 // Pads with a compile-time plan; others take the runtime plan.
-#define FH_FIXED_PADS(X) X(8, 96) X(8, 192) X(8, 288) X(8, 384) X(4, 480) X(4, 600)
+#define FH_FIXED_PADS(X) X(4, 96) X(4, 192) X(4, 288) X(4, 384) X(4, 480) X(4, 600)
 
 hipError_t launch_prep_rows(const float *imgs, int n, int B, int P, const float *means, float2 *slices,
                             const FftPlan &plan, const float2 *tw, hipStream_t s)
@@ -174,7 +177,7 @@ hipError_t launch_prep_rows(const float *imgs, int n, int B, int P, const float 
 __device__ __forceinline__ float ctf_sign(int x, int y, float defocus, const CtfConst &cc)
 {
 #pragma clang fp contract(off)
-    const float hyp = sqrt_rn((float)(x * x + y * y));  // Util::hypot_fast
+    const float hyp = sqrt_int_rn(x * x + y * y);  // Util::hypot_fast
     const float s = hyp * cc.ds;
     const float gam = -cc.g1 * s * s * s * s + cc.g2 * defocus * s * s;
     const double t = (double)(gam - cc.acac);
