@@ -245,9 +245,11 @@ static void free_chunk(fh_reconstructor *h)
 static void free_bins(fh_reconstructor *h)
 {
     BinBuffers &b = h->bins;
-    for (int **p : {&b.counts, &b.offsets, &b.cursors, &b.item_off, &b.tile, &b.list, &b.ctl})
+    for (int **p : {&b.counts, &b.offsets, &b.item_off, &b.tile, &b.list, &b.ctl})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
     if (b.items) { (void)hipFree(b.items); b.items = nullptr; }
+    if (b.recs) { (void)hipFree(b.recs); b.recs = nullptr; }
+    b.rec_cap = 0;
     if (b.part) { (void)hipFree(b.part); b.part = nullptr; }
     b.part_cap = 0;
     h->bins_cap_slices = 0;
@@ -307,17 +309,18 @@ static int ensure_bins(fh_reconstructor *h, long long nslices)
     const BrickGeom g = brick_geom(h->P);
     h->geom = g;
     BinBuffers &b = h->bins;
-    // at most 5 bricks per brick column per slice (see k_bin)
+    // at most 5 bricks per brick column per slice (4 shown, see k_bin_cols)
     b.list_cap = 5LL * g.nb * g.nb * nslices;
     b.item_cap = b.list_cap / std::max(1, h->tune.slices_per_item) + g.nkeys + 1;
     FH_TRY(h, hipMalloc(&b.counts, sizeof(int) * g.nkeys));
     FH_TRY(h, hipMalloc(&b.offsets, sizeof(int) * (g.nkeys + 1)));
-    FH_TRY(h, hipMalloc(&b.cursors, sizeof(int) * g.nkeys));
     FH_TRY(h, hipMalloc(&b.item_off, sizeof(int) * (g.nkeys + 1)));
     FH_TRY(h, hipMalloc(&b.tile, sizeof(int) * 2 * scan_tiles(g)));
     FH_TRY(h, hipMalloc(&b.list, sizeof(int) * b.list_cap));
     FH_TRY(h, hipMalloc(&b.items, sizeof(WorkItem) * b.item_cap));
-    FH_TRY(h, hipMalloc(&b.ctl, sizeof(int) * 4));
+    FH_TRY(h, hipMalloc(&b.ctl, sizeof(int) * 8));
+    FH_TRY(h, hipMalloc(&b.recs, sizeof(SliceRec) * 3 * (size_t)nslices));
+    b.rec_cap = nslices;
     h->bins_cap_slices = nslices;
     return FH_OK;
 }
@@ -1009,7 +1012,7 @@ long long fh_footprint_bytes(int P, int nsym, int chunk)
     const long long nslices = (long long)chunk * nsym;
     const long long list = 5LL * g.nb * g.nb * nslices;
     const long long items = list / 64 + g.nkeys + 1;
-    const long long bins = 4LL * list + 16LL * items + 16LL * g.nkeys;
+    const long long bins = 4LL * list + 16LL * items + 16LL * g.nkeys + 3LL * (long long)sizeof(SliceRec) * nslices;
     // partial bricks: about 1.1 (P/8)^2 (slice, brick) pairs per slice, one
     // item per 64 pairs of a key plus one per key the slices reach
     const long long part_items = (long long)(1.1 * (P / 8.0) * (P / 8.0) * nslices / 64) + g.nkeys / 2;

@@ -48,16 +48,28 @@ struct WorkItem {
     int pad;
 };
 
+// Per-slice plane record of the binning, grouped by dominant normal axis a:
+// t3d rows u, v and the column slopes r1 = -n[b1]/n[a], r2 = -n[b2]/n[a]
+// (n = u x v in double), so the plane's a coordinate is r1 p_b1 + r2 p_b2.
+struct SliceRec {
+    float u[3], v[3];
+    double r1, r2;
+    int q;    // slice index
+    int pad;
+};
+
 // Device buffers of the binning stage and the partial bricks.
 struct BinBuffers {
     int *counts;      // nkeys
     int *offsets;     // nkeys + 1
-    int *cursors;     // nkeys
     int *item_off;    // nkeys + 1: first work item of each key
     int *tile;        // 2 per scan tile of 1024 keys: count and item sums / offsets
     int *list;        // capacity
     WorkItem *items;  // item capacity
-    int *ctl;         // [0] work counter, [1] total items, [2] total pairs, [3] overflow flag
+    int *ctl;         // [0] work counter, [1] total items, [2] total pairs, [3] overflow flag,
+                      // [4..6] slices per dominant axis
+    SliceRec *recs;   // 3 segments of rec_cap: the slices of each dominant axis
+    long long rec_cap;
     float *part;      // partial bricks, PART_FLOATS per work item
     long long list_cap;
     long long item_cap;

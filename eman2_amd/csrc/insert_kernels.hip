@@ -485,82 +485,118 @@ hipError_t launch_agree(const float2 *slices, int nsym, const float *t3d, int ns
 // along the dominant normal axis at most 4 bricks qualify.  A slice is
 // binned under key 3 * brick + (its dominant normal axis), so that every
 // work item's slices share one column direction.
-struct SliceFrame {
-    double u[3], v[3], n[3];
-    int a, b1, b2;
-};
+//
+// The binning is owned by brick columns: one workgroup per (column, axis)
+// runs over the slices of that dominant axis and counts / lists the
+// column's bricks in LDS, so no (slice, brick) pair touches a global atomic.
 
-__device__ __forceinline__ void slice_frame(const float *m, SliceFrame &f)
+// Per slice: the plane record, appended to the segment of its dominant axis
+// (one atomic per wave and axis).  Weight-0 slices are skipped
+// (reconstructor.cpp:1088).
+__global__ void __launch_bounds__(256) k_slice_recs(const float *__restrict__ t3d, const float *__restrict__ wq,
+                                                    int nslices, BinBuffers bb)
 {
-    for (int d = 0; d < 3; d++) {
-        f.u[d] = m[d];
-        f.v[d] = m[3 + d];
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    int a = -1;
+    SliceRec r;
+    if (q < nslices && wq[q] != 0.0f) {
+        const float *m = t3d + 9 * (size_t)q;
+        double u[3], v[3];
+        for (int d = 0; d < 3; d++) {
+            r.u[d] = m[d];
+            r.v[d] = m[3 + d];
+            u[d] = m[d];
+            v[d] = m[3 + d];
+        }
+        const double n[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+        const double a0 = fabs(n[0]), a1 = fabs(n[1]), a2 = fabs(n[2]);
+        a = (a0 >= a1 && a0 >= a2) ? 0 : (a1 >= a2 ? 1 : 2);
+        const int b1 = a == 0 ? 1 : 0, b2 = a == 2 ? 1 : 2;
+        r.r1 = -n[b1] / n[a];
+        r.r2 = -n[b2] / n[a];
+        r.q = q;
+        r.pad = 0;
     }
-    f.n[0] = f.u[1] * f.v[2] - f.u[2] * f.v[1];
-    f.n[1] = f.u[2] * f.v[0] - f.u[0] * f.v[2];
-    f.n[2] = f.u[0] * f.v[1] - f.u[1] * f.v[0];
-    const double a0 = fabs(f.n[0]), a1 = fabs(f.n[1]), a2 = fabs(f.n[2]);
-    f.a = (a0 >= a1 && a0 >= a2) ? 0 : (a1 >= a2 ? 1 : 2);
-    f.b1 = f.a == 0 ? 1 : 0;
-    f.b2 = f.a == 2 ? 1 : 2;
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++) {
+        const unsigned long long m = __ballot(a == ax);
+        if (m == 0) continue;
+        const int leader = __ffsll((long long)m) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&bb.ctl[4 + ax], __popcll(m));
+        base = __shfl(base, leader, 64);
+        if (a == ax) {
+            const int rank = __popcll(m & ((1ull << lane) - 1ull));
+            bb.recs[(size_t)ax * bb.rec_cap + base + rank] = r;
+        }
+    }
 }
 
 // Does the slice's pixel domain (half-disk x>=0 of radius sqrt(H^2+H), or
-// the half-square with corners) come within reach of brick region R?
-__device__ __forceinline__ bool brick_in_domain(const SliceFrame &f, const int bc[3], int H, int BS, int corners)
+// the half-square with corners) come within reach of the brick?
+__device__ __forceinline__ bool brick_in_domain(const SliceRec &f, const int bc[3], int H, int BS, int corners,
+                                                double rr)
 {
     double c[3];
     for (int d = 0; d < 3; d++) c[d] = -H - 1 + BS * bc[d] + 0.5 * BS;
-    const double s = c[0] * f.u[0] + c[1] * f.u[1] + c[2] * f.u[2];
-    const double t = c[0] * f.v[0] + c[1] * f.v[1] + c[2] * f.v[2];
+    const double s = c[0] * (double)f.u[0] + c[1] * (double)f.u[1] + c[2] * (double)f.u[2];
+    const double t = c[0] * (double)f.v[0] + c[1] * (double)f.v[1] + c[2] * (double)f.v[2];
     const double rh = 0.8660254037844386 * BS + 0.01;
     if (s < -rh) return false;
     if (corners) return s <= H + rh && t >= -H - rh && t <= H - 1 + rh;
-    const double rr = sqrt((double)H * H + H) + rh;
     return s * s + t * t <= rr * rr;
 }
 
-template <int MODE>  // 0 count, 1 fill
-__global__ void __launch_bounds__(256) k_bin(const float *__restrict__ t3d, const float *__restrict__ wq, int nslices,
-                                             InsertConst c, BrickGeom g, BinBuffers bb)
+// One workgroup per (brick column (c1, c2), axis a): MODE 0 counts the
+// slices of each brick of the column, MODE 1 lists them from the key
+// offsets on.  LDS: one counter per brick of the column.
+template <int MODE>
+__global__ void __launch_bounds__(256) k_bin_cols(InsertConst c, BrickGeom g, BinBuffers bb)
 {
+    extern __shared__ int colcnt[];
     const int nb = g.nb, BS = BRICK, H = c.H;
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
-    const int q = blockIdx.y;
-    if (col >= nb * nb || q >= nslices) return;
-    if (wq[q] == 0.0f) return;  // weight 0 slices are skipped (reconstructor.cpp:1088)
-    SliceFrame f;
-    slice_frame(t3d + 9 * (size_t)q, f);
+    const int col = blockIdx.x, A = blockIdx.y;
     const int c1 = col % nb, c2 = col / nb;
+    const int b1 = A == 0 ? 1 : 0, b2 = A == 2 ? 1 : 2;
+    int bc[3];
+    bc[b1] = c1;
+    bc[b2] = c2;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+        bc[A] = i;
+        colcnt[i] = MODE == 0 ? 0 : bb.offsets[3 * ((bc[2] * nb + bc[1]) * nb + bc[0]) + A];
+    }
+    __syncthreads();
     const double dlt = 1e-3;
     const double L1 = -H - 1 + BS * c1 - dlt, U1 = -H - 1 + BS * c1 + BS + dlt;
     const double L2 = -H - 1 + BS * c2 - dlt, U2 = -H - 1 + BS * c2 + BS + dlt;
-    const double na = f.n[f.a], n1 = f.n[f.b1], n2 = f.n[f.b2];
-    double amin = 1e30, amax = -1e30;
-    const double p1s[2] = {L1, U1}, p2s[2] = {L2, U2};
-    for (int i = 0; i < 2; i++)
-        for (int j = 0; j < 2; j++) {
-            const double pa = -(n1 * p1s[i] + n2 * p2s[j]) / na;
-            amin = fmin(amin, pa);
-            amax = fmax(amax, pa);
+    const double rr = sqrt((double)H * H + H) + 0.8660254037844386 * BS + 0.01;
+    const int n = bb.ctl[4 + A];
+    const SliceRec *recs = bb.recs + (size_t)A * bb.rec_cap;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const SliceRec f = recs[i];
+        const double pa0 = f.r1 * L1 + f.r2 * L2, pa1 = f.r1 * U1 + f.r2 * L2;
+        const double pa2 = f.r1 * L1 + f.r2 * U2, pa3 = f.r1 * U1 + f.r2 * U2;
+        const double amin = fmin(fmin(pa0, pa1), fmin(pa2, pa3)), amax = fmax(fmax(pa0, pa1), fmax(pa2, pa3));
+        int lo = (int)floor((amin - dlt + H + 1) / BS);
+        int hi = (int)floor((amax + dlt + H + 1) / BS);
+        if (lo < 0) lo = 0;
+        if (hi > nb - 1) hi = nb - 1;
+        for (int ca = lo; ca <= hi; ca++) {
+            bc[A] = ca;
+            if (!brick_in_domain(f, bc, H, BS, c.corners, rr)) continue;
+            const int pos = atomicAdd(&colcnt[ca], 1);
+            if (MODE == 1) {
+                if (pos < bb.list_cap) bb.list[pos] = f.q;
+                else bb.ctl[3] = 1;
+            }
         }
-    int lo = (int)floor((amin - dlt + H + 1) / BS);
-    int hi = (int)floor((amax + dlt + H + 1) / BS);
-    if (lo < 0) lo = 0;
-    if (hi > nb - 1) hi = nb - 1;
-    for (int ca = lo; ca <= hi; ca++) {
-        int bc[3];
-        bc[f.a] = ca;
-        bc[f.b1] = c1;
-        bc[f.b2] = c2;
-        if (!brick_in_domain(f, bc, H, BS, c.corners)) continue;
-        const int key = 3 * ((bc[2] * nb + bc[1]) * nb + bc[0]) + f.a;
-        if (MODE == 0) {
-            atomicAdd(&bb.counts[key], 1);
-        } else {
-            const long long pos = (long long)bb.offsets[key] + atomicAdd(&bb.cursors[key], 1);
-            if (pos < bb.list_cap) bb.list[pos] = q;
-            else bb.ctl[3] = 1;
+    }
+    if (MODE == 0) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+            bc[A] = i;
+            bb.counts[3 * ((bc[2] * nb + bc[1]) * nb + bc[0]) + A] = colcnt[i];
         }
     }
 }
@@ -682,7 +718,6 @@ __global__ void __launch_bounds__(256) k_scan_apply(BrickGeom g, BinBuffers bb, 
         if (k + t < g.nkeys) {
             bb.offsets[k + t] = a;
             bb.item_off[k + t] = b;
-            bb.cursors[k + t] = 0;
         }
         a += cv[t];
         b += (cv[t] + K - 1) / K;
@@ -714,11 +749,14 @@ hipError_t launch_bin(const float *t3d, const float *wq, int nslices, InsertCons
                       int K, hipStream_t s)
 {
     hipError_t e;
+    if (nslices > b.rec_cap) return hipErrorInvalidValue;
     if ((e = launch_fill_i32(b.counts, 0, g.nkeys, s)) != hipSuccess) return e;
-    if ((e = launch_fill_i32(b.ctl, 0, 4, s)) != hipSuccess) return e;
+    if ((e = launch_fill_i32(b.ctl, 0, 8, s)) != hipSuccess) return e;
+    const dim3 cgrid(g.nb * g.nb, 3);
+    const size_t clds = sizeof(int) * g.nb;
     if (nslices > 0) {
-        dim3 grid((g.nb * g.nb + 255) / 256, nslices);
-        hipLaunchKernelGGL(k_bin<0>, grid, dim3(256), 0, s, t3d, wq, nslices, c, g, b);
+        hipLaunchKernelGGL(k_slice_recs, dim3((nslices + 255) / 256), dim3(256), 0, s, t3d, wq, nslices, b);
+        hipLaunchKernelGGL(k_bin_cols<0>, cgrid, dim3(256), clds, s, c, g, b);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     const int nt = scan_tiles(g);
@@ -727,8 +765,7 @@ hipError_t launch_bin(const float *t3d, const float *wq, int nslices, InsertCons
     hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(256), 0, s, g, b, K);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (nslices > 0) {
-        dim3 grid((g.nb * g.nb + 255) / 256, nslices);
-        hipLaunchKernelGGL(k_bin<1>, grid, dim3(256), 0, s, t3d, wq, nslices, c, g, b);
+        hipLaunchKernelGGL(k_bin_cols<1>, cgrid, dim3(256), clds, s, c, g, b);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     hipLaunchKernelGGL(k_bin_items, dim3((g.nkeys + 255) / 256), dim3(256), 0, s, g, b, K);

@@ -1,4 +1,4 @@
-"""CPU emulation of the brick binning (k_bin in insert_kernels.hip): brick b
+"""CPU emulation of the brick binning (k_bin_cols in insert_kernels.hip): brick b
 owns the stencil origins [-P/2-1 + 8b, -P/2-1 + 8b + 8) per axis, and a slice
 is listed under every brick that may own one of its pixels' origins
 (libEM/reconstructor.cpp:1166-1209 for the pixel loop).  Restates the
@@ -23,7 +23,7 @@ def _frame(m):
 
 
 def bricks_for_slice(m, H, BS, nb, corners=False):
-    """k_bin: bricks a slice's pixels can reach (insert_kernels.hip)."""
+    """k_bin_cols: bricks a slice is listed under (insert_kernels.hip)."""
     u, v, n, a, b1, b2 = _frame(m)
     out = set()
     dlt = 1e-3
@@ -31,7 +31,8 @@ def bricks_for_slice(m, H, BS, nb, corners=False):
         for c1 in range(nb):
             L1, U1 = -H - 1 + BS * c1 - dlt, -H - 1 + BS * c1 + BS + dlt
             L2, U2 = -H - 1 + BS * c2 - dlt, -H - 1 + BS * c2 + BS + dlt
-            pas = [-(n[b1] * p1 + n[b2] * p2) / n[a] for p1 in (L1, U1) for p2 in (L2, U2)]
+            r1, r2 = -n[b1] / n[a], -n[b2] / n[a]
+            pas = [r1 * p1 + r2 * p2 for p1 in (L1, U1) for p2 in (L2, U2)]
             lo = max(math.floor((min(pas) - dlt + H + 1) / BS), 0)
             hi = min(math.floor((max(pas) + dlt + H + 1) / BS), nb - 1)
             for ca in range(lo, hi + 1):
