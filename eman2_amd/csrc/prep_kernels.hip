@@ -225,19 +225,24 @@ __global__ void __launch_bounds__(256) k_prep_cols(int B, int Pr, int off, float
     if constexpr (NF != 0) fft_fixed<T, NF>(A, tw);
     else R = fft_tile<T>(A, Bf, plan, tw);
     const float df = defocus ? defocus[p] : 0.f;
-    for (int idx = threadIdx.x; idx < T * P; idx += blockDim.x) {
-        const int y = idx / T;
+    // the sign depends on x^2 + y^2 only: rows y and P - y (signed y and -y,
+    // 0 < y < P/2) share one evaluation
+    for (int idx = threadIdx.x; idx < T * (H + 1); idx += blockDim.x) {
+        const int y = idx / T;  // 0 .. P/2; row P/2 is signed -P/2
         const int t = idx - y * T;
         const int x = x0 + t;
         if (x > H) continue;
-        float2 v = R[y * T + t];
-        if (defocus) {
-            const float sg = ctf_sign(x, y < H ? y : y - P, df, cc);
-            v = make_float2(v.x * sg - v.y * 0.0f, v.x * 0.0f + v.y * sg);
+        const float sg = defocus ? ctf_sign(x, y, df, cc) : 1.f;
+#pragma unroll
+        for (int m = 0; m < 2; m++) {
+            const int row = m == 0 ? y : P - y;
+            if (m == 1 && (y == 0 || y == H)) break;
+            float2 v = R[row * T + t];
+            if (defocus) v = make_float2(v.x * sg - v.y * 0.0f, v.x * 0.0f + v.y * sg);
+            v.x = v.x * scale;
+            v.y = v.y * scale;
+            sl[(size_t)row * (H + 1) + x] = v;
         }
-        v.x = v.x * scale;
-        v.y = v.y * scale;
-        sl[(size_t)y * (H + 1) + x] = v;
     }
 }
 
