@@ -32,14 +32,16 @@ bool make_plan(int N, FftPlan *p)
     return true;
 }
 
+// Transforms per block: 4 for the row pass and the finish passes (the
+// in-place tile, 5 P float2 with the twiddles, lets 8 blocks share a CU),
+// 8 for the column pass up to P = 400 (its CTF epilogue keeps it busier per
+// block).  Pads above 3200 / T take the ping-pong form.
 static int pick_T(int P)
 {
-    // 4 transforms per block: the in-place tile (5 P float2 with the
-    // twiddles) lets 8 blocks share a CU; pads above 3200 / 4 take the
-    // ping-pong form
     (void)P;
     return 4;
 }
+static int pick_T_cols(int P) { return P <= 400 ? 8 : 4; }
 
 // twiddles + one tile (in place) or two (ping-pong), see fft_tile
 static size_t fft_lds_bytes(int P, int T)
@@ -141,6 +143,7 @@ __global__ void __launch_bounds__(256) k_prep_rows(const float *__restrict__ img
 
 // Pads with a compile-time plan; others take the runtime plan.
 #define FH_FIXED_PADS(X) X(4, 96) X(4, 192) X(4, 288) X(4, 384) X(4, 480) X(4, 600)
+#define FH_FIXED_PADS_COLS(X) X(8, 96) X(8, 192) X(8, 288) X(8, 384) X(4, 480) X(4, 600)
 
 hipError_t launch_prep_rows(const float *imgs, int n, int B, int P, const float *means, float2 *slices,
                             const FftPlan &plan, const float2 *tw, hipStream_t s)
@@ -253,7 +256,7 @@ hipError_t launch_prep_cols(int n, int B, int P, float2 *slices, const float *de
     const int d = B - P;
     const int fl = d >= 0 ? d / 2 : -((-d + 1) / 2);
     const int off = -fl;
-    const int T = pick_T(P);
+    const int T = pick_T_cols(P);
     const int H = P / 2;
     const size_t lds = fft_lds_bytes(P, T);
     dim3 grid((H + 1 + T - 1) / T, n);
@@ -263,7 +266,7 @@ hipError_t launch_prep_cols(int n, int B, int P, float2 *slices, const float *de
                            tw);                                                                                   \
         return hipGetLastError();                                                                                 \
     }
-    FH_FIXED_PADS(FH_COLS_CASE)
+    FH_FIXED_PADS_COLS(FH_COLS_CASE)
 #undef FH_COLS_CASE
     if (T == 8)
         hipLaunchKernelGGL((k_prep_cols<8, 0>), grid, dim3(256), lds, s, B, P, off, slices, defocus, cc, scale, plan, tw);
