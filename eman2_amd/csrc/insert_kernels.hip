@@ -893,27 +893,36 @@ struct PendPix {
     bool pend;
 };
 
-// One collision-free pass of the pending pixels of a window: one lane per
-// stencil origin adds its 8 corners; pixels carried over from earlier
-// windows take precedence over the window's new ones (`fresh`), so a
+// Election of a window: one lane per stencil origin among the pending
+// pixels wins the right to add its 8 corners; pixels carried over from
+// earlier windows take precedence over the window's new ones (`fresh`), so a
 // carried pixel is added in the next window it takes part in unless it
 // collides with another carried pixel.  The losers stay pending.
-template <int A>
-__device__ __forceinline__ void scatter_pass(PendPix &st, bool fresh, const int lo[3], const InsertConst &c, int lane,
-                                             float *acc, unsigned char *tag, const float *etab)
+__device__ __forceinline__ bool scatter_elect(const PendPix &st, bool fresh, const int lo[3], int lane,
+                                              unsigned char *tag)
 {
-    const int f0 = (int)floorf(st.px), f1 = (int)floorf(st.py), f2 = (int)floorf(st.pz);
-    const int i0 = f0 - lo[0], j0 = f1 - lo[1], k0 = f2 - lo[2];  // in [0, BRICK-1] while pending
-    const int ti = st.pend ? i0 + BRICK * (j0 + BRICK * k0) : 0;
+    const int i0 = (int)floorf(st.px) - lo[0], j0 = (int)floorf(st.py) - lo[1], k0 = (int)floorf(st.pz) - lo[2];
+    const int ti = st.pend ? i0 + BRICK * (j0 + BRICK * k0) : 0;  // in [0, BRICK^3) while pending
     if (st.pend && fresh) tag[ti] = (unsigned char)lane;
     wave_sync_lds();
     if (st.pend && !fresh) tag[ti] = (unsigned char)lane;
     wave_sync_lds();
     const bool win = st.pend && tag[ti] == (unsigned char)lane;
     wave_sync_lds();
+    return win;
+}
+
+// The winners add their 8 corners: lanes with different origins can still
+// meet on a cell through different corners, so the corners go one after
+// another with a wave-level ordering point between them.
+template <int A>
+__device__ __forceinline__ void scatter_add(const PendPix &st, bool win, const int lo[3], const InsertConst &c,
+                                            float *acc, const float *etab)
+{
+    const int f0 = (int)floorf(st.px), f1 = (int)floorf(st.py), f2 = (int)floorf(st.pz);
     float wg[8];
     corner_weights8(c.mode, f0, f1, f2, st.px, st.py, st.pz, c.h, st.pw, etab, wg);
-    const int cbase = acc_slot<A>(i0, j0, k0);
+    const int cbase = acc_slot<A>(f0 - lo[0], f1 - lo[1], f2 - lo[2]);
     float *const p_re = acc + cbase, *const p_im = acc + ACC_N + cbase, *const p_w = acc + 2 * ACC_N + cbase;
 #pragma unroll
     for (int cI = 0; cI < 8; cI++) {
@@ -926,7 +935,6 @@ __device__ __forceinline__ void scatter_pass(PendPix &st, bool fresh, const int 
         }
         wave_sync_lds();
     }
-    st.pend = st.pend && !win;
 }
 
 // Up to GROUP slices of an item in one stream of 64-pixel windows: lane
@@ -942,7 +950,8 @@ __device__ __forceinline__ void scatter_group(const float2 *__restrict__ slices,
                                               const float *__restrict__ t3d, const float *__restrict__ wq,
                                               const int *__restrict__ list, int first, int ng, const InsertConst &c,
                                               const int lo[3], int lane, float *acc, unsigned char *tag, int *mark,
-                                              float4 *rec, const float *etab, const float *rtab, PendPix &st)
+                                              float4 *rec, const float *etab, const float *rtab, PendPix &st,
+                                              bool &fresh)
 {
     const int H = c.H, P = c.P;
     const unsigned slice_elems = (unsigned)P * (unsigned)(H + 1);
@@ -994,9 +1003,14 @@ __device__ __forceinline__ void scatter_group(const float2 *__restrict__ slices,
     const int excl = incl - cnt;
     // row record read by the slots of this row: (x - slot, y) as two int16
     const int rowrec = (int)(((unsigned)(xs - excl) & 0xffffu) | ((unsigned)y << 16));
+    // Software-pipelined windows: the election of the pending pixels comes
+    // first, then the lanes that will be free after this window load their
+    // next pixels (slots next, next+1, ... in lane order), and the winners'
+    // corner updates run while those slice reads are in flight.
     for (int next = 0; next < total;) {
-        // the lanes without a pending pixel take slots next, next+1, ... in lane order
-        const unsigned long long fm = __ballot(!st.pend);
+        const bool win = scatter_elect(st, fresh, lo, lane, tag);
+        const bool freed = !st.pend || win;
+        const unsigned long long fm = __ballot(freed);
         const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(fm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)fm, 0u));
         // row of slot next+lane: the last non-empty row starting at or before it
         mark[lane] = -1;
@@ -1008,8 +1022,10 @@ __device__ __forceinline__ void scatter_group(const float2 *__restrict__ slices,
         const int rl = __builtin_amdgcn_ds_bpermute(rank << 2, rk);  // row of slot next+rank
         const int rr = __builtin_amdgcn_ds_bpermute(rl << 2, rowrec);
         const int s = next + rank;
-        const bool fresh = !st.pend && s < total;
-        if (fresh) {
+        PendPix ns;
+        ns.pend = false;
+        bool ncj = false;
+        if (freed && s < total) {
             const float4 ra = rec[2 * (rl >> 4)], rb = rec[2 * (rl >> 4) + 1];
             const int x = (int)(short)(rr & 0xffff) + s, ry = rr >> 16;
             const float pm[6] = {ra.x, ra.y, ra.z, ra.w, rb.x, rb.y};
@@ -1020,20 +1036,33 @@ __device__ __forceinline__ void scatter_group(const float2 *__restrict__ slices,
             // ones past P/2 only reach cells that are never merged
             if ((unsigned)x <= (unsigned)H && (unsigned)(f0 - lo[0]) < (unsigned)BRICK &&
                 (unsigned)(f1 - lo[1]) < (unsigned)BRICK && (unsigned)(f2 - lo[2]) < (unsigned)BRICK) {
-                const float2 dt =
-                    slice_value(slices + (unsigned long long)(unsigned)__float_as_int(rb.w) * slice_elems, x, ry, P, H);
-                st.re = dt.x;
-                st.im = dt.y;
-                st.px = px;
-                st.py = py;
-                st.pz = pz;
-                st.pw = rb.z;
-                st.pend = true;
+                // slice_value() with the conjugation applied after the corner
+                // updates, so that nothing waits for the read before then
+                const bool cj = x == 0 && ry < 0;
+                const int row = cj ? -ry : (ry < 0 ? P + ry : ry);
+                const float2 dt = (slices + (unsigned long long)(unsigned)__float_as_int(rb.w) *
+                                                slice_elems)[row * (H + 1) + x];
+                ns.re = dt.x;
+                ns.im = dt.y;
+                ncj = cj;
+                ns.px = px;
+                ns.py = py;
+                ns.pz = pz;
+                ns.pw = rb.z;
+                ns.pend = true;
             }
         }
         wave_sync_lds();  // rec reads before the next group rewrites them
+        if (__ballot(win)) scatter_add<A>(st, win, lo, c, acc, etab);
+        if (freed) {
+            st.pend = ns.pend;
+            if (ns.pend) {
+                st = ns;
+                if (ncj) st.im = -st.im;
+            }
+        }
+        fresh = freed && ns.pend;
         next += __popcll(fm);
-        scatter_pass<A>(st, fresh, lo, c, lane, acc, tag, etab);
     }
 }
 
@@ -1051,10 +1080,16 @@ __device__ __forceinline__ void scatter_item(const float2 *__restrict__ slices, 
     st.py = (float)lo[1];
     st.pz = (float)lo[2];
     st.pend = false;
+    bool fresh = false;
     for (int s0 = 0; s0 < w.count; s0 += GROUP)
         scatter_group<A>(slices, nslices, nsym, t3d, wq, list, w.start + s0, min(GROUP, w.count - s0), c, lo, lane,
-                         acc, tag, mark, rec, etab, rtab, st);
-    while (__ballot(st.pend)) scatter_pass<A>(st, false, lo, c, lane, acc, tag, etab);
+                         acc, tag, mark, rec, etab, rtab, st, fresh);
+    while (__ballot(st.pend)) {
+        const bool win = scatter_elect(st, fresh, lo, lane, tag);
+        scatter_add<A>(st, win, lo, c, acc, etab);
+        st.pend = st.pend && !win;
+        fresh = false;
+    }
     wave_sync_lds();
 #pragma unroll
     for (int wv = 0; wv < (ACC_N + 63) / 64; wv++) {
