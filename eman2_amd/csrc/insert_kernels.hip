@@ -945,24 +945,46 @@ __device__ __forceinline__ void scatter_add(const PendPix &st, bool win, const i
 // to the lanes that hold no pending pixel.
 constexpr int GROUP = 4;
 
+// The slice of lane (g, r) in a group: index, t3d rows 0-1, weight.  Loaded
+// one group ahead, so the reads are in flight while the previous group's
+// windows run.
+struct GroupSlice {
+    int q;  // -1: no slice
+    float mm[6];
+    float wgt;
+};
+
+// Slice record of lane (g, r) for the group starting at item slice s: the
+// index from the 64 list entries held in qbuf (entries qbase .. qbase+63).
+__device__ __forceinline__ void fetch_group_slice(GroupSlice &gs, int s, int count, int qbuf, int qbase, int lane,
+                                                  int nslices, const float *__restrict__ t3d,
+                                                  const float *__restrict__ wq)
+{
+    const int idx = s + (lane >> 4);
+    const int qi = __builtin_amdgcn_ds_bpermute((idx - qbase) << 2, qbuf);
+    const int q = idx < count ? qi : -1;
+    const bool qv = q >= 0 && q < nslices;  // never written list entries are skipped
+    gs.q = qv ? q : -1;
+#pragma unroll
+    for (int d = 0; d < 6; d++) gs.mm[d] = qv ? t3d[9 * (size_t)q + d] : 0.f;
+    gs.wgt = qv ? wq[q] : 0.f;
+}
+
 template <int A>
-__device__ __forceinline__ void scatter_group(const float2 *__restrict__ slices, int nslices, int nsym,
-                                              const float *__restrict__ t3d, const float *__restrict__ wq,
-                                              const int *__restrict__ list, int first, int ng, const InsertConst &c,
-                                              const int lo[3], int lane, float *acc, unsigned char *tag, int *mark,
-                                              float4 *rec, const float *etab, const float *rtab, PendPix &st,
-                                              bool &fresh)
+__device__ __forceinline__ void scatter_group(const float2 *__restrict__ slices, int nsym, const GroupSlice &gs,
+                                              const InsertConst &c, const int lo[3], int lane, float *acc,
+                                              unsigned char *tag, int *mark, float4 *rec, const float *etab,
+                                              const float *rtab, PendPix &st, bool &fresh)
 {
     const int H = c.H, P = c.P;
     const unsigned slice_elems = (unsigned)P * (unsigned)(H + 1);
     const int g = lane >> 4, r = lane & 15;
-    int q = -1;
-    if (g < ng) q = list[first + g];
-    const bool qv = q >= 0 && q < nslices;  // never written list entries are skipped
+    const int q = gs.q;
+    const bool qv = q >= 0;
     float mm[6];
 #pragma unroll
-    for (int d = 0; d < 6; d++) mm[d] = qv ? t3d[9 * (size_t)q + d] : 0.f;
-    const float wgt = qv ? wq[q] : 0.f;
+    for (int d = 0; d < 6; d++) mm[d] = gs.mm[d];
+    const float wgt = gs.wgt;
     if (r == 0) {
         rec[2 * g] = make_float4(mm[0], mm[1], mm[2], mm[3]);
         rec[2 * g + 1] = make_float4(mm[4], mm[5], wgt, __int_as_float(qv ? q / nsym : 0));  // particle
@@ -1081,9 +1103,23 @@ __device__ __forceinline__ void scatter_item(const float2 *__restrict__ slices, 
     st.pz = (float)lo[2];
     st.pend = false;
     bool fresh = false;
-    for (int s0 = 0; s0 < w.count; s0 += GROUP)
-        scatter_group<A>(slices, nslices, nsym, t3d, wq, list, w.start + s0, min(GROUP, w.count - s0), c, lo, lane,
-                         acc, tag, mark, rec, etab, rtab, st, fresh);
+    // list entries 64 at a time; each group's slice records fetched one group ahead
+    int qbase = 0;
+    int qbuf = lane < w.count ? list[w.start + lane] : -1;
+    GroupSlice nxt;
+    fetch_group_slice(nxt, 0, w.count, qbuf, qbase, lane, nslices, t3d, wq);
+    for (int s0 = 0; s0 < w.count; s0 += GROUP) {
+        const GroupSlice cur = nxt;
+        const int s1 = s0 + GROUP;
+        if (s1 < w.count) {
+            if (s1 + GROUP > qbase + 64) {
+                qbase = s1;
+                qbuf = qbase + lane < w.count ? list[w.start + qbase + lane] : -1;
+            }
+            fetch_group_slice(nxt, s1, w.count, qbuf, qbase, lane, nslices, t3d, wq);
+        }
+        scatter_group<A>(slices, nsym, cur, c, lo, lane, acc, tag, mark, rec, etab, rtab, st, fresh);
+    }
     while (__ballot(st.pend)) {
         const bool win = scatter_elect(st, fresh, lo, lane, tag);
         scatter_add<A>(st, win, lo, c, acc, etab);
