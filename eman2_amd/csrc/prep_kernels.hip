@@ -141,6 +141,93 @@ __global__ void __launch_bounds__(256) k_prep_rows(const float *__restrict__ img
     }
 }
 
+// Persistent row pass for the compile-time plans: each block walks tiles
+// (particle, 2T rows) and reads the next tile's pixels into registers while
+// it transforms the current one, so the global read latency hides behind
+// the FFT.  Same arithmetic as k_prep_rows.
+template <int T, int NF>
+__global__ void __launch_bounds__(256) k_prep_rows_pipe(const float *__restrict__ imgs, int B, int off, int n,
+                                                        const float *__restrict__ means, float2 *__restrict__ slices,
+                                                        const float2 *__restrict__ twg)
+{
+    constexpr int P = NF, H = P / 2, K = (T * P + 255) / 256;
+    extern __shared__ float2 smem[];
+    float2 *tw = smem;
+    float2 *A = tw + P;
+    load_twiddles(tw, twg, P);
+    const int nrb = (B + 2 * T - 1) / (2 * T);
+    const int ntiles = nrb * n;
+    // raw pixels of a tile; out-of-image slots hold the mean (zero after the subtraction)
+    float2 v[K];
+    float vmean = 0.f;
+    auto fetch = [&](int tile) {
+        const int p = tile / nrb, b0 = (tile - p * nrb) * 2 * T;
+        const float mean = means[p];
+        const float *img = imgs + (size_t)p * B * B;
+        vmean = mean;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int idx = threadIdx.x + 256 * k;
+            const int t = idx % T, xs = idx / T;
+            const int xp = xs + H < P ? xs + H : xs + H - P;
+            const int bx = xp - off;
+            const int r0 = b0 + 2 * t;
+            const bool in = idx < T * P && bx >= 0 && bx < B;
+            float re = mean, im = mean;
+            if (in && r0 < B) re = img[(size_t)r0 * B + bx];
+            if (in && r0 + 1 < B) im = img[(size_t)(r0 + 1) * B + bx];
+            v[k] = make_float2(re, im);
+        }
+    };
+    int tile = blockIdx.x;
+    if (tile < ntiles) fetch(tile);
+    for (; tile < ntiles; tile += gridDim.x) {
+        const int p = tile / nrb, b0 = (tile - p * nrb) * 2 * T;
+        const float mean = vmean;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int idx = threadIdx.x + 256 * k;
+            if (idx < T * P) A[idx] = make_float2(v[k].x - mean, v[k].y - mean);
+        }
+        __syncthreads();
+        if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
+        fft_fixed<T, NF>(A, tw);
+        float2 *out = slices + (size_t)p * P * (H + 1);
+        for (int idx = threadIdx.x; idx < T * (H + 1); idx += 256) {
+            const int t = idx % T;
+            const int k = idx / T;
+            const int r0 = b0 + 2 * t;
+            if (r0 >= B) continue;
+            const float2 z1 = A[k * T + t];
+            const float2 z2 = A[(k == 0 ? 0 : P - k) * T + t];
+            const float2 a = make_float2(0.5f * (z1.x + z2.x), 0.5f * (z1.y - z2.y));
+            const float2 c = make_float2(0.5f * (z1.y + z2.y), -0.5f * (z1.x - z2.x));
+            int ys0 = r0 + off + H;  // < 2P
+            if (ys0 >= P) ys0 -= P;
+            out[(size_t)ys0 * (H + 1) + k] = a;
+            if (r0 + 1 < B) {
+                const int ys1 = ys0 + 1 == P ? 0 : ys0 + 1;
+                out[(size_t)ys1 * (H + 1) + k] = c;
+            }
+        }
+        __syncthreads();  // the tile is read before the next one is stored
+    }
+}
+
+// Compute units of the current device (persistent grids), cached per device.
+static int device_cus()
+{
+    static int cus[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (cus[dev] == 0) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
+        cus[dev] = v;
+    }
+    return cus[dev];
+}
+
 // Pads with a compile-time plan; others take the runtime plan.
 #define FH_FIXED_PADS(X) X(4, 96) X(4, 192) X(4, 288) X(4, 384) X(4, 480) X(4, 600)
 #define FH_FIXED_PADS_COLS(X) X(8, 96) X(8, 192) X(8, 288) X(8, 384) X(4, 480) X(4, 600)
@@ -156,9 +243,16 @@ hipError_t launch_prep_rows(const float *imgs, int n, int B, int P, const float 
     const int T = pick_T(P);
     const size_t lds = fft_lds_bytes(P, T);
     dim3 grid((B + 2 * T - 1) / (2 * T), n);
+    const int ntiles = (int)grid.x * n;
 #define FH_ROWS_CASE(TT, NN)                                                                                    \
     if (T == TT && P == NN) {                                                                                   \
-        hipLaunchKernelGGL((k_prep_rows<TT, NN>), grid, dim3(256), lds, s, imgs, B, P, off, means, slices, plan, tw); \
+        int per = 0;                                                                                            \
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_prep_rows_pipe<TT, NN>, 256, lds) != hipSuccess || \
+            per < 1)                                                                                            \
+            per = 1;                                                                                            \
+        const int pgrid = std::min(ntiles, device_cus() * per);                                                 \
+        hipLaunchKernelGGL((k_prep_rows_pipe<TT, NN>), dim3(pgrid), dim3(256), lds, s, imgs, B, off, n, means, slices, \
+                           tw);                                                                                 \
         return hipGetLastError();                                                                               \
     }
     FH_FIXED_PADS(FH_ROWS_CASE)
