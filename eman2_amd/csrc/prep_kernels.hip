@@ -329,16 +329,15 @@ __global__ void __launch_bounds__(256) k_prep_cols(int B, int Pr, int off, float
         const int t = idx - y * T;
         const int x = x0 + t;
         if (x > H) continue;
-        const float sg = defocus ? ctf_sign(x, y, df, cc) : 1.f;
+        // (v * (sg + 0i)) * scale with sg = +-1 equals v * (sg * scale) exactly
+        // (up to the sign of a zero)
+        const float ss = defocus ? ctf_sign(x, y, df, cc) * scale : scale;
 #pragma unroll
         for (int m = 0; m < 2; m++) {
             const int row = m == 0 ? y : P - y;
             if (m == 1 && (y == 0 || y == H)) break;
-            float2 v = R[row * T + t];
-            if (defocus) v = make_float2(v.x * sg - v.y * 0.0f, v.x * 0.0f + v.y * sg);
-            v.x = v.x * scale;
-            v.y = v.y * scale;
-            sl[(size_t)row * (H + 1) + x] = v;
+            const float2 v = R[row * T + t];
+            sl[(size_t)row * (H + 1) + x] = make_float2(v.x * ss, v.y * ss);
         }
     }
 }
