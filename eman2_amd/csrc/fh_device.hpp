@@ -305,7 +305,7 @@ __device__ __forceinline__ void stockham_pass_fixed(float2 *buf, const float2 *_
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 float2 a = buf[(j + r * NR) * T + t];
-                if (r > 0) a = cmul(a, tw[r * k * TWM]);
+                if (NS > 1 && r > 0) a = cmul(a, tw[r * k * TWM]);  // the first pass has k = 0: unit twiddles
                 v[gi][r] = a;
             }
             dft<R>(v[gi]);
