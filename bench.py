@@ -63,6 +63,9 @@ def parse():
     ap.add_argument("--serve-jobs", type=int, default=40,
                     help="jobs in the serving measurement (0 = skip); Poisson arrivals at --serve-rate")
     ap.add_argument("--serve-rate", type=float, default=4.0)
+    ap.add_argument("--workload", default="insert", choices=["insert", "refine"],
+                    help="insert: batched insertion (the metric); refine: insert, slice agreement of every "
+                         "particle, re-insertion of img x reconstruct_norm, finish() (400 px -> 600 by default)")
     return ap.parse_args()
 
 
@@ -155,8 +158,91 @@ def cpu_baseline(imgs_host, mats, dfs, P, sym, mode, apix, budget_s):
     }
 
 
+def refine(a):
+    """Two-pass refinement flow (e2make3d.py:391-539): insert all particles,
+    determine_slice_agreement(sub=True) for every particle, clear, re-insert
+    each image multiplied by its reconstruct_norm, finish() with the 3-D
+    inverse FFT.  Particles HBM-resident; the finished map is copied to the
+    host as finish() returns it."""
+    import torch
+
+    from eman2_amd import Reconstructors
+    from eman2_amd import synthetic as S
+
+    dev = "cuda:0"
+    P, B, n = a.pad, a.box, a.particles
+    print("[bench] refine flow: generating %d particles %dpx on %s" % (n, B, dev), file=sys.stderr, flush=True)
+    ph = S.phantom(400, 96.0 * B / 256.0, seed=2)
+    imgs, mats, dfs = S.particles_device(ph, n, B, seed=3, apix=a.apix, snr=a.snr, ctf=CTF, device=dev)
+    torch.cuda.synchronize()
+    w = np.ones(n, np.float32)
+    ctf = dict(CTF, apix=a.apix)
+    r = Reconstructors.get("fourier_hip", {"size": (P, P, P), "sym": a.sym, "mode": a.mode})
+    ptr = int(imgs.data_ptr())
+    stages = {"insert1_ms": [], "agree_ms": [], "insert2_ms": [], "finish_ms": []}
+
+    def step(record):
+        t = [time.perf_counter()]
+        r.setup()
+        r.insert_batch(ptr, mats, w, defocus=dfs, ctf=ctf, box=B)
+        r.sync()
+        t.append(time.perf_counter())
+        q = torch.empty((n, 5), dtype=torch.float32, device=dev)
+        r.agreement_batch(ptr, mats, w, defocus=dfs, ctf=ctf, sub=True, apix=a.apix, box=B, out=int(q.data_ptr()))
+        t.append(time.perf_counter())
+        scaled = imgs * q[:, 0].view(-1, 1, 1)
+        r.setup()
+        r.insert_batch(int(scaled.data_ptr()), mats, w, defocus=dfs, ctf=ctf, box=B)
+        r.sync()
+        t.append(time.perf_counter())
+        r.finish(True, return_numpy=True)
+        t.append(time.perf_counter())
+        del scaled
+        if record:
+            for k, i in zip(stages, range(4)):
+                stages[k].append((t[i + 1] - t[i]) * 1e3)
+        return t[-1] - t[0]
+
+    for i in range(a.warmup):
+        print("[bench] refine warmup %d" % i, file=sys.stderr, flush=True)
+        step(False)
+    total = 0.0
+    for i in range(a.steps):
+        total += step(True)
+        print("[bench] refine step %d %s" % (i, {k: round(v[-1], 1) for k, v in stages.items()}), file=sys.stderr,
+              flush=True)
+    value = n * a.steps / total
+    nsym = {"c": 1, "d": 2}[a.sym[0].lower()] * int(a.sym[1:])
+    H = P // 2
+    # SURVEY-style algorithmic bytes: two insertions, the agreement pass and finish
+    b_ins = 4 * B * B + nsym * npix(P) * 96
+    b_agree = 4 * B * B + 2 * nsym * npix(P) * 96 + 12 * (H + 2) * P * nsym
+    b_fin = 20 * (H + 1) * P * P + 3 * 16 * (H + 1) * P * P + 4 * P ** 3
+    job_bytes = n * (2 * b_ins + b_agree) + b_fin
+    ag = sum(stages["agree_ms"]) / a.steps
+    out = {
+        "metric": "particles/s through the refine flow (insert, slice agreement, re-insert, finish)",
+        "value": round(value, 1), "unit": "particles/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(total / a.steps * 1e3, 2), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "%d x %dpx particles, pad %d, %s, %s, CTF phase flip, SNR %g, HBM-resident" % (
+            n, B, P, a.sym, a.mode, a.snr), "box": B, "pad": P, "particles": n, "sym": a.sym, "mode": a.mode},
+        "stage_ms_per_step": {k: round(sum(v) / a.steps, 2) for k, v in stages.items()},
+        "agreement_particles_per_s": round(n / (ag * 1e-3), 1),
+        "pipeline_roofline": {"effective_GBs": round(job_bytes * a.steps / total / 1e9, 1),
+                              "frac": round(job_bytes * a.steps / total / 1e9 / HBM_PEAK_GBS, 4),
+                              "bytes_per_job": job_bytes},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
+    if a.workload == "refine":
+        if a.gpus != 1:
+            print("bench.py: this build measures one GPU only; --gpus %d is not supported" % a.gpus, file=sys.stderr)
+            sys.exit(2)
+        return refine(a)
     if a.gpus != 1:
         if int(os.environ.get("RANK", "0")) == 0:
             print("bench.py: this build measures one GPU only (the path shards by particle; see the design notes); "

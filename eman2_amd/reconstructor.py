@@ -312,15 +312,26 @@ class FourierHipReconstructor:
                     img.set_attr(k, v)
         return res
 
-    def agreement_batch(self, imgs, xforms, weights=None, defocus=None, ctf=None, sub=True, apix=1.0):
+    def agreement_batch(self, imgs, xforms, weights=None, defocus=None, ctf=None, sub=True, apix=1.0, box=None,
+                        out=None):
+        """determine_slice_agreement for a batch of raw particles: numpy
+        (n,box,box) on the host, returning (n, 5) = norm, absqual,
+        absqual_lowres, qual, weight; or a device pointer (int) with ``box``
+        and ``out`` a device pointer to n x 5 floats, filled in place."""
         n, m, w, d, c = self._batch_args(imgs, xforms, weights, defocus, ctf)
-        a = _arr(imgs)
-        out = np.zeros((n, 5), np.float32)
-        self._check(self._lib.fh_agreement_batch(self._need(), _ptr(a), n, a.shape[-1], _ptr(m), _ptr(w),
+        if isinstance(imgs, int):
+            if out is None:
+                raise _lib.FourierHipError(3, "device particles need a device output pointer")
+            where, ptr, B, optr, res = _lib.FH_DEVICE, ctypes.c_void_p(imgs), int(box), ctypes.c_void_p(out), None
+        else:
+            a = _arr(imgs)
+            res = np.zeros((n, 5), np.float32)
+            where, ptr, B, optr = _lib.FH_HOST, _ptr(a), a.shape[-1], _ptr(res)
+        self._check(self._lib.fh_agreement_batch(self._need(), ptr, n, B, _ptr(m), _ptr(w),
                                                  None if d is None else _ptr(d),
                                                  None if c is None else ctypes.byref(c), int(bool(sub)),
-                                                 float(apix), _ptr(out), _lib.FH_HOST))
-        return out
+                                                 float(apix), optr, where))
+        return res
 
     # ---------------------------------------------------------------- finish
     def finish(self, doift=True, return_numpy=False):
