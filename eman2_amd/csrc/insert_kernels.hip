@@ -361,23 +361,32 @@ __device__ void own_contribution(const float2 *S, const float *t3dp, int nsym, i
         }
 }
 
-// One block row per (particle, symmetry op) slice; eight double sums per
-// particle accumulated with float64 atomics into acc[p][8].
+// One block row per (particle, symmetry op) slice, AGREE_PIX compare pixels
+// per thread; eight double sums per particle accumulated with float64
+// atomics into acc[p][8].
+constexpr int AGREE_PIX = 4;
 __global__ void __launch_bounds__(256) k_agree_batch(const float2 *__restrict__ slices, int nsym,
                                                      const float *__restrict__ t3d, const float *__restrict__ wq,
                                                      InsertConst c, float rmax, int sub,
                                                      const float2 *__restrict__ data, const float *__restrict__ norm,
-                                                     const float *__restrict__ exptab, const float *__restrict__ rtab,
+                                                     const float *__restrict__ exptab, const float *__restrict__ rtab_g,
                                                      double *__restrict__ acc_out)
 {
     const int H = c.H, P = c.P;
     const int q = blockIdx.y, p = q / nsym;
-    const int pidx = blockIdx.x * blockDim.x + threadIdx.x;
     const int W = H + 2;
     const float2 *S = slices + (size_t)p * P * (H + 1);
     const float *t3dp = t3d + 9 * (size_t)p * nsym;
     const float weight = wq[q];
+    // the fast_exp and fl(i/P) tables in LDS
+    extern __shared__ float agree_tabs[];
+    float *etab = agree_tabs, *rtab = agree_tabs + EXPN;
+    for (int i = threadIdx.x; i < EXPN; i += blockDim.x) etab[i] = exptab[i];
+    for (int i = threadIdx.x; i < P + 2; i += blockDim.x) rtab[i] = rtab_g[i];
+    __syncthreads();
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int it = 0; it < AGREE_PIX; it++) {
+    const int pidx = (blockIdx.x * AGREE_PIX + it) * blockDim.x + threadIdx.x;
     if (pidx < W * P) {
         const int x = pidx % W, y = pidx / W - H;
         const float *m = t3d + 9 * (size_t)q;
@@ -443,6 +452,7 @@ __global__ void __launch_bounds__(256) k_agree_batch(const float2 *__restrict__ 
             }
         }
     }
+    }
     __shared__ double red[8][256];
 #pragma unroll
     for (int k = 0; k < 8; k++) red[k][threadIdx.x] = acc[k];
@@ -462,8 +472,9 @@ hipError_t launch_agree_batch(const float2 *slices, int nsym, const float *t3d, 
 {
     if (nslices <= 0) return hipSuccess;
     c.corners = 0;
-    hipLaunchKernelGGL(k_agree_batch, dim3(agree_blocks(c.P), nslices), dim3(256), 0, s, slices, nsym, t3d, wq, c, rmax,
-                       sub, data, norm, exptab, rtab, acc);
+    const int nb = ((c.P / 2 + 2) * c.P + 256 * AGREE_PIX - 1) / (256 * AGREE_PIX);
+    hipLaunchKernelGGL(k_agree_batch, dim3(nb, nslices), dim3(256), sizeof(float) * (EXPN + c.P + 2), s, slices, nsym,
+                       t3d, wq, c, rmax, sub, data, norm, exptab, rtab, acc);
     return hipGetLastError();
 }
 
