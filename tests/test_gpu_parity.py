@@ -332,6 +332,29 @@ def test_agreement_batch_matches_sequential(oracle):
             assert abs(got[i, j] - o[k]) <= 1e-4 * max(1.0, abs(o[k])), (i, k, got[i, j], o[k])
 
 
+def test_agreement_batch_device_pointers():
+    """Device-resident particles and output (FH_DEVICE) give the host path's
+    results: the same kernels, only the staging differs."""
+    import torch
+    P, B, n = 32, 24, 200
+    ph = S.phantom(10, 7.0, seed=45)
+    imgs, mats, dfs = S.particles(ph, n, B, seed=46, snr=0.5, ctf=dict(voltage=300.0, cs=2.7, ampcont=10.0))
+    w = np.ones(n, np.float32)
+    ctf = dict(voltage=300.0, cs=2.7, ampcont=10.0, apix=1.0)
+    r = recon(P)
+    r.insert_batch(imgs, list(mats), w, defocus=dfs, ctf=ctf)
+    host = r.agreement_batch(imgs[:40], mats[:40], w[:40], defocus=dfs[:40], ctf=ctf, sub=True, apix=1.0)
+    dimg = torch.from_numpy(np.ascontiguousarray(imgs[:40], np.float32)).to("cuda:0")
+    dout = torch.zeros((40, 5), dtype=torch.float32, device="cuda:0")
+    res = r.agreement_batch(int(dimg.data_ptr()), mats[:40], w[:40], defocus=dfs[:40], ctf=ctf, sub=True, apix=1.0,
+                            box=B, out=int(dout.data_ptr()))
+    r.sync()
+    assert res is None
+    np.testing.assert_allclose(dout.cpu().numpy(), host, rtol=1e-6, atol=1e-7)
+    with pytest.raises(RuntimeError, match="device output"):
+        r.agreement_batch(int(dimg.data_ptr()), mats[:40], w[:40], box=B)
+
+
 def test_transform_with_translation_rejected():
     P = 16
     r = recon(P)
