@@ -332,27 +332,51 @@ def test_agreement_batch_matches_sequential(oracle):
             assert abs(got[i, j] - o[k]) <= 1e-4 * max(1.0, abs(o[k])), (i, k, got[i, j], o[k])
 
 
+def _hip_runtime():
+    """The HIP runtime libfourier_hip.so is linked against (already loaded)."""
+    import ctypes
+    with open("/proc/self/maps") as f:
+        paths = {line.split()[-1] for line in f if "libamdhip64" in line}
+    assert paths, "libamdhip64 not loaded"
+    rocm = sorted(q for q in paths if q.startswith("/opt/rocm"))  # the system runtime, not a bundled copy
+    hip = ctypes.CDLL((rocm or sorted(paths))[0])
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    return hip
+
+
 def test_agreement_batch_device_pointers():
     """Device-resident particles and output (FH_DEVICE) give the host path's
     results: the same kernels, only the staging differs."""
-    import torch
-    P, B, n = 32, 24, 200
+    import ctypes
+    P, B, n, m = 32, 24, 200, 40
     ph = S.phantom(10, 7.0, seed=45)
     imgs, mats, dfs = S.particles(ph, n, B, seed=46, snr=0.5, ctf=dict(voltage=300.0, cs=2.7, ampcont=10.0))
     w = np.ones(n, np.float32)
     ctf = dict(voltage=300.0, cs=2.7, ampcont=10.0, apix=1.0)
     r = recon(P)
     r.insert_batch(imgs, list(mats), w, defocus=dfs, ctf=ctf)
-    host = r.agreement_batch(imgs[:40], mats[:40], w[:40], defocus=dfs[:40], ctf=ctf, sub=True, apix=1.0)
-    dimg = torch.from_numpy(np.ascontiguousarray(imgs[:40], np.float32)).to("cuda:0")
-    dout = torch.zeros((40, 5), dtype=torch.float32, device="cuda:0")
-    res = r.agreement_batch(int(dimg.data_ptr()), mats[:40], w[:40], defocus=dfs[:40], ctf=ctf, sub=True, apix=1.0,
-                            box=B, out=int(dout.data_ptr()))
-    r.sync()
-    assert res is None
-    np.testing.assert_allclose(dout.cpu().numpy(), host, rtol=1e-6, atol=1e-7)
-    with pytest.raises(RuntimeError, match="device output"):
-        r.agreement_batch(int(dimg.data_ptr()), mats[:40], w[:40], box=B)
+    host = r.agreement_batch(imgs[:m], mats[:m], w[:m], defocus=dfs[:m], ctf=ctf, sub=True, apix=1.0)
+    hip = _hip_runtime()
+    src = np.ascontiguousarray(imgs[:m], np.float32)
+    dimg, dout = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(dimg), src.nbytes) == 0
+    assert hip.hipMalloc(ctypes.byref(dout), m * 5 * 4) == 0
+    try:
+        assert hip.hipMemcpy(dimg, src.ctypes.data, src.nbytes, 1) == 0  # host to device
+        res = r.agreement_batch(dimg.value, mats[:m], w[:m], defocus=dfs[:m], ctf=ctf, sub=True, apix=1.0, box=B,
+                                out=dout.value)
+        r.sync()
+        got = np.zeros((m, 5), np.float32)
+        assert hip.hipMemcpy(got.ctypes.data, dout, got.nbytes, 2) == 0  # device to host
+        assert res is None
+        np.testing.assert_allclose(got, host, rtol=1e-6, atol=1e-7)
+        with pytest.raises(RuntimeError, match="device output"):
+            r.agreement_batch(dimg.value, mats[:m], w[:m], box=B)
+    finally:
+        hip.hipFree(dimg)
+        hip.hipFree(dout)
 
 
 def test_transform_with_translation_rejected():
