@@ -53,6 +53,7 @@ def parse():
     ap.add_argument("--spi", type=int, default=None, help="slices per work item")
     ap.add_argument("--chunk", type=int, default=None)
     ap.add_argument("--bpc", type=int, default=None, help="persistent workgroups per CU")
+    ap.add_argument("--graph", type=int, default=None, help="1: each chunk as one captured hipGraph")
     ap.add_argument("--plain-sample", type=int, default=200)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -265,7 +266,8 @@ def main():
 
     r = Reconstructors.get("fourier_hip", {"size": (P, P, P), "sym": a.sym, "mode": a.mode})
     tune = r.get_tuning()
-    for k, v in (("waves_per_block", a.wpb), ("slices_per_item", a.spi), ("chunk", a.chunk), ("blocks_per_cu", a.bpc)):
+    for k, v in (("waves_per_block", a.wpb), ("slices_per_item", a.spi), ("chunk", a.chunk), ("blocks_per_cu", a.bpc),
+                 ("graph", a.graph)):
         if v is not None:
             tune[k] = v
     r.set_tuning(**tune)
@@ -303,7 +305,8 @@ def main():
     launches = sum(s["chunks"] for s in stats)
     avg_ms = kern_ms / max(1, launches)
     per_launch_bytes = bytes_per_particle * (a.steps * n) / max(1, launches)
-    achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
+    # the graph form does not split per-kernel times: no kernel figure then
+    achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None
     traffic = None
     # HBM bytes per particle of the same kernel from the FETCH_SIZE /
     # WRITE_SIZE runs (scaled by a 1 GiB copy), when measured on this workload
@@ -313,8 +316,8 @@ def main():
             t = json.load(f)
         if t.get("P") == P and t.get("sym") == a.sym and t.get("box") == B:
             traffic = round(t["hbm_bytes_per_particle"] * (a.steps * n) / max(1, launches))
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+    roof = {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": traffic,
             "kernel": "k_scatter_insert", "avg_launch_ms": round(avg_ms, 3),
             "algorithmic_bytes_per_launch": int(per_launch_bytes)}
     # whole-pipeline effective bandwidth with the 4*B^2 raw read (BASELINE.md formula)

@@ -33,7 +33,8 @@ class Ctf(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("waves_per_block", _c_int), ("slices_per_item", _c_int), ("chunk", _c_int), ("blocks_per_cu", _c_int)]
+    _fields_ = [("waves_per_block", _c_int), ("slices_per_item", _c_int), ("chunk", _c_int), ("blocks_per_cu", _c_int),
+                ("graph", _c_int)]
 
 
 class Stats(ctypes.Structure):

@@ -55,6 +55,7 @@ def main():
     ap.add_argument("--spi", default="256,512,1024,2048")
     ap.add_argument("--bpc", default="8")
     ap.add_argument("--chunks", default="0,25000", help="0 = automatic (48 GiB of staging)")
+    ap.add_argument("--graphs", default="0,1", help="1 = each chunk as one captured hipGraph")
     ap.add_argument("--write", action="store_true", help="update tuning_table.json")
     a = ap.parse_args()
     import torch
@@ -73,9 +74,10 @@ def main():
     ptr = int(imgs.data_ptr())
     results = []
     grid = itertools.product([int(x) for x in a.wpb.split(",")], [int(x) for x in a.spi.split(",")],
-                             [int(x) for x in a.bpc.split(",")], [int(x) for x in a.chunks.split(",")])
-    for wpb, spi, bpc, chunk in grid:
-        r.set_tuning(waves_per_block=wpb, slices_per_item=spi, chunk=chunk, blocks_per_cu=bpc)
+                             [int(x) for x in a.bpc.split(",")], [int(x) for x in a.chunks.split(",")],
+                             [int(x) for x in a.graphs.split(",")])
+    for wpb, spi, bpc, chunk, graph in grid:
+        r.set_tuning(waves_per_block=wpb, slices_per_item=spi, chunk=chunk, blocks_per_cu=bpc, graph=graph)
         r.setup()
         r.insert_batch(ptr, mats, w, defocus=dfs, ctf=ctf, box=a.box)  # warm
         r.setup()
@@ -84,7 +86,7 @@ def main():
         r.sync()
         dt = time.perf_counter() - t0
         st = r.get_stats()
-        rec = {"waves_per_block": wpb, "slices_per_item": spi, "blocks_per_cu": bpc, "chunk": chunk,
+        rec = {"waves_per_block": wpb, "slices_per_item": spi, "blocks_per_cu": bpc, "chunk": chunk, "graph": graph,
                "particles_per_s": a.particles / dt, "insert_kernel_ms_per_particle": st["insert_kernel_ms"] / a.particles}
         results.append(rec)
         print(json.dumps(rec), flush=True)

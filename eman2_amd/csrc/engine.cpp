@@ -212,7 +212,7 @@ struct fh_reconstructor {
     BrickGeom geom{};
     BinBuffers bins{};
     long long bins_cap_slices = 0;
-    fh_tuning tune{2, 1024, 0, 8};
+    fh_tuning tune{2, 1024, 0, 8, 0};
     fh_stats stats{};
     hipEvent_t ev[6] = {};
 };
@@ -535,12 +535,21 @@ int fh_clear(fh_reconstructor *h)
     return FH_OK;
 }
 
+// prep kernels of n particles (box B) into h->slices; dfd: device defocus or null
+static hipError_t prep_kernels(fh_reconstructor *h, int n, int B, const float *dev_imgs, const float *dfd,
+                               const CtfConst &cc)
+{
+    hipError_t e;
+    if ((e = launch_edge_mean(dev_imgs, n, B, h->means, h->stream)) != hipSuccess) return e;
+    if ((e = launch_prep_rows(dev_imgs, n, B, h->P, h->means, h->slices, h->plan, h->tw, h->stream)) != hipSuccess)
+        return e;
+    return launch_prep_cols(n, B, h->P, h->slices, dfd, cc, h->scale, h->plan, h->tw, h->stream);
+}
+
 // prep of n particles already in h->imgs (box B) into h->slices
 static int prep_chunk(fh_reconstructor *h, int n, int B, const float *dev_imgs, const float *defocus_host,
                       const fh_ctf *ctf)
 {
-    FH_TRY(h, launch_edge_mean(dev_imgs, n, B, h->means, h->stream));
-    FH_TRY(h, launch_prep_rows(dev_imgs, n, B, h->P, h->means, h->slices, h->plan, h->tw, h->stream));
     const float *dfd = nullptr;
     CtfConst cc{};
     if (defocus_host && ctf) {
@@ -548,7 +557,72 @@ static int prep_chunk(fh_reconstructor *h, int n, int B, const float *dev_imgs, 
         dfd = h->dfs;
         cc = ctf_constants(*ctf, h->P);
     }
-    FH_TRY(h, launch_prep_cols(n, B, h->P, h->slices, dfd, cc, h->scale, h->plan, h->tw, h->stream));
+    FH_TRY(h, prep_kernels(h, n, B, dev_imgs, dfd, cc));
+    return FH_OK;
+}
+
+// Graph form of the batched insertion for device-resident particles: per
+// chunk, the host data (rotations, weights, defocus) is copied first, then
+// the whole launch sequence -- prep, compose, binning, the persistent
+// scatter and the merge -- is captured into one hipGraph and launched.  No
+// host read sits inside: the partial-brick buffer holds the binning's
+// worst-case item count, and the scatter takes its item count on the
+// device.  The captured sequence holds kernels only (no memset nodes).
+static int insert_batch_graph(fh_reconstructor *h, const float *imgs, int n, int box, const std::vector<float> &rot9,
+                              const float *weights, const float *defocus, const fh_ctf *ctf, int chunk)
+{
+    int rc = ensure_partials(h, h->bins.item_cap);
+    if (rc) return rc;
+    const InsertConst c = iconst(h);
+    fh_stats st{};
+    const int wpb = h->tune.waves_per_block;
+    const int grid = h->cus * h->tune.blocks_per_cu;
+    const CtfConst cc = (defocus && ctf) ? ctf_constants(*ctf, h->P) : CtfConst{};
+    FH_TRY(h, hipEventRecord(h->ev[0], h->stream));
+    for (int c0 = 0; c0 < n; c0 += chunk) {
+        const int m = std::min(chunk, n - c0);
+        const float *dimgs = imgs + (size_t)c0 * box * box;
+        FH_TRY(h, hipMemcpyAsync(h->rots, &rot9[9 * (size_t)c0], sizeof(float) * 9 * m, hipMemcpyHostToDevice, h->stream));
+        FH_TRY(h, hipMemcpyAsync(h->wts, weights + c0, sizeof(float) * m, hipMemcpyHostToDevice, h->stream));
+        const float *dfd = nullptr;
+        if (defocus && ctf) {
+            FH_TRY(h, hipMemcpyAsync(h->dfs, defocus + c0, sizeof(float) * m, hipMemcpyHostToDevice, h->stream));
+            dfd = h->dfs;
+        }
+        FH_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
+        hipError_t e = prep_kernels(h, m, box, dimgs, dfd, cc);
+        if (e == hipSuccess) e = launch_compose(h->rots, h->wts, m, h->syms_d, h->nsym, h->t3d, h->wq, h->stream);
+        if (e == hipSuccess)
+            e = launch_bin(h->t3d, h->wq, m * h->nsym, c, h->geom, h->bins, h->tune.slices_per_item, h->stream);
+        if (e == hipSuccess)
+            e = launch_scatter_insert(h->slices, m * h->nsym, h->nsym, h->t3d, h->wq, c, h->geom, h->bins, h->exptab,
+                                      h->rtab, grid, wpb, h->stream);
+        if (e == hipSuccess) e = launch_merge_partials(h->geom, h->bins, c, h->data, h->norm, h->stream);
+        hipGraph_t g = nullptr;
+        const hipError_t ee = hipStreamEndCapture(h->stream, &g);
+        if (e == hipSuccess) e = ee;
+        hipGraphExec_t ge = nullptr;
+        if (e == hipSuccess) e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        if (e == hipSuccess) e = hipGraphLaunch(ge, h->stream);
+        // the counters of this chunk (and the end of its graph) before the next capture
+        int ctl[4] = {0, 0, 0, 0};
+        if (e == hipSuccess) e = hipMemcpyAsync(ctl, h->bins.ctl, sizeof(ctl), hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (ge) (void)hipGraphExecDestroy(ge);
+        if (g) (void)hipGraphDestroy(g);
+        FH_TRY(h, e);
+        if (ctl[3]) return fail(h, FH_ERR_DEVICE, "internal error: brick list capacity exceeded");
+        st.slices += (long long)m * h->nsym;
+        st.pairs += ctl[2];
+        st.items += ctl[1];
+        st.chunks += 1;
+    }
+    FH_TRY(h, hipEventRecord(h->ev[5], h->stream));
+    FH_TRY(h, hipEventSynchronize(h->ev[5]));
+    float ms;
+    FH_TRY(h, hipEventElapsedTime(&ms, h->ev[0], h->ev[5]));
+    st.total_ms = ms;  // per-stage times are not split out in the graph form
+    h->stats = st;
     return FH_OK;
 }
 
@@ -658,6 +732,8 @@ int fh_insert_batch(fh_reconstructor *h, const float *imgs, int n, int box, cons
     if (rc) return rc;
     rc = ensure_bins(h, (long long)chunk * h->nsym);
     if (rc) return rc;
+    if (h->tune.graph && where == FH_DEVICE)
+        return insert_batch_graph(h, imgs, n, box, rot9, weights, defocus, ctf, chunk);
     const InsertConst c = iconst(h);
     fh_stats st{};
     float ms;
@@ -975,6 +1051,7 @@ int fh_set_tuning(fh_reconstructor *h, const fh_tuning *t)
         return fail(h, FH_ERR_INVALID_PARAM, "waves_per_block must be in [1, 4]");
     if (t->slices_per_item < 1 || t->chunk < 0 || t->blocks_per_cu < 1)
         return fail(h, FH_ERR_INVALID_PARAM, "tuning values must be positive (chunk 0: automatic)");
+    if (t->graph != 0 && t->graph != 1) return fail(h, FH_ERR_INVALID_PARAM, "graph must be 0 or 1");
     if (t->slices_per_item != h->tune.slices_per_item) free_bins(h);
     h->tune = *t;
     return FH_OK;
@@ -1005,7 +1082,7 @@ int fh_sync(fh_reconstructor *h)
 
 long long fh_footprint_bytes(int P, int nsym, int chunk)
 {
-    if (chunk <= 0) chunk = chunk_for(fh_tuning{1, 1, 0, 1}, P, P);
+    if (chunk <= 0) chunk = chunk_for(fh_tuning{1, 1, 0, 1, 0}, P, P);
     const long long v = 12LL * P * P * (P / 2 + 1);
     const long long sl = 8LL * chunk * P * (P / 2 + 1) + 4LL * chunk * P * P;
     const BrickGeom g = brick_geom(P);

@@ -22,7 +22,7 @@ import numpy as np
 from . import _lib
 from .transform import Transform
 
-TUNING_KEYS = ("waves_per_block", "slices_per_item", "chunk", "blocks_per_cu")
+TUNING_KEYS = ("waves_per_block", "slices_per_item", "chunk", "blocks_per_cu", "graph")
 
 
 class EMData:
@@ -357,8 +357,8 @@ class FourierHipReconstructor:
         return data, norm
 
     # ---------------------------------------------------------------- tuning
-    def set_tuning(self, waves_per_block=2, slices_per_item=1024, chunk=0, blocks_per_cu=8):
-        t = _lib.Tuning(int(waves_per_block), int(slices_per_item), int(chunk), int(blocks_per_cu))
+    def set_tuning(self, waves_per_block=2, slices_per_item=1024, chunk=0, blocks_per_cu=8, graph=0):
+        t = _lib.Tuning(int(waves_per_block), int(slices_per_item), int(chunk), int(blocks_per_cu), int(graph))
         self._check(self._lib.fh_set_tuning(self._need(), ctypes.byref(t)))
 
     def get_tuning(self):

@@ -80,6 +80,7 @@ typedef struct {
     int slices_per_item; /* slices a work item adds before writing its partial brick */
     int chunk;           /* particles per batched launch; 0 = as many as fit in 48 GiB of staging */
     int blocks_per_cu;   /* persistent grid: workgroups per compute unit */
+    int graph;           /* 1: device-resident batches run each chunk as one captured hipGraph */
 } fh_tuning;
 
 /* Timings and counters of the last batched call (device time, ms). */

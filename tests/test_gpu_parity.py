@@ -379,6 +379,46 @@ def test_agreement_batch_device_pointers():
         hip.hipFree(dout)
 
 
+@pytest.mark.parametrize("sym,mode", [("c1", "gauss_2"), ("d7", "trilinear")])
+def test_graph_batch_matches_eager(sym, mode):
+    """The hipGraph form (device-resident particles, one captured graph per
+    chunk, 4 chunks) against the eager form on the same device particles,
+    three graph runs with eager runs and synchronisations between them;
+    every graph run is compared with the eager result."""
+    import ctypes
+    P, B, n = 48, 32, 200
+    ph = S.phantom(12, 10.0, seed=47)
+    imgs, mats, dfs = S.particles(ph, n, B, seed=48, snr=0.5, ctf=dict(voltage=300.0, cs=2.7, ampcont=10.0))
+    w = np.ones(n, np.float32)
+    ctf = dict(voltage=300.0, cs=2.7, ampcont=10.0, apix=1.0)
+    hip = _hip_runtime()
+    src = np.ascontiguousarray(imgs, np.float32)
+    dimg = ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(dimg), src.nbytes) == 0
+    try:
+        assert hip.hipMemcpy(dimg, src.ctypes.data, src.nbytes, 1) == 0
+        r = recon(P, sym, mode)
+        t = r.get_tuning()
+        r.set_tuning(**dict(t, chunk=64, graph=0))
+        r.setup()
+        r.insert_batch(dimg.value, list(mats), w, defocus=dfs, ctf=ctf, box=B)
+        ed, en = r.get_state()
+        for k in range(3):
+            r.set_tuning(**dict(t, chunk=64, graph=1))
+            r.setup()
+            r.insert_batch(dimg.value, list(mats), w, defocus=dfs, ctf=ctf, box=B)
+            gd, gn = r.get_state()
+            st = r.get_stats()
+            assert st["chunks"] == 4 and st["total_ms"] > 0
+            assert_state_close(gd, gn, ed, en, tol=1e-5)
+            r.set_tuning(**dict(t, chunk=64, graph=0))  # eager work between the graph runs
+            r.setup()
+            r.insert_batch(dimg.value, list(mats), w, defocus=dfs, ctf=ctf, box=B)
+            r.sync()
+    finally:
+        hip.hipFree(dimg)
+
+
 def test_transform_with_translation_rejected():
     P = 16
     r = recon(P)
