@@ -361,10 +361,128 @@ __device__ void own_contribution(const float2 *S, const float *t3dp, int nsym, i
         }
 }
 
-// One block row per (particle, symmetry op) slice, AGREE_PIX compare pixels
-// per thread; eight double sums per particle accumulated with float64
-// atomics into acc[p][8].
-constexpr int AGREE_PIX = 4;
+// Tiles of compare pixels: each workgroup takes a 32 x 16 tile of one
+// (particle, symmetry op) slice and stages the slice around it (3-pixel
+// halo) in LDS -- pixel positions, values and the insertion loop's pixel
+// filters -- so that the own-contribution gather of the slice's own plane
+// reads LDS; the other symmetry planes, the conjugate cells and windows
+// that leave the patch take the global gather.  Same arithmetic and order
+// as gather_cell.
+constexpr int AT_X = 32, AT_Y = 16, AT_HALO = 3;
+constexpr int AT_PX = AT_X + 2 * AT_HALO, AT_PY = AT_Y + 2 * AT_HALO;
+constexpr int AT_PATCH = AT_PX * AT_PY;
+
+struct AgreePatch {
+    const float4 *pos;   // xx, yy, zz, usable (1.0f) -- radius cut and origin range of the insertion loop
+    const float2 *val;   // slice_value
+    int x0, y0;          // pixel of patch entry 0
+};
+
+__device__ __forceinline__ void gather_cell_patch(const float2 *__restrict__ S, const float *m, const AgreePatch &pt,
+                                                  int ci, int cj, int ck, const InsertConst &c, float weight,
+                                                  const float *etab, const float *rtab, float &re, float &im,
+                                                  float &w)
+{
+    const int H = c.H;
+    const float n0 = m[1] * m[5] - m[2] * m[4], n1 = m[2] * m[3] - m[0] * m[5], n2 = m[0] * m[4] - m[1] * m[3];
+    const float dist = n0 * ci + n1 * cj + n2 * ck;
+    if (fabsf(dist) > 1.7330f) return;  // |p - c|_inf < 1 on the plane => |n.c| < sqrt(3)
+    const float s = m[0] * ci + m[1] * cj + m[2] * ck;
+    const float t = m[3] * ci + m[4] * cj + m[5] * ck;
+    const int xa = max(0, (int)ceilf(s - 1.7330f)), xb = min(H, (int)floorf(s + 1.7330f));
+    const int ya = max(-H, (int)ceilf(t - 1.7330f)), yb = min(H - 1, (int)floorf(t + 1.7330f));
+    if (xa < pt.x0 || xb >= pt.x0 + AT_PX || ya < pt.y0 || yb >= pt.y0 + AT_PY) {
+        gather_cell(S, m, ci, cj, ck, c, weight, etab, rtab, re, im, w);
+        return;
+    }
+    for (int y = ya; y <= yb; y++)
+        for (int x = xa; x <= xb; x++) {
+            const int e = (y - pt.y0) * AT_PX + (x - pt.x0);
+            const float4 q = pt.pos[e];
+            if (q.w == 0.f) continue;  // radius cut / origin out of range
+            const int x0 = (int)floorf(q.x), y0 = (int)floorf(q.y), z0 = (int)floorf(q.z);
+            if ((unsigned)(ci - x0) > 1u || (unsigned)(cj - y0) > 1u || (unsigned)(ck - z0) > 1u) continue;
+            const float gg = corner_weight(c.mode, ci, cj, ck, q.x, q.y, q.z, c.h, weight, etab);
+            const float2 dt = pt.val[e];
+            re += dt.x * gg;
+            im += dt.y * gg;
+            w += gg;
+        }
+}
+
+// own_contribution() with the slice's own plane gathered from the patch.
+__device__ void own_contribution_tile(const float2 *S, const float *t3dp, int nsym, int own, const AgreePatch &pt,
+                                      int xs, int ys, int zs, const InsertConst &c, float weight, const float *etab,
+                                      const float *rtab, float &ore, float &oim, float &ow)
+{
+    const int H = c.H, P = c.P;
+    auto cell_all = [&](int ci, int cj, int ck, float &r, float &i, float &w) {
+        for (int sop = 0; sop < nsym; sop++) {
+            if (sop == own) gather_cell_patch(S, t3dp + 9 * sop, pt, ci, cj, ck, c, weight, etab, rtab, r, i, w);
+            else gather_cell(S, t3dp + 9 * sop, ci, cj, ck, c, weight, etab, rtab, r, i, w);
+        }
+    };
+    ore = oim = ow = 0.f;
+    int jd[2], njd = 0, jn[2], njn = 0, kd[2], nkd = 0, kn[2], nkn = 0;
+    if (ys <= H) jd[njd++] = ys;
+    if (ys >= H) jd[njd++] = ys - P;
+    if (ys <= H) jn[njn++] = -ys;
+    if (ys >= H && P - ys <= H && P - ys != -ys) jn[njn++] = P - ys;
+    if (zs <= H) kd[nkd++] = zs;
+    if (zs >= H) kd[nkd++] = zs - P;
+    if (zs <= H) kn[nkn++] = -zs;
+    if (zs >= H && P - zs <= H && P - zs != -zs) kn[nkn++] = P - zs;
+    if (ys == 0 && zs == 0 && (xs == 0 || xs == H)) {
+        float r = 0, i = 0, w = 0;
+        cell_all(xs, 0, 0, r, i, w);
+        if (xs == H) cell_all(-H, 0, 0, r, i, w);
+        ore = r;
+        ow = w;
+        return;
+    }
+    for (int a = 0; a < njd; a++)
+        for (int b = 0; b < nkd; b++) {  // direct: value and weight
+            float r = 0, i = 0, w = 0;
+            cell_all(xs, jd[a], kd[b], r, i, w);
+            ore += r;
+            oim += i;
+            ow += w;
+        }
+    for (int a = 0; a < njn; a++)
+        for (int b = 0; b < nkn; b++) {
+            float r = 0, i = 0, w = 0;
+            if (xs == 0 || xs == H) {
+                // conjugate copy of the double insertion: value only
+                cell_all(xs, jn[a], kn[b], r, i, w);
+                ore += r;
+                oim -= i;
+                if (xs == H) {  // x = -P/2 cells: conjugate added twice, weight once
+                    float r2 = 0, i2 = 0, w2 = 0;
+                    cell_all(-H, jn[a], kn[b], r2, i2, w2);
+                    ore += 2.f * r2;
+                    oim -= 2.f * i2;
+                    ow += w2;
+                }
+            } else {
+                cell_all(-xs, jn[a], kn[b], r, i, w);
+                ore += r;
+                oim -= i;
+                ow += w;
+            }
+        }
+}
+
+// Sum of a double over the wave (xor butterflies on 32-bit halves).
+__device__ __forceinline__ double wave_sum_f64(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// One workgroup per (32 x 16 compare-pixel tile, particle x symmetry op
+// slice); eight double sums per particle accumulated with float64 atomics
+// into acc[p][8].
 __global__ void __launch_bounds__(256) k_agree_batch(const float2 *__restrict__ slices, int nsym,
                                                      const float *__restrict__ t3d, const float *__restrict__ wq,
                                                      InsertConst c, float rmax, int sub,
@@ -373,97 +491,113 @@ __global__ void __launch_bounds__(256) k_agree_batch(const float2 *__restrict__ 
                                                      double *__restrict__ acc_out)
 {
     const int H = c.H, P = c.P;
-    const int q = blockIdx.y, p = q / nsym;
+    const int q = blockIdx.y, p = q / nsym, own = q - p * nsym;
     const int W = H + 2;
+    const int ntx = (W + AT_X - 1) / AT_X;
+    const int tx0 = (blockIdx.x % ntx) * AT_X, ty0 = (int)(blockIdx.x / ntx) * AT_Y - H;
     const float2 *S = slices + (size_t)p * P * (H + 1);
     const float *t3dp = t3d + 9 * (size_t)p * nsym;
+    const float *m = t3d + 9 * (size_t)q;
     const float weight = wq[q];
-    // the fast_exp and fl(i/P) tables in LDS
-    extern __shared__ float agree_tabs[];
-    float *etab = agree_tabs, *rtab = agree_tabs + EXPN;
+    // LDS: patch positions, patch values, fast_exp and fl(i/P) tables
+    extern __shared__ float4 agree_lds[];
+    float4 *ppos = agree_lds;
+    float2 *pval = reinterpret_cast<float2 *>(ppos + AT_PATCH);
+    float *etab = reinterpret_cast<float *>(pval + AT_PATCH), *rtab = etab + EXPN;
     for (int i = threadIdx.x; i < EXPN; i += blockDim.x) etab[i] = exptab[i];
     for (int i = threadIdx.x; i < P + 2; i += blockDim.x) rtab[i] = rtab_g[i];
     __syncthreads();
+    AgreePatch pt{ppos, pval, tx0 - AT_HALO, ty0 - AT_HALO};
+    for (int e = threadIdx.x; e < AT_PATCH; e += blockDim.x) {
+        const int x = pt.x0 + e % AT_PX, y = pt.y0 + e / AT_PX;
+        float4 pos = make_float4(0.f, 0.f, 0.f, 0.f);
+        float2 val = make_float2(0.f, 0.f);
+        // the insertion loop's pixels: x in [0, P/2], y in [-P/2, P/2), radius cut (corners off here)
+        if (x >= 0 && x <= H && y >= -H && y <= H - 1 && x * x + y * y <= H * H + H) {
+            float xx, yy, zz;
+            pixel_coords(x, y, m, rtab, H, c.Pf, xx, yy, zz);
+            const int x0 = (int)floorf(xx), y0 = (int)floorf(yy), z0 = (int)floorf(zz);
+            const bool in = !(x0 < -H - 1 || y0 < -H - 1 || z0 < -H - 1 || x0 > H || y0 > H || z0 > H);
+            pos = make_float4(xx, yy, zz, in ? 1.f : 0.f);
+            if (in) val = slice_value(S, x, y, P, H);
+        }
+        ppos[e] = pos;
+        pval[e] = val;
+    }
+    __syncthreads();
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int it = 0; it < AGREE_PIX; it++) {
-    const int pidx = (blockIdx.x * AGREE_PIX + it) * blockDim.x + threadIdx.x;
-    if (pidx < W * P) {
-        const int x = pidx % W, y = pidx / W - H;
-        const float *m = t3d + 9 * (size_t)q;
-        if (!(x == 0 && y == 0)) {
-            const float rx = rtab[x + H], ry = rtab[y + H];
-            const float c0 = rx * m[0] + ry * m[3];
-            const float c1 = rx * m[1] + ry * m[4];
-            const float c2 = rx * m[2] + ry * m[5];
-            if (!(fabsf(c0) > 0.5f || fabsf(c1) >= 0.5f || fabsf(c2) >= 0.5f)) {
-                const float xx = c0 * c.Pf, yy = c1 * c.Pf, zz = c2 * c.Pf;
-                const float *Sf = reinterpret_cast<const float *>(S);
-                const int slen = (P + 2) * P;
-                const int fidx = 2 * x + (P + 2) * (y < 0 ? P + y : y);
-                const float d20 = fidx < slen ? Sf[fidx] : 0.f;
-                const float d21 = fidx + 1 < slen ? Sf[fidx + 1] : 0.f;
-                const int x0 = round_ref(xx), y0 = round_ref(yy), z0 = round_ref(zz);
-                int xs, ysn, zsn;
-                if (x0 < 0) {
-                    xs = -x0;
-                    ysn = y0 <= 0 ? -y0 : P - y0;
-                    zsn = z0 <= 0 ? -z0 : P - z0;
-                } else {
-                    xs = x0;
-                    ysn = y0 < 0 ? P + y0 : y0;
-                    zsn = z0 < 0 ? P + z0 : z0;
-                }
-                const size_t ci = cidx(xs, ysn, zsn, P, H);
-                float vr = 0.f, vi = 0.f;
-                const bool inb = !(abs(x0) >= H + 1 || abs(y0) > H || abs(z0) > H);
-                if (inb) {
-                    const float2 v = data[ci];
-                    vr = v.x;
-                    vi = v.y;
-                }
-                float nv = norm[ci];
-                if (sub) {
-                    float ore, oim, ow;
-                    own_contribution(S, t3dp, nsym, xs, ysn, zsn, c, weight, exptab, rtab, ore, oim, ow);
-                    if (inb) {
-                        vr = vr - ore;
-                        vi = vi - oim;
-                    }
-                    nv = nv - ow;
-                }
-                if (x0 < 0) vi = -vi;  // get_complex_at conjugate for x<0
-                if (nv != 0.f) {
-                    const float dt0 = __fdiv_rn(vr, nv), dt1 = __fdiv_rn(vi, nv), dt2 = nv;
-                    const float r = sqrt_int_rn(x * x + y * y);  // Util::hypot_fast
-                    const float dotv = (dt0 * d20 + dt1 * d21) * dt2;
-                    const float pw = (dt0 * dt0 + dt1 * dt1) * dt2;
-                    const float pw2 = (d20 * d20 + d21 * d21) * dt2;
-                    if (r > 3 && r < rmax) {
-                        acc[4] += dotv;
-                        acc[5] += pw;
-                        acc[6] += pw2;
-                    }
-                    acc[1] += dt2;
-                    acc[0] += dotv;
-                    acc[2] += pw;
-                    acc[3] += pw2;
-                    acc[7] += 1.0;
-                }
+    for (int it = 0; it < AT_Y * AT_X / 256; it++) {
+        const int x = tx0 + (threadIdx.x % AT_X), y = ty0 + (int)(threadIdx.x / AT_X) + it * (256 / AT_X);
+        if (x >= W || y >= H || (x == 0 && y == 0)) continue;
+        const float rx = rtab[x + H], ry = rtab[y + H];
+        const float c0 = rx * m[0] + ry * m[3];
+        const float c1 = rx * m[1] + ry * m[4];
+        const float c2 = rx * m[2] + ry * m[5];
+        if (fabsf(c0) > 0.5f || fabsf(c1) >= 0.5f || fabsf(c2) >= 0.5f) continue;
+        const float xx = c0 * c.Pf, yy = c1 * c.Pf, zz = c2 * c.Pf;
+        const float *Sf = reinterpret_cast<const float *>(S);
+        const int slen = (P + 2) * P;
+        const int fidx = 2 * x + (P + 2) * (y < 0 ? P + y : y);
+        const float d20 = fidx < slen ? Sf[fidx] : 0.f;
+        const float d21 = fidx + 1 < slen ? Sf[fidx + 1] : 0.f;
+        const int x0 = round_ref(xx), y0 = round_ref(yy), z0 = round_ref(zz);
+        int xs, ysn, zsn;
+        if (x0 < 0) {
+            xs = -x0;
+            ysn = y0 <= 0 ? -y0 : P - y0;
+            zsn = z0 <= 0 ? -z0 : P - z0;
+        } else {
+            xs = x0;
+            ysn = y0 < 0 ? P + y0 : y0;
+            zsn = z0 < 0 ? P + z0 : z0;
+        }
+        const size_t ci = cidx(xs, ysn, zsn, P, H);
+        float vr = 0.f, vi = 0.f;
+        const bool inb = !(abs(x0) >= H + 1 || abs(y0) > H || abs(z0) > H);
+        if (inb) {
+            const float2 v = data[ci];
+            vr = v.x;
+            vi = v.y;
+        }
+        float nv = norm[ci];
+        if (sub) {
+            float ore, oim, ow;
+            own_contribution_tile(S, t3dp, nsym, own, pt, xs, ysn, zsn, c, weight, etab, rtab, ore, oim, ow);
+            if (inb) {
+                vr = vr - ore;
+                vi = vi - oim;
             }
+            nv = nv - ow;
+        }
+        if (x0 < 0) vi = -vi;  // get_complex_at conjugate for x<0
+        if (nv != 0.f) {
+            const float dt0 = __fdiv_rn(vr, nv), dt1 = __fdiv_rn(vi, nv), dt2 = nv;
+            const float r = sqrt_int_rn(x * x + y * y);  // Util::hypot_fast
+            const float dotv = (dt0 * d20 + dt1 * d21) * dt2;
+            const float pw = (dt0 * dt0 + dt1 * dt1) * dt2;
+            const float pw2 = (d20 * d20 + d21 * d21) * dt2;
+            if (r > 3 && r < rmax) {
+                acc[4] += dotv;
+                acc[5] += pw;
+                acc[6] += pw2;
+            }
+            acc[1] += dt2;
+            acc[0] += dotv;
+            acc[2] += pw;
+            acc[3] += pw2;
+            acc[7] += 1.0;
         }
     }
-    }
-    __shared__ double red[8][256];
+    __shared__ double red[4][8];
 #pragma unroll
-    for (int k = 0; k < 8; k++) red[k][threadIdx.x] = acc[k];
+    for (int k = 0; k < 8; k++) {
+        const double v = wave_sum_f64(acc[k]);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][k] = v;
+    }
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s)
-#pragma unroll
-            for (int k = 0; k < 8; k++) red[k][threadIdx.x] += red[k][threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x < 8) unsafeAtomicAdd(&acc_out[(size_t)p * 8 + threadIdx.x], red[threadIdx.x][0]);
+    if (threadIdx.x < 8)
+        unsafeAtomicAdd(&acc_out[(size_t)p * 8 + threadIdx.x],
+                        red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x]);
 }
 
 hipError_t launch_agree_batch(const float2 *slices, int nsym, const float *t3d, const float *wq, int nslices,
@@ -472,9 +606,10 @@ hipError_t launch_agree_batch(const float2 *slices, int nsym, const float *t3d, 
 {
     if (nslices <= 0) return hipSuccess;
     c.corners = 0;
-    const int nb = ((c.P / 2 + 2) * c.P + 256 * AGREE_PIX - 1) / (256 * AGREE_PIX);
-    hipLaunchKernelGGL(k_agree_batch, dim3(nb, nslices), dim3(256), sizeof(float) * (EXPN + c.P + 2), s, slices, nsym,
-                       t3d, wq, c, rmax, sub, data, norm, exptab, rtab, acc);
+    const int ntx = (c.P / 2 + 2 + AT_X - 1) / AT_X, nty = (c.P + AT_Y - 1) / AT_Y;
+    const size_t lds = (sizeof(float4) + sizeof(float2)) * AT_PATCH + sizeof(float) * (EXPN + c.P + 2);
+    hipLaunchKernelGGL(k_agree_batch, dim3(ntx * nty, nslices), dim3(256), lds, s, slices, nsym, t3d, wq, c, rmax, sub,
+                       data, norm, exptab, rtab, acc);
     return hipGetLastError();
 }
 
