@@ -75,6 +75,9 @@ EXPORTS = {
     "fh_preprocess_slice": (_c_int, [_vp, _vp, _vp]),
     "fh_insert_slice": (_c_int, [_vp, _vp, _c_int, _vp, _c_float]),
     "fh_insert_batch": (_c_int, [_vp, _vp, _c_int, _c_int, _vp, _vp, _vp, ctypes.POINTER(Ctf), _c_int]),
+    "fh_insert_slice_ssnr": (_c_int, [_vp, _vp, _c_int, _vp, _c_float, _vp, _c_int]),
+    "fh_insert_batch_ssnr": (_c_int, [_vp, _vp, _c_int, _c_int, _vp, _vp, _vp, _c_int, _vp,
+                                      ctypes.POINTER(Ctf), _c_int]),
     "fh_prep_batch": (_c_int, [_vp, _vp, _c_int, _c_int, _vp, ctypes.POINTER(Ctf), _vp, _c_int]),
     "fh_insert_plain": (_c_int, [_vp, _vp, _c_int, _c_int, _vp, _vp, _vp, ctypes.POINTER(Ctf)]),
     "fh_determine_slice_agreement": (_c_int, [_vp, _vp, _c_int, _vp, _c_float, _c_int, _c_float, _vp]),
@@ -87,6 +90,7 @@ EXPORTS = {
     "fh_get_stats": (_c_int, [_vp, ctypes.POINTER(Stats)]),
     "fh_sync": (_c_int, [_vp]),
     "fh_footprint_bytes": (_c_ll, [_c_int, _c_int, _c_int]),
+    "fh_get_symmetries": (_c_int, [ctypes.c_char_p, ctypes.c_void_p, _c_int]),
 }
 
 _LIB = None

@@ -22,8 +22,12 @@ TABLE = os.path.join(HERE, "tuning_table.json")
 
 
 def nsym_of(sym):
-    s = sym.lower()
-    return int(s[1:]) * (2 if s[0] == "d" else 1)
+    """Number of symmetry operators of a symmetry name (Symmetry3D::get_nsym)."""
+    from . import _lib
+    n = _lib.load().fh_get_symmetries(str(sym).encode(), None, 0)
+    if n < 0:
+        raise ValueError("unknown symmetry %r" % (sym,))
+    return n
 
 
 def load_table():

@@ -107,24 +107,92 @@ void euler_eman(double az, double alt, double phi, float *m9)
     m9[8] = (float)cos(altp);
 }
 
-// Symmetry3D::get_symmetries for c<n>/d<n> (libEM/symmetry.cpp:75-142,
-// 1407-1418 CSym::get_sym, 1452-1470 DSym::get_sym)
+// Symmetry3D::get_symmetries (libEM/symmetry.cpp:75-142 name parse,
+// :1240-1246): c<n> (CSym::get_sym :1407-1418), d<n> (DSym::get_sym
+// :1452-1470), h<n>[:<nstart>:<daz>:<tz>[:<maxtilt>]] (HSym::get_sym
+// :1648-1671, nstart 1 only, as the reference), tet / oct / icos / icos5
+// (TetrahedralSym :2007-2030, OctahedralSym :1950-1975, IcosahedralSym
+// :1822-1866).  Each operator is the 3x3 rotation of the eman-Euler
+// Transform the reference builds; the helical z translation is dropped
+// because insertion uses only the 3x3 part (transform.h:547-553).
 bool sym_ops(const std::string &name, std::vector<float> &out)
 {
-    if (name.size() < 2) return false;
-    char k = (char)tolower((unsigned char)name[0]);
-    if (k != 'c' && k != 'd') return false;
-    for (size_t i = 1; i < name.size(); i++)
-        if (!isdigit((unsigned char)name[i])) return false;
-    int n = atoi(name.c_str() + 1);
-    if (n <= 0) return false;
+    if (name.empty() || name.size() > 63) return false;
+    std::string low(name);
+    for (char &ch : low) ch = (char)tolower((unsigned char)ch);
     out.clear();
+    auto push = [&](double az, double alt, double phi) {
+        float m[9];
+        euler_eman(az, alt, phi, m);
+        out.insert(out.end(), m, m + 9);
+    };
+    const double rad2deg = 180.0 / 3.141592653589793238462643383279502884197169399;  // EMConsts::rad2deg
+    if (low == "tet") {
+        // faces: one along z (az 0, phi 0/120/240), three at acos(-1/3)
+        // (az 0/120/240, phi 60/180/300); ::acos(double) of the float -1/3
+        const float l1 = (float)(rad2deg * acos((double)(-1.0f / 3.0f)));
+        for (int r = 0; r < 4; r++)
+            for (int c = 0; c < 3; c++) {
+                if (r == 0) push(0.0, 0.0, 120.0 * c);
+                else push(120.0 * (r - 1), (double)l1, 60.0 + 120.0 * c);
+            }
+        return true;
+    }
+    if (low == "oct") {
+        // a cube face along z: alt 0, four alt-90 rows (az 0/90/180/270), alt 180;
+        // phi 0/90/180/270 in each row
+        for (int r = 0; r < 6; r++)
+            for (int c = 0; c < 4; c++) {
+                double az = (r >= 1 && r <= 4) ? 90.0 * (r - 1) : 0.0;
+                double alt = r == 0 ? 0.0 : (r == 5 ? 180.0 : 90.0);
+                push(az, alt, 90.0 * c);
+            }
+        return true;
+    }
+    if (low == "icos" || low == "icos5") {
+        // a pentagon normal to z; levels 0, atan(2), 180 - atan(2), 180 (::atan(double)
+        // of the float 2); the 234-degree alignment offset adds 90 to az and
+        // subtracts 90 from phi (symmetry.cpp:1820, 1845-1849)
+        const float l1 = (float)(rad2deg * atan((double)2.0f));
+        const float l2 = 180.0f - l1;
+        for (int r = 0; r < 12; r++)
+            for (int c = 0; c < 5; c++) {
+                float ring = c == 0 ? 0.0f : 360.0f - 72.0f * (float)c;  // 0, 288, 216, 144, 72
+                float az, alt, phi;
+                if (r == 0) { az = 0.0f; alt = 0.0f; phi = ring; }
+                else if (r <= 5) { az = 72.0f * (float)(r - 1); alt = l1; phi = c == 0 ? 36.0f : 396.0f - 72.0f * (float)c; }
+                else if (r <= 10) { az = 36.0f + 72.0f * (float)(r - 6); alt = l2; phi = ring; }
+                else { az = 0.0f; alt = 180.0f; phi = ring; }
+                push((double)(az + 90.0f), (double)alt, (double)(phi - 90.0f));
+            }
+        return true;
+    }
+    const char k = low[0];
+    if (k == 'h') {
+        // the reference's two sscanf forms and defaults (nsym 1, nstart 1, daz 0, tz 0, maxtilt 90)
+        int nsym = 1, nstart = 1;
+        float daz = 0.0f, tz = 0.0f, maxtilt = 90.0f;
+        if (sscanf(low.c_str() + 1, "%d:%d:%f:%f:%f", &nsym, &nstart, &daz, &tz, &maxtilt) < 4)
+            sscanf(low.c_str() + 1, "%d,%d,%f,%f,%f", &nsym, &nstart, &daz, &tz, &maxtilt);
+        if (nstart != 1 || nsym <= 0) return false;
+        for (int i = 0; i < nsym; i++) {
+            int ii = (i + 1) / 2;  // 0, 1, -1, 2, -2, ...
+            if (i > 1 && i % 2 == 0) ii = -ii;
+            double az = (double)(ii % nstart) * (360.0 / nstart) + floor((double)((float)ii / (float)nstart)) * (double)daz;
+            push(az, 0.0, 0.0);
+        }
+        return true;
+    }
+    if (k != 'c' && k != 'd') return false;
+    if (low.size() < 2) return false;
+    for (size_t i = 1; i < low.size(); i++)
+        if (!isdigit((unsigned char)low[i])) return false;
+    int n = atoi(low.c_str() + 1);
+    if (n <= 0) return false;
     if (k == 'c') {
         for (int i = 0; i < n; i++) {
             float az = (float)(i % n) * 360.0f / (float)n;
-            float m[9];
-            euler_eman((double)az, 0.0, 0.0, m);
-            out.insert(out.end(), m, m + 9);
+            push((double)az, 0.0, 0.0);
         }
         return true;
     }
@@ -138,9 +206,7 @@ bool sym_ops(const std::string &name, std::vector<float> &out)
             az = (float)(i % nsym) * 360.0f / (float)(nsym / 2);
             alt = 0.0f;
         }
-        float m[9];
-        euler_eman((double)az, (double)alt, 0.0, m);
-        out.insert(out.end(), m, m + 9);
+        push((double)az, (double)alt, 0.0);
     }
     return true;
 }
@@ -187,7 +253,7 @@ struct fh_reconstructor {
     // parameters
     int P = 0, H = 0, device = 0;
     std::string sym, mode_name;
-    int mode = 0, corners = 0, sqrtnorm = 0, verbose = 0;
+    int mode = 0, corners = 0, sqrtnorm = 0, verbose = 0, usessnr = 0;
     std::vector<float> syms;  // nsym x 9
     int nsym = 1;
     // device state
@@ -205,6 +271,7 @@ struct fh_reconstructor {
     float *imgs = nullptr, *means = nullptr, *rots = nullptr, *wts = nullptr, *dfs = nullptr;
     float2 *slices = nullptr;
     float *t3d = nullptr, *wq = nullptr;
+    float *rw = nullptr;        // class_ssnr radial weights, (P/2+1) per particle (usessnr)
     double *partial = nullptr;  // agreement partial sums
     double *acc8 = nullptr;     // batched agreement sums, 8 per particle
     int acc8_cap = 0;
@@ -236,7 +303,7 @@ static size_t vol_elems(int P) { return (size_t)P * P * (P / 2 + 1); }
 
 static void free_chunk(fh_reconstructor *h)
 {
-    for (float **p : {&h->imgs, &h->means, &h->rots, &h->wts, &h->dfs, &h->t3d, &h->wq})
+    for (float **p : {&h->imgs, &h->means, &h->rots, &h->wts, &h->dfs, &h->t3d, &h->wq, &h->rw})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
     if (h->slices) { (void)hipFree(h->slices); h->slices = nullptr; }
     h->cap_n = h->cap_B = 0;
@@ -276,6 +343,7 @@ static int ensure_chunk(fh_reconstructor *h, int n, int B)
     FH_TRY(h, hipMalloc(&h->slices, sizeof(float2) * (size_t)n * P * (H + 1)));
     FH_TRY(h, hipMalloc(&h->t3d, sizeof(float) * 9 * ns));
     FH_TRY(h, hipMalloc(&h->wq, sizeof(float) * ns));
+    if (h->usessnr) FH_TRY(h, hipMalloc(&h->rw, sizeof(float) * (size_t)n * (H + 1)));
     h->cap_n = n;
     h->cap_B = B;
     return FH_OK;
@@ -383,14 +451,16 @@ int fh_create(const fh_params *p, fh_reconstructor **out)
         g_create_error = "NotExistingObjectException: inserter mode '" + mode + "' (fourier_hip supports gauss_2, trilinear)";
         return FH_ERR_NOT_EXISTING;
     }
-    if (p->usessnr) {
-        g_create_error = "InvalidParameterException: usessnr is not supported by fourier_hip";
+    if (p->usessnr && p->corners) {
+        // with corners the reference indexes class_ssnr up to sqrt(2) times past its end
+        // (reconstructor.cpp:1171-1175)
+        g_create_error = "InvalidParameterException: usessnr with corners is not supported by fourier_hip";
         return FH_ERR_INVALID_PARAM;
     }
     std::string sym = p->sym ? p->sym : "c1";
     std::vector<float> ops;
     if (!sym_ops(sym, ops)) {
-        g_create_error = "NotExistingObjectException: symmetry '" + sym + "' (fourier_hip supports c<n>, d<n>)";
+        g_create_error = "NotExistingObjectException: symmetry '" + sym + "' (fourier_hip supports c<n>, d<n>, h<n>, tet, oct, icos)";
         return FH_ERR_NOT_EXISTING;
     }
     fh_reconstructor *h = new fh_reconstructor();
@@ -401,6 +471,7 @@ int fh_create(const fh_params *p, fh_reconstructor **out)
     h->mode_name = mode;
     h->mode = imode;
     h->corners = p->corners;
+    h->usessnr = p->usessnr ? 1 : 0;
     h->sqrtnorm = p->sqrtnorm;
     h->verbose = p->verbose;
     h->syms = ops;
@@ -569,7 +640,8 @@ static int prep_chunk(fh_reconstructor *h, int n, int B, const float *dev_imgs, 
 // worst-case item count, and the scatter takes its item count on the
 // device.  The captured sequence holds kernels only (no memset nodes).
 static int insert_batch_graph(fh_reconstructor *h, const float *imgs, int n, int box, const std::vector<float> &rot9,
-                              const float *weights, const float *defocus, const fh_ctf *ctf, int chunk)
+                              const float *weights, const float *defocus, const fh_ctf *ctf, int chunk,
+                              const float *rw_host)
 {
     int rc = ensure_partials(h, h->bins.item_cap);
     if (rc) return rc;
@@ -584,6 +656,9 @@ static int insert_batch_graph(fh_reconstructor *h, const float *imgs, int n, int
         const float *dimgs = imgs + (size_t)c0 * box * box;
         FH_TRY(h, hipMemcpyAsync(h->rots, &rot9[9 * (size_t)c0], sizeof(float) * 9 * m, hipMemcpyHostToDevice, h->stream));
         FH_TRY(h, hipMemcpyAsync(h->wts, weights + c0, sizeof(float) * m, hipMemcpyHostToDevice, h->stream));
+        if (rw_host)
+            FH_TRY(h, hipMemcpyAsync(h->rw, rw_host + (size_t)c0 * (h->H + 1), sizeof(float) * (size_t)m * (h->H + 1),
+                                     hipMemcpyHostToDevice, h->stream));
         const float *dfd = nullptr;
         if (defocus && ctf) {
             FH_TRY(h, hipMemcpyAsync(h->dfs, defocus + c0, sizeof(float) * m, hipMemcpyHostToDevice, h->stream));
@@ -596,7 +671,7 @@ static int insert_batch_graph(fh_reconstructor *h, const float *imgs, int n, int
             e = launch_bin(h->t3d, h->wq, m * h->nsym, c, h->geom, h->bins, h->tune.slices_per_item, h->stream);
         if (e == hipSuccess)
             e = launch_scatter_insert(h->slices, m * h->nsym, h->nsym, h->t3d, h->wq, c, h->geom, h->bins, h->exptab,
-                                      h->rtab, grid, wpb, h->stream);
+                                      h->rtab, rw_host ? h->rw : nullptr, grid, wpb, h->stream);
         if (e == hipSuccess) e = launch_merge_partials(h->geom, h->bins, c, h->data, h->norm, h->stream);
         hipGraph_t g = nullptr;
         const hipError_t ee = hipStreamEndCapture(h->stream, &g);
@@ -660,24 +735,44 @@ int fh_preprocess_slice(fh_reconstructor *h, const float *img, float *slice_out)
 }
 
 // plain insertion of one prepared slice (h->slices + idx) with rotation rot9
-static int insert_slice_plain(fh_reconstructor *h, const float2 *slice_d, const float *rot9, float weight, bool sync_each)
+static int insert_slice_plain(fh_reconstructor *h, const float2 *slice_d, const float *rot9, float weight, bool sync_each,
+                              const float *rw_d = nullptr)
 {
     const InsertConst c = iconst(h);
     for (int s = 0; s < h->nsym; s++) {
         float t[9];
         compose9(rot9, &h->syms[9 * s], t);
-        FH_TRY(h, launch_insert_plain(slice_d, c, t, weight, h->data, h->norm, h->exptab, h->rtab, h->stream));
+        FH_TRY(h, launch_insert_plain(slice_d, c, t, weight, h->data, h->norm, h->exptab, h->rtab, rw_d, h->stream));
         if (sync_each) FH_TRY(h, hipStreamSynchronize(h->stream));
     }
     return FH_OK;
 }
 
-int fh_insert_slice(fh_reconstructor *h, const float *data, int is_preprocessed, const float xform[12], float weight)
+// class_ssnr weighting (reconstructor.cpp:1160-1184): the weight of slice
+// radius r = hypot_fast_int(x, y) is max(0, ssnr[int(r * sscale)]) with
+// sscale = 2 (len - 1) / ny, for r in [0, P/2] (the radius cut keeps r <= P/2).
+static void ssnr_radial(int P, const float *ssnr, int len, float *out)
+{
+    const float iny = (float)P;
+    const float sscale = (float)(2.0 * (double)(len - 1) / (double)iny);
+    for (int r = 0; r <= P / 2; r++) {
+        int i = (int)((float)r * sscale);
+        if (i > len - 1) i = len - 1;
+        out[r] = std::max(0.0f, ssnr[i]);
+    }
+}
+
+static int insert_slice_impl(fh_reconstructor *h, const float *data, int is_preprocessed, const float xform[12],
+                             float weight, const float *ssnr, int ssnr_len)
 {
     if (!h) return FH_ERR_NULL;
     if (!data || !xform) return fail(h, FH_ERR_NULL, "NullPointerException: EMData pointer (input image) is NULL");
+    if (ssnr && ssnr_len < 1) return fail(h, FH_ERR_INVALID_PARAM, "InvalidParameterException: empty class_ssnr");
     std::lock_guard<std::mutex> lk(h->mu);
     if (!h->ready) return fail(h, FH_ERR_STATE, "insert_slice() before setup()");
+    // usessnr: weight -1 (the SSNR flag) with class_ssnr, else 0 (reconstructor.cpp:1082-1088)
+    const bool use_ssnr = h->usessnr && ssnr;
+    if (h->usessnr) weight = ssnr ? -1.0f : 0.0f;
     if (weight == 0) return FH_SKIPPED;
     FH_TRY(h, hipSetDevice(h->device));
     const int P = h->P, H = h->H;
@@ -694,10 +789,29 @@ int fh_insert_slice(fh_reconstructor *h, const float *data, int is_preprocessed,
         FH_TRY(h, launch_prep_rows(h->imgs, 1, P, P, h->means, h->slices, h->plan, h->tw, h->stream));
         FH_TRY(h, launch_prep_cols(1, P, P, h->slices, nullptr, CtfConst{}, h->scale, h->plan, h->tw, h->stream));
     }
-    rc = insert_slice_plain(h, h->slices, rot9.data(), weight, false);
+    if (use_ssnr) {
+        std::vector<float> rw(H + 1);
+        ssnr_radial(P, ssnr, ssnr_len, rw.data());
+        FH_TRY(h, hipMemcpyAsync(h->rw, rw.data(), sizeof(float) * (H + 1), hipMemcpyHostToDevice, h->stream));
+        // the pageable source must stay alive until the copy has been taken
+        FH_TRY(h, hipStreamSynchronize(h->stream));
+    }
+    rc = insert_slice_plain(h, h->slices, rot9.data(), weight, false, use_ssnr ? h->rw : nullptr);
     if (rc) return rc;
     FH_TRY(h, hipStreamSynchronize(h->stream));
     return FH_OK;
+}
+
+int fh_insert_slice(fh_reconstructor *h, const float *data, int is_preprocessed, const float xform[12], float weight)
+{
+    return insert_slice_impl(h, data, is_preprocessed, xform, weight, nullptr, 0);
+}
+
+int fh_insert_slice_ssnr(fh_reconstructor *h, const float *data, int is_preprocessed, const float xform[12],
+                         float weight, const float *class_ssnr, int ssnr_len)
+{
+    if (h && !class_ssnr) return fail(h, FH_ERR_NULL, "NullPointerException: class_ssnr is NULL");
+    return insert_slice_impl(h, data, is_preprocessed, xform, weight, class_ssnr, ssnr_len);
 }
 
 static int check_batch_args(fh_reconstructor *h, const float *imgs, int n, int box, const float *xforms,
@@ -712,11 +826,28 @@ static int check_batch_args(fh_reconstructor *h, const float *imgs, int n, int b
     return FH_OK;
 }
 
-int fh_insert_batch(fh_reconstructor *h, const float *imgs, int n, int box, const float *xforms, const float *weights,
-                    const float *defocus, const fh_ctf *ctf, int where)
+static int insert_batch_impl(fh_reconstructor *h, const float *imgs, int n, int box, const float *xforms,
+                             const float *weights, const float *defocus, const fh_ctf *ctf, int where,
+                             const float *ssnr, int ssnr_len)
 {
     if (!h) return FH_ERR_NULL;
     std::lock_guard<std::mutex> lk(h->mu);
+    if (h->usessnr && !ssnr) {
+        // usessnr without class_ssnr: every particle has weight 0 (reconstructor.cpp:1082-1088)
+        if (!h->ready) return fail(h, FH_ERR_STATE, "insert before setup()");
+        return FH_SKIPPED;
+    }
+    if (ssnr && ssnr_len < 1) return fail(h, FH_ERR_INVALID_PARAM, "InvalidParameterException: empty class_ssnr");
+    // with usessnr every particle carries the SSNR flag weight -1 and its radial table
+    std::vector<float> rw_host, flag;
+    if (h->usessnr) {
+        if (!imgs || !xforms) return fail(h, FH_ERR_NULL, "NullPointerException: batch input is NULL");
+        rw_host.resize((size_t)std::max(n, 0) * (h->H + 1));
+        for (int i = 0; i < n; i++) ssnr_radial(h->P, ssnr + (size_t)i * ssnr_len, ssnr_len, &rw_host[(size_t)i * (h->H + 1)]);
+        flag.assign((size_t)std::max(n, 1), -1.0f);
+        weights = flag.data();
+    }
+    const float *rwh = h->usessnr ? rw_host.data() : nullptr;
     int rc = check_batch_args(h, imgs, n, box, xforms, weights);
     if (rc) return rc;
     FH_TRY(h, hipSetDevice(h->device));
@@ -733,7 +864,7 @@ int fh_insert_batch(fh_reconstructor *h, const float *imgs, int n, int box, cons
     rc = ensure_bins(h, (long long)chunk * h->nsym);
     if (rc) return rc;
     if (h->tune.graph && where == FH_DEVICE)
-        return insert_batch_graph(h, imgs, n, box, rot9, weights, defocus, ctf, chunk);
+        return insert_batch_graph(h, imgs, n, box, rot9, weights, defocus, ctf, chunk, rwh);
     const InsertConst c = iconst(h);
     fh_stats st{};
     float ms;
@@ -754,6 +885,9 @@ int fh_insert_batch(fh_reconstructor *h, const float *imgs, int n, int box, cons
         if (rc) return rc;
         FH_TRY(h, hipMemcpyAsync(h->rots, &rot9[9 * (size_t)c0], sizeof(float) * 9 * m, hipMemcpyHostToDevice, h->stream));
         FH_TRY(h, hipMemcpyAsync(h->wts, weights + c0, sizeof(float) * m, hipMemcpyHostToDevice, h->stream));
+        if (rwh)
+            FH_TRY(h, hipMemcpyAsync(h->rw, rwh + (size_t)c0 * (h->H + 1), sizeof(float) * (size_t)m * (h->H + 1),
+                                     hipMemcpyHostToDevice, h->stream));
         FH_TRY(h, launch_compose(h->rots, h->wts, m, h->syms_d, h->nsym, h->t3d, h->wq, h->stream));
         FH_TRY(h, hipEventRecord(h->ev[1], h->stream));
         FH_TRY(h, launch_bin(h->t3d, h->wq, m * h->nsym, c, h->geom, h->bins, h->tune.slices_per_item, h->stream));
@@ -768,8 +902,8 @@ int fh_insert_batch(fh_reconstructor *h, const float *imgs, int n, int box, cons
         FH_TRY(h, hipEventRecord(h->ev[3], h->stream));
         if (ctl[1] > 0) {
             FH_TRY(h, launch_scatter_insert(h->slices, m * h->nsym, h->nsym, h->t3d, h->wq, c, h->geom, h->bins,
-                                            h->exptab, h->rtab, std::min(grid, (ctl[1] + wpb - 1) / wpb), wpb,
-                                            h->stream));
+                                            h->exptab, h->rtab, rwh ? h->rw : nullptr,
+                                            std::min(grid, (ctl[1] + wpb - 1) / wpb), wpb, h->stream));
         }
         FH_TRY(h, hipEventRecord(h->ev[4], h->stream));
         if (ctl[1] > 0) FH_TRY(h, launch_merge_partials(h->geom, h->bins, c, h->data, h->norm, h->stream));
@@ -794,6 +928,22 @@ int fh_insert_batch(fh_reconstructor *h, const float *imgs, int n, int box, cons
     }
     h->stats = st;
     return FH_OK;
+}
+
+int fh_insert_batch(fh_reconstructor *h, const float *imgs, int n, int box, const float *xforms, const float *weights,
+                    const float *defocus, const fh_ctf *ctf, int where)
+{
+    return insert_batch_impl(h, imgs, n, box, xforms, weights, defocus, ctf, where, nullptr, 0);
+}
+
+int fh_insert_batch_ssnr(fh_reconstructor *h, const float *imgs, int n, int box, const float *xforms,
+                         const float *weights, const float *class_ssnr, int ssnr_len, const float *defocus,
+                         const fh_ctf *ctf, int where)
+{
+    if (h && !class_ssnr) return fail(h, FH_ERR_NULL, "NullPointerException: class_ssnr is NULL");
+    if (h && !h->usessnr)  // usessnr off: class_ssnr is ignored, the weights apply
+        return insert_batch_impl(h, imgs, n, box, xforms, weights, defocus, ctf, where, nullptr, 0);
+    return insert_batch_impl(h, imgs, n, box, xforms, weights, defocus, ctf, where, class_ssnr, ssnr_len);
 }
 
 int fh_prep_batch(fh_reconstructor *h, const float *imgs, int n, int box, const float *defocus, const fh_ctf *ctf,
@@ -833,6 +983,7 @@ int fh_insert_plain(fh_reconstructor *h, const float *imgs, int n, int box, cons
     std::lock_guard<std::mutex> lk(h->mu);
     int rc = check_batch_args(h, imgs, n, box, xforms, weights);
     if (rc) return rc;
+    if (h->usessnr) return FH_SKIPPED;  // no class_ssnr: weight 0 (reconstructor.cpp:1082-1088)
     FH_TRY(h, hipSetDevice(h->device));
     std::vector<float> rot9;
     rc = check_xforms(h, xforms, n, 1, rot9);
@@ -891,7 +1042,7 @@ static int agree_one(fh_reconstructor *h, const float2 *slice_d, const float *ro
     FH_TRY(h, hipMemcpyAsync(h->t3d, t.data(), sizeof(float) * t.size(), hipMemcpyHostToDevice, h->stream));
     if (sub)
         for (int s = 0; s < h->nsym; s++)
-            FH_TRY(h, launch_insert_plain(slice_d, c, &t[9 * (size_t)s], -weight, h->data, h->norm, h->exptab, h->rtab,
+            FH_TRY(h, launch_insert_plain(slice_d, c, &t[9 * (size_t)s], -weight, h->data, h->norm, h->exptab, h->rtab, nullptr,
                                           h->stream));
     FH_TRY(h, launch_agree(slice_d, h->nsym, h->t3d, h->nsym, c, agree_rmax(h->P, apix), h->data, h->norm, h->rtab,
                            h->partial, h->stream));
@@ -899,7 +1050,7 @@ static int agree_one(fh_reconstructor *h, const float2 *slice_d, const float *ro
     FH_TRY(h, hipMemcpyAsync(part.data(), h->partial, sizeof(double) * part.size(), hipMemcpyDeviceToHost, h->stream));
     if (sub)
         for (int s = 0; s < h->nsym; s++)
-            FH_TRY(h, launch_insert_plain(slice_d, c, &t[9 * (size_t)s], weight, h->data, h->norm, h->exptab, h->rtab,
+            FH_TRY(h, launch_insert_plain(slice_d, c, &t[9 * (size_t)s], weight, h->data, h->norm, h->exptab, h->rtab, nullptr,
                                           h->stream));
     FH_TRY(h, hipStreamSynchronize(h->stream));
     double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -1078,6 +1229,18 @@ int fh_sync(fh_reconstructor *h)
     FH_TRY(h, hipSetDevice(h->device));
     FH_TRY(h, hipStreamSynchronize(h->stream));
     return FH_OK;
+}
+
+int fh_get_symmetries(const char *sym, float *out9, int max_ops)
+{
+    std::vector<float> ops;
+    if (!sym || !sym_ops(sym, ops)) return -FH_ERR_NOT_EXISTING;
+    const int n = (int)ops.size() / 9;
+    if (out9) {
+        if (max_ops < n) return -FH_ERR_INVALID_PARAM;
+        memcpy(out9, ops.data(), sizeof(float) * ops.size());
+    }
+    return n;
 }
 
 long long fh_footprint_bytes(int P, int nsym, int chunk)

@@ -108,17 +108,18 @@ hipError_t launch_scale_f32(float *p, float v, size_t n, hipStream_t s);
 // ---- insertion (insert_kernels.hip) ----
 hipError_t launch_compose(const float *rots, const float *weights, int n, const float *syms, int nsym,
                           float *t3d, float *wq, hipStream_t s);
+// rw: optional radial weights ((P/2+1) floats, class_ssnr weighting) replacing weight
 hipError_t launch_insert_plain(const float2 *slice, InsertConst c, const float *t3d9, float weight,
                                float2 *data, float *norm, const float *exptab, const float *rtab,
-                               hipStream_t s);
+                               const float *rw, hipStream_t s);
 hipError_t launch_bin(const float *t3d, const float *wq, int nslices, InsertConst c, BrickGeom g,
                       BinBuffers b, int slices_per_item, hipStream_t s);
 int scan_tiles(const BrickGeom &g);
 size_t scatter_lds_bytes(int P, int waves_per_block);
 hipError_t launch_scatter_insert(const float2 *slices, int nslices, int nsym, const float *t3d,
                                  const float *wq, InsertConst c, BrickGeom g, BinBuffers b,
-                                 const float *exptab, const float *rtab, int grid, int waves_per_block,
-                                 hipStream_t s);
+                                 const float *exptab, const float *rtab, const float *rw, int grid,
+                                 int waves_per_block, hipStream_t s);
 hipError_t launch_merge_partials(BrickGeom g, BinBuffers b, InsertConst c, float2 *data, float *norm,
                                  hipStream_t s);
 int agree_blocks(int P);

@@ -148,16 +148,30 @@ __device__ __forceinline__ void add_complex_atomic(float2 *data, float *norm, in
 }
 
 // ------------------------------------------------------------ plain form
+// Util::hypot_fast_int(x, y) (libEM/util.cpp:764-785): round(hypot(x, y)),
+// exact for 0 <= x^2 + y^2 < 2^24 (sqrt of an integer is never a half-integer).
+__device__ __forceinline__ int hypot_int(int x, int y)
+{
+    const int n = x * x + y * y;
+    int r = (int)__builtin_amdgcn_sqrtf((float)n);
+    r -= (r * r > n);
+    r += ((r + 1) * (r + 1) <= n);
+    return r + (n > r * r + r);
+}
+
 __global__ void __launch_bounds__(256) k_insert_plain(const float2 *__restrict__ S, InsertConst c, Mat9 m,
                                                       float weight, float2 *data, float *norm,
                                                       const float *__restrict__ exptab,
-                                                      const float *__restrict__ rtab_g)
+                                                      const float *__restrict__ rtab_g,
+                                                      const float *__restrict__ rw)
 {
     const int H = c.H, P = c.P;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = (int)blockIdx.y - H;
     if (x > H) return;
     if (!c.corners && x * x + y * y > H * H + H) return;  // hypot_fast_int(x,y) > P/2
+    // class_ssnr weighting (reconstructor.cpp:1170-1184): the weight of radius r
+    if (rw) weight = rw[hypot_int(x, y)];
     float xx, yy, zz;
     pixel_coords(x, y, m.m, rtab_g, H, c.Pf, xx, yy, zz);
     const int x0 = (int)floorf(xx), y0 = (int)floorf(yy), z0 = (int)floorf(zz);
@@ -172,12 +186,12 @@ __global__ void __launch_bounds__(256) k_insert_plain(const float2 *__restrict__
 }
 
 hipError_t launch_insert_plain(const float2 *slice, InsertConst c, const float *t3d9, float weight, float2 *data,
-                               float *norm, const float *exptab, const float *rtab, hipStream_t s)
+                               float *norm, const float *exptab, const float *rtab, const float *rw, hipStream_t s)
 {
     Mat9 m;
     for (int i = 0; i < 9; i++) m.m[i] = t3d9[i];
     dim3 grid((c.H + 1 + 63) / 64, c.P);
-    hipLaunchKernelGGL(k_insert_plain, grid, dim3(64), 0, s, slice, c, m, weight, data, norm, exptab, rtab);
+    hipLaunchKernelGGL(k_insert_plain, grid, dim3(64), 0, s, slice, c, m, weight, data, norm, exptab, rtab, rw);
     return hipGetLastError();
 }
 
@@ -1116,11 +1130,12 @@ __device__ __forceinline__ void fetch_group_slice(GroupSlice &gs, int s, int cou
     gs.wgt = qv ? wq[q] : 0.f;
 }
 
-template <int A>
+template <int A, bool SSNR>
 __device__ __forceinline__ void scatter_group(const float2 *__restrict__ slices, int nsym, const GroupSlice &gs,
                                               const InsertConst &c, const int lo[3], int lane, float *acc,
                                               unsigned char *tag, int *mark, float4 *rec, const float *etab,
-                                              const float *rtab, PendPix &st, bool &fresh)
+                                              const float *rtab, const float *__restrict__ rw, PendPix &st,
+                                              bool &fresh)
 {
     const int H = c.H, P = c.P;
     const unsigned slice_elems = (unsigned)P * (unsigned)(H + 1);
@@ -1216,7 +1231,10 @@ __device__ __forceinline__ void scatter_group(const float2 *__restrict__ slices,
                 ns.px = px;
                 ns.py = py;
                 ns.pz = pz;
-                ns.pw = rb.z;
+                // class_ssnr weighting: the particle's weight at radius hypot_fast_int(x, y)
+                ns.pw = SSNR ? rw[(unsigned long long)(unsigned)__float_as_int(rb.w) * (unsigned)(H + 1) +
+                                  (unsigned)hypot_int(x, ry)]
+                             : rb.z;
                 ns.pend = true;
             }
         }
@@ -1235,12 +1253,13 @@ __device__ __forceinline__ void scatter_group(const float2 *__restrict__ slices,
 }
 
 // Partial brick layout in HBM: [field re/im/w][z][y][x] of the brick's 9^3 cells.
-template <int A>
+template <int A, bool SSNR>
 __device__ __forceinline__ void scatter_item(const float2 *__restrict__ slices, int nslices, int nsym,
                                              const float *__restrict__ t3d, const float *__restrict__ wq,
                                              const InsertConst &c, const int *__restrict__ list, const WorkItem &w,
                                              const int lo[3], int lane, float *acc, unsigned char *tag, int *mark,
-                                             float4 *rec, const float *etab, const float *rtab, float *out)
+                                             float4 *rec, const float *etab, const float *rtab,
+                                             const float *__restrict__ rw, float *out)
 {
     PendPix st;
     st.re = st.im = st.pw = 0.f;
@@ -1264,7 +1283,7 @@ __device__ __forceinline__ void scatter_item(const float2 *__restrict__ slices, 
             }
             fetch_group_slice(nxt, s1, w.count, qbuf, qbase, lane, nslices, t3d, wq);
         }
-        scatter_group<A>(slices, nsym, cur, c, lo, lane, acc, tag, mark, rec, etab, rtab, st, fresh);
+        scatter_group<A, SSNR>(slices, nsym, cur, c, lo, lane, acc, tag, mark, rec, etab, rtab, rw, st, fresh);
     }
     while (__ballot(st.pend)) {
         const bool win = scatter_elect(st, fresh, lo, lane, tag);
@@ -1287,11 +1306,15 @@ __device__ __forceinline__ void scatter_item(const float2 *__restrict__ slices, 
     wave_sync_lds();
 }
 
+// SSNR: per-pixel weights from the particles' radial class_ssnr tables rw
+// ((P/2+1) floats per particle) instead of the slice weight.
+template <bool SSNR>
 __global__ void __launch_bounds__(256) k_scatter_insert(const float2 *__restrict__ slices, int nslices, int nsym,
                                                         const float *__restrict__ t3d, const float *__restrict__ wq,
                                                         InsertConst c, BrickGeom g, BinBuffers bb,
                                                         const float *__restrict__ exptab_g,
-                                                        const float *__restrict__ rtab_g)
+                                                        const float *__restrict__ rtab_g,
+                                                        const float *__restrict__ rw)
 {
     extern __shared__ float4 lds4[];
     char *base = reinterpret_cast<char *>(lds4);
@@ -1322,9 +1345,9 @@ __global__ void __launch_bounds__(256) k_scatter_insert(const float2 *__restrict
             if (lane + 64 * wv < 3 * ACC_N) acc[lane + 64 * wv] = 0.f;
         wave_sync_lds();
         float *out = bb.part + (size_t)it * PART_FLOATS;
-        if (A == 0) scatter_item<0>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, tag, mark, rec, etab, rtab, out);
-        else if (A == 1) scatter_item<1>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, tag, mark, rec, etab, rtab, out);
-        else scatter_item<2>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, tag, mark, rec, etab, rtab, out);
+        if (A == 0) scatter_item<0, SSNR>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, tag, mark, rec, etab, rtab, rw, out);
+        else if (A == 1) scatter_item<1, SSNR>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, tag, mark, rec, etab, rtab, rw, out);
+        else scatter_item<2, SSNR>(slices, nslices, nsym, t3d, wq, c, bb.list, w, lo, lane, acc, tag, mark, rec, etab, rtab, rw, out);
     }
 }
 
@@ -1335,11 +1358,17 @@ size_t scatter_lds_bytes(int P, int waves_per_block)
 
 hipError_t launch_scatter_insert(const float2 *slices, int nslices, int nsym, const float *t3d, const float *wq,
                                  InsertConst c, BrickGeom g, BinBuffers b, const float *exptab, const float *rtab,
-                                 int grid, int waves_per_block, hipStream_t s)
+                                 const float *rw, int grid, int waves_per_block, hipStream_t s)
 {
     if (waves_per_block < 1 || waves_per_block > 4) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_scatter_insert, dim3(grid), dim3(64 * waves_per_block), scatter_lds_bytes(c.P, waves_per_block),
-                       s, slices, nslices, nsym, t3d, wq, c, g, b, exptab, rtab);
+    if (rw)
+        hipLaunchKernelGGL(k_scatter_insert<true>, dim3(grid), dim3(64 * waves_per_block),
+                           scatter_lds_bytes(c.P, waves_per_block), s, slices, nslices, nsym, t3d, wq, c, g, b, exptab,
+                           rtab, rw);
+    else
+        hipLaunchKernelGGL(k_scatter_insert<false>, dim3(grid), dim3(64 * waves_per_block),
+                           scatter_lds_bytes(c.P, waves_per_block), s, slices, nslices, nsym, t3d, wq, c, g, b, exptab,
+                           rtab, rw);
     return hipGetLastError();
 }
 

@@ -242,6 +242,11 @@ class FourierHipReconstructor:
         if a.shape != want:
             raise _lib.FourierHipError(3, "slice must be %s, got %s" % (want, a.shape))
         m = _xform12(xform)
+        if isinstance(img, EMData) and img.has_attr("class_ssnr"):
+            # usessnr weights the slice by its class_ssnr (reconstructor.cpp:1080-1088)
+            ss = np.ascontiguousarray(img.get_attr("class_ssnr"), np.float32).ravel()
+            return self._check(self._lib.fh_insert_slice_ssnr(self._need(), _ptr(a), pre, _ptr(m), float(weight),
+                                                              _ptr(ss), int(ss.size)))
         return self._check(self._lib.fh_insert_slice(self._need(), _ptr(a), pre, _ptr(m), float(weight)))
 
     def _batch_args(self, imgs, xforms, weights, defocus, ctf):
@@ -260,11 +265,18 @@ class FourierHipReconstructor:
                          float(ctf.get("ampcont", 10.0)), float(ctf["apix"]))
         return n, m, w, d, c
 
-    def insert_batch(self, imgs, xforms, weights=None, defocus=None, ctf=None, box=None):
+    def insert_batch(self, imgs, xforms, weights=None, defocus=None, ctf=None, box=None, class_ssnr=None):
         """Batched insertion of raw particles (numpy (n,box,box) on the host,
         or a device pointer (int) with ``box`` given) with the caller prep
-        fused in (edge-mean subtract, pad to size, optional CTF phase flip)."""
+        fused in (edge-mean subtract, pad to size, optional CTF phase flip).
+        ``class_ssnr`` (n, len): every particle's class_ssnr attribute, used
+        for per-radius weights when the reconstructor has ``usessnr``."""
         n, m, w, d, c = self._batch_args(imgs, xforms, weights, defocus, ctf)
+        ss = None
+        if class_ssnr is not None:
+            ss = np.ascontiguousarray(class_ssnr, np.float32)
+            if ss.ndim != 2 or ss.shape[0] != n:
+                raise _lib.FourierHipError(1, "class_ssnr must be (n, len) for n particles")
         if isinstance(imgs, int):
             where, ptr, B = _lib.FH_DEVICE, ctypes.c_void_p(imgs), int(box)
         else:
@@ -272,6 +284,10 @@ class FourierHipReconstructor:
             if a.shape[0] != n:
                 raise _lib.FourierHipError(3, "image / transform count mismatch")
             where, ptr, B = _lib.FH_HOST, _ptr(a), a.shape[-1]
+        if ss is not None:
+            return self._check(self._lib.fh_insert_batch_ssnr(self._need(), ptr, n, B, _ptr(m), _ptr(w), _ptr(ss),
+                                                              int(ss.shape[1]), None if d is None else _ptr(d),
+                                                              None if c is None else ctypes.byref(c), where))
         return self._check(self._lib.fh_insert_batch(self._need(), ptr, n, B, _ptr(m), _ptr(w),
                                                      None if d is None else _ptr(d),
                                                      None if c is None else ctypes.byref(c), where))

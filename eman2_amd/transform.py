@@ -105,3 +105,21 @@ class Transform:
 
     def __repr__(self):
         return "Transform(%s)" % np.array2string(self.matrix, precision=5)
+
+
+def get_symmetries(sym):
+    """Symmetry3D::get_symmetries (libEM/symmetry.cpp:1240-1246): the
+    rotation operators of a symmetry name (c<n>, d<n>, h<n>[:1:daz:tz], tet,
+    oct, icos) as an (n, 3, 3) float32 array, from the library's own table.
+    Unknown names raise NotExistingObjectException as Factory::get does."""
+    import ctypes
+    from . import _lib
+    L = _lib.load()
+    n = L.fh_get_symmetries(str(sym).encode(), None, 0)
+    if n < 0:
+        raise _lib.FourierHipError(-n, "symmetry %r (c<n>, d<n>, h<n>, tet, oct, icos)" % (sym,))
+    out = np.zeros((n, 3, 3), np.float32)
+    rc = L.fh_get_symmetries(str(sym).encode(), out.ctypes.data_as(ctypes.c_void_p), n)
+    if rc != n:
+        raise _lib.FourierHipError(-rc if rc < 0 else 6, "symmetry table of %r" % (sym,))
+    return out

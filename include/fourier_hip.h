@@ -53,11 +53,11 @@ typedef struct fh_reconstructor fh_reconstructor;
  * (libEM/reconstructor.h:501-517) restricted to what this path implements. */
 typedef struct {
     int nx, ny, nz;        /* "size": padded real-space box, nx==ny==nz, even */
-    const char *sym;       /* "sym": c<n> or d<n> (symmetry.cpp:75-142) */
+    const char *sym;       /* "sym": c<n>, d<n>, h<n>[:1:daz:tz], tet, oct, icos (symmetry.cpp:75-142) */
     const char *mode;      /* "mode": "gauss_2" or "trilinear" */
     int corners;           /* "corners" (reconstructor.cpp:1175-1181) */
     int sqrtnorm;          /* "sqrtnorm" (reconstructor.cpp:1564) */
-    int usessnr;           /* "usessnr": must be 0 (no class_ssnr support) */
+    int usessnr;           /* "usessnr": weight slices by their class_ssnr (not with corners) */
     int verbose;           /* "verbose" */
     int device;            /* HIP device ordinal */
 } fh_params;
@@ -120,6 +120,15 @@ int fh_preprocess_slice(fh_reconstructor *h, const float *img, float *slice_out)
 int fh_insert_slice(fh_reconstructor *h, const float *data, int is_preprocessed,
                     const float xform[12], float weight);
 
+/* insert_slice of a slice carrying a "class_ssnr" attribute
+ * (reconstructor.cpp:1080-1088,1158-1184): with usessnr set, each pixel's
+ * weight is max(0, class_ssnr[int(r * 2 (len-1) / P)]) at slice radius
+ * r = hypot_fast_int(x, y) and `weight` is ignored; with usessnr off the
+ * class_ssnr is ignored and `weight` applies, as in the reference.  (Without
+ * class_ssnr, fh_insert_slice on a usessnr handle returns FH_SKIPPED.) */
+int fh_insert_slice_ssnr(fh_reconstructor *h, const float *data, int is_preprocessed,
+                         const float xform[12], float weight, const float *class_ssnr, int ssnr_len);
+
 /* Batched insertion of n raw particles (box x box, box <= P) with the
  * caller prep of e2make3dpar.py:630-654,704-716 fused in: edge-mean
  * subtract, centre zero-pad to P, optional CTF_SIGN flip (defocus != NULL
@@ -128,6 +137,15 @@ int fh_insert_slice(fh_reconstructor *h, const float *data, int is_preprocessed,
 int fh_insert_batch(fh_reconstructor *h, const float *imgs, int n, int box,
                     const float *xforms, const float *weights, const float *defocus,
                     const fh_ctf *ctf, int where);
+
+/* fh_insert_batch for particles that all carry a class_ssnr of ssnr_len
+ * values (host, n x ssnr_len): the per-radius weighting of
+ * fh_insert_slice_ssnr on a usessnr handle, `weights` otherwise.  (On a
+ * usessnr handle fh_insert_batch and fh_insert_plain return FH_SKIPPED:
+ * particles without class_ssnr have weight 0.) */
+int fh_insert_batch_ssnr(fh_reconstructor *h, const float *imgs, int n, int box,
+                         const float *xforms, const float *weights, const float *class_ssnr,
+                         int ssnr_len, const float *defocus, const fh_ctf *ctf, int where);
 
 /* The prep stage of fh_insert_batch alone: n raw particles -> n slices
  * (host, n x P x (P+2) floats).  The caller prep of e2make3dpar.py:630-654,
@@ -175,6 +193,11 @@ int fh_get_tuning(const fh_reconstructor *h, fh_tuning *t);
 int fh_get_stats(const fh_reconstructor *h, fh_stats *s);
 /* wait for all queued device work of this handle */
 int fh_sync(fh_reconstructor *h);
+/* Symmetry3D::get_symmetries (libEM/symmetry.cpp:1240-1246): the rotation
+ * operators of a symmetry name as row-major 3x3 matrices; returns their
+ * count (out9 NULL: count only), -FH_ERR_NOT_EXISTING for an unknown name,
+ * -FH_ERR_INVALID_PARAM when max_ops is too small.  No device needed. */
+int fh_get_symmetries(const char *sym, float *out9, int max_ops);
 /* device bytes this handle holds / would hold after setup for size P
  * (chunk 0: the automatic chunk for particles of box P) */
 long long fh_footprint_bytes(int P, int nsym, int chunk);

@@ -47,6 +47,7 @@
 
 /* EMConsts (libEM/emobject.cpp:53-57) */
 static const double OR_PI = 3.141592653589793238462643383279502884197169399;
+#define OR_RAD2DEG (180.0 / OR_PI) /* EMConsts::rad2deg, libEM/emobject.cpp:60 */
 #define OR_DEG2RAD (OR_PI / 180.0)
 #define OR_I2G ((float)(4.0 / (M_PI * M_PI)))
 /* Transform::ERR_LIMIT (libEM/transform.cpp:57) */
@@ -220,37 +221,67 @@ void or_vec_times_transform(float x, float y, float z, const float m[12], float 
 /* Symmetry3D                                                                */
 /* ======================================================================== */
 
-static int parse_sym(const char *sym, char *kind, int *n)
+/* Symmetry names (libEM/symmetry.cpp:75-142): c<n>, d<n>, h<n>[:<nstart>:<daz>:<tz>[:<maxtilt>]]
+ * (or comma-separated), tet, oct, icos, icos5 (= icos).  kind is 'c', 'd',
+ * 'h', 't', 'o' or 'i'. */
+typedef struct {
+    char kind;
+    int n;        /* c/d/h: the n of the name */
+    float daz;    /* h: azimuth step (degrees) */
+} or_sym_spec;
+
+static int parse_sym(const char *sym, or_sym_spec *sp)
 {
-    /* libEM/symmetry.cpp:75-142 (c<n>, d<n>) */
     if (!sym || !sym[0]) return -1;
-    char c = (char)tolower((unsigned char)sym[0]);
+    char low[64];
+    size_t len = strlen(sym);
+    if (len >= sizeof(low)) return -1;
+    for (size_t i = 0; i <= len; i++) low[i] = (char)tolower((unsigned char)sym[i]);
+    sp->daz = 0.0f;
+    if (!strcmp(low, "tet") || !strcmp(low, "oct") || !strcmp(low, "icos") || !strcmp(low, "icos5")) {
+        sp->kind = low[0];
+        sp->n = low[0] == 't' ? 12 : low[0] == 'o' ? 24 : 60; /* symmetry.h:736,834,899 */
+        return 0;
+    }
+    char c = low[0];
+    if (c == 'h') {
+        /* symmetry.cpp:98-116: defaults nsym 1, nstart 1, daz 0, tz 0, maxtilt 90 */
+        int nsym = 1, nstart = 1;
+        float daz = 0.0f, tz = 0.0f, maxtilt = 90.0f;
+        if (sscanf(low + 1, "%d:%d:%f:%f:%f", &nsym, &nstart, &daz, &tz, &maxtilt) < 4)
+            sscanf(low + 1, "%d,%d,%f,%f,%f", &nsym, &nstart, &daz, &tz, &maxtilt);
+        if (nstart != 1) return -1; /* HSym::get_sym throws for nstart != 1 (symmetry.cpp:1656) */
+        if (nsym <= 0) return -1;
+        sp->kind = 'h';
+        sp->n = nsym;
+        sp->daz = daz;
+        return 0;
+    }
     if (c != 'c' && c != 'd') return -1;
-    if (!sym[1]) return -1;
-    for (const char *p = sym + 1; *p; p++)
+    if (!low[1]) return -1;
+    for (const char *p = low + 1; *p; p++)
         if (!isdigit((unsigned char)*p)) return -1;
-    int v = atoi(sym + 1);
+    int v = atoi(low + 1);
     if (v <= 0) return -1;
-    *kind = c;
-    *n = v;
+    sp->kind = c;
+    sp->n = v;
     return 0;
 }
 
 int or_sym_nsym(const char *sym)
 {
-    /* libEM/symmetry.h:289 (C: n), :397 (D: 2n) */
-    char k;
-    int n;
-    if (parse_sym(sym, &k, &n)) return -1;
-    return k == 'c' ? n : 2 * n;
+    /* libEM/symmetry.h:289 (C: n), :397 (D: 2n), :519 (H: nsym), :736/834/899 (12/24/60) */
+    or_sym_spec sp;
+    if (parse_sym(sym, &sp)) return -1;
+    return sp.kind == 'd' ? 2 * sp.n : sp.n;
 }
 
 int or_sym_get(const char *sym, float *out)
 {
-    char k;
-    int n;
-    if (parse_sym(sym, &k, &n)) return -1;
-    if (k == 'c') {
+    or_sym_spec sp;
+    if (parse_sym(sym, &sp)) return -1;
+    const int n = sp.n;
+    if (sp.kind == 'c') {
         /* CSym::get_sym libEM/symmetry.cpp:1407-1418 */
         for (int i = 0; i < n; i++) {
             float az = (float)(i % n) * 360.0f / (float)n;
@@ -258,20 +289,80 @@ int or_sym_get(const char *sym, float *out)
         }
         return n;
     }
-    /* DSym::get_sym libEM/symmetry.cpp:1452-1470 */
-    int nsym = 2 * n;
-    for (int i = 0; i < nsym; i++) {
-        float az, alt;
-        if (i >= nsym / 2) {
-            az = (float)((i % nsym) - nsym / 2) * 360.0f / (float)(nsym / 2);
-            alt = 180.0f;
-        } else {
-            az = (float)(i % nsym) * 360.0f / (float)(nsym / 2);
-            alt = 0.0f;
+    if (sp.kind == 'd') {
+        /* DSym::get_sym libEM/symmetry.cpp:1452-1470 */
+        int nsym = 2 * n;
+        for (int i = 0; i < nsym; i++) {
+            float az, alt;
+            if (i >= nsym / 2) {
+                az = (float)((i % nsym) - nsym / 2) * 360.0f / (float)(nsym / 2);
+                alt = 180.0f;
+            } else {
+                az = (float)(i % nsym) * 360.0f / (float)(nsym / 2);
+                alt = 0.0f;
+            }
+            or_euler_eman((double)az, (double)alt, 0.0, out + 12 * i);
         }
-        or_euler_eman((double)az, (double)alt, 0.0, out + 12 * i);
+        return nsym;
     }
-    return nsym;
+    if (sp.kind == 'h') {
+        /* HSym::get_sym libEM/symmetry.cpp:1648-1671 with nstart = 1: the i-th
+         * operator turns by k*daz about z, k = 0, 1, -1, 2, -2, ... (the
+         * z translation k*tz/apix does not enter the slice coordinates:
+         * Vec3f*Transform uses the 3x3 part, transform.h:547-553). */
+        for (int i = 0; i < n; i++) {
+            int k = (i + 1) / 2;
+            if (i > 1 && i % 2 == 0) k = -k;
+            /* (k % 1) * (360.0 / 1) + floor(float(k) / 1) * daz, in double (::floor) */
+            double az = (double)(k % 1) * 360.0 + floor((double)((float)k / 1.0f)) * (double)sp.daz;
+            or_euler_eman(az, 0.0, 0.0, out + 12 * i);
+        }
+        return n;
+    }
+    /* Platonic groups: (az, alt, phi) tables of TetrahedralSym /
+     * OctahedralSym / IcosahedralSym::get_sym (symmetry.cpp:2007-2030,
+     * 1950-1975, 1822-1866), the level angles computed as the reference does:
+     * the C library's double acos/atan of a float argument (symmetry.cpp sees
+     * only ::acos(double) and ::atan(double) from <cmath>), times the double
+     * degree factor, rounded to float. */
+    float ang[60][3];
+    if (sp.kind == 't') {
+        const float l1 = (float)(OR_RAD2DEG * acos((double)(-1.0f / 3.0f)));
+        const float lv[4] = {0.0f, l1, l1, l1};
+        for (int i = 0; i < 12; i++) {
+            int r = i / 3, c3 = i % 3;
+            ang[i][0] = r == 0 ? 0.0f : 120.0f * (float)(r - 1);
+            ang[i][1] = lv[r];
+            ang[i][2] = r == 0 ? 120.0f * (float)c3 : 60.0f + 120.0f * (float)c3;
+        }
+    } else if (sp.kind == 'o') {
+        /* rows: alt 0 (az 0), alt 90 with az 0/90/180/270, alt 180 (az 0); phi 0/90/180/270 */
+        for (int i = 0; i < 24; i++) {
+            int r = i / 4, c4 = i % 4;
+            ang[i][0] = (r >= 1 && r <= 4) ? 90.0f * (float)(r - 1) : 0.0f;
+            ang[i][1] = r == 0 ? 0.0f : r == 5 ? 180.0f : 90.0f;
+            ang[i][2] = 90.0f * (float)c4;
+        }
+    } else {
+        const float l1 = (float)(OR_RAD2DEG * atan((double)2.0f));
+        const float l2 = 180.0f - l1;
+        for (int i = 0; i < 60; i++) {
+            int r = i / 5, c5 = i % 5;
+            float az, alt, phi;
+            float phi5 = c5 == 0 ? 0.0f : 360.0f - 72.0f * (float)c5; /* 0, 288, 216, 144, 72 */
+            if (r == 0) { az = 0.0f; alt = 0.0f; phi = phi5; }
+            else if (r <= 5) { az = 72.0f * (float)(r - 1); alt = l1; phi = c5 == 0 ? 36.0f : 396.0f - 72.0f * (float)c5; }
+            else if (r <= 10) { az = 36.0f + 72.0f * (float)(r - 6); alt = l2; phi = phi5; }
+            else { az = 0.0f; alt = 180.0f; phi = phi5; }
+            /* get_az_alignment_offset() == 234 (symmetry.cpp:1820, 1845-1849) */
+            ang[i][0] = az + 90.0f;
+            ang[i][1] = alt;
+            ang[i][2] = phi - 90.0f;
+        }
+    }
+    for (int i = 0; i < n; i++)
+        or_euler_eman((double)ang[i][0], (double)ang[i][1], (double)ang[i][2], out + 12 * i);
+    return n;
 }
 
 /* ======================================================================== */
@@ -721,10 +812,15 @@ static inline void slice_get(const float *s, int P, int x, int y, float *re, flo
     /* FourierReconstructor::do_insert_slice_work (libEM/reconstructor.cpp:1122-1210) */          \
     static void insert_work##SUFFIX(int P, ACC *data, ACC *norm, const float *slice,               \
                                     const float rot[12], const float *syms, int nsym,              \
-                                    float weight, int mode, int corners)                           \
+                                    float weight, int mode, int corners,                           \
+                                    const float *ssnr, int nssnr)                                  \
     {                                                                                              \
         const float inx = (float)(P + 2), iny = (float)P;                                          \
         const int nx = P + 2, ny = P, nz = P;                                                      \
+        /* class_ssnr weighting, the weight < 0 branch (:1160-1184) */                             \
+        float sscale = 1.0f;                                                                       \
+        if (ssnr) sscale = (float)(2.0 * (double)(nssnr - 1) / (double)iny);                      \
+        float rweight = weight;                                                                    \
         for (int s = 0; s < nsym; s++) {                                                           \
             float t3d[12];                                                                         \
             or_transform_mult(rot, syms + 12 * s, t3d);                                            \
@@ -732,9 +828,15 @@ static inline void slice_get(const float *s, int P, int x, int y, float *re, flo
                 for (int x = 0; x < inx / 2; x++) {                                                \
                     float rx = (float)x / (inx - 2.0f);                                            \
                     float ry = (float)y / iny;                                                     \
-                    if (!corners) {                                                                \
+                    if (corners) {                                                                 \
+                        if (ssnr) {                                                                \
+                            int r = or_hypot_fast_int(x, y);                                       \
+                            rweight = fmaxf(0.0f, ssnr[(int)((float)r * sscale)]);                 \
+                        }                                                                          \
+                    } else {                                                                       \
                         int r = or_hypot_fast_int(x, y);                                           \
                         if (r > iny / 2 && fabsf(inx - iny) < 3) continue;                         \
+                        if (ssnr) rweight = fmaxf(0.0f, ssnr[(int)((float)r * sscale)]);           \
                     }                                                                              \
                     float c[3];                                                                    \
                     or_vec_times_transform(rx, ry, 0.0f, t3d, c);                                  \
@@ -743,7 +845,7 @@ static inline void slice_get(const float *s, int P, int x, int y, float *re, flo
                     float zz = c[2] * (float)nz;                                                   \
                     float dr, di;                                                                  \
                     slice_get(slice, P, x, y, &dr, &di);                                           \
-                    insert_pixel##SUFFIX(data, norm, P, mode, xx, yy, zz, dr, di, weight);         \
+                    insert_pixel##SUFFIX(data, norm, P, mode, xx, yy, zz, dr, di, rweight);        \
                 }                                                                                  \
             }                                                                                      \
         }                                                                                          \
@@ -760,7 +862,22 @@ void or_insert_slice_work(int P, float *data, float *norm, const float *slice, c
     float r[12];
     memcpy(r, rot, sizeof(r));
     or_rotation_only(r);
-    insert_work_f32(P, data, norm, slice, r, syms, nsym, weight, mode, corners);
+    insert_work_f32(P, data, norm, slice, r, syms, nsym, weight, mode, corners, NULL, 0);
+}
+
+/* insert_slice on a usessnr reconstructor of a slice carrying class_ssnr
+ * (libEM/reconstructor.cpp:1080-1088: weight -1, the SSNR flag; :1160-1184:
+ * per-radius weight max(0, ssnr[int(r * sscale)])).  With corners the
+ * reference indexes past the end of class_ssnr; not restated here. */
+int or_insert_slice_ssnr(int P, float *data, float *norm, const float *slice, const float rot[12],
+                         const float *syms, int nsym, const float *ssnr, int nssnr, int mode)
+{
+    if (!ssnr || nssnr < 1) return -1;
+    float r[12];
+    memcpy(r, rot, sizeof(r));
+    or_rotation_only(r);
+    insert_work_f32(P, data, norm, slice, r, syms, nsym, -1.0f, mode, 0, ssnr, nssnr);
+    return 0;
 }
 
 void or_insert_slice_work_f64(int P, double *data, double *norm, const float *slice,
@@ -771,7 +888,7 @@ void or_insert_slice_work_f64(int P, double *data, double *norm, const float *sl
     float r[12];
     memcpy(r, rot, sizeof(r));
     or_rotation_only(r);
-    insert_work_f64(P, data, norm, slice, r, syms, nsym, weight, mode, corners);
+    insert_work_f64(P, data, norm, slice, r, syms, nsym, weight, mode, corners, NULL, 0);
 }
 
 static double now_s(void)
@@ -808,7 +925,7 @@ double or_insert_batch_mt(int P, float *data, float *norm, const float *slices, 
             memcpy(r, rots + 12 * (size_t)i, sizeof(r));
             if (weights[i] == 0) continue;
             or_rotation_only(r);
-            insert_work_f32(P, pd[t], pn[t], slices + sl * i, r, syms, nsym, weights[i], mode, 0);
+            insert_work_f32(P, pd[t], pn[t], slices + sl * i, r, syms, nsym, weights[i], mode, 0, NULL, 0);
         }
     }
     for (int t = 0; t < nthreads; t++) {
@@ -919,7 +1036,7 @@ void or_determine_slice_agreement(int P, float *data, float *norm, float *slice,
     float rot[12];
     memcpy(rot, rot_in, sizeof(rot));
     or_rotation_only(rot);
-    if (sub) insert_work_f32(P, data, norm, slice, rot, syms, nsym, -weight, mode, 0);
+    if (sub) insert_work_f32(P, data, norm, slice, rot, syms, nsym, -weight, mode, 0, NULL, 0);
 
     /* do_compare_slice_work, libEM/reconstructor.cpp:1259-1381 */
     const float inx = (float)(P + 2), iny = (float)P;
@@ -983,7 +1100,7 @@ void or_determine_slice_agreement(int P, float *data, float *norm, float *slice,
     out[3] = (float)(dot * rw / ((rw - 1.0) * dot + 1.0));
     out[4] = (float)(vweight / (float)(nval));
 
-    if (sub) insert_work_f32(P, data, norm, slice, rot, syms, nsym, weight, mode, 0);
+    if (sub) insert_work_f32(P, data, norm, slice, rot, syms, nsym, weight, mode, 0, NULL, 0);
 }
 
 /* ======================================================================== */

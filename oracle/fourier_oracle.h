@@ -80,6 +80,8 @@ void or_prep_particle(const float *raw, int B, int P, const or_ctf *ctf_or_null,
 void or_insert_slice_work(int P, float *data, float *norm, const float *slice,
                           const float rot[12], const float *syms, int nsym,
                           float weight, int mode, int corners);
+int or_insert_slice_ssnr(int P, float *data, float *norm, const float *slice, const float rot[12],
+                         const float *syms, int nsym, const float *ssnr, int nssnr, int mode);
 void or_insert_slice_work_f64(int P, double *data, double *norm, const float *slice,
                               const float rot[12], const float *syms, int nsym,
                               float weight, int mode, int corners);

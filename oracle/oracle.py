@@ -60,6 +60,7 @@ def lib(fast=False):
         "or_rotation_only": (v, [_fp]),
         "or_sym_nsym": (i, [ctypes.c_char_p]),
         "or_sym_get": (i, [ctypes.c_char_p, _fp]),
+        "or_insert_slice_ssnr": (i, [i, _fp, _fp, _fp, _fp, _fp, i, _fp, i, i]),
         "or_edge_mean": (f, [_fp, i, i]),
         "or_clip_2d": (v, [_fp, i, i, i, i, i, i, _fp]),
         "or_fft_r2c_1d": (v, [_fp, i, _fp]),
@@ -293,6 +294,16 @@ class Volume:
             lib().or_insert_slice_work(self.P, _f(self.data), _f(self.norm), _f(s), _f(r),
                                       _f(self.syms), len(self.syms), float(weight),
                                       self.mode, int(corners))
+
+    def insert_ssnr(self, slice_, rot, ssnr):
+        """insert_slice on a usessnr reconstructor of a slice with class_ssnr"""
+        assert not self.fp64
+        s = f32(slice_)
+        r = f32(rot).reshape(12)
+        ss = f32(ssnr).ravel()
+        rc = lib().or_insert_slice_ssnr(self.P, _f(self.data), _f(self.norm), _f(s), _f(r), _f(self.syms),
+                                        len(self.syms), _f(ss), int(ss.size), self.mode)
+        assert rc == 0
 
     def setup_seed(self, seed, seed_weight):
         assert not self.fp64

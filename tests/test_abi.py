@@ -5,6 +5,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 from eman2_amd import _lib
@@ -41,9 +42,9 @@ def _create(**kw):
     (dict(nx=96, ny=96, nz=64), 3, "cubic"),
     (dict(nx=98, ny=98, nz=98), 3, "2^a 3^b 5^c"),
     (dict(sym=b"x5"), 2, "symmetry"),
-    (dict(sym=b"icos"), 2, "symmetry"),
+    (dict(sym=b"icos2"), 2, "symmetry"),
     (dict(mode=b"gauss_5"), 2, "mode"),
-    (dict(usessnr=1), 1, "usessnr"),
+    (dict(usessnr=1, corners=1), 1, "usessnr"),
 ])
 def test_create_validation(kw, code, word):
     rc, msg = _create(**kw)
@@ -77,3 +78,28 @@ def test_footprint_grows_with_size():
     a, b = L.fh_footprint_bytes(96, 1, 256), L.fh_footprint_bytes(384, 1, 256)
     assert 0 < a < b
     assert b >= 12 * 384 * 384 * 193
+
+
+SYM_NAMES = ["c1", "C5", "d2", "d7", "h5:1:30:2", "h4,1,12.5,3", "tet", "oct", "icos", "icos5"]
+
+
+@pytest.mark.parametrize("sym", SYM_NAMES)
+def test_symmetry_table_matches_oracle(sym):
+    """fh_get_symmetries is the oracle's Symmetry3D::get_symmetries bit for bit."""
+    from eman2_amd.transform import get_symmetries
+    from oracle import oracle as O
+    ops = get_symmetries(sym)
+    ref = O.sym_get(sym)[:, :, :3]
+    assert ops.shape == ref.shape
+    np.testing.assert_array_equal(ops, ref)
+
+
+def test_symmetry_unknown_and_small_buffer():
+    from eman2_amd.transform import get_symmetries
+    L = _lib.load()
+    for s in (b"x3", b"ico", b"h3:2:10:1", b""):
+        assert L.fh_get_symmetries(s, None, 0) == -2
+    buf = (ctypes.c_float * 9)()
+    assert L.fh_get_symmetries(b"c4", ctypes.cast(buf, ctypes.c_void_p), 1) == -1
+    with pytest.raises(_lib.FourierHipError, match="NotExistingObjectException"):
+        get_symmetries("ico")

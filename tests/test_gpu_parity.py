@@ -150,7 +150,8 @@ def test_batch_ctf(oracle):
 
 
 @pytest.mark.parametrize("sym,mode", [("c4", "gauss_2"), ("d2", "trilinear"), ("d7", "gauss_2"), ("d7", "trilinear"),
-                                      ("c1", "trilinear")])
+                                      ("c1", "trilinear"), ("tet", "gauss_2"), ("oct", "trilinear"),
+                                      ("icos", "gauss_2"), ("h6:1:25.5:3", "trilinear")])
 def test_batch_symmetry_modes(oracle, sym, mode):
     P, B = 48, 32
     ph = S.phantom(10, 10.0, seed=5)
@@ -162,6 +163,39 @@ def test_batch_symmetry_modes(oracle, sym, mode):
     v = oracle_volume(oracle, P, sym, mode, [oracle.prep_particle(im, P) for im in imgs], mats, w)
     assert_state_close(gd, gn, v.data, v.norm)
     assert_map_close(r.finish(True, return_numpy=True), v.finish(True))
+
+
+@pytest.mark.parametrize("sym,mode", [("d2", "gauss_2"), ("c1", "trilinear")])
+def test_usessnr_batch_and_slice(oracle, sym, mode):
+    """usessnr: per-radius class_ssnr weights (reconstructor.cpp:1080-1088,
+    1160-1184) in the batched scatter and the single-slice plain kernel."""
+    from eman2_amd.reconstructor import EMData
+    P, B, n = 48, 32, 30
+    ph = S.phantom(10, 10.0, seed=7)
+    imgs, mats, _ = S.particles(ph, n, B, seed=8, snr=1.0)
+    ss = np.random.default_rng(9).uniform(-0.2, 2.0, (n, 17)).astype(np.float32)
+    slices = [oracle.prep_particle(im, P) for im in imgs]
+    v = oracle.Volume(P, sym, mode)
+    for sl, m, q in zip(slices, mats, ss):
+        v.insert_ssnr(sl, m, q)
+    r = recon(P, sym, mode, usessnr=True, slices_per_item=8)
+    r.insert_batch(imgs, list(mats), None, class_ssnr=ss)
+    gd, gn = r.get_state()
+    assert_state_close(gd, gn, v.data, v.norm)
+    r2 = recon(P, sym, mode, usessnr=True)
+    for sl, m, q in zip(slices, mats, ss):
+        assert r2.insert_slice(EMData(sl, True, {"reconstruct_preproc": 1, "class_ssnr": q}), m, 5.0) == 0
+        # no class_ssnr on a usessnr reconstructor: weight 0, skipped
+        assert r2.insert_slice(EMData(sl, True, {"reconstruct_preproc": 1}), m, 5.0) == -1
+    gd2, gn2 = r2.get_state()
+    assert_state_close(gd2, gn2, v.data, v.norm)
+    assert_map_close(r.finish(True, return_numpy=True), v.finish(True))
+    # usessnr off: class_ssnr is ignored and the weight applies
+    r3 = recon(P, sym, mode)
+    r3.insert_batch(imgs, list(mats), np.full(n, 0.5, np.float32), class_ssnr=ss)
+    w = oracle_volume(oracle, P, sym, mode, slices, mats, np.full(n, 0.5, np.float32))
+    gd3, gn3 = r3.get_state()
+    assert_state_close(gd3, gn3, w.data, w.norm)
 
 
 def test_plain_matches_batch():

@@ -149,7 +149,8 @@ def test_transform_compose(oracle):
     np.testing.assert_array_equal(r[:, 3], 0)
 
 
-@pytest.mark.parametrize("sym,n", [("c1", 1), ("c4", 4), ("d2", 4), ("d7", 14), ("C3", 3)])
+@pytest.mark.parametrize("sym,n", [("c1", 1), ("c4", 4), ("d2", 4), ("d7", 14), ("C3", 3),
+                                   ("tet", 12), ("oct", 24), ("icos", 60), ("ICOS5", 60)])
 def test_symmetry_group(oracle, sym, n):
     ops = oracle.sym_get(sym)[:, :, :3].astype(np.float64)
     assert len(ops) == n
@@ -164,8 +165,62 @@ def test_symmetry_group(oracle, sym, n):
     np.testing.assert_allclose(ops[0], np.eye(3), atol=1e-7)
 
 
+def _angle_classes(ops):
+    """rotation angle (degrees, rounded) -> count, and the unit axes per angle"""
+    counts, axes = {}, {}
+    for R in ops:
+        ang = int(round(np.degrees(np.arccos(np.clip((np.trace(R) - 1) / 2, -1, 1)))))
+        counts[ang] = counts.get(ang, 0) + 1
+        if ang:
+            w, v = np.linalg.eig(R)
+            ax = np.real(v[:, np.argmin(np.abs(w - 1))])
+            axes.setdefault(ang, []).append(ax / np.linalg.norm(ax))
+    return counts, axes
+
+
+def _has_axis(axes, ang, e):
+    return any(abs(abs(np.dot(a, e)) - 1) < 1e-5 for a in axes.get(ang, []))
+
+
+@pytest.mark.parametrize("sym,classes,zfold", [
+    # conjugacy classes of A4, S4 and A5 by rotation angle (textbook), and the
+    # axis the reference puts on z: a tetrahedron face (3-fold), a cube face
+    # (4-fold), a pentagon (5-fold) -- symmetry.cpp:2010-2013, 1953-1955, 1825-1828
+    ("tet", {0: 1, 180: 3, 120: 8}, 120),
+    ("oct", {0: 1, 180: 9, 120: 8, 90: 6}, 90),
+    ("icos", {0: 1, 180: 15, 120: 20, 72: 12, 144: 12}, 72),
+])
+def test_platonic_groups(oracle, sym, classes, zfold):
+    ops = oracle.sym_get(sym)[:, :, :3].astype(np.float64)
+    counts, axes = _angle_classes(ops)
+    assert counts == classes
+    assert _has_axis(axes, zfold, np.array([0, 0, 1.0]))
+    for R in ops:
+        assert np.abs(R @ R.T - np.eye(3)).max() < 1e-6 and abs(np.linalg.det(R) - 1) < 1e-6
+    if sym == "oct":  # cube faces on x and y too
+        assert _has_axis(axes, 90, np.array([1.0, 0, 0])) and _has_axis(axes, 90, np.array([0, 1.0, 0]))
+    if sym == "icos":
+        # the 234-degree alignment offset (az + 90, phi - 90; symmetry.cpp:1845-1849)
+        # turns the table's 2-fold on x onto y
+        assert _has_axis(axes, 180, np.array([0, 1.0, 0]))
+        assert not _has_axis(axes, 180, np.array([1.0, 0, 0]))
+    if sym == "tet":  # the face-to-face angle acos(-1/3) is the table's second level
+        np.testing.assert_allclose(ops[3][2, 2], -1.0 / 3.0, atol=1e-7)
+
+
+@pytest.mark.parametrize("name,daz", [("h5:1:30:2", 30.0), ("h4,1,12.5,3", 12.5), ("H3:1:7.5:1:60", 7.5)])
+def test_helical_ops(oracle, name, daz):
+    """HSym::get_sym (symmetry.cpp:1648-1671, nstart 1): operator i turns by
+    k*daz about z with k = 0, 1, -1, 2, -2, ..."""
+    ops = oracle.sym_get(name)[:, :, :3].astype(np.float64)
+    ks = [0, 1, -1, 2, -2, 3, -3][:len(ops)]
+    for R, k in zip(ops, ks):
+        t = np.radians(k * daz)
+        np.testing.assert_allclose(R, [[np.cos(t), np.sin(t), 0], [-np.sin(t), np.cos(t), 0], [0, 0, 1]], atol=2e-7)
+
+
 def test_bad_symmetry_rejected(oracle):
-    for s in ("x3", "c", "c0", "d-2", ""):
+    for s in ("x3", "c", "c0", "d-2", "", "ico", "tetra", "h3:2:10:1", "h0"):
         with pytest.raises(ValueError):
             oracle.sym_get(s)
 
@@ -290,7 +345,7 @@ def _brute_force(slice_, P, rot, syms, weight, mode):
                                 g = math.exp(-(di * di + dj * dj + dk * dk) * math.pi ** 2 / 4)
                             else:
                                 g = (1 - abs(di)) * (1 - abs(dj)) * (1 - abs(dk))
-                            g *= weight
+                            g *= weight(x, y) if callable(weight) else weight
                             V[i + h, j + h, k + h] += val * g
                             W[i + h, j + h, k + h] += g
                             A[i + h, j + h, k + h] += abs(val) * abs(g)
@@ -344,6 +399,44 @@ def test_insert_brute_force(oracle, sym, mode):
     rt = 0.0102 if mode == "gauss_2" else 2e-5
     assert np.all(np.abs(v.norm - norm) <= rt * np.abs(norm) + 1e-6)
     assert np.all(np.abs(cplx(v.data) - data) <= rt * bound + 1e-5)
+
+
+@pytest.mark.parametrize("step,mode", [(1, "gauss_2"), (2, "trilinear")])
+def test_insert_ssnr_brute_force(oracle, step, mode):
+    """usessnr: a slice's pixels at Fourier radius r = round(hypot(x, y)) are
+    weighted by its class_ssnr sampled at that radius, Nyquist (r = P/2) at
+    the last entry, negative values clamped to 0 (reconstructor.cpp:1160-1184).
+    With len = step * P/2 + 1 entries the sample of radius r is entry step*r."""
+    P = 16
+    h = P // 2
+    rng = np.random.default_rng(18)
+    sl = oracle.prep_particle(rng.standard_normal((12, 12)).astype(np.float32), P)
+    rot = euler_eman_matrix(*(rng.random(3) * (360, 180, 360)))
+    ssnr = np.linspace(-0.3, 2.0, step * h + 1).astype(np.float32)
+    rng.shuffle(ssnr)
+    v = oracle.Volume(P, "c1", mode)
+    v.insert_ssnr(sl, rot, ssnr)
+
+    def wfun(x, y):
+        r = int(math.floor(math.hypot(x, y) + 0.5))
+        return max(0.0, float(ssnr[step * r]))
+    data, norm, bound = _brute_force(sl, P, rot, oracle.sym_get("c1"), wfun, mode)
+    rt = 0.0102 if mode == "gauss_2" else 2e-5
+    assert np.all(np.abs(v.norm - norm) <= rt * np.abs(norm) + 1e-6)
+    assert np.all(np.abs(cplx(v.data) - data) <= rt * bound + 1e-5)
+
+
+def test_insert_ssnr_constant_is_weight(oracle):
+    """a constant class_ssnr c inserts exactly like weight c"""
+    P = 16
+    rng = np.random.default_rng(19)
+    sl = oracle.prep_particle(rng.standard_normal((12, 12)).astype(np.float32), P)
+    rot = euler_eman_matrix(*(rng.random(3) * (360, 180, 360)))
+    a, b = oracle.Volume(P, "d2", "gauss_2"), oracle.Volume(P, "d2", "gauss_2")
+    a.insert_ssnr(sl, rot, np.full(7, 0.625, np.float32))
+    b.insert(sl, rot, 0.625)
+    np.testing.assert_array_equal(a.data, b.data)
+    np.testing.assert_array_equal(a.norm, b.norm)
 
 
 def test_insert_remove_is_zero(oracle):
