@@ -82,15 +82,6 @@ void rotation_only(const float *in12, float *out9)
         for (int j = 0; j < 3; j++) out9[3 * i + j] = m[4 * i + j];
 }
 
-// preprocess_slice applies any 2-D translation / scale / mirror to the image
-// (libEM/reconstructor.cpp:1026-1035); only the identity case is supported.
-bool pure_rotation(const float *m)
-{
-    if (fabsf(m[3]) > ERR_LIMIT || fabsf(m[7]) > ERR_LIMIT || fabsf(m[11]) > ERR_LIMIT) return false;
-    float d = determinant(m);
-    return d == 1.0f;
-}
-
 // eman Euler -> 3x3 (libEM/transform.cpp:650-667)
 void euler_eman(double az, double alt, double phi, float *m9)
 {
@@ -211,6 +202,54 @@ bool sym_ops(const std::string &name, std::vector<float> &out)
     return true;
 }
 
+// preprocess_slice's 2-D transform (libEM/reconstructor.cpp:1031-1035):
+// tmp = t with its rotation reset by set_rotation({"type": "eman"}), which
+// keeps scale and x mirror (transform.cpp:527-707: scale and mirror taken
+// first, the eman (0,0,0) matrix, scale re-applied, row 0 negated for a
+// mirror) and the translation.  is_identity(tmp) (transform.cpp:222-235)
+// -> the image is used as is; otherwise the "xform" processor samples it at
+// inverse(tmp) . (x, y) (Transform::invert, transform.cpp:1324-1356, in
+// double).  Returns false for identity, else inverse rows m00 m01 m03 m10
+// m11 m13.
+bool xform2d_inverse(const float *m, float inv6[6])
+{
+    const float det0 = determinant(m);
+    const bool mirror = det0 < 0;
+    const float scale = get_scale(m);
+    float t[12];
+    memcpy(t, m, sizeof(t));
+    float e[9];
+    euler_eman(0.0, 0.0, 0.0, e);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) t[4 * i + j] = e[3 * i + j];
+    if (scale != 1.0f)
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) t[4 * i + j] *= scale;
+    if (mirror)
+        for (int j = 0; j < 3; j++) t[j] *= -1.0f;
+    bool ident = true;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 4; j++) {
+            float c = t[4 * i + j];
+            apply_precision(c, ERR_LIMIT);
+            if (i == j ? c != 1.0f : c != 0.0f) ident = false;
+        }
+    if (ident) return false;
+    const double m00 = t[0], m01 = t[1], m02 = t[2], m10 = t[4], m11 = t[5], m12 = t[6];
+    const double m20 = t[8], m21 = t[9], m22 = t[10], v0 = t[3], v1 = t[7], v2 = t[11];
+    const double cof00 = m11 * m22 - m12 * m21, cof11 = m22 * m00 - m20 * m02, cof01 = m10 * m22 - m20 * m12;
+    const double cof02 = m10 * m21 - m20 * m11, cof10 = m01 * m22 - m02 * m21, cof20 = m01 * m12 - m02 * m11;
+    const double cof21 = m00 * m12 - m10 * m02;
+    const double det = m00 * cof00 + m02 * cof02 - m01 * cof01;
+    inv6[0] = (float)(cof00 / det);
+    inv6[1] = -(float)(cof10 / det);
+    inv6[2] = (float)((-cof00 * v0 + cof10 * v1 - cof20 * v2) / det);
+    inv6[3] = -(float)(cof01 / det);
+    inv6[4] = (float)(cof11 / det);
+    inv6[5] = (float)((cof01 * v0 - cof11 * v1 + cof21 * v2) / det);
+    return true;
+}
+
 // t3d = a * b in FP32, reference order (libEM/transform.cpp:1384-1395)
 void compose9(const float *a, const float *b, float *r)
 {
@@ -272,6 +311,9 @@ struct fh_reconstructor {
     float2 *slices = nullptr;
     float *t3d = nullptr, *wq = nullptr;
     float *rw = nullptr;        // class_ssnr radial weights, (P/2+1) per particle (usessnr)
+    float *ximgs = nullptr;     // 2-D transformed padded particles, P x P each
+    float *xfd = nullptr;       // their inverse 2-D transforms, 6 per particle
+    int ximgs_cap = 0;
     double *partial = nullptr;  // agreement partial sums
     double *acc8 = nullptr;     // batched agreement sums, 8 per particle
     int acc8_cap = 0;
@@ -354,10 +396,10 @@ static int ensure_chunk(fh_reconstructor *h, int n, int B)
 // Larger chunks put more slices into each work item, so fewer partial
 // bricks are written and merged per particle.
 constexpr long long AUTO_CHUNK_BYTES = 48LL << 30;
-static int chunk_for(const fh_tuning &t, int P, int box)
+static int chunk_for(const fh_tuning &t, int P, int box, long long extra = 0)
 {
     if (t.chunk > 0) return t.chunk;
-    const long long per = 8LL * P * (P / 2 + 1) + 4LL * box * box;
+    const long long per = 8LL * P * (P / 2 + 1) + 4LL * box * box + extra;
     return (int)std::max(256LL, std::min(65536LL, AUTO_CHUNK_BYTES / per));
 }
 
@@ -523,6 +565,8 @@ int fh_destroy(fh_reconstructor *h)
         free_chunk(h);
         free_bins(h);
         free_volume(h);
+        if (h->ximgs) (void)hipFree(h->ximgs);
+        if (h->xfd) (void)hipFree(h->xfd);
         if (h->partial) (void)hipFree(h->partial);
         if (h->acc8) (void)hipFree(h->acc8);
         for (void *p : {(void *)h->exptab, (void *)h->rtab, (void *)h->tw, (void *)h->syms_d})
@@ -608,18 +652,43 @@ int fh_clear(fh_reconstructor *h)
 
 // prep kernels of n particles (box B) into h->slices; dfd: device defocus or null
 static hipError_t prep_kernels(fh_reconstructor *h, int n, int B, const float *dev_imgs, const float *dfd,
-                               const CtfConst &cc)
+                               const CtfConst &cc, const float *xfd = nullptr)
 {
     hipError_t e;
     if ((e = launch_edge_mean(dev_imgs, n, B, h->means, h->stream)) != hipSuccess) return e;
+    if (xfd) {
+        // pad + 2-D resampling into P x P images, then the prep of those (zero mean)
+        if ((e = launch_xform2d(dev_imgs, n, B, h->P, h->means, xfd, h->ximgs, h->stream)) != hipSuccess) return e;
+        if ((e = launch_fill_f32(h->means, 0.f, n, h->stream)) != hipSuccess) return e;
+        if ((e = launch_prep_rows(h->ximgs, n, h->P, h->P, h->means, h->slices, h->plan, h->tw, h->stream)) !=
+            hipSuccess)
+            return e;
+        return launch_prep_cols(n, h->P, h->P, h->slices, dfd, cc, h->scale, h->plan, h->tw, h->stream);
+    }
     if ((e = launch_prep_rows(dev_imgs, n, B, h->P, h->means, h->slices, h->plan, h->tw, h->stream)) != hipSuccess)
         return e;
     return launch_prep_cols(n, B, h->P, h->slices, dfd, cc, h->scale, h->plan, h->tw, h->stream);
 }
 
-// prep of n particles already in h->imgs (box B) into h->slices
+// staging of n 2-D transformed particles (P x P) and their inverse transforms
+static int ensure_ximgs(fh_reconstructor *h, int n)
+{
+    if (n <= h->ximgs_cap) return FH_OK;
+    if (h->ximgs) (void)hipFree(h->ximgs);
+    if (h->xfd) (void)hipFree(h->xfd);
+    h->ximgs = nullptr;
+    h->xfd = nullptr;
+    h->ximgs_cap = 0;
+    FH_TRY(h, hipMalloc(&h->ximgs, sizeof(float) * (size_t)n * h->P * h->P));
+    FH_TRY(h, hipMalloc(&h->xfd, sizeof(float) * 6 * (size_t)n));
+    h->ximgs_cap = n;
+    return FH_OK;
+}
+
+// prep of n particles already in h->imgs (box B) into h->slices; xf_host:
+// their inverse 2-D transforms (n x 6) or null when all are identities
 static int prep_chunk(fh_reconstructor *h, int n, int B, const float *dev_imgs, const float *defocus_host,
-                      const fh_ctf *ctf)
+                      const fh_ctf *ctf, const float *xf_host = nullptr)
 {
     const float *dfd = nullptr;
     CtfConst cc{};
@@ -628,7 +697,33 @@ static int prep_chunk(fh_reconstructor *h, int n, int B, const float *dev_imgs, 
         dfd = h->dfs;
         cc = ctf_constants(*ctf, h->P);
     }
-    FH_TRY(h, prep_kernels(h, n, B, dev_imgs, dfd, cc));
+    const float *xfd = nullptr;
+    if (xf_host) {
+        int rc = ensure_ximgs(h, n);
+        if (rc) return rc;
+        FH_TRY(h, hipMemcpyAsync(h->xfd, xf_host, sizeof(float) * 6 * n, hipMemcpyHostToDevice, h->stream));
+        xfd = h->xfd;
+    }
+    FH_TRY(h, prep_kernels(h, n, B, dev_imgs, dfd, cc, xfd));
+    return FH_OK;
+}
+
+// preprocess_slice of one raw P x P image already in h->imgs (no edge-mean
+// step): 2-D transform when xf6 is given, phase origin, r2c FFT, scale
+static int prep_single_raw(fh_reconstructor *h, const float *xf6)
+{
+    const int P = h->P;
+    FH_TRY(h, launch_fill_f32(h->means, 0.f, 1, h->stream));
+    const float *src = h->imgs;
+    if (xf6) {
+        int rc = ensure_ximgs(h, 1);
+        if (rc) return rc;
+        FH_TRY(h, hipMemcpyAsync(h->xfd, xf6, sizeof(float) * 6, hipMemcpyHostToDevice, h->stream));
+        FH_TRY(h, launch_xform2d(h->imgs, 1, P, P, h->means, h->xfd, h->ximgs, h->stream));
+        src = h->ximgs;
+    }
+    FH_TRY(h, launch_prep_rows(src, 1, P, P, h->means, h->slices, h->plan, h->tw, h->stream));
+    FH_TRY(h, launch_prep_cols(1, P, P, h->slices, nullptr, CtfConst{}, h->scale, h->plan, h->tw, h->stream));
     return FH_OK;
 }
 
@@ -641,10 +736,11 @@ static int prep_chunk(fh_reconstructor *h, int n, int B, const float *dev_imgs, 
 // device.  The captured sequence holds kernels only (no memset nodes).
 static int insert_batch_graph(fh_reconstructor *h, const float *imgs, int n, int box, const std::vector<float> &rot9,
                               const float *weights, const float *defocus, const fh_ctf *ctf, int chunk,
-                              const float *rw_host)
+                              const float *rw_host, const float *xf_host)
 {
     int rc = ensure_partials(h, h->bins.item_cap);
     if (rc) return rc;
+    if (xf_host && (rc = ensure_ximgs(h, chunk))) return rc;
     const InsertConst c = iconst(h);
     fh_stats st{};
     const int wpb = h->tune.waves_per_block;
@@ -659,13 +755,16 @@ static int insert_batch_graph(fh_reconstructor *h, const float *imgs, int n, int
         if (rw_host)
             FH_TRY(h, hipMemcpyAsync(h->rw, rw_host + (size_t)c0 * (h->H + 1), sizeof(float) * (size_t)m * (h->H + 1),
                                      hipMemcpyHostToDevice, h->stream));
+        if (xf_host)
+            FH_TRY(h, hipMemcpyAsync(h->xfd, xf_host + 6 * (size_t)c0, sizeof(float) * 6 * m, hipMemcpyHostToDevice,
+                                     h->stream));
         const float *dfd = nullptr;
         if (defocus && ctf) {
             FH_TRY(h, hipMemcpyAsync(h->dfs, defocus + c0, sizeof(float) * m, hipMemcpyHostToDevice, h->stream));
             dfd = h->dfs;
         }
         FH_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
-        hipError_t e = prep_kernels(h, m, box, dimgs, dfd, cc);
+        hipError_t e = prep_kernels(h, m, box, dimgs, dfd, cc, xf_host ? h->xfd : nullptr);
         if (e == hipSuccess) e = launch_compose(h->rots, h->wts, m, h->syms_d, h->nsym, h->t3d, h->wq, h->stream);
         if (e == hipSuccess)
             e = launch_bin(h->t3d, h->wq, m * h->nsym, c, h->geom, h->bins, h->tune.slices_per_item, h->stream);
@@ -701,17 +800,49 @@ static int insert_batch_graph(fh_reconstructor *h, const float *imgs, int n, int
     return FH_OK;
 }
 
-static int check_xforms(fh_reconstructor *h, const float *xforms, int n, int need_identity_2d, std::vector<float> &rot9)
+// rot9: the rotations insert_slice uses (scale, mirror, translation
+// stripped).  need_2d: the images still go through preprocess_slice, whose
+// 2-D part (translation / scale / mirror) is collected into xf6 (n x 6
+// inverse rows; left empty when every particle's is the identity); with
+// xf6 null a non-identity 2-D part is refused.
+static int check_xforms(fh_reconstructor *h, const float *xforms, int n, int need_2d, std::vector<float> &rot9,
+                        std::vector<float> *xf6 = nullptr)
 {
     rot9.resize((size_t)9 * n);
+    if (xf6) xf6->clear();
+    bool any = false;
+    std::vector<float> xf;
+    if (need_2d && xf6) xf.resize((size_t)6 * n);
     for (int i = 0; i < n; i++) {
         const float *m = xforms + 12 * (size_t)i;
-        if (need_identity_2d && !pure_rotation(m))
-            return fail(h, FH_ERR_INVALID_PARAM,
-                        "InvalidParameterException: particle " + std::to_string(i) +
-                            ": translation/scale/mirror in preprocess_slice is not supported by fourier_hip");
+        if (need_2d) {
+            if (determinant(m) == 0.0f)
+                return fail(h, FH_ERR_INVALID_PARAM, "UnexpectedBehaviorException: particle " + std::to_string(i) +
+                                                         ": the determinant of the Transform is 0");
+            float inv[6] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f};
+            const bool non_identity = xform2d_inverse(m, inv);
+            if (non_identity && !xf6)
+                return fail(h, FH_ERR_INVALID_PARAM,
+                            "InvalidParameterException: particle " + std::to_string(i) +
+                                ": translation/scale/mirror in preprocess_slice is not supported on this path");
+            any = any || non_identity;
+            if (xf6) memcpy(&xf[6 * (size_t)i], inv, sizeof(inv));
+        }
         rotation_only(m, &rot9[9 * (size_t)i]);
     }
+    if (xf6 && any) xf6->swap(xf);
+    return FH_OK;
+}
+
+// e2make3dpar.py:704-716 flips the CTF on the padded image before
+// preprocess_slice resamples it; the fused prep applies the flip in Fourier
+// space after the resampling, which equals it only for identity 2-D parts.
+static int check_xform_ctf(fh_reconstructor *h, const std::vector<float> &xf6, const float *defocus, const fh_ctf *ctf)
+{
+    if (!xf6.empty() && defocus && ctf)
+        return fail(h, FH_ERR_INVALID_PARAM,
+                    "InvalidParameterException: CTF phase flip together with a 2-D translation/scale/mirror is not "
+                    "supported by fourier_hip (flip the particles first, as phase-flipped stacks are)");
     return FH_OK;
 }
 
@@ -776,8 +907,8 @@ static int insert_slice_impl(fh_reconstructor *h, const float *data, int is_prep
     if (weight == 0) return FH_SKIPPED;
     FH_TRY(h, hipSetDevice(h->device));
     const int P = h->P, H = h->H;
-    std::vector<float> rot9;
-    int rc = check_xforms(h, xform, 1, !is_preprocessed, rot9);
+    std::vector<float> rot9, xf6;
+    int rc = check_xforms(h, xform, 1, !is_preprocessed, rot9, &xf6);
     if (rc) return rc;
     rc = ensure_chunk(h, std::max(1, h->cap_n), std::max(P, h->cap_B));
     if (rc) return rc;
@@ -785,9 +916,7 @@ static int insert_slice_impl(fh_reconstructor *h, const float *data, int is_prep
         FH_TRY(h, hipMemcpyAsync(h->slices, data, sizeof(float2) * P * (H + 1), hipMemcpyHostToDevice, h->stream));
     } else {
         FH_TRY(h, hipMemcpyAsync(h->imgs, data, sizeof(float) * P * P, hipMemcpyHostToDevice, h->stream));
-        FH_TRY(h, launch_fill_f32(h->means, 0.f, 1, h->stream));
-        FH_TRY(h, launch_prep_rows(h->imgs, 1, P, P, h->means, h->slices, h->plan, h->tw, h->stream));
-        FH_TRY(h, launch_prep_cols(1, P, P, h->slices, nullptr, CtfConst{}, h->scale, h->plan, h->tw, h->stream));
+        if ((rc = prep_single_raw(h, xf6.empty() ? nullptr : xf6.data()))) return rc;
     }
     if (use_ssnr) {
         std::vector<float> rw(H + 1);
@@ -851,20 +980,22 @@ static int insert_batch_impl(fh_reconstructor *h, const float *imgs, int n, int 
     int rc = check_batch_args(h, imgs, n, box, xforms, weights);
     if (rc) return rc;
     FH_TRY(h, hipSetDevice(h->device));
-    std::vector<float> rot9;
-    rc = check_xforms(h, xforms, n, 1, rot9);
+    std::vector<float> rot9, xf6;
+    rc = check_xforms(h, xforms, n, 1, rot9, &xf6);
     if (rc) return rc;
+    if ((rc = check_xform_ctf(h, xf6, defocus, ctf))) return rc;
+    const float *xfh = xf6.empty() ? nullptr : xf6.data();
     // the brick list (at most 5 nb^2 entries per slice) is indexed with int
     const BrickGeom bg = brick_geom(h->P);
     const long long max_slices = std::max(1LL, (long long)INT32_MAX / (5LL * bg.nb * bg.nb) - 1);
-    const int chunk = (int)std::max(1LL, std::min({(long long)chunk_for(h->tune, h->P, box),
+    const int chunk = (int)std::max(1LL, std::min({(long long)chunk_for(h->tune, h->P, box, xfh ? 4LL * h->P * h->P : 0),
                                                    (long long)std::max(n, 1), max_slices / h->nsym}));
     rc = ensure_chunk(h, chunk, box);
     if (rc) return rc;
     rc = ensure_bins(h, (long long)chunk * h->nsym);
     if (rc) return rc;
     if (h->tune.graph && where == FH_DEVICE)
-        return insert_batch_graph(h, imgs, n, box, rot9, weights, defocus, ctf, chunk, rwh);
+        return insert_batch_graph(h, imgs, n, box, rot9, weights, defocus, ctf, chunk, rwh, xfh);
     const InsertConst c = iconst(h);
     fh_stats st{};
     float ms;
@@ -881,7 +1012,7 @@ static int insert_batch_impl(fh_reconstructor *h, const float *imgs, int n, int 
             FH_TRY(h, hipMemcpyAsync(h->imgs, src, sizeof(float) * (size_t)m * box * box, hipMemcpyHostToDevice, h->stream));
             dimgs = h->imgs;
         }
-        rc = prep_chunk(h, m, box, dimgs, defocus ? defocus + c0 : nullptr, ctf);
+        rc = prep_chunk(h, m, box, dimgs, defocus ? defocus + c0 : nullptr, ctf, xfh ? xfh + 6 * (size_t)c0 : nullptr);
         if (rc) return rc;
         FH_TRY(h, hipMemcpyAsync(h->rots, &rot9[9 * (size_t)c0], sizeof(float) * 9 * m, hipMemcpyHostToDevice, h->stream));
         FH_TRY(h, hipMemcpyAsync(h->wts, weights + c0, sizeof(float) * m, hipMemcpyHostToDevice, h->stream));
@@ -985,16 +1116,18 @@ int fh_insert_plain(fh_reconstructor *h, const float *imgs, int n, int box, cons
     if (rc) return rc;
     if (h->usessnr) return FH_SKIPPED;  // no class_ssnr: weight 0 (reconstructor.cpp:1082-1088)
     FH_TRY(h, hipSetDevice(h->device));
-    std::vector<float> rot9;
-    rc = check_xforms(h, xforms, n, 1, rot9);
+    std::vector<float> rot9, xf6;
+    rc = check_xforms(h, xforms, n, 1, rot9, &xf6);
     if (rc) return rc;
+    if ((rc = check_xform_ctf(h, xf6, defocus, ctf))) return rc;
     rc = ensure_chunk(h, std::max(1, h->cap_n), std::max(box, h->cap_B));
     if (rc) return rc;
     for (int i = 0; i < n; i++) {
         if (weights[i] == 0) continue;
         // per-slice host->device copy (reconstructor.cpp:1074-1078)
         FH_TRY(h, hipMemcpy(h->imgs, imgs + (size_t)i * box * box, sizeof(float) * box * box, hipMemcpyHostToDevice));
-        rc = prep_chunk(h, 1, box, h->imgs, defocus ? defocus + i : nullptr, ctf);
+        rc = prep_chunk(h, 1, box, h->imgs, defocus ? defocus + i : nullptr, ctf,
+                        xf6.empty() ? nullptr : &xf6[6 * (size_t)i]);
         if (rc) return rc;
         // one launch per symmetry op, synchronised (cuda_reconstructor.cu:171-193)
         rc = insert_slice_plain(h, h->slices, &rot9[9 * (size_t)i], weights[i], true);
@@ -1075,8 +1208,8 @@ int fh_determine_slice_agreement(fh_reconstructor *h, const float *data, int is_
     if (!h->ready) return fail(h, FH_ERR_STATE, "determine_slice_agreement() before setup()");
     FH_TRY(h, hipSetDevice(h->device));
     const int P = h->P, H = h->H;
-    std::vector<float> rot9;
-    int rc = check_xforms(h, xform, 1, !is_preprocessed, rot9);
+    std::vector<float> rot9, xf6;
+    int rc = check_xforms(h, xform, 1, !is_preprocessed, rot9, &xf6);
     if (rc) return rc;
     if ((rc = ensure_chunk(h, std::max(1, h->cap_n), std::max(P, h->cap_B)))) return rc;
     if ((rc = ensure_partial(h))) return rc;
@@ -1084,9 +1217,7 @@ int fh_determine_slice_agreement(fh_reconstructor *h, const float *data, int is_
         FH_TRY(h, hipMemcpyAsync(h->slices, data, sizeof(float2) * P * (H + 1), hipMemcpyHostToDevice, h->stream));
     } else {
         FH_TRY(h, hipMemcpyAsync(h->imgs, data, sizeof(float) * P * P, hipMemcpyHostToDevice, h->stream));
-        FH_TRY(h, launch_fill_f32(h->means, 0.f, 1, h->stream));
-        FH_TRY(h, launch_prep_rows(h->imgs, 1, P, P, h->means, h->slices, h->plan, h->tw, h->stream));
-        FH_TRY(h, launch_prep_cols(1, P, P, h->slices, nullptr, CtfConst{}, h->scale, h->plan, h->tw, h->stream));
+        if ((rc = prep_single_raw(h, xf6.empty() ? nullptr : xf6.data()))) return rc;
     }
     return agree_one(h, h->slices, rot9.data(), weight, sub, apix, out);
 }
@@ -1100,9 +1231,11 @@ int fh_agreement_batch(fh_reconstructor *h, const float *imgs, int n, int box, c
     if (rc) return rc;
     if (!out) return fail(h, FH_ERR_NULL, "NullPointerException: output is NULL");
     FH_TRY(h, hipSetDevice(h->device));
-    std::vector<float> rot9;
-    if ((rc = check_xforms(h, xforms, n, 1, rot9))) return rc;
-    const int chunk = std::max(1, std::min(chunk_for(h->tune, h->P, box), std::max(n, 1)));
+    std::vector<float> rot9, xf6;
+    if ((rc = check_xforms(h, xforms, n, 1, rot9, &xf6))) return rc;
+    if ((rc = check_xform_ctf(h, xf6, defocus, ctf))) return rc;
+    const float *xfh = xf6.empty() ? nullptr : xf6.data();
+    const int chunk = std::max(1, std::min(chunk_for(h->tune, h->P, box, xfh ? 4LL * h->P * h->P : 0), std::max(n, 1)));
     if ((rc = ensure_chunk(h, chunk, box))) return rc;
     if (!h->acc8 || h->acc8_cap < chunk) {
         if (h->acc8) (void)hipFree(h->acc8);
@@ -1122,7 +1255,8 @@ int fh_agreement_batch(fh_reconstructor *h, const float *imgs, int n, int box, c
             FH_TRY(h, hipMemcpyAsync(h->imgs, src, sizeof(float) * (size_t)m * box * box, hipMemcpyHostToDevice, h->stream));
             dimgs = h->imgs;
         }
-        if ((rc = prep_chunk(h, m, box, dimgs, defocus ? defocus + c0 : nullptr, ctf))) return rc;
+        if ((rc = prep_chunk(h, m, box, dimgs, defocus ? defocus + c0 : nullptr, ctf, xfh ? xfh + 6 * (size_t)c0 : nullptr)))
+            return rc;
         FH_TRY(h, hipMemcpyAsync(h->rots, &rot9[9 * (size_t)c0], sizeof(float) * 9 * m, hipMemcpyHostToDevice, h->stream));
         FH_TRY(h, hipMemcpyAsync(h->wts, weights + c0, sizeof(float) * m, hipMemcpyHostToDevice, h->stream));
         FH_TRY(h, launch_compose(h->rots, h->wts, m, h->syms_d, h->nsym, h->t3d, h->wq, h->stream));

@@ -106,6 +106,9 @@ hipError_t launch_seed_weights(float *norm, int P, float w, hipStream_t s);
 hipError_t launch_scale_f32(float *p, float v, size_t n, hipStream_t s);
 
 // ---- insertion (insert_kernels.hip) ----
+// preprocess_slice's 2-D resampling of n padded particles (xf: n x 6 inverse rows) -> n x P x P
+hipError_t launch_xform2d(const float *imgs, int n, int B, int P, const float *means, const float *xf, float *out,
+                          hipStream_t s);
 hipError_t launch_compose(const float *rots, const float *weights, int n, const float *syms, int nsym,
                           float *t3d, float *wq, hipStream_t s);
 // rw: optional radial weights ((P/2+1) floats, class_ssnr weighting) replacing weight

@@ -52,6 +52,54 @@ __global__ void k_compose(const float *__restrict__ rots, const float *__restric
     wq[q] = weights[p];
 }
 
+// The 2-D part of preprocess_slice (libEM/reconstructor.cpp:1031-1035): the
+// "xform" processor's real-space resampling (TransformProcessor::transform,
+// libEM/processor.cpp:11025-11075) of the centre zero-padded, edge-mean
+// subtracted particle.  Output pixel (i, j) samples the padded image at
+// inv . (i - P/2, j - P/2) + (P/2, P/2) bilinearly (Util::bilinear_interpolate,
+// libEM/util.h:543-547; the last row/column reuse their own pixel), 0 outside.
+// xf holds inv's m00 m01 m03 m10 m11 m13 per particle; float operation
+// order as the reference (no contraction in this unit).
+__global__ void __launch_bounds__(256) k_xform2d(const float *__restrict__ imgs, int B, int P, int off,
+                                                 const float *__restrict__ means, const float *__restrict__ xf,
+                                                 float *__restrict__ out)
+{
+    const int p = blockIdx.y;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= P * P) return;
+    const int i = idx % P, j = idx / P;
+    const float *m = xf + 6 * (size_t)p;
+    const float x = (float)(i - P / 2), y = (float)(j - P / 2);
+    const float x2 = (m[0] * x + m[1] * y + m[2]) + (float)(P / 2);
+    const float y2 = (m[3] * x + m[4] * y + m[5]) + (float)(P / 2);
+    float v = 0.0f;
+    if (!(x2 < 0 || x2 >= (float)P || y2 < 0 || y2 >= (float)P)) {
+        const int ii = (int)x2, jj = (int)y2;  // fast_floor of a non-negative value
+        const int i1 = ii == P - 1 ? ii : ii + 1, j1 = jj == P - 1 ? jj : jj + 1;
+        const float *d = imgs + (size_t)p * B * B;
+        const float mean = means[p];
+        // the padded image: particle pixel minus its edge mean inside the box, 0 outside
+        auto px = [&](int a, int b) -> float {
+            const int xa = a - off, yb = b - off;
+            return ((unsigned)xa < (unsigned)B && (unsigned)yb < (unsigned)B) ? d[(size_t)yb * B + xa] - mean : 0.0f;
+        };
+        const float p1 = px(ii, jj), p2 = px(i1, jj), p3 = px(ii, j1), p4 = px(i1, j1);
+        const float t = x2 - (float)ii, u = y2 - (float)jj;
+        v = (1 - t) * (1 - u) * p1 + t * (1 - u) * p2 + (1 - t) * u * p3 + t * u * p4;
+    }
+    out[(size_t)p * P * P + idx] = v;
+}
+
+hipError_t launch_xform2d(const float *imgs, int n, int B, int P, const float *means, const float *xf, float *out,
+                          hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const int d = B - P;
+    const int fl = d >= 0 ? d / 2 : -((-d + 1) / 2);  // clip_inplace Region((B-P)//2, ...)
+    hipLaunchKernelGGL(k_xform2d, dim3((P * P + 255) / 256, n), dim3(256), 0, s, imgs, B, P, -fl, means, xf, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_compose(const float *rots, const float *weights, int n, const float *syms, int nsym,
                           float *t3d, float *wq, hipStream_t s)
 {

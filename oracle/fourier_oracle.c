@@ -669,13 +669,95 @@ void or_preprocess_slice(const float *img, int P, float *out)
     free(t);
 }
 
+/* preprocess_slice's 2-D part (libEM/reconstructor.cpp:1031-1035): tmp = t
+ * with the rotation reset by set_rotation({"type":"eman"}) -- scale and x
+ * mirror measured first and re-applied, translation kept
+ * (libEM/transform.cpp:527-707) -- then a copy when tmp.is_identity()
+ * (:222-235), else the "xform" processor: each output pixel samples the image
+ * bilinearly at inverse(tmp) . (i - nx/2, j - ny/2) + (nx/2, ny/2), 0 outside
+ * (TransformProcessor::transform, libEM/processor.cpp:11025-11075;
+ * Transform::invert :1324-1356; Util::bilinear_interpolate util.h:543-547).
+ * Returns 1 when the image was resampled, 0 for the identity copy. */
+int or_xform2d(const float *img, int nx, int ny, const float t_in[12], float *out)
+{
+    float t[12];
+    memcpy(t, t_in, sizeof(t));
+    const float det0 = get_determinant(t);
+    const int mirror = det0 < 0;
+    const float scale = get_scale(t);
+    float e[12];
+    or_euler_eman(0.0, 0.0, 0.0, e);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) t[4 * i + j] = e[4 * i + j];
+    if (scale != 1.0f)
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) t[4 * i + j] *= scale;
+    if (mirror)
+        for (int j = 0; j < 3; j++) t[j] *= -1.0f;
+    int ident = 1;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 4; j++) {
+            float c = t[4 * i + j];
+            apply_precision(&c, OR_ERR_LIMIT);
+            if (i == j ? c != 1.0f : c != 0.0f) ident = 0;
+        }
+    if (ident) {
+        memcpy(out, img, sizeof(float) * (size_t)nx * ny);
+        return 0;
+    }
+    double m00 = t[0], m01 = t[1], m02 = t[2], m10 = t[4], m11 = t[5], m12 = t[6];
+    double m20 = t[8], m21 = t[9], m22 = t[10], v0 = t[3], v1 = t[7], v2 = t[11];
+    double cof00 = m11 * m22 - m12 * m21, cof11 = m22 * m00 - m20 * m02, cof01 = m10 * m22 - m20 * m12;
+    double cof02 = m10 * m21 - m20 * m11, cof10 = m01 * m22 - m02 * m21, cof20 = m01 * m12 - m02 * m11;
+    double cof21 = m00 * m12 - m10 * m02;
+    double det = m00 * cof00 + m02 * cof02 - m01 * cof01;
+    float a00 = (float)(cof00 / det), a01 = -(float)(cof10 / det);
+    float a03 = (float)((-cof00 * v0 + cof10 * v1 - cof20 * v2) / det);
+    float a10 = -(float)(cof01 / det), a11 = (float)(cof11 / det);
+    float a13 = (float)((cof01 * v0 - cof11 * v1 + cof21 * v2) / det);
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+            float cx = (float)(i - nx / 2), cy = (float)(j - ny / 2);
+            float x2 = a00 * cx + a01 * cy + a03;
+            float y2 = a10 * cx + a11 * cy + a13;
+            x2 += (float)(nx / 2);
+            y2 += (float)(ny / 2);
+            if (x2 < 0 || x2 >= nx || y2 < 0 || y2 >= ny) {
+                out[i + j * nx] = 0;
+                continue;
+            }
+            int ii = x2 < 0 ? (int)x2 - 1 : (int)x2, jj = y2 < 0 ? (int)y2 - 1 : (int)y2;
+            int k0 = ii + jj * nx, k1 = k0 + 1, k2 = k0 + nx, k3 = k0 + nx + 1;
+            if (ii == nx - 1) { k1--; k3--; }
+            if (jj == ny - 1) { k2 -= nx; k3 -= nx; }
+            float tt = x2 - ii, u = y2 - jj;
+            out[i + j * nx] = (1 - tt) * (1 - u) * img[k0] + tt * (1 - u) * img[k1] + (1 - tt) * u * img[k2] +
+                              tt * u * img[k3];
+        }
+    return 1;
+}
+
 static int floordiv2(int a)
 {
     /* python old_div(a,2) for ints == floor division */
     return a >= 0 ? a / 2 : -((-a + 1) / 2);
 }
 
+static void prep_particle_impl(const float *raw, int B, int P, const or_ctf *ctf, const float *xf, float *out);
+
 void or_prep_particle(const float *raw, int B, int P, const or_ctf *ctf, float *out)
+{
+    prep_particle_impl(raw, B, P, ctf, NULL, out);
+}
+
+/* the same with the particle's Transform given to preprocess_slice
+ * (e2make3dpar.py:716): its 2-D part resamples the padded, flipped image */
+void or_prep_particle_xform(const float *raw, int B, int P, const or_ctf *ctf, const float xf[12], float *out)
+{
+    prep_particle_impl(raw, B, P, ctf, xf, out);
+}
+
+static void prep_particle_impl(const float *raw, int B, int P, const or_ctf *ctf, const float *xf, float *out)
 {
     /* programs/e2make3dpar.py:630-654 (edge-mean subtract, clip/pad),
      * :704-714 (CTF_SIGN flip: fft, mult, ift), :716 preprocess_slice */
@@ -699,6 +781,12 @@ void or_prep_particle(const float *raw, int B, int P, const or_ctf *ctf, float *
         or_fft_c2r_2d(F, P, P, pad);
         free(F);
         free(sg);
+    }
+    if (xf) {
+        float *tr = (float *)malloc(sizeof(float) * P * P);
+        or_xform2d(pad, P, P, xf, tr);
+        memcpy(pad, tr, sizeof(float) * P * P);
+        free(tr);
     }
     or_preprocess_slice(pad, P, out);
     free(pad);

@@ -74,6 +74,8 @@ void or_ctf_amp_2d(const or_ctf *c, int ny, float *out /* (ny+2) x ny interleave
 void or_preprocess_slice(const float *img, int P, float *out /* (P+2)*P */);
 void or_prep_particle(const float *raw, int B, int P, const or_ctf *ctf_or_null,
                       float *out /* (P+2)*P */);
+void or_prep_particle_xform(const float *raw, int B, int P, const or_ctf *ctf, const float xf[12], float *out);
+int or_xform2d(const float *img, int nx, int ny, const float t[12], float *out);
 
 /* ---- volume insertion (libEM/reconstructor.cpp:1122-1210,
  *      libEM/reconstructor_tools.cpp:128-239, libEM/emdata_core.h:368-413) */

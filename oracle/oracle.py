@@ -60,6 +60,8 @@ def lib(fast=False):
         "or_rotation_only": (v, [_fp]),
         "or_sym_nsym": (i, [ctypes.c_char_p]),
         "or_sym_get": (i, [ctypes.c_char_p, _fp]),
+        "or_xform2d": (i, [_fp, i, i, _fp, _fp]),
+        "or_prep_particle_xform": (v, [_fp, i, i, ctypes.POINTER(CtfStruct), _fp, _fp]),
         "or_insert_slice_ssnr": (i, [i, _fp, _fp, _fp, _fp, _fp, i, _fp, i, i]),
         "or_edge_mean": (f, [_fp, i, i]),
         "or_clip_2d": (v, [_fp, i, i, i, i, i, i, _fp]),
@@ -259,13 +261,26 @@ def preprocess_slice(img):
     return out
 
 
-def prep_particle(raw, P, ctf=None):
+def prep_particle(raw, P, ctf=None, xform=None):
+    """caller prep + preprocess_slice; xform: the particle's 3x4 Transform,
+    whose 2-D part (translation / scale / mirror) preprocess_slice applies"""
     raw = f32(raw)
     B = raw.shape[0]
     out = np.zeros((P, P + 2), np.float32)
     cp = ctypes.byref(ctf_struct(ctf)) if ctf is not None else None
-    lib().or_prep_particle(_f(raw), B, P, cp, _f(out))
+    if xform is None:
+        lib().or_prep_particle(_f(raw), B, P, cp, _f(out))
+    else:
+        lib().or_prep_particle_xform(_f(raw), B, P, cp, _f(f32(xform).reshape(12)), _f(out))
     return out
+
+
+def xform2d(img, xform):
+    """preprocess_slice's 2-D resampling of a real image; (image, resampled?)"""
+    img = f32(img)
+    out = np.zeros_like(img)
+    r = lib().or_xform2d(_f(img), img.shape[1], img.shape[0], _f(f32(xform).reshape(12)), _f(out))
+    return out, bool(r)
 
 
 # ---------------------------------------------------------------- volume

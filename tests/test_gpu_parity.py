@@ -198,6 +198,85 @@ def test_usessnr_batch_and_slice(oracle, sym, mode):
     assert_state_close(gd3, gn3, w.data, w.norm)
 
 
+def _with_2d(mats, seed, mirror_scale=False):
+    """the particles' Transforms with 2-D translations (and some x mirrors /
+    scales), the part preprocess_slice applies to the image"""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i, m in enumerate(mats):
+        t = np.array(m, np.float32).reshape(3, 4).copy()
+        t[:2, 3] = rng.uniform(-3.0, 3.0, 2)
+        if mirror_scale and i % 3 == 1:
+            t[0, :] *= -1
+        if mirror_scale and i % 3 == 2:
+            t[:, :3] *= np.float32(1.25)
+        out.append(t)
+    return np.array(out, np.float32)
+
+
+@pytest.mark.parametrize("mirror_scale", [False, True])
+def test_2d_transforms_in_preprocess(oracle, mirror_scale):
+    """preprocess_slice's translation / mirror / scale (reconstructor.cpp:1031-1035)
+    on the batched, single-slice and agreement paths."""
+    from eman2_amd.reconstructor import EMData
+    P, B, n = 48, 32, 24
+    ph = S.phantom(10, 10.0, seed=11)
+    imgs, mats, _ = S.particles(ph, n, B, seed=12, snr=1.0)
+    xfs = _with_2d(mats, 13, mirror_scale)
+    w = np.linspace(0.5, 1.5, n).astype(np.float32)
+    slices = [oracle.prep_particle(im, P, xform=t) for im, t in zip(imgs, xfs)]
+    v = oracle_volume(oracle, P, "d2", "gauss_2", slices, xfs, w)
+    r = recon(P, "d2", "gauss_2", slices_per_item=8)
+    r.insert_batch(imgs, list(xfs), w)
+    gd, gn = r.get_state()
+    assert_state_close(gd, gn, v.data, v.norm)
+    # raw P x P images (zero border: edge mean 0) through insert_slice
+    r2 = recon(P, "d2", "gauss_2")
+    o = (P - B) // 2
+    for im, t, wi in zip(imgs, xfs, w):
+        pad = np.zeros((P, P), np.float32)
+        pad[o:o + B, o:o + B] = im - oracle.edge_mean(im)
+        r2.insert_slice(EMData(pad), t, float(wi))
+    gd2, gn2 = r2.get_state()
+    v2 = oracle.Volume(P, "d2", "gauss_2")
+    for im, t, wi in zip(imgs, xfs, w):
+        pad = np.zeros((P, P), np.float32)
+        pad[o:o + B, o:o + B] = im - oracle.edge_mean(im)
+        v2.insert(oracle.prep_particle(pad, P, xform=t), t, float(wi))
+    assert_state_close(gd2, gn2, v2.data, v2.norm)
+    assert_map_close(r.finish(True, return_numpy=True), v.finish(True))
+
+
+def test_2d_transforms_in_agreement(oracle):
+    """agreement_batch of translated / mirrored / scaled particles against a
+    well-covered volume (sparse volumes keep summation-order residues of the
+    subtracted slice, which decide whether a voxel is compared)"""
+    P, B, n = 32, 24, 500
+    ph = S.phantom(10, 7.0, seed=45)
+    imgs, mats, _ = S.particles(ph, n, B, seed=46, snr=0.5)
+    xfs = _with_2d(mats, 47, True)
+    w = np.ones(n, np.float32)
+    r = recon(P)
+    r.insert_batch(imgs, list(xfs), w)
+    got = r.agreement_batch(imgs[:12], xfs[:12], w[:12], sub=True, apix=1.0)
+    v = oracle_volume(oracle, P, "c1", "gauss_2", [oracle.prep_particle(im, P, xform=t) for im, t in zip(imgs, xfs)],
+                      xfs, w)
+    for i in range(12):
+        a = v.agreement(oracle.prep_particle(imgs[i], P, xform=xfs[i]), xfs[i], 1.0, sub=True, apix=1.0)
+        for j, k in enumerate(AGREE_KEYS):
+            assert abs(got[i, j] - a[k]) <= 1e-4 * max(1.0, abs(a[k])), (i, k, got[i, j], a[k])
+
+
+def test_2d_transform_with_ctf_rejected():
+    from eman2_amd import _lib
+    P, B = 48, 32
+    ph = S.phantom(5, 10.0, seed=14)
+    imgs, mats, dfs = S.particles(ph, 4, B, seed=15, snr=1.0, ctf=dict(CTF))
+    r = recon(P)
+    with pytest.raises(_lib.FourierHipError, match="InvalidParameterException"):
+        r.insert_batch(imgs, list(_with_2d(mats, 16)), np.ones(4, np.float32), defocus=dfs, ctf=CTF)
+
+
 def test_plain_matches_batch():
     P = 96
     imgs, mats, _ = config1_inputs(60)
@@ -453,9 +532,10 @@ def test_graph_batch_matches_eager(sym, mode):
         hip.hipFree(dimg)
 
 
-def test_transform_with_translation_rejected():
+def test_singular_transform_rejected():
     P = 16
     r = recon(P)
     img = np.zeros((P, P), np.float32)
-    with pytest.raises(RuntimeError, match="InvalidParameter"):
-        r.insert_slice(img, Transform({"type": "eman", "az": 10, "tx": 2.0}), 1.0)
+    assert r.insert_slice(img, Transform({"type": "eman", "az": 10, "tx": 2.0}), 1.0) == 0
+    with pytest.raises(RuntimeError, match="determinant"):
+        r.insert_slice(img, np.zeros((3, 4), np.float32), 1.0)

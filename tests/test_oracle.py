@@ -225,6 +225,67 @@ def test_bad_symmetry_rejected(oracle):
             oracle.sym_get(s)
 
 
+# ------------------------------------------------------------------ preprocess_slice 2-D part
+def _xf(tx=0.0, ty=0.0, az=0.0, alt=0.0, phi=0.0, scale=1.0, mirror=False):
+    m = euler_eman_matrix(az, alt, phi).astype(np.float32)
+    m[:, :3] *= np.float32(scale)
+    m[:, 3] = (tx, ty, 0.0)
+    if mirror:
+        m[0, :] *= -1  # Transform::set_mirror toggles row 0 (transform.cpp:1269-1279)
+    return m
+
+
+def test_xform2d_closed_forms(oracle):
+    """TransformProcessor's 2-D resampling: a translation t moves the image
+    content by +t (zero fill), a half-pixel shift averages neighbours, an x
+    mirror maps column i to P - i, a scale of 2 magnifies about the centre,
+    the identity is a copy; the rotation part is reset by preprocess_slice."""
+    P = 16
+    rng = np.random.default_rng(21)
+    img = rng.standard_normal((P, P)).astype(np.float32)
+    out, r = oracle.xform2d(img, _xf())
+    assert not r and np.array_equal(out, img)
+    out, r = oracle.xform2d(img, _xf(3, -2))
+    ref = np.zeros_like(img)
+    ref[0:P - 2, 3:] = img[2:, :P - 3]
+    assert r and np.array_equal(out, ref)
+    out2, _ = oracle.xform2d(img, _xf(3, -2, az=30, alt=40, phi=50))
+    assert np.array_equal(out2, out)
+    out, _ = oracle.xform2d(img, _xf(0.5, 0))
+    ref = np.zeros_like(img)
+    ref[:, 1:] = 0.5 * img[:, :-1] + 0.5 * img[:, 1:]
+    np.testing.assert_allclose(out, ref, atol=1e-6)
+    out, _ = oracle.xform2d(img, _xf(mirror=True))
+    ref = np.zeros_like(img)
+    ref[:, 1:] = img[:, :0:-1]
+    assert np.array_equal(out, ref)
+    out, _ = oracle.xform2d(img, _xf(scale=2.0))
+    c = P // 2
+    for j in range(P):
+        for i in range(P):
+            x, y = 0.5 * (i - c) + c, 0.5 * (j - c) + c
+            i0, j0 = int(x), int(y)
+            t, u = x - i0, y - j0
+            i1, j1 = min(i0 + 1, P - 1), min(j0 + 1, P - 1)
+            v = (1 - t) * (1 - u) * img[j0, i0] + t * (1 - u) * img[j0, i1] + (1 - t) * u * img[j1, i0] + t * u * img[j1, i1]
+            assert abs(out[j, i] - v) < 1e-5
+
+
+def test_prep_translation_is_phase_ramp(oracle):
+    """Fourier shift theorem: an integer translation d of a zero-padded
+    particle multiplies its slice by exp(-2 pi i k.d / P)."""
+    P, B = 32, 20
+    rng = np.random.default_rng(22)
+    raw = rng.standard_normal((B, B)).astype(np.float32)
+    a = cplx(oracle.prep_particle(raw, P))
+    dx, dy = 3, -4
+    b = cplx(oracle.prep_particle(raw, P, xform=_xf(dx, dy)))
+    kx = np.arange(P // 2 + 1)[None, :]
+    ky = np.fft.fftfreq(P, 1.0 / P)[:, None]
+    ramp = np.exp(-2j * np.pi * (kx * dx + ky * dy) / P)
+    assert np.abs(b - a * ramp).max() <= 2e-5 * np.abs(a).max()
+
+
 # ------------------------------------------------------------------ prep
 def test_edge_mean_closed_form(oracle):
     img = np.random.default_rng(5).random((20, 16)).astype(np.float32)
