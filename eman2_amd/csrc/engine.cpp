@@ -273,10 +273,12 @@ CtfConst ctf_constants(const fh_ctf &c, int ny)
     return k;
 }
 
-bool smooth_235(int n)
+// sizes whose prime factors the FFT codelets cover: every even entry of
+// EMAN2's good_box_sizes (libpyEM/EMAN2.py:660) has factors 2, 3, 5, 7, 11, 13
+bool fft_size_ok(int n)
 {
     if (n < 2) return false;
-    for (int p : {2, 3, 5})
+    for (int p : {2, 3, 5, 7, 11, 13})
         while (n % p == 0) n /= p;
     return n == 1;
 }
@@ -476,8 +478,8 @@ int fh_create(const fh_params *p, fh_reconstructor **out)
         g_create_error = "ImageDimensionException: fourier_hip requires a cubic size";
         return FH_ERR_IMAGE_DIM;
     }
-    if (p->nx < 8 || p->nx % 2 || !smooth_235(p->nx)) {
-        g_create_error = "ImageDimensionException: size must be even, >= 8 and 2^a 3^b 5^c (got " +
+    if (p->nx < 8 || p->nx % 2 || !fft_size_ok(p->nx)) {
+        g_create_error = "ImageDimensionException: size must be even, >= 8 and 13-smooth (2^a 3^b 5^c 7^d 11^e 13^f, got " +
                          std::to_string(p->nx) + ")";
         return FH_ERR_IMAGE_DIM;
     }

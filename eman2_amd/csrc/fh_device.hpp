@@ -102,6 +102,65 @@ __device__ __forceinline__ void dft<5>(float2 *v)
     v[3] = csub(r2, mul_mi(i2));
 }
 
+// Odd prime radices 7, 11, 13 (the remaining factors of EMAN2's good box
+// sizes): X[0] = sum v; for k = 1..(R-1)/2, with p_j = v_j + v_{R-j} and
+// m_j = v_j - v_{R-j}: X[k] = v_0 + sum_j cos(2 pi jk/R) p_j - i sum_j
+// sin(2 pi jk/R) m_j and X[R-k] the same with +i.
+constexpr float prime_cos(int R, int k)
+{
+    return R == 7 ? (k == 1 ? 0.6234898018587336f : k == 2 ? -0.22252093395631434f : -0.900968867902419f)
+         : R == 11 ? (k == 1 ? 0.8412535328311812f : k == 2 ? 0.41541501300188644f : k == 3 ? -0.142314838273285f
+                     : k == 4 ? -0.654860733945285f : -0.9594929736144974f)
+         : (k == 1 ? 0.8854560256532099f : k == 2 ? 0.5680647467311559f : k == 3 ? 0.120536680255323f
+            : k == 4 ? -0.35460488704253545f : k == 5 ? -0.7485107481711012f : -0.970941817426052f);
+}
+constexpr float prime_sin(int R, int k)
+{
+    return R == 7 ? (k == 1 ? 0.7818314824680298f : k == 2 ? 0.9749279121818236f : 0.43388373911755823f)
+         : R == 11 ? (k == 1 ? 0.5406408174555976f : k == 2 ? 0.9096319953545183f : k == 3 ? 0.9898214418809328f
+                     : k == 4 ? 0.7557495743542583f : 0.28173255684142967f)
+         : (k == 1 ? 0.4647231720437685f : k == 2 ? 0.8229838658936564f : k == 3 ? 0.992708874098054f
+            : k == 4 ? 0.9350162426854148f : k == 5 ? 0.6631226582407952f : 0.23931566428755768f);
+}
+
+template <int R>
+__device__ __forceinline__ void dft_prime(float2 *v)
+{
+    constexpr int H = (R - 1) / 2;
+    float2 p[H], m[H];
+    const float2 x0 = v[0];
+    float2 sum = v[0];
+#pragma unroll
+    for (int j = 0; j < H; j++) {
+        p[j] = cadd(v[j + 1], v[R - 1 - j]);
+        m[j] = csub(v[j + 1], v[R - 1 - j]);
+        sum = cadd(sum, p[j]);
+    }
+#pragma unroll
+    for (int k = 1; k <= H; k++) {
+        float2 re = x0, im = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int j = 1; j <= H; j++) {
+            const int a = (j * k) % R;  // cos/sin(2 pi a / R), folded into (0, R/2)
+            const float cc = a <= H ? prime_cos(R, a) : prime_cos(R, R - a);
+            const float ss = a <= H ? prime_sin(R, a) : -prime_sin(R, R - a);
+            re.x += cc * p[j - 1].x;
+            re.y += cc * p[j - 1].y;
+            im.x += ss * m[j - 1].x;
+            im.y += ss * m[j - 1].y;
+        }
+        v[k] = cadd(re, mul_mi(im));
+        v[R - k] = csub(re, mul_mi(im));
+    }
+    v[0] = sum;
+}
+template <>
+__device__ __forceinline__ void dft<7>(float2 *v) { dft_prime<7>(v); }
+template <>
+__device__ __forceinline__ void dft<11>(float2 *v) { dft_prime<11>(v); }
+template <>
+__device__ __forceinline__ void dft<13>(float2 *v) { dft_prime<13>(v); }
+
 // 6 = 2 x 3: DFT3 of the even and odd halves, X[k] = E[k] + W6^k O[k],
 // X[k+3] = E[k] - W6^k O[k].
 template <>
@@ -230,6 +289,9 @@ __device__ void fft_lds_inplace(float2 *buf, const FftPlan &plan, const float2 *
         case 4: stockham_pass_inplace<4, T, 4>(buf, plan.N, Ns, mag, twm, tw); break;
         case 5: stockham_pass_inplace<5, T, 3>(buf, plan.N, Ns, mag, twm, tw); break;
         case 6: stockham_pass_inplace<6, T, 3>(buf, plan.N, Ns, mag, twm, tw); break;
+        case 7: stockham_pass_inplace<7, T, 2>(buf, plan.N, Ns, mag, twm, tw); break;
+        case 11: stockham_pass_inplace<11, T, 2>(buf, plan.N, Ns, mag, twm, tw); break;
+        case 13: stockham_pass_inplace<13, T, 1>(buf, plan.N, Ns, mag, twm, tw); break;
         default: stockham_pass_inplace<8, T, 2>(buf, plan.N, Ns, mag, twm, tw); break;
         }
     }
@@ -250,6 +312,9 @@ __device__ float2 *fft_lds(float2 *a, float2 *b, const FftPlan &plan, const floa
         case 4: stockham_pass<4, T>(a, b, plan.N, Ns, mag, twm, tw); break;
         case 5: stockham_pass<5, T>(a, b, plan.N, Ns, mag, twm, tw); break;
         case 6: stockham_pass<6, T>(a, b, plan.N, Ns, mag, twm, tw); break;
+        case 7: stockham_pass<7, T>(a, b, plan.N, Ns, mag, twm, tw); break;
+        case 11: stockham_pass<11, T>(a, b, plan.N, Ns, mag, twm, tw); break;
+        case 13: stockham_pass<13, T>(a, b, plan.N, Ns, mag, twm, tw); break;
         default: stockham_pass<8, T>(a, b, plan.N, Ns, mag, twm, tw); break;
         }
         __syncthreads();
@@ -268,9 +333,9 @@ __device__ float2 *fft_lds(float2 *a, float2 *b, const FftPlan &plan, const floa
 // ---------------------------------------------------------------------------
 constexpr int plan_radix(int N, int p)
 {
-    const int order[6] = {8, 6, 4, 2, 3, 5};
+    const int order[9] = {8, 6, 4, 2, 3, 5, 7, 11, 13};
     int m = N, k = 0;
-    for (int oi = 0; oi < 6; oi++)
+    for (int oi = 0; oi < 9; oi++)
         while (m % order[oi] == 0) {
             if (k == p) return order[oi];
             k++;

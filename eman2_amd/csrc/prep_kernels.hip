@@ -21,6 +21,9 @@ bool make_plan(int N, FftPlan *p)
     while (m % 2 == 0 && p->npass < 16) { p->radix[p->npass++] = 2; m /= 2; }
     while (m % 3 == 0 && p->npass < 16) { p->radix[p->npass++] = 3; m /= 3; }
     while (m % 5 == 0 && p->npass < 16) { p->radix[p->npass++] = 5; m /= 5; }
+    while (m % 7 == 0 && p->npass < 16) { p->radix[p->npass++] = 7; m /= 7; }
+    while (m % 11 == 0 && p->npass < 16) { p->radix[p->npass++] = 11; m /= 11; }
+    while (m % 13 == 0 && p->npass < 16) { p->radix[p->npass++] = 13; m /= 13; }
     if (m != 1 || N <= 1 || N >= 65536) return false;
     int ns = 1;
     for (int i = 0; i < p->npass; i++) {

@@ -40,7 +40,7 @@ def _create(**kw):
 @pytest.mark.parametrize("kw,code,word", [
     (dict(nx=97, ny=97, nz=97), 3, "ImageDimension"),
     (dict(nx=96, ny=96, nz=64), 3, "cubic"),
-    (dict(nx=98, ny=98, nz=98), 3, "2^a 3^b 5^c"),
+    (dict(nx=34, ny=34, nz=34), 3, "13-smooth"),
     (dict(sym=b"x5"), 2, "symmetry"),
     (dict(sym=b"icos2"), 2, "symmetry"),
     (dict(mode=b"gauss_5"), 2, "mode"),

@@ -46,7 +46,7 @@ def assert_map_close(a, b, tol=1e-4, fmin=0.9999):
 
 
 # ------------------------------------------------------------------ prep
-@pytest.mark.parametrize("P", [16, 96, 384])
+@pytest.mark.parametrize("P", [16, 96, 384, 56, 88, 104, 168, 448])
 def test_preprocess_slice(oracle, P):
     img = np.random.default_rng(P).standard_normal((P, P)).astype(np.float32)
     r = recon(P)
@@ -275,6 +275,21 @@ def test_2d_transform_with_ctf_rejected():
     r = recon(P)
     with pytest.raises(_lib.FourierHipError, match="InvalidParameterException"):
         r.insert_batch(imgs, list(_with_2d(mats, 16)), np.ones(4, np.float32), defocus=dfs, ctf=CTF)
+
+
+@pytest.mark.parametrize("P,B", [(56, 40), (88, 60), (104, 64), (168, 112)])
+def test_radix_7_11_13_pads(oracle, P, B):
+    """good_box_sizes pads with factors 7, 11, 13 (libpyEM/EMAN2.py:660):
+    prep, batched insertion and finish() with the 3-D inverse FFT"""
+    ph = S.phantom(10, B / 4.0, seed=P)
+    imgs, mats, _ = S.particles(ph, 30, B, seed=P + 1, snr=1.0)
+    w = np.ones(30, np.float32)
+    r = recon(P, "c1", "gauss_2")
+    r.insert_batch(imgs, list(mats), w)
+    gd, gn = r.get_state()
+    v = oracle_volume(oracle, P, "c1", "gauss_2", [oracle.prep_particle(im, P) for im in imgs], mats, w)
+    assert_state_close(gd, gn, v.data, v.norm)
+    assert_map_close(r.finish(True, return_numpy=True), v.finish(True))
 
 
 def test_plain_matches_batch():
