@@ -1138,6 +1138,36 @@ int fh_insert_plain(fh_reconstructor *h, const float *imgs, int n, int box, cons
     return FH_OK;
 }
 
+int fh_projection(fh_reconstructor *h, const float xform[12], int ret_fourier, float *out)
+{
+    if (!h) return FH_ERR_NULL;
+    if (!xform || !out) return fail(h, FH_ERR_NULL, "NullPointerException: projection output is NULL");
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (!h->ready) return fail(h, FH_ERR_STATE, "projection() before setup() or after finish()");
+    FH_TRY(h, hipSetDevice(h->device));
+    const int P = h->P, H = h->H;
+    // only the rotation enters the slice (reconstructor.cpp:1398-1401); the
+    // translation (Transform::get_trans, transform.cpp:1077-1092) shifts it
+    const bool mirror = determinant(xform) < 0;
+    float tx = mirror ? -xform[3] : xform[3], ty = xform[7];
+    apply_precision(tx, ERR_LIMIT);
+    apply_precision(ty, ERR_LIMIT);
+    float rot9[9];
+    rotation_only(xform, rot9);
+    int rc = ensure_chunk(h, std::max(1, h->cap_n), std::max(P, h->cap_B));
+    if (rc) return rc;
+    const InsertConst c = iconst(h);
+    FH_TRY(h, launch_project(h->data, h->norm, c, rot9, h->rtab, tx, ty, ret_fourier ? 0 : 1, h->slices, h->stream));
+    if (ret_fourier) {
+        FH_TRY(h, hipMemcpyAsync(out, h->slices, sizeof(float2) * P * (H + 1), hipMemcpyDeviceToHost, h->stream));
+    } else {
+        FH_TRY(h, launch_ift2d(h->slices, P, h->imgs, h->plan, h->tw, h->stream));
+        FH_TRY(h, hipMemcpyAsync(out, h->imgs, sizeof(float) * P * P, hipMemcpyDeviceToHost, h->stream));
+    }
+    FH_TRY(h, hipStreamSynchronize(h->stream));
+    return FH_OK;
+}
+
 // rmax of do_compare_slice_work (libEM/reconstructor.cpp:1272-1274)
 static float agree_rmax(int P, float apix)
 {

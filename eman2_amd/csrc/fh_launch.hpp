@@ -109,6 +109,12 @@ hipError_t launch_scale_f32(float *p, float v, size_t n, hipStream_t s);
 // preprocess_slice's 2-D resampling of n padded particles (xf: n x 6 inverse rows) -> n x P x P
 hipError_t launch_xform2d(const float *imgs, int n, int B, int P, const float *means, const float *xf, float *out,
                           hipStream_t s);
+// projection(): the (P, P/2+1) complex slice of rotation rot9 (3x3 rows), shifted by
+// (xshift, yshift); center: phase origin to centre for the real-space output
+hipError_t launch_project(const float2 *data, const float *norm, InsertConst c, const float *rot9, const float *rtab,
+                          float xshift, float yshift, int center, float2 *out, hipStream_t s);
+// 2-D do_ift of one complex image in place (data is overwritten) into P x P floats
+hipError_t launch_ift2d(float2 *data, int P, float *out, const FftPlan &plan, const float2 *tw, hipStream_t s);
 hipError_t launch_compose(const float *rots, const float *weights, int n, const float *syms, int nsym,
                           float *t3d, float *wq, hipStream_t s);
 // rw: optional radial weights ((P/2+1) floats, class_ssnr weighting) replacing weight

@@ -675,6 +675,99 @@ hipError_t launch_agree_batch(const float2 *slices, int nsym, const float *t3d, 
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------ projection
+// FourierReconstructor::projection (libEM/reconstructor.cpp:1383-1445): the
+// slice pixel (x, y) takes the nearest voxel's value over its weight
+// (pixel_at, :1448-1461) when the weight is at least 0.005, halved on the
+// x = 0 column.  The result is what set_complex_at (emdata_core.cpp:171-205)
+// leaves in the (P+2) x P image after the reference's loop: on the x = 0 and
+// x = P/2 columns each pixel also writes its conjugate at -y, so a stored
+// row holds the later of the two iterations that write it (y > 0 comes after
+// y < 0), rows 0 and P/2 keep the real part only.
+__device__ __forceinline__ bool project_pixel(int x, int y, const Mat9 &m, const InsertConst &c,
+                                              const float *__restrict__ rtab, const float2 *__restrict__ data,
+                                              const float *__restrict__ norm, float &re, float &im)
+{
+    const int H = c.H, P = c.P;
+    const float rx = rtab[x + H], ry = rtab[y + H];
+    const float c0 = rx * m.m[0] + ry * m.m[3];
+    const float c1 = rx * m.m[1] + ry * m.m[4];
+    const float c2 = rx * m.m[2] + ry * m.m[5];
+    if (fabsf(c0) > 0.5f || fabsf(c1) >= 0.5f || fabsf(c2) >= 0.5f) return false;
+    const int x0 = round_ref(c0 * c.Pf), y0 = round_ref(c1 * c.Pf), z0 = round_ref(c2 * c.Pf);
+    size_t ci;
+    if (x0 < 0) ci = cidx(-x0, y0 <= 0 ? -y0 : P - y0, z0 <= 0 ? -z0 : P - z0, P, H);
+    else ci = cidx(x0, y0 < 0 ? P + y0 : y0, z0 < 0 ? P + z0 : z0, P, H);
+    const float nv = norm[ci];
+    if ((double)nv < 0.005) return false;  // dt[2] < 0.005, a double comparison
+    const float2 v = data[ci];
+    re = __fdiv_rn(v.x, nv);
+    im = __fdiv_rn(x0 < 0 ? -v.y : v.y, nv);
+    if (x == 0 && y != 0 && y != -H) {
+        re /= 2;
+        im /= 2;
+    }
+    return true;
+}
+
+// shift: the translation's phase ramp of the "xform" processor's complex
+// branch (libEM/processor.cpp:11081-11176; with no rotation its bilinear
+// weights are exactly 1, 0, 0, 0), the reference's long double pi
+// arithmetic restated in double.  center: xform.phaseorigin.tocenter on the
+// complex image, x (-1)^(x + row) (sparx/emdata_sparx.cpp:8451-8460).
+__global__ void __launch_bounds__(256) k_project(const float2 *__restrict__ data, const float *__restrict__ norm,
+                                                 InsertConst c, Mat9 m, const float *__restrict__ rtab, float xshift,
+                                                 float yshift, int shift, int center, float2 *__restrict__ out)
+{
+    const int H = c.H, P = c.P;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= P * (H + 1)) return;
+    const int x = idx % (H + 1), r = idx / (H + 1);
+    float re = 0.f, im = 0.f, pr, pim;
+    if (x != 0 && x != H) {
+        if (!project_pixel(x, r < H ? r : r - P, m, c, rtab, data, norm, re, im)) re = im = 0.f;
+    } else if (r == 0 || r == H) {
+        if (project_pixel(x, r == 0 ? 0 : -H, m, c, rtab, data, norm, pr, pim)) re = pr;
+    } else {
+        const int a = r < H ? r : P - r;
+        if (project_pixel(x, a, m, c, rtab, data, norm, pr, pim)) {
+            re = pr;
+            im = r < H ? pim : -pim;
+        } else if (project_pixel(x, -a, m, c, rtab, data, norm, pr, pim)) {
+            re = pr;
+            im = r < H ? -pim : pim;
+        }
+    }
+    if (shift) {
+        const double pi = 3.141592653589793238462643383279502884;
+        const int ky = r >= H + 1 ? r - P : r;
+        const float phaseY = (float)(-2 * pi * ky * (double)yshift / P);
+        const float phase = (float)((double)phaseY - 2 * pi * x * (double)xshift / P);
+        const double cp = cos((double)phase), sp = sin((double)phase);
+        const float rb = (float)((double)re * cp - (double)im * sp);
+        const float ib = (float)((double)re * sp + (double)im * cp);
+        re = rb;
+        im = ib;
+    }
+    if (center && ((x + r) & 1)) {
+        re = -re;
+        im = -im;
+    }
+    out[idx] = make_float2(re, im);
+}
+
+hipError_t launch_project(const float2 *data, const float *norm, InsertConst c, const float *rot9, const float *rtab,
+                          float xshift, float yshift, int center, float2 *out, hipStream_t s)
+{
+    Mat9 m;
+    for (int i = 0; i < 9; i++) m.m[i] = rot9[i];
+    const int n = c.P * (c.H + 1);
+    const int shift = xshift != 0.0f || yshift != 0.0f;
+    hipLaunchKernelGGL(k_project, dim3((n + 255) / 256), dim3(256), 0, s, data, norm, c, m, rtab, xshift, yshift, shift,
+                       center, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_agree(const float2 *slices, int nsym, const float *t3d, int nslices, InsertConst c, float rmax,
                         const float2 *data, const float *norm, const float *rtab, double *partial, hipStream_t s)
 {

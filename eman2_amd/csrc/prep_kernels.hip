@@ -492,16 +492,19 @@ hipError_t launch_fft3d_inverse_zy(float2 *data, int P, const FftPlan &plan, con
 
 // c2r along x for row pairs, x 1/P^3 (do_ift_inplace, emdata_transform.cpp:392-425),
 // depad, and phase origin to centre (fftshift) on the write.
+// nz planes of P rows; shift: fftshift of the written volume (3-D finish),
+// else rows written in place (2-D projection).
 template <int T>
-__global__ void __launch_bounds__(256) k_c2r_x(const float2 *__restrict__ data, int P, float *__restrict__ out,
-                                               float scale, FftPlan plan, const float2 *__restrict__ twg)
+__global__ void __launch_bounds__(256) k_c2r_x(const float2 *__restrict__ data, int P, int nz, int shift,
+                                               float *__restrict__ out, float scale, FftPlan plan,
+                                               const float2 *__restrict__ twg)
 {
     extern __shared__ float2 smem[];
     float2 *tw = smem;
     float2 *A = tw + P;
     float2 *Bf = A + T * P;
     const int H = P / 2;
-    const size_t nrows = (size_t)P * P;
+    const size_t nrows = (size_t)nz * P;
     const size_t row0 = (size_t)blockIdx.x * 2 * T;
     load_twiddles(tw, twg, P);
     for (int idx = threadIdx.x; idx < T * P; idx += blockDim.x) {
@@ -531,6 +534,10 @@ __global__ void __launch_bounds__(256) k_c2r_x(const float2 *__restrict__ data, 
         for (int q = 0; q < 2; q++) {
             const size_t r = ra + q;
             if (r >= nrows) break;
+            if (!shift) {
+                out[r * P + x] = (q == 0 ? z.x : z.y) * scale;
+                continue;
+            }
             const int y = (int)(r % P), zz = (int)(r / P);
             const int yc = y + H < P ? y + H : y + H - P;
             const int zc = zz + H < P ? zz + H : zz + H - P;
@@ -539,18 +546,42 @@ __global__ void __launch_bounds__(256) k_c2r_x(const float2 *__restrict__ data, 
     }
 }
 
-hipError_t launch_c2r_x(const float2 *data, int P, float *out, float scale, const FftPlan &plan, const float2 *tw,
-                        hipStream_t s)
+static hipError_t c2r_rows(const float2 *data, int P, int nz, int shift, float *out, float scale,
+                           const FftPlan &plan, const float2 *tw, hipStream_t s)
 {
     const int T = pick_T(P);
     const size_t lds = fft_lds_bytes(P, T);
-    const size_t nrows = (size_t)P * P;
+    const size_t nrows = (size_t)nz * P;
     dim3 grid((unsigned)((nrows + 2 * T - 1) / (2 * T)));
     if (T == 8)
-        hipLaunchKernelGGL(k_c2r_x<8>, grid, dim3(256), lds, s, data, P, out, scale, plan, tw);
+        hipLaunchKernelGGL(k_c2r_x<8>, grid, dim3(256), lds, s, data, P, nz, shift, out, scale, plan, tw);
     else
-        hipLaunchKernelGGL(k_c2r_x<4>, grid, dim3(256), lds, s, data, P, out, scale, plan, tw);
+        hipLaunchKernelGGL(k_c2r_x<4>, grid, dim3(256), lds, s, data, P, nz, shift, out, scale, plan, tw);
     return hipGetLastError();
+}
+
+hipError_t launch_c2r_x(const float2 *data, int P, float *out, float scale, const FftPlan &plan, const float2 *tw,
+                        hipStream_t s)
+{
+    return c2r_rows(data, P, P, 1, out, scale, plan, tw, s);
+}
+
+// 2-D do_ift of one (P+2) x P complex image (emdata_transform.cpp:330-375):
+// inverse FFT along y of the P/2+1 columns, c2r along x, x 1/P^2.
+hipError_t launch_ift2d(float2 *data, int P, float *out, const FftPlan &plan, const float2 *tw, hipStream_t s)
+{
+    const int H = P / 2;
+    const int T = pick_T(P);
+    const size_t lds = fft_lds_bytes(P, T);
+    const size_t row = H + 1;
+    dim3 grid((H + 1 + T - 1) / T, 1);
+    if (T == 8)
+        hipLaunchKernelGGL(k_fft_axis_inv<8>, grid, dim3(256), lds, s, data, P, row, (size_t)0, plan, tw);
+    else
+        hipLaunchKernelGGL(k_fft_axis_inv<4>, grid, dim3(256), lds, s, data, P, row, (size_t)0, plan, tw);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return c2r_rows(data, P, 1, 0, out, 1.0f / ((float)P * (float)P), plan, tw, s);
 }
 
 // ---------------------------------------------------------------- fills

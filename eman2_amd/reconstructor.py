@@ -311,6 +311,16 @@ class FourierHipReconstructor:
                                                      None if d is None else _ptr(d),
                                                      None if c is None else ctypes.byref(c)))
 
+    def projection(self, xform, ret_fourier=0):
+        """FourierReconstructor::projection (libEM/reconstructor.cpp:1383-1445):
+        a new EMData, the complex (P, P+2) slice when ret_fourier, else the
+        real (P, P) image."""
+        m = _xform12(xform)
+        P = self.P
+        out = np.zeros((P, P + 2) if ret_fourier else (P, P), np.float32)
+        self._check(self._lib.fh_projection(self._need(), _ptr(m), int(bool(ret_fourier)), _ptr(out)))
+        return EMData(out, bool(ret_fourier))
+
     def determine_slice_agreement(self, img, xform, weight=1.0, sub=True):
         pre = int(isinstance(img, EMData) and bool(img.get_attr_default("reconstruct_preproc", 0)))
         a = _arr(img)

@@ -176,6 +176,13 @@ int fh_agreement_batch(fh_reconstructor *h, const float *imgs, int n, int box,
                        const float *xforms, const float *weights, const float *defocus,
                        const fh_ctf *ctf, int sub, float apix, float *out, int where);
 
+/* FourierReconstructor::projection (reconstructor.cpp:1383-1445): the
+ * Fourier slice of the accumulating volume at the rotation of xform (value
+ * over weight of the nearest voxel, weights below 0.005 left out), shifted
+ * by xform's translation.  out (host): ret_fourier -> P x (P+2) slice floats,
+ * else the P x P real-space image (phase origin to centre, inverse FFT). */
+int fh_projection(fh_reconstructor *h, const float xform[12], int ret_fourier, float *out);
+
 /* FourierReconstructor::finish (reconstructor.cpp:1552-1641):
  * normalize_threed, then (doift) 3-D c2r, x 1/P^3, depad, phase origin to
  * centre.  out (host): P^3 floats if doift, else the normalised complex

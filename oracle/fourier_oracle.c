@@ -1213,3 +1213,109 @@ void or_setup_seed(int P, float *data, float *norm, const float *seed, float see
     NV(nnx - 1, 0, nz / 2) = seed_weight;
 #undef NV
 }
+
+/* ======================================================================== */
+/* FourierReconstructor::projection (libEM/reconstructor.cpp:1383-1445)      */
+/* ======================================================================== */
+
+/* EMData::set_complex_at(x, y) on a 2-D (P+2) x P complex image
+ * (libEM/emdata_core.cpp:171-205): the x = 0 and x = nx/2-1 columns also
+ * write the conjugate at -y, rows 0 and P/2 keep the real part only. */
+static void set_complex_2d(float *r, int P, int x, int y, float re, float im)
+{
+    const int nx = P + 2, ny = P;
+    if (abs(x) >= nx / 2 || abs(y) > ny / 2) return;
+    if (x == 0 || x == nx / 2 - 1) {
+        const int c = x == 0 ? 0 : nx - 2;
+        if (y == 0) { r[c] = re; r[c + 1] = 0; }
+        else if (y == ny / 2 || y == -ny / 2) { r[ny / 2 * nx + c] = re; r[ny / 2 * nx + c + 1] = 0; }
+        else if (y > 0) {
+            r[y * nx + c] = r[(ny - y) * nx + c] = re;
+            r[y * nx + c + 1] = im;
+            r[(ny - y) * nx + c + 1] = -im;
+        } else {
+            r[(ny + y) * nx + c] = r[-y * nx + c] = re;
+            r[(ny + y) * nx + c + 1] = im;
+            r[-y * nx + c + 1] = -im;
+        }
+    } else if (x > 0 && y >= 0) { r[x * 2 + y * nx] = re; r[x * 2 + y * nx + 1] = im; }
+    else if (x > 0 && y < 0) { r[x * 2 + (ny + y) * nx] = re; r[x * 2 + (ny + y) * nx + 1] = im; }
+}
+
+/* out: ret_fourier -> (P+2) x P complex slice; else P x P real image.
+ * The translation of `euler` is applied by the "xform" processor's complex
+ * branch (libEM/processor.cpp:11081-11176) with no rotation, where the
+ * bilinear weights are exact (1, 0, 0, 0) and only the phase ramp remains;
+ * its long double pi arithmetic is kept (sparx/emconstants.h:38). */
+void or_projection(int P, const float *data, const float *norm, const float euler[12], int ret_fourier, float *out)
+{
+    const int nx = P + 2, ny = P, nz = P;
+    float rot[12];
+    memcpy(rot, euler, sizeof(rot));
+    /* translation of euler (Transform::get_trans, transform.cpp:1077-1092) */
+    const int mirror = get_determinant(euler) < 0;
+    float tx = mirror ? -euler[3] : euler[3], ty = euler[7];
+    apply_precision(&tx, OR_ERR_LIMIT);
+    apply_precision(&ty, OR_ERR_LIMIT);
+    or_rotation_only(rot);
+    float *ret = (float *)calloc((size_t)nx * ny, sizeof(float));
+    for (int y = -ny / 2; y < ny / 2; y++) {
+        for (int x = 0; x <= nx / 2; x++) {
+            float rx = (float)x / (float)(nx - 2);
+            float ry = (float)y / (float)ny;
+            float c[3];
+            or_vec_times_transform(rx, ry, 0.0f, rot, c);
+            float xx = c[0], yy = c[1], zz = c[2];
+            if (fabsf(xx) > 0.5 || fabsf(yy) >= 0.5 || fabsf(zz) >= 0.5) continue;
+            xx = xx * (float)(nx - 2);
+            yy = yy * (float)ny;
+            zz = zz * (float)nz;
+            /* pixel_at (reconstructor.cpp:1448-1461) */
+            int x0 = or_round_f(xx), y0 = or_round_f(yy), z0 = or_round_f(zz);
+            float vr, vi;
+            vol_get(data, P, x0, y0, z0, &vr, &vi);
+            float nv = norm[vol_index_fast(P, x0, y0, z0) / 2];
+            float d0 = vr / nv, d1 = vi / nv;
+            if ((double)nv < 0.005) continue;
+            if ((x == 0 || x == nx / 2) && (y != 0 && y != -ny / 2)) { d0 /= 2; d1 /= 2; }
+            set_complex_2d(ret, P, x, y, d0, d1);
+        }
+    }
+    /* translation: the complex branch of TransformProcessor::transform */
+    /* (a zero translation leaves every value as it is, up to the sign of zeros) */
+    if (tx != 0.0f || ty != 0.0f) {
+        float *des = (float *)calloc((size_t)nx * ny, sizeof(float));
+        const long double pi = 3.141592653589793238462643383279502884197L;
+        const float xshift = tx, yshift = ty;
+        for (int kyN = 0; kyN < ny; kyN++) {
+            int kyNew = kyN >= nx / 2 ? kyN - ny : kyN;
+            float phaseY = (float)(-2 * pi * kyNew * yshift / ny);
+            for (int kxN = 0; kxN < nx / 2; kxN++) {
+                int kxNew = kxN;
+                /* with no rotation the lower-left neighbour is the element itself, weight 1 */
+                float tempR = ret[2 * kxN + kyN * nx], tempI = ret[2 * kxN + 1 + kyN * nx];
+                float phase = (float)(phaseY - 2 * pi * kxNew * xshift / ny);
+                float rb = (float)(tempR * cos((double)phase) - tempI * sin((double)phase));
+                float ib = (float)(tempR * sin((double)phase) + tempI * cos((double)phase));
+                des[2 * kxN + nx * kyN] = rb / 1.0f;
+                des[2 * kxN + 1 + nx * kyN] = ib / 1.0f;
+            }
+        }
+        memcpy(ret, des, sizeof(float) * nx * ny);
+        free(des);
+    }
+    if (ret_fourier) {
+        memcpy(out, ret, sizeof(float) * nx * ny);
+    } else {
+        /* xform.phaseorigin.tocenter on a complex image: center_origin_fft,
+         * x (-1)^(kx + row) (libEM/sparx/emdata_sparx.cpp:8451-8460), then do_ift */
+        for (int r = 0; r < ny; r++)
+            for (int k = 0; k < nx / 2; k++)
+                if ((k + r) % 2) {
+                    ret[2 * k + r * nx] *= -1.0f;
+                    ret[2 * k + 1 + r * nx] *= -1.0f;
+                }
+        or_fft_c2r_2d(ret, P, P, out);
+    }
+    free(ret);
+}

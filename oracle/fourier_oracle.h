@@ -110,4 +110,6 @@ void or_setup_seed(int P, float *data, float *norm, const float *seed, float see
 #ifdef __cplusplus
 }
 #endif
+void or_projection(int P, const float *data, const float *norm, const float euler[12], int ret_fourier, float *out);
+
 #endif

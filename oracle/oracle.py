@@ -61,6 +61,7 @@ def lib(fast=False):
         "or_sym_nsym": (i, [ctypes.c_char_p]),
         "or_sym_get": (i, [ctypes.c_char_p, _fp]),
         "or_xform2d": (i, [_fp, i, i, _fp, _fp]),
+        "or_projection": (v, [i, _fp, _fp, _fp, i, _fp]),
         "or_prep_particle_xform": (v, [_fp, i, i, ctypes.POINTER(CtfStruct), _fp, _fp]),
         "or_insert_slice_ssnr": (i, [i, _fp, _fp, _fp, _fp, _fp, i, _fp, i, i]),
         "or_edge_mean": (f, [_fp, i, i]),
@@ -319,6 +320,15 @@ class Volume:
         rc = lib().or_insert_slice_ssnr(self.P, _f(self.data), _f(self.norm), _f(s), _f(r), _f(self.syms),
                                         len(self.syms), _f(ss), int(ss.size), self.mode)
         assert rc == 0
+
+    def projection(self, euler, ret_fourier=True):
+        """FourierReconstructor::projection: (P, P+2) slice or (P, P) image"""
+        assert not self.fp64
+        P = self.P
+        out = np.zeros((P, P + 2) if ret_fourier else (P, P), np.float32)
+        lib().or_projection(P, _f(self.data), _f(self.norm), _f(f32(euler).reshape(12)), int(bool(ret_fourier)),
+                            _f(out))
+        return out
 
     def setup_seed(self, seed, seed_weight):
         assert not self.fp64

@@ -292,6 +292,28 @@ def test_radix_7_11_13_pads(oracle, P, B):
     assert_map_close(r.finish(True, return_numpy=True), v.finish(True))
 
 
+@pytest.mark.parametrize("ret_fourier", [1, 0])
+@pytest.mark.parametrize("P", [48, 56])
+def test_projection(oracle, ret_fourier, P):
+    """projection() (reconstructor.cpp:1383-1445) of the same volume state on
+    the GPU and in the oracle, with and without translations"""
+    B, n = 32, 200
+    ph = S.phantom(10, 10.0, seed=21)
+    imgs, mats, _ = S.particles(ph, n, B, seed=22, snr=2.0)
+    v = oracle_volume(oracle, P, "c1", "gauss_2", [oracle.prep_particle(im, P) for im in imgs], mats,
+                      np.ones(n, np.float32))
+    r = recon(P)
+    r.setup_seedandweights(v.data, v.norm)
+    rng = np.random.default_rng(23)
+    for k in range(6):
+        t = euler_eman_matrix(*(rng.random(3) * (360, 180, 360))).astype(np.float32)
+        if k % 2:
+            t[:2, 3] = rng.uniform(-4, 4, 2)
+        g = r.projection(t, ret_fourier).numpy()
+        o = v.projection(t, ret_fourier)
+        assert max_rel_err(g, o) <= (1e-6 if ret_fourier else 2e-5), (k, max_rel_err(g, o))
+
+
 def test_plain_matches_batch():
     P = 96
     imgs, mats, _ = config1_inputs(60)

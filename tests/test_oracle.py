@@ -631,3 +631,46 @@ def test_setup_seed_weights(oracle):
     n = v.norm
     assert n[1, 1, 1] == 2.0 and n[1, 1, 0] == 1.0 and n[1, 1, P // 2] == 1.0
     assert n[0, P // 2, 0] == 2.0 and n[P // 2, 0, 0] == 2.0
+
+
+# ------------------------------------------------------------------ projection
+def _blob(P, cx=2, cy=-1, sigma=3.0):
+    yy, xx = np.mgrid[0:P, 0:P]
+    return np.exp(-((xx - P // 2 - cx) ** 2 + (yy - P // 2 - cy) ** 2) / (2 * sigma ** 2)).astype(np.float32)
+
+
+def test_projection_inverts_identity_insertion(oracle):
+    """projection() (reconstructor.cpp:1383-1445) is the inverse of a single
+    identity-orientation trilinear insertion: the Fourier slice comes back
+    inside the insertion radius (x = 0 column halved back), the real-space
+    image comes back times preprocess_slice's sqrt((P+2)/P), and a
+    translation (dx, dy) shifts it circularly by (dx, dy)."""
+    P, H = 32, 16
+    img = _blob(P)
+    sl = oracle.preprocess_slice(img)
+    v = oracle.Volume(P, "c1", "trilinear")
+    I = np.eye(3, 4, dtype=np.float32)
+    v.insert(sl, I, 1.0)
+    A, B = cplx(v.projection(I, True)), cplx(sl)
+    ky = np.fft.fftfreq(P, 1.0 / P)[:, None]
+    kx = np.arange(H + 1)[None, :]
+    inside = kx ** 2 + ky ** 2 <= H * H + H
+    assert np.abs((A - B) * inside).max() <= 5e-6 * np.abs(B).max()
+    assert np.all(A[~inside] == 0)
+    rs = v.projection(I, False)
+    np.testing.assert_allclose(rs, img * np.sqrt((P + 2) / P), atol=2e-6)
+    t = I.copy()
+    t[:, 3] = (3, -2, 0)
+    np.testing.assert_allclose(v.projection(t, False), np.roll(rs, (-2, 3), axis=(0, 1)), atol=1e-6)
+
+
+def test_projection_weight_threshold(oracle):
+    """voxels with weight below 0.005 are left out (reconstructor.cpp:1428)"""
+    P = 16
+    sl = oracle.preprocess_slice(_blob(P, 0, 0, 2.0))
+    v = oracle.Volume(P, "c1", "trilinear")
+    I = np.eye(3, 4, dtype=np.float32)
+    v.insert(sl, I, 0.004)
+    assert np.all(v.projection(I, True) == 0)
+    v.insert(sl, I, 0.002)
+    assert np.abs(v.projection(I, True)).max() > 0
