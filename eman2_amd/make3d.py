@@ -1,0 +1,119 @@
+"""The caller loop of e2make3d.py's reconstruct() (programs/e2make3d.py:385-539)
+driving a reconstructor in batches: niter passes of insertion; between
+passes every particle's agreement with the volume (determine_slice_agreement,
+its own contribution removed unless it is excluded), its norm multiplied by
+reconstruct_norm, absolute qualities turned into relative ones by a running
+average over particles sorted by weight, and a quality cutoff that leaves
+the worst particles out of the next pass.  The last pass ends with finish().
+
+The reconstructor is anything with setup() / setup_seed() / insert_batch() /
+agreement_batch() / finish() -- the "fourier_hip" reconstructor here.  The
+reference multiplies each cached preprocessed slice by its norm; the batched
+form scales the raw particle instead, which is the same up to float
+rounding (edge-mean subtraction, padding and the FFT are linear).
+"""
+import numpy as np
+
+
+AGREE_KEYS = ("reconstruct_norm", "reconstruct_absqual", "reconstruct_absqual_lowres", "reconstruct_qual",
+              "reconstruct_weight")
+
+
+def relative_quality(weights, absqual):
+    """e2make3d.py:494-517: particles sorted by (weight, absqual, index); each
+    absolute quality over the mean of its sorted neighbours [i-10, i+10);
+    weight-0 particles get -1 and stay out of the returned list."""
+    n = len(weights)
+    order = sorted(range(n), key=lambda i: (float(weights[i]), float(absqual[i]), 0, i))
+    qual = np.zeros(n)
+    qlist = []
+    for k, i in enumerate(order):
+        if weights[i] == 0:
+            qual[i] = -1
+            continue
+        sub = [float(absqual[order[j]]) for j in range(max(0, k - 10), min(k + 10, n))]
+        mean = sum(sub) / len(sub)
+        try:
+            q = float(absqual[i]) / mean
+        except ZeroDivisionError:
+            q = -1
+        qual[i] = q
+        qlist.append(q)
+    return qual, qlist
+
+
+def quality_cutoff(absqual, qlist, keep, keepsig, keepabs):
+    """e2make3d.py:519-539"""
+    if keepabs:
+        sq = np.array(sorted(float(a) for a in absqual))
+        if keepsig:
+            return float(sq.mean() - sq.std() * keepabs)
+        return float(sq[-int(keep * len(qlist))])
+    if keepsig:
+        qmean = sum(qlist) / len(qlist)
+        qsigma = np.sqrt(sum(q * q for q in qlist) / len(qlist) - qmean ** 2)
+        return float(qmean - qsigma * keep)
+    ql = sorted(qlist)
+    return float(ql[-int(keep * len(ql))])
+
+
+def reconstruct(recon, imgs, xforms, weights=None, niter=2, keep=1.0, keepsig=False, keepabs=False,
+                start=None, startweight=0.0, apix=1.0, defocus=None, ctf=None):
+    """e2make3d.py reconstruct() over numpy particles (n, box, box) and their
+    3x4 Transforms.  Returns (map, info): the finished map and per-particle
+    norm / qualities plus the included / excluded indices of the last pass
+    and ptcl_repr (the summed weight of the inserted particles)."""
+    imgs = np.ascontiguousarray(imgs, np.float32)
+    n = len(imgs)
+    xf = np.ascontiguousarray(xforms, np.float32).reshape(n, 3, 4)
+    w = np.ones(n, np.float32) if weights is None else np.ascontiguousarray(weights, np.float32)
+    norm = np.ones(n, np.float64)
+    absqual = np.zeros(n)
+    qual = np.zeros(n)
+    qcutoff = None
+    out = None
+    included, excluded = [], []
+    ptcl = 0.0
+    for it in range(niter):
+        if start is not None:
+            recon.setup_seed(start, startweight)
+        else:
+            recon.setup()
+        # the quality test is skipped (try/except) until a cutoff exists
+        if qcutoff is None:
+            take = np.ones(n, bool)
+        else:
+            q = absqual if keepabs else qual
+            take = ~((q < qcutoff) | (w == 0))
+        if it == niter - 1:
+            included = [int(i) for i in np.nonzero(take)[0]]
+            excluded = [int(i) for i in np.nonzero(~take)[0]]
+        idx = np.nonzero(take & (w != 0))[0]
+        ptcl = float(w[idx].sum())
+        scaled = imgs[idx] * norm[idx].astype(np.float32)[:, None, None]
+        if len(idx):
+            recon.insert_batch(scaled, xf[idx], w[idx],
+                               defocus=None if defocus is None else np.asarray(defocus, np.float32)[idx], ctf=ctf)
+        if it != niter - 1:
+            # agreement of every particle; its own slice is removed unless it is excluded
+            dosub = take & (w != 0)
+            res = np.zeros((n, 5), np.float32)
+            scaled_all = imgs * norm.astype(np.float32)[:, None, None]
+            for flag in (True, False):
+                sel = np.nonzero(dosub == flag)[0]
+                if len(sel) == 0:
+                    continue
+                res[sel] = recon.agreement_batch(scaled_all[sel], xf[sel], w[sel],
+                                                 defocus=None if defocus is None else
+                                                 np.asarray(defocus, np.float32)[sel],
+                                                 ctf=ctf, sub=flag, apix=apix)
+            norm *= res[:, 0]
+            absqual = res[:, 1].astype(np.float64)
+            qual, qlist = relative_quality(w, absqual)
+            if qlist:
+                qcutoff = quality_cutoff(absqual, qlist, keep, keepsig, keepabs)
+        else:
+            out = recon.finish(True)
+    info = dict(norm=norm, reconstruct_absqual=absqual, reconstruct_qual=qual, qcutoff=qcutoff,
+                threed_ptcl_idxs=included, threed_excl_ptcl_idxs=excluded, ptcl_repr=ptcl)
+    return out, info

@@ -1,0 +1,73 @@
+"""The e2make3d.py reconstruct() loop (programs/e2make3d.py:385-539) over the
+HIP reconstructor against the same loop over the oracle: per-particle norms
+and qualities within 1e-4, final maps max|a-b|/max|b| <= 1e-4 with FSC >=
+0.9999; the quality cutoff leaves out the requested share."""
+import numpy as np
+import pytest
+
+from eman2_amd import make3d
+from eman2_amd import synthetic as S
+from eman2_amd.metrics import fsc, max_rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+class OracleRecon:
+    """the oracle behind the reconstructor calls make3d.reconstruct makes"""
+
+    def __init__(self, O, P, sym="c1", mode="gauss_2"):
+        self.O, self.P, self.sym, self.mode = O, P, sym, mode
+
+    def setup(self):
+        self.v = self.O.Volume(self.P, self.sym, self.mode)
+
+    def insert_batch(self, imgs, xforms, weights, defocus=None, ctf=None):
+        for im, m, w in zip(imgs, xforms, weights):
+            self.v.insert(self.O.prep_particle(im, self.P), m, float(w))
+
+    def agreement_batch(self, imgs, xforms, weights, defocus=None, ctf=None, sub=True, apix=1.0):
+        out = np.zeros((len(imgs), 5), np.float32)
+        for i, (im, m, w) in enumerate(zip(imgs, xforms, weights)):
+            r = self.v.agreement(self.O.prep_particle(im, self.P), m, float(w), sub=sub, apix=apix)
+            out[i] = [r[k] for k in make3d.AGREE_KEYS]
+        return out
+
+    def finish(self, doift=True):
+        return self.v.finish(True)
+
+
+def _inputs(n=300, P=32, B=24):
+    ph = S.phantom(10, 7.0, seed=61)
+    imgs, mats, _ = S.particles(ph, n, B, seed=62, snr=0.5)
+    w = np.ones(n, np.float32)
+    w[::17] = 0.0  # some particles with weight 0
+    return P, imgs, mats, w
+
+
+def test_make3d_loop_matches_oracle(oracle):
+    from eman2_amd import Reconstructors
+    P, imgs, mats, w = _inputs()
+    print("make3d P=%d n=%d" % (P, len(imgs)), flush=True)
+    r = Reconstructors.get("fourier_hip", dict(size=(P, P, P), sym="c1", mode="gauss_2"))
+    gmap, ginfo = make3d.reconstruct(r, imgs, mats, w, niter=2, keep=1.0)
+    omap, oinfo = make3d.reconstruct(OracleRecon(oracle, P), imgs, mats, w, niter=2, keep=1.0)
+    g = gmap.numpy() if hasattr(gmap, "numpy") else np.asarray(gmap)
+    assert max_rel_err(g, omap) <= 1e-4
+    assert fsc(g, omap).min() >= 0.9999
+    np.testing.assert_allclose(ginfo["norm"], oinfo["norm"], rtol=1e-4)
+    np.testing.assert_allclose(ginfo["reconstruct_absqual"], oinfo["reconstruct_absqual"], rtol=1e-4, atol=1e-6)
+    assert ginfo["ptcl_repr"] == oinfo["ptcl_repr"] == float(w.sum())
+
+
+def test_make3d_quality_cutoff():
+    from eman2_amd import Reconstructors
+    P, imgs, mats, w = _inputs()
+    r = Reconstructors.get("fourier_hip", dict(size=(P, P, P), sym="c1", mode="gauss_2"))
+    _, info = make3d.reconstruct(r, imgs, mats, w, niter=3, keep=0.8)
+    n = len(imgs)
+    nz = int((w != 0).sum())
+    excl = set(info["threed_excl_ptcl_idxs"])
+    assert set(np.nonzero(w == 0)[0]) <= excl
+    # qcutoff = the int(0.8 * len) -th largest relative quality: the rest are left out
+    assert len(excl) - (n - nz) == nz - int(0.8 * nz)
+    assert len(info["threed_ptcl_idxs"]) + len(excl) == n
