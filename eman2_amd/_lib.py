@@ -77,6 +77,7 @@ EXPORTS = {
     "fh_insert_batch": (_c_int, [_vp, _vp, _c_int, _c_int, _vp, _vp, _vp, ctypes.POINTER(Ctf), _c_int]),
     "fh_insert_slice_ssnr": (_c_int, [_vp, _vp, _c_int, _vp, _c_float, _vp, _c_int]),
     "fh_projection": (_c_int, [_vp, _vp, _c_int, _vp]),
+    "fh_preprocess_slice_xform": (_c_int, [_vp, _vp, _vp, _vp]),
     "fh_insert_batch_ssnr": (_c_int, [_vp, _vp, _c_int, _c_int, _vp, _vp, _vp, _c_int, _vp,
                                       ctypes.POINTER(Ctf), _c_int]),
     "fh_prep_batch": (_c_int, [_vp, _vp, _c_int, _c_int, _vp, ctypes.POINTER(Ctf), _vp, _c_int]),

@@ -848,23 +848,29 @@ static int check_xform_ctf(fh_reconstructor *h, const std::vector<float> &xf6, c
     return FH_OK;
 }
 
-int fh_preprocess_slice(fh_reconstructor *h, const float *img, float *slice_out)
+int fh_preprocess_slice_xform(fh_reconstructor *h, const float *img, const float xform[12], float *slice_out)
 {
     if (!h) return FH_ERR_NULL;
     if (!img || !slice_out) return fail(h, FH_ERR_NULL, "NullPointerException: image is NULL");
     std::lock_guard<std::mutex> lk(h->mu);
     FH_TRY(h, hipSetDevice(h->device));
     const int P = h->P, H = h->H;
-    int rc = ensure_chunk(h, std::max(1, h->cap_n), std::max(P, h->cap_B));
+    std::vector<float> rot9, xf6;
+    int rc;
+    if (xform && (rc = check_xforms(h, xform, 1, 1, rot9, &xf6))) return rc;
+    rc = ensure_chunk(h, std::max(1, h->cap_n), std::max(P, h->cap_B));
     if (rc) return rc;
     FH_TRY(h, hipMemcpyAsync(h->imgs, img, sizeof(float) * P * P, hipMemcpyHostToDevice, h->stream));
-    // preprocess_slice has no edge-mean step: subtract a zero mean
-    FH_TRY(h, launch_fill_f32(h->means, 0.f, 1, h->stream));
-    FH_TRY(h, launch_prep_rows(h->imgs, 1, P, P, h->means, h->slices, h->plan, h->tw, h->stream));
-    FH_TRY(h, launch_prep_cols(1, P, P, h->slices, nullptr, CtfConst{}, h->scale, h->plan, h->tw, h->stream));
+    // preprocess_slice has no edge-mean step
+    if ((rc = prep_single_raw(h, xf6.empty() ? nullptr : xf6.data()))) return rc;
     FH_TRY(h, hipMemcpyAsync(slice_out, h->slices, sizeof(float2) * P * (H + 1), hipMemcpyDeviceToHost, h->stream));
     FH_TRY(h, hipStreamSynchronize(h->stream));
     return FH_OK;
+}
+
+int fh_preprocess_slice(fh_reconstructor *h, const float *img, float *slice_out)
+{
+    return fh_preprocess_slice_xform(h, img, nullptr, slice_out);
 }
 
 // plain insertion of one prepared slice (h->slices + idx) with rotation rot9

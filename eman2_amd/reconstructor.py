@@ -224,10 +224,12 @@ class FourierHipReconstructor:
         P = self.P
         if a.shape != (P, P):
             raise _lib.FourierHipError(3, "image must be %dx%d" % (P, P))
-        if xform is not None and not (xform if isinstance(xform, Transform) else Transform(xform)).is_pure_rotation():
-            raise _lib.FourierHipError(1, "translation/scale/mirror in preprocess_slice is not supported")
         out = np.zeros((P, P + 2), np.float32)
-        self._check(self._lib.fh_preprocess_slice(self._need(), _ptr(a), _ptr(out)))
+        if xform is None:
+            self._check(self._lib.fh_preprocess_slice(self._need(), _ptr(a), _ptr(out)))
+        else:  # the Transform's 2-D part (translation / scale / mirror) resamples the image first
+            m = _xform12(xform)
+            self._check(self._lib.fh_preprocess_slice_xform(self._need(), _ptr(a), _ptr(m), _ptr(out)))
         e = EMData(out, complex_=True, attrs={"reconstruct_preproc": 1})
         if isinstance(img, EMData):
             for k, v in img.attr.items():

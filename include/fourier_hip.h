@@ -112,6 +112,10 @@ int fh_clear(fh_reconstructor *h);
 /* FourierReconstructor::preprocess_slice (reconstructor.cpp:1022-1061) for
  * a pure rotation: host P x P image -> host slice. */
 int fh_preprocess_slice(fh_reconstructor *h, const float *img, float *slice_out);
+/* The same with the Transform preprocess_slice receives: its 2-D part
+ * (translation, scale, x mirror; reconstructor.cpp:1031-1035) resamples the
+ * image first ("xform" processor, processor.cpp:11025-11075). */
+int fh_preprocess_slice_xform(fh_reconstructor *h, const float *img, const float xform[12], float *slice_out);
 
 /* FourierReconstructor::insert_slice (reconstructor.cpp:1064-1119).
  * data is a host P x P image (is_preprocessed = 0) or a host slice

@@ -53,6 +53,15 @@ def test_preprocess_slice(oracle, P):
     g = r.preprocess_slice(img).numpy()
     o = oracle.preprocess_slice(img)
     assert max_rel_err(g, o) <= 2e-5
+    # with a Transform whose 2-D part is a translation, mirror and scale
+    t = euler_eman_matrix(30.0, 40.0, 50.0).astype(np.float32)
+    t[0, :] *= -1
+    t[:, :3] *= np.float32(1.1)
+    t[:2, 3] = (2.5, -1.25)
+    g = r.preprocess_slice(img, t).numpy()
+    tr, moved = oracle.xform2d(img, t)
+    assert moved
+    assert max_rel_err(g, oracle.preprocess_slice(tr)) <= 2e-5
 
 
 def test_prep_ctf_sign_many_particles(oracle):
