@@ -491,8 +491,10 @@ int fh_create(const fh_params *p, fh_reconstructor **out)
     int imode;
     if (mode == "gauss_2") imode = 0;
     else if (mode == "trilinear") imode = 1;
+    else if (mode == "gauss_var") imode = 2;
     else {
-        g_create_error = "NotExistingObjectException: inserter mode '" + mode + "' (fourier_hip supports gauss_2, trilinear)";
+        g_create_error = "NotExistingObjectException: inserter mode '" + mode +
+                         "' (fourier_hip supports gauss_2, trilinear, gauss_var)";
         return FH_ERR_NOT_EXISTING;
     }
     if (p->usessnr && p->corners) {
@@ -531,8 +533,8 @@ int fh_create(const fh_params *p, fh_reconstructor **out)
     for (auto &ev : h->ev) (void)hipEventCreate(&ev);
     // constant tables: fast_exp (util.cpp:792-794), fl(i/P), twiddles, syms
     const int P = h->P, H = h->H;
-    std::vector<float> et(400), rt(P + 2);
-    for (int i = 0; i < 400; i++) et[i] = (float)exp(-i / 50.0);
+    std::vector<float> et(1000), rt(P + 2);
+    for (int i = 0; i < 1000; i++) et[i] = (float)exp(-i / 50.0);
     for (int i = 0; i <= P + 1; i++) rt[i] = (float)(i - H) / (float)P;  // fl(i/P), i in [-H, H+1]
     std::vector<float2> tw(P);
     for (int m = 0; m < P; m++) {
@@ -541,11 +543,11 @@ int fh_create(const fh_params *p, fh_reconstructor **out)
     }
     make_plan(P, &h->plan);
     h->scale = (float)sqrt((double)(1.0f / (float)P * (float)(P + 2)));
-    if ((e = hipMalloc(&h->exptab, sizeof(float) * 400)) != hipSuccess ||
+    if ((e = hipMalloc(&h->exptab, sizeof(float) * 1000)) != hipSuccess ||
         (e = hipMalloc(&h->rtab, sizeof(float) * (P + 2))) != hipSuccess ||
         (e = hipMalloc(&h->tw, sizeof(float2) * P)) != hipSuccess ||
         (e = hipMalloc(&h->syms_d, sizeof(float) * ops.size())) != hipSuccess ||
-        (e = hipMemcpy(h->exptab, et.data(), sizeof(float) * 400, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(h->exptab, et.data(), sizeof(float) * 1000, hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(h->rtab, rt.data(), sizeof(float) * (P + 2), hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(h->tw, tw.data(), sizeof(float2) * P, hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(h->syms_d, ops.data(), sizeof(float) * ops.size(), hipMemcpyHostToDevice)) != hipSuccess) {
@@ -1002,7 +1004,7 @@ static int insert_batch_impl(fh_reconstructor *h, const float *imgs, int n, int 
     if (rc) return rc;
     rc = ensure_bins(h, (long long)chunk * h->nsym);
     if (rc) return rc;
-    if (h->tune.graph && where == FH_DEVICE)
+    if (h->tune.graph && where == FH_DEVICE && h->mode != 2)
         return insert_batch_graph(h, imgs, n, box, rot9, weights, defocus, ctf, chunk, rwh, xfh);
     const InsertConst c = iconst(h);
     fh_stats st{};
@@ -1029,6 +1031,23 @@ static int insert_batch_impl(fh_reconstructor *h, const float *imgs, int n, int 
                                      hipMemcpyHostToDevice, h->stream));
         FH_TRY(h, launch_compose(h->rots, h->wts, m, h->syms_d, h->nsym, h->t3d, h->wq, h->stream));
         FH_TRY(h, hipEventRecord(h->ev[1], h->stream));
+        if (h->mode == 2) {
+            // gauss_var's 6x6x6 stencil: every slice through the global-atomic form
+            FH_TRY(h, launch_insert_plain_batch(h->slices, m * h->nsym, h->nsym, h->t3d, h->wq, c, h->data, h->norm,
+                                                h->exptab, h->rtab, rwh ? h->rw : nullptr, h->stream));
+            FH_TRY(h, hipEventRecord(h->ev[5], h->stream));
+            FH_TRY(h, hipEventSynchronize(h->ev[5]));
+            FH_TRY(h, hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
+            st.prep_ms += ms;
+            FH_TRY(h, hipEventElapsedTime(&ms, h->ev[1], h->ev[5]));
+            st.insert_kernel_ms += ms;
+            st.insert_ms += ms;
+            FH_TRY(h, hipEventElapsedTime(&ms, h->ev[0], h->ev[5]));
+            st.total_ms += ms;
+            st.slices += (long long)m * h->nsym;
+            st.chunks += 1;
+            continue;
+        }
         FH_TRY(h, launch_bin(h->t3d, h->wq, m * h->nsym, c, h->geom, h->bins, h->tune.slices_per_item, h->stream));
         FH_TRY(h, hipEventRecord(h->ev[2], h->stream));
         // the work item count sizes the partial-brick buffer
@@ -1269,6 +1288,10 @@ int fh_agreement_batch(fh_reconstructor *h, const float *imgs, int n, int box, c
     if (rc) return rc;
     if (!out) return fail(h, FH_ERR_NULL, "NullPointerException: output is NULL");
     FH_TRY(h, hipSetDevice(h->device));
+    if (h->mode == 2)
+        return fail(h, FH_ERR_INVALID_PARAM,
+                    "InvalidParameterException: the batched agreement gathers 2x2x2 stencils; use "
+                    "determine_slice_agreement for gauss_var");
     std::vector<float> rot9, xf6;
     if ((rc = check_xforms(h, xforms, n, 1, rot9, &xf6))) return rc;
     if ((rc = check_xform_ctf(h, xf6, defocus, ctf))) return rc;

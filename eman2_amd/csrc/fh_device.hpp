@@ -1,6 +1,7 @@
 // Device helpers shared by the fourier_hip kernels (gfx950, wave64).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <float.h>
 #include <stdint.h>
 
 #include "fh_launch.hpp"

@@ -30,7 +30,7 @@ struct InsertConst {
     int H;       // P/2
     float Pf;    // (float)P
     float h;     // 1.0f / EMConsts::I2G
-    int mode;    // 0 gauss_2, 1 trilinear
+    int mode;    // 0 gauss_2, 1 trilinear, 2 gauss_var
     int corners;
 };
 
@@ -115,6 +115,10 @@ hipError_t launch_project(const float2 *data, const float *norm, InsertConst c, 
                           float xshift, float yshift, int center, float2 *out, hipStream_t s);
 // 2-D do_ift of one complex image in place (data is overwritten) into P x P floats
 hipError_t launch_ift2d(float2 *data, int P, float *out, const FftPlan &plan, const float2 *tw, hipStream_t s);
+// every slice of a batch through the plain global-atomic insertion (any mode)
+hipError_t launch_insert_plain_batch(const float2 *slices, int nslices, int nsym, const float *t3d, const float *wq,
+                                     InsertConst c, float2 *data, float *norm, const float *exptab, const float *rtab,
+                                     const float *rw, hipStream_t s);
 hipError_t launch_compose(const float *rots, const float *weights, int n, const float *syms, int nsym,
                           float *t3d, float *wq, hipStream_t s);
 // rw: optional radial weights ((P/2+1) floats, class_ssnr weighting) replacing weight

@@ -195,6 +195,35 @@ __device__ __forceinline__ void add_complex_atomic(float2 *data, float *norm, in
     gadd(&norm[i], gg);
 }
 
+// ------------------------------------------------------------ gauss_var
+// FourierInserter3DMode6::insert_pixel (libEM/reconstructor_tools.cpp:382-433):
+// a 6x6x6 stencil from floor(p - 2.5), clamped to [-P/2, P/2] on each axis,
+// weight fast_exp(-r h) w with h = 1 / (|p|^2 / 4000 + 0.15), at least 0.1;
+// Util::fast_exp's full 1000-entry table, exp() past it (util.cpp:788-800).
+__device__ void insert_gauss_var(float2 dt, float xx, float yy, float zz, float w, int P, int H,
+                                 const float *__restrict__ etab, float2 *data, float *norm)
+{
+    int x0 = (int)floor((double)xx - 2.5), y0 = (int)floor((double)yy - 2.5), z0 = (int)floor((double)zz - 2.5);
+    if (x0 < -H - 4 || y0 < -H - 4 || z0 < -H - 4 || x0 > H + 3 || y0 > H + 3 || z0 > H + 3) return;
+    const int x1 = min(x0 + 5, H), y1 = min(y0 + 5, H), z1 = min(z0 + 5, H);
+    x0 = max(x0, -H);
+    y0 = max(y0, -H);
+    z0 = max(z0, -H);
+    float h = (float)(1.0f / ((double)((xx * xx + yy * yy + zz * zz) / 4000) + .15));
+    h = h < 0.1 ? (float)0.1 : h;
+    for (int k = z0; k <= z1; k++)
+        for (int j = y0; j <= y1; j++)
+            for (int i = x0; i <= x1; i++) {
+                const float dx = (float)i - xx, dy = (float)j - yy, dz = (float)k - zz;
+                const float f = -(dx * dx + dy * dy + dz * dz) * h;
+                const float gg = (f > 0 || f < -19.98) ? expf(f) : etab[(int)(-(double)f * 50.0 + 0.5)];
+                // subnormal weights are dropped (flush-to-zero: the reference's -ffast-math
+                // build and the float atomics do not keep them)
+                if (fabsf(gg * w) < FLT_MIN) continue;
+                add_complex_atomic(data, norm, P, H, i, j, k, dt.x * gg * w, dt.y * gg * w, gg * w);
+            }
+}
+
 // ------------------------------------------------------------ plain form
 // Util::hypot_fast_int(x, y) (libEM/util.cpp:764-785): round(hypot(x, y)),
 // exact for 0 <= x^2 + y^2 < 2^24 (sqrt of an integer is never a half-integer).
@@ -222,6 +251,10 @@ __global__ void __launch_bounds__(256) k_insert_plain(const float2 *__restrict__
     if (rw) weight = rw[hypot_int(x, y)];
     float xx, yy, zz;
     pixel_coords(x, y, m.m, rtab_g, H, c.Pf, xx, yy, zz);
+    if (c.mode == 2) {
+        insert_gauss_var(slice_value(S, x, y, P, H), xx, yy, zz, weight, P, H, exptab, data, norm);
+        return;
+    }
     const int x0 = (int)floorf(xx), y0 = (int)floorf(yy), z0 = (int)floorf(zz);
     if (x0 < -H - 1 || y0 < -H - 1 || z0 < -H - 1 || x0 > H || y0 > H || z0 > H) return;
     const float2 dt = slice_value(S, x, y, P, H);
@@ -231,6 +264,57 @@ __global__ void __launch_bounds__(256) k_insert_plain(const float2 *__restrict__
                 const float gg = corner_weight(c.mode, i, j, k, xx, yy, zz, c.h, weight, exptab);
                 add_complex_atomic(data, norm, P, H, i, j, k, dt.x * gg, dt.y * gg, gg);
             }
+}
+
+// The plain (global-atomic) form over every slice of a batch in one launch:
+// block (xb, q) covers pixels x in [64 xb, 64 xb + 64) of rows y of slice q
+// = p nsym + s, with t3d[q] and weight wq[q] (or the particle's class_ssnr
+// radial weights).  Used for the inserter modes the brick scatter does not
+// take (gauss_var's 6x6x6 stencil).
+__global__ void __launch_bounds__(64) k_insert_plain_batch(const float2 *__restrict__ slices, int nsym, int nxb,
+                                                           const float *__restrict__ t3d, const float *__restrict__ wq,
+                                                           InsertConst c, float2 *data, float *norm,
+                                                           const float *__restrict__ exptab,
+                                                           const float *__restrict__ rtab_g,
+                                                           const float *__restrict__ rw)
+{
+    const int H = c.H, P = c.P;
+    const int q = blockIdx.x / nxb;
+    const int x = (blockIdx.x - q * nxb) * 64 + threadIdx.x;
+    const int y = (int)blockIdx.y - H;
+    const float wgt = wq[q];
+    if (x > H || wgt == 0.0f) return;
+    if (!c.corners && x * x + y * y > H * H + H) return;  // hypot_fast_int(x,y) > P/2
+    const int p = q / nsym;
+    const float weight = rw ? rw[(size_t)p * (H + 1) + hypot_int(x, y)] : wgt;
+    const float2 *S = slices + (size_t)p * P * (H + 1);
+    const float *m = t3d + 9 * (size_t)q;
+    float xx, yy, zz;
+    pixel_coords(x, y, m, rtab_g, H, c.Pf, xx, yy, zz);
+    const float2 dt = slice_value(S, x, y, P, H);
+    if (c.mode == 2) {
+        insert_gauss_var(dt, xx, yy, zz, weight, P, H, exptab, data, norm);
+        return;
+    }
+    const int x0 = (int)floorf(xx), y0 = (int)floorf(yy), z0 = (int)floorf(zz);
+    if (x0 < -H - 1 || y0 < -H - 1 || z0 < -H - 1 || x0 > H || y0 > H || z0 > H) return;
+    for (int k = z0; k <= z0 + 1; k++)
+        for (int j = y0; j <= y0 + 1; j++)
+            for (int i = x0; i <= x0 + 1; i++) {
+                const float gg = corner_weight(c.mode, i, j, k, xx, yy, zz, c.h, weight, exptab);
+                add_complex_atomic(data, norm, P, H, i, j, k, dt.x * gg, dt.y * gg, gg);
+            }
+}
+
+hipError_t launch_insert_plain_batch(const float2 *slices, int nslices, int nsym, const float *t3d, const float *wq,
+                                     InsertConst c, float2 *data, float *norm, const float *exptab, const float *rtab,
+                                     const float *rw, hipStream_t s)
+{
+    if (nslices <= 0) return hipSuccess;
+    const int nxb = (c.H + 1 + 63) / 64;
+    hipLaunchKernelGGL(k_insert_plain_batch, dim3(nxb * nslices, c.P), dim3(64), 0, s, slices, nsym, nxb, t3d, wq, c,
+                       data, norm, exptab, rtab, rw);
+    return hipGetLastError();
 }
 
 hipError_t launch_insert_plain(const float2 *slice, InsertConst c, const float *t3d9, float weight, float2 *data,

@@ -31,6 +31,7 @@
  */
 #include "fourier_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -873,6 +874,38 @@ static inline void slice_get(const float *s, int P, int x, int y, float *re, flo
     {                                                                                              \
         const int nx2 = P / 2, ny2 = P / 2, nz2 = P / 2;                                           \
         const size_t nxyz = (size_t)(P + 2) * P * P;                                               \
+        if (mode == OR_MODE_GAUSSVAR) {                                                            \
+            /* FourierInserter3DMode6 (gauss_var, reconstructor_tools.cpp:382-433): a 6x6x6      \
+             * stencil from floor(p - 2.5), clamped to [-P/2, P/2], Gaussian width               \
+             * h = 1 / (|p|^2 / 4000 + 0.15) >= 0.1, weight gg * w */                           \
+            int a0 = (int)floor((double)xx - 2.5), b0 = (int)floor((double)yy - 2.5);             \
+            int c0 = (int)floor((double)zz - 2.5);                                                \
+            if (a0 < -nx2 - 4 || b0 < -ny2 - 4 || c0 < -nz2 - 4 || a0 > nx2 + 3 || b0 > ny2 + 3 || \
+                c0 > nz2 + 3)                                                                      \
+                return;                                                                            \
+            int a1 = a0 + 5, b1 = b0 + 5, c1 = c0 + 5;                                             \
+            if (a0 < -nx2) a0 = -nx2;                                                              \
+            if (a1 > nx2) a1 = nx2;                                                                \
+            if (b0 < -ny2) b0 = -ny2;                                                              \
+            if (b1 > ny2) b1 = ny2;                                                                \
+            if (c0 < -nz2) c0 = -nz2;                                                              \
+            if (c1 > nz2) c1 = nz2;                                                                \
+            float hv = (float)(1.0f / ((double)(hypot3sq(xx, yy, zz) / 4000) + .15));             \
+            hv = hv < 0.1 ? (float)0.1 : hv;                                                       \
+            for (int k = c0; k <= c1; k++)                                                         \
+                for (int j = b0; j <= b1; j++)                                                     \
+                    for (int i = a0; i <= a1; i++) {                                               \
+                        float r = hypot3sq((float)i - xx, (float)j - yy, (float)k - zz);           \
+                        float gg = or_fast_exp(-r * hv);                                           \
+                        /* subnormal weights are dropped (flush-to-zero, as the reference's      \
+                         * -ffast-math build and the GPU's float atomics treat them) */          \
+                        if (fabsf(gg * weight) < FLT_MIN) continue;                                \
+                        size_t off = add_complex_at_fast##SUFFIX(data, P, i, j, k, dr * gg * weight, \
+                                                                 di * gg * weight);                \
+                        norm[off / 2] += (ACC)(gg * weight);                                       \
+                    }                                                                              \
+            return;                                                                                \
+        }                                                                                          \
         int x0 = (int)floorf(xx);                                                                  \
         int y0 = (int)floorf(yy);                                                                  \
         int z0 = (int)floorf(zz);                                                                  \

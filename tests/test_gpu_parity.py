@@ -160,7 +160,8 @@ def test_batch_ctf(oracle):
 
 @pytest.mark.parametrize("sym,mode", [("c4", "gauss_2"), ("d2", "trilinear"), ("d7", "gauss_2"), ("d7", "trilinear"),
                                       ("c1", "trilinear"), ("tet", "gauss_2"), ("oct", "trilinear"),
-                                      ("icos", "gauss_2"), ("h6:1:25.5:3", "trilinear")])
+                                      ("icos", "gauss_2"), ("h6:1:25.5:3", "trilinear"),
+                                      ("c1", "gauss_var"), ("d2", "gauss_var")])
 def test_batch_symmetry_modes(oracle, sym, mode):
     P, B = 48, 32
     ph = S.phantom(10, 10.0, seed=5)
@@ -321,6 +322,30 @@ def test_projection(oracle, ret_fourier, P):
         g = r.projection(t, ret_fourier).numpy()
         o = v.projection(t, ret_fourier)
         assert max_rel_err(g, o) <= (1e-6 if ret_fourier else 2e-5), (k, max_rel_err(g, o))
+
+
+def test_gauss_var_single_slice_and_agreement(oracle):
+    """gauss_var (reconstructor_tools.cpp:382-433) through insert_slice and
+    determine_slice_agreement; the batched agreement refuses it"""
+    from eman2_amd import _lib
+    from eman2_amd.reconstructor import EMData
+    P, B, n = 32, 24, 60
+    ph = S.phantom(8, 7.0, seed=71)
+    imgs, mats, _ = S.particles(ph, n, B, seed=72, snr=1.0)
+    slices = [oracle.prep_particle(im, P) for im in imgs]
+    r = recon(P, "c1", "gauss_var")
+    v = oracle.Volume(P, "c1", "gauss_var")
+    for sl, m in zip(slices, mats):
+        r.insert_slice(EMData(sl, True, {"reconstruct_preproc": 1}), m, 1.0)
+        v.insert(sl, m, 1.0)
+    gd, gn = r.get_state()
+    assert_state_close(gd, gn, v.data, v.norm)
+    got = r.determine_slice_agreement(EMData(slices[0], True, {"reconstruct_preproc": 1}), mats[0], 1.0, True)
+    ref = v.agreement(slices[0], mats[0], 1.0, sub=True, apix=1.0)
+    for k in ("reconstruct_norm", "reconstruct_absqual"):
+        assert abs(got[k] - ref[k]) <= 1e-4 * max(1.0, abs(ref[k])), (k, got[k], ref[k])
+    with pytest.raises(_lib.FourierHipError, match="gauss_var"):
+        r.agreement_batch(imgs[:2], mats[:2], np.ones(2, np.float32))
 
 
 def test_plain_matches_batch():

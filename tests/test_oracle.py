@@ -393,17 +393,29 @@ def _brute_force(slice_, P, rot, syms, weight, mode):
                 rx, ry = f(x) / f(P), f(y) / f(P)
                 c = [f(f(rx * M[0, d]) + f(ry * M[1, d])) * f(P) for d in range(3)]
                 val = np.conj(S[-y, 0]) if (x == 0 and y < 0) else S[y % P, x]
-                x0, y0, z0 = (int(math.floor(q)) for q in c)
-                if min(x0, y0, z0) < -h - 1 or max(x0, y0, z0) > h:
-                    continue
-                for k in (z0, z0 + 1):
-                    for j in (y0, y0 + 1):
-                        for i in (x0, x0 + 1):
+                if mode == "gauss_var":
+                    # 6 cells per axis from floor(p - 2.5), inside [-P/2, P/2]
+                    lo = [int(math.floor(float(q) - 2.5)) for q in c]
+                    if min(lo) < -h - 4 or max(lo) > h + 3:
+                        continue
+                    rng3 = [range(max(a, -h), min(a + 5, h) + 1) for a in lo]
+                    hv = 1.0 / (sum(float(q) ** 2 for q in c) / 4000.0 + 0.15)
+                    hv = max(hv, 0.1)
+                else:
+                    x0, y0, z0 = (int(math.floor(q)) for q in c)
+                    if min(x0, y0, z0) < -h - 1 or max(x0, y0, z0) > h:
+                        continue
+                    rng3 = [(x0, x0 + 1), (y0, y0 + 1), (z0, z0 + 1)]
+                for k in rng3[2]:
+                    for j in rng3[1]:
+                        for i in rng3[0]:
                             if abs(i) > h or abs(j) > h or abs(k) > h:
                                 continue
                             di, dj, dk = i - float(c[0]), j - float(c[1]), k - float(c[2])
                             if mode == "gauss_2":
                                 g = math.exp(-(di * di + dj * dj + dk * dk) * math.pi ** 2 / 4)
+                            elif mode == "gauss_var":
+                                g = math.exp(-(di * di + dj * dj + dk * dk) * hv)
                             else:
                                 g = (1 - abs(di)) * (1 - abs(dj)) * (1 - abs(dk))
                             g *= weight(x, y) if callable(weight) else weight
@@ -447,7 +459,8 @@ def _brute_force(slice_, P, rot, syms, weight, mode):
     return data, norm, bound
 
 
-@pytest.mark.parametrize("sym,mode", [("c1", "gauss_2"), ("d2", "gauss_2"), ("c1", "trilinear"), ("c4", "trilinear")])
+@pytest.mark.parametrize("sym,mode", [("c1", "gauss_2"), ("d2", "gauss_2"), ("c1", "trilinear"), ("c4", "trilinear"),
+                                      ("c1", "gauss_var"), ("d2", "gauss_var")])
 def test_insert_brute_force(oracle, sym, mode):
     P = 16
     rng = np.random.default_rng(8)
@@ -456,8 +469,8 @@ def test_insert_brute_force(oracle, sym, mode):
     v = oracle.Volume(P, sym, mode)
     v.insert(sl, rot, 0.8)
     data, norm, bound = _brute_force(sl, P, rot, oracle.sym_get(sym), 0.8, mode)
-    # gauss_2 uses exp(-round(50 r^2 pi^2/4)/50): <= 1.005% per term
-    rt = 0.0102 if mode == "gauss_2" else 2e-5
+    # gauss_2 / gauss_var use exp(-round(50 f)/50) from a table: <= 1.005% per term
+    rt = 2e-5 if mode == "trilinear" else 0.0102
     assert np.all(np.abs(v.norm - norm) <= rt * np.abs(norm) + 1e-6)
     assert np.all(np.abs(cplx(v.data) - data) <= rt * bound + 1e-5)
 
