@@ -492,9 +492,12 @@ int fh_create(const fh_params *p, fh_reconstructor **out)
     if (mode == "gauss_2") imode = 0;
     else if (mode == "trilinear") imode = 1;
     else if (mode == "gauss_var") imode = 2;
+    else if (mode == "gauss_5") imode = 3;
+    else if (mode == "nearest_neighbor") imode = 4;
+    else if (mode == "gauss_3") imode = 5;
     else {
         g_create_error = "NotExistingObjectException: inserter mode '" + mode +
-                         "' (fourier_hip supports gauss_2, trilinear, gauss_var)";
+                         "' (fourier_hip supports gauss_2, trilinear, gauss_var, gauss_5, gauss_3, nearest_neighbor)";
         return FH_ERR_NOT_EXISTING;
     }
     if (p->usessnr && p->corners) {
@@ -1004,7 +1007,7 @@ static int insert_batch_impl(fh_reconstructor *h, const float *imgs, int n, int 
     if (rc) return rc;
     rc = ensure_bins(h, (long long)chunk * h->nsym);
     if (rc) return rc;
-    if (h->tune.graph && where == FH_DEVICE && h->mode != 2)
+    if (h->tune.graph && where == FH_DEVICE && h->mode < 2)
         return insert_batch_graph(h, imgs, n, box, rot9, weights, defocus, ctf, chunk, rwh, xfh);
     const InsertConst c = iconst(h);
     fh_stats st{};
@@ -1031,8 +1034,8 @@ static int insert_batch_impl(fh_reconstructor *h, const float *imgs, int n, int 
                                      hipMemcpyHostToDevice, h->stream));
         FH_TRY(h, launch_compose(h->rots, h->wts, m, h->syms_d, h->nsym, h->t3d, h->wq, h->stream));
         FH_TRY(h, hipEventRecord(h->ev[1], h->stream));
-        if (h->mode == 2) {
-            // gauss_var's 6x6x6 stencil: every slice through the global-atomic form
+        if (h->mode >= 2) {
+            // wider or other stencils (gauss_var, gauss_5, gauss_3, nearest_neighbor): the global-atomic form
             FH_TRY(h, launch_insert_plain_batch(h->slices, m * h->nsym, h->nsym, h->t3d, h->wq, c, h->data, h->norm,
                                                 h->exptab, h->rtab, rwh ? h->rw : nullptr, h->stream));
             FH_TRY(h, hipEventRecord(h->ev[5], h->stream));
@@ -1288,10 +1291,10 @@ int fh_agreement_batch(fh_reconstructor *h, const float *imgs, int n, int box, c
     if (rc) return rc;
     if (!out) return fail(h, FH_ERR_NULL, "NullPointerException: output is NULL");
     FH_TRY(h, hipSetDevice(h->device));
-    if (h->mode == 2)
+    if (h->mode >= 2)
         return fail(h, FH_ERR_INVALID_PARAM,
                     "InvalidParameterException: the batched agreement gathers 2x2x2 stencils; use "
-                    "determine_slice_agreement for gauss_var");
+                    "determine_slice_agreement for " + h->mode_name);
     std::vector<float> rot9, xf6;
     if ((rc = check_xforms(h, xforms, n, 1, rot9, &xf6))) return rc;
     if ((rc = check_xform_ctf(h, xf6, defocus, ctf))) return rc;

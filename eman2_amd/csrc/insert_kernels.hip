@@ -195,32 +195,66 @@ __device__ __forceinline__ void add_complex_atomic(float2 *data, float *norm, in
     gadd(&norm[i], gg);
 }
 
-// ------------------------------------------------------------ gauss_var
-// FourierInserter3DMode6::insert_pixel (libEM/reconstructor_tools.cpp:382-433):
-// a 6x6x6 stencil from floor(p - 2.5), clamped to [-P/2, P/2] on each axis,
-// weight fast_exp(-r h) w with h = 1 / (|p|^2 / 4000 + 0.15), at least 0.1;
-// Util::fast_exp's full 1000-entry table, exp() past it (util.cpp:788-800).
-__device__ void insert_gauss_var(float2 dt, float xx, float yy, float zz, float w, int P, int H,
-                                 const float *__restrict__ etab, float2 *data, float *norm)
+// ------------------------------------------------------------ other inserter modes
+// Util::fast_exp (util.cpp:788-800) with its full 1000-entry table
+__device__ __forceinline__ float fast_exp_full(float f, const float *__restrict__ etab)
 {
-    int x0 = (int)floor((double)xx - 2.5), y0 = (int)floor((double)yy - 2.5), z0 = (int)floor((double)zz - 2.5);
-    if (x0 < -H - 4 || y0 < -H - 4 || z0 < -H - 4 || x0 > H + 3 || y0 > H + 3 || z0 > H + 3) return;
-    const int x1 = min(x0 + 5, H), y1 = min(y0 + 5, H), z1 = min(z0 + 5, H);
+    return (f > 0 || f < -19.98) ? expf(f) : etab[(int)(-(double)f * 50.0 + 0.5)];
+}
+
+// The inserters the brick scatter does not take, in the global-atomic form
+// (libEM/reconstructor_tools.cpp):
+//   mode 2 gauss_var (Mode6, :382-433): 6x6x6 stencil from floor(p - 2.5),
+//     h = 1 / (|p|^2 / 4000 + 0.15) at least 0.1, weight fast_exp(-r h) w;
+//   mode 3 gauss_5 (Mode5): the same stencil with h = 1 / I5G;
+//   mode 4 nearest_neighbor (Mode1, :113-126): the voxel floor(p + 0.5);
+//   mode 5 gauss_3 (Mode3): 3x3x3 from floor(p - 0.5), h = 32 / ((8 + |p|) I3G),
+//     weight fast_exp(-r h) w / (1 + 6 e^-h + 12 e^-2h + 8 e^-3h).
+// Stencils are clamped to [-P/2, P/2] per axis; subnormal weights are
+// dropped (flush-to-zero: the reference's -ffast-math build and the float
+// atomics do not keep them).
+__device__ void insert_wide(int mode, float2 dt, float xx, float yy, float zz, float w, int P, int H,
+                            const float *__restrict__ etab, float2 *data, float *norm)
+{
+    if (mode == 4) {
+        const int x0 = (int)floor((double)(xx + 0.5f)), y0 = (int)floor((double)(yy + 0.5f));
+        const int z0 = (int)floor((double)(zz + 0.5f));
+        add_complex_atomic(data, norm, P, H, x0, y0, z0, dt.x * w, dt.y * w, w);
+        return;
+    }
+    const double off = mode == 5 ? 0.5 : 2.5;
+    const int span = mode == 5 ? 2 : 5, lo = mode == 5 ? 2 : 4, hi = mode == 5 ? 1 : 3;
+    int x0 = (int)floor((double)xx - off), y0 = (int)floor((double)yy - off), z0 = (int)floor((double)zz - off);
+    if (x0 < -H - lo || y0 < -H - lo || z0 < -H - lo || x0 > H + hi || y0 > H + hi || z0 > H + hi) return;
+    const int x1 = min(x0 + span, H), y1 = min(y0 + span, H), z1 = min(z0 + span, H);
     x0 = max(x0, -H);
     y0 = max(y0, -H);
     z0 = max(z0, -H);
-    float h = (float)(1.0f / ((double)((xx * xx + yy * yy + zz * zz) / 4000) + .15));
-    h = h < 0.1 ? (float)0.1 : h;
+    float h, ww = w;
+    if (mode == 2) {
+        h = (float)(1.0f / ((double)((xx * xx + yy * yy + zz * zz) / 4000) + .15));
+        h = h < 0.1 ? (float)0.1 : h;
+    } else if (mode == 3) {
+        h = 1.0f / (float)(10.4 / (M_PI * M_PI));  // 1.0f / EMConsts::I5G
+    } else {
+        const float I3G = (float)(6.4 / (M_PI * M_PI));
+        h = 32.0f / ((8.0f + sqrt_rn(xx * xx + yy * yy + zz * zz)) * I3G);  // Util::hypot3
+        ww = w / (1.0f + 6.0f * fast_exp_full(-h, etab) + 12 * fast_exp_full(-h * 2.0f, etab) +
+                  8 * fast_exp_full(-h * 3.0f, etab));
+    }
     for (int k = z0; k <= z1; k++)
         for (int j = y0; j <= y1; j++)
             for (int i = x0; i <= x1; i++) {
                 const float dx = (float)i - xx, dy = (float)j - yy, dz = (float)k - zz;
-                const float f = -(dx * dx + dy * dy + dz * dz) * h;
-                const float gg = (f > 0 || f < -19.98) ? expf(f) : etab[(int)(-(double)f * 50.0 + 0.5)];
-                // subnormal weights are dropped (flush-to-zero: the reference's -ffast-math
-                // build and the float atomics do not keep them)
-                if (fabsf(gg * w) < FLT_MIN) continue;
-                add_complex_atomic(data, norm, P, H, i, j, k, dt.x * gg * w, dt.y * gg * w, gg * w);
+                const float gg = fast_exp_full(-(dx * dx + dy * dy + dz * dz) * h, etab);
+                if (mode == 5) {  // gg = fast_exp(-r h) * w; dt * gg; norm += gg
+                    const float g = gg * ww;
+                    if (fabsf(g) < FLT_MIN) continue;
+                    add_complex_atomic(data, norm, P, H, i, j, k, dt.x * g, dt.y * g, g);
+                } else {  // dt * gg * w; norm += gg * w
+                    if (fabsf(gg * ww) < FLT_MIN) continue;
+                    add_complex_atomic(data, norm, P, H, i, j, k, dt.x * gg * ww, dt.y * gg * ww, gg * ww);
+                }
             }
 }
 
@@ -251,8 +285,8 @@ __global__ void __launch_bounds__(256) k_insert_plain(const float2 *__restrict__
     if (rw) weight = rw[hypot_int(x, y)];
     float xx, yy, zz;
     pixel_coords(x, y, m.m, rtab_g, H, c.Pf, xx, yy, zz);
-    if (c.mode == 2) {
-        insert_gauss_var(slice_value(S, x, y, P, H), xx, yy, zz, weight, P, H, exptab, data, norm);
+    if (c.mode >= 2) {
+        insert_wide(c.mode, slice_value(S, x, y, P, H), xx, yy, zz, weight, P, H, exptab, data, norm);
         return;
     }
     const int x0 = (int)floorf(xx), y0 = (int)floorf(yy), z0 = (int)floorf(zz);
@@ -292,8 +326,8 @@ __global__ void __launch_bounds__(64) k_insert_plain_batch(const float2 *__restr
     float xx, yy, zz;
     pixel_coords(x, y, m, rtab_g, H, c.Pf, xx, yy, zz);
     const float2 dt = slice_value(S, x, y, P, H);
-    if (c.mode == 2) {
-        insert_gauss_var(dt, xx, yy, zz, weight, P, H, exptab, data, norm);
+    if (c.mode >= 2) {
+        insert_wide(c.mode, dt, xx, yy, zz, weight, P, H, exptab, data, norm);
         return;
     }
     const int x0 = (int)floorf(xx), y0 = (int)floorf(yy), z0 = (int)floorf(zz);

@@ -51,6 +51,8 @@ static const double OR_PI = 3.141592653589793238462643383279502884197169399;
 #define OR_RAD2DEG (180.0 / OR_PI) /* EMConsts::rad2deg, libEM/emobject.cpp:60 */
 #define OR_DEG2RAD (OR_PI / 180.0)
 #define OR_I2G ((float)(4.0 / (M_PI * M_PI)))
+#define OR_I3G ((float)(6.4 / (M_PI * M_PI))) /* EMConsts::I3G, libEM/emobject.cpp:54 */
+#define OR_I5G ((float)(10.4 / (M_PI * M_PI))) /* EMConsts::I5G, libEM/emobject.cpp:56 */
 /* Transform::ERR_LIMIT (libEM/transform.cpp:57) */
 static const float OR_ERR_LIMIT = 0.000001f;
 
@@ -874,24 +876,40 @@ static inline void slice_get(const float *s, int P, int x, int y, float *re, flo
     {                                                                                              \
         const int nx2 = P / 2, ny2 = P / 2, nz2 = P / 2;                                           \
         const size_t nxyz = (size_t)(P + 2) * P * P;                                               \
-        if (mode == OR_MODE_GAUSSVAR) {                                                            \
-            /* FourierInserter3DMode6 (gauss_var, reconstructor_tools.cpp:382-433): a 6x6x6      \
-             * stencil from floor(p - 2.5), clamped to [-P/2, P/2], Gaussian width               \
-             * h = 1 / (|p|^2 / 4000 + 0.15) >= 0.1, weight gg * w */                           \
-            int a0 = (int)floor((double)xx - 2.5), b0 = (int)floor((double)yy - 2.5);             \
-            int c0 = (int)floor((double)zz - 2.5);                                                \
-            if (a0 < -nx2 - 4 || b0 < -ny2 - 4 || c0 < -nz2 - 4 || a0 > nx2 + 3 || b0 > ny2 + 3 || \
-                c0 > nz2 + 3)                                                                      \
+        if (mode == OR_MODE_NEAREST) {                                                             \
+            /* FourierInserter3DMode1 (nearest_neighbor, reconstructor_tools.cpp:113-126) */         \
+            int a = (int)floor(xx + 0.5f), b = (int)floor(yy + 0.5f), c = (int)floor(zz + 0.5f);  \
+            size_t off = add_complex_at_fast##SUFFIX(data, P, a, b, c, dr * weight, di * weight);  \
+            if (off != nxyz) norm[off / 2] += (ACC)weight;                                         \
+            return;                                                                                \
+        }                                                                                          \
+        if (mode == OR_MODE_GAUSSVAR || mode == OR_MODE_GAUSS5 || mode == OR_MODE_GAUSS3) {        \
+            /* Mode6 gauss_var (:382-433), Mode5 gauss_5, Mode3 gauss_3 */                         \
+            const int g3 = mode == OR_MODE_GAUSS3;                                                 \
+            const double sh = g3 ? .5 : 2.5;                                                       \
+            const int span = g3 ? 2 : 5, lo = g3 ? 2 : 4, hi = g3 ? 1 : 3;                         \
+            int a0 = (int)floor(xx - sh), b0 = (int)floor(yy - sh), c0 = (int)floor(zz - sh);      \
+            if (a0 < -nx2 - lo || b0 < -ny2 - lo || c0 < -nz2 - lo || a0 > nx2 + hi ||             \
+                b0 > ny2 + hi || c0 > nz2 + hi)                                                    \
                 return;                                                                            \
-            int a1 = a0 + 5, b1 = b0 + 5, c1 = c0 + 5;                                             \
+            int a1 = a0 + span, b1 = b0 + span, c1 = c0 + span;                                    \
             if (a0 < -nx2) a0 = -nx2;                                                              \
             if (a1 > nx2) a1 = nx2;                                                                \
             if (b0 < -ny2) b0 = -ny2;                                                              \
             if (b1 > ny2) b1 = ny2;                                                                \
             if (c0 < -nz2) c0 = -nz2;                                                              \
             if (c1 > nz2) c1 = nz2;                                                                \
-            float hv = (float)(1.0f / ((double)(hypot3sq(xx, yy, zz) / 4000) + .15));             \
-            hv = hv < 0.1 ? (float)0.1 : hv;                                                       \
+            float hv, w = weight;                                                                  \
+            if (mode == OR_MODE_GAUSSVAR) {                                                        \
+                hv = (float)(1.0f / ((double)(hypot3sq(xx, yy, zz) / 4000) + .15));                \
+                hv = hv < 0.1 ? (float)0.1 : hv;                                                   \
+            } else if (mode == OR_MODE_GAUSS5) {                                                   \
+                hv = 1.0f / OR_I5G;                                                                \
+            } else {                                                                               \
+                hv = 32.0f / ((8.0f + sqrtf(hypot3sq(xx, yy, zz))) * OR_I3G);                      \
+                w = weight / (1.0f + 6.0f * or_fast_exp(-hv) + 12 * or_fast_exp(-hv * 2.0f) +      \
+                              8 * or_fast_exp(-hv * 3.0f));                                        \
+            }                                                                                      \
             for (int k = c0; k <= c1; k++)                                                         \
                 for (int j = b0; j <= b1; j++)                                                     \
                     for (int i = a0; i <= a1; i++) {                                               \
@@ -899,10 +917,17 @@ static inline void slice_get(const float *s, int P, int x, int y, float *re, flo
                         float gg = or_fast_exp(-r * hv);                                           \
                         /* subnormal weights are dropped (flush-to-zero, as the reference's      \
                          * -ffast-math build and the GPU's float atomics treat them) */          \
-                        if (fabsf(gg * weight) < FLT_MIN) continue;                                \
-                        size_t off = add_complex_at_fast##SUFFIX(data, P, i, j, k, dr * gg * weight, \
-                                                                 di * gg * weight);                \
-                        norm[off / 2] += (ACC)(gg * weight);                                       \
+                        if (g3) {                                                                  \
+                            float g = gg * w;                                                      \
+                            if (fabsf(g) < FLT_MIN) continue;                                      \
+                            size_t off = add_complex_at_fast##SUFFIX(data, P, i, j, k, dr * g, di * g); \
+                            norm[off / 2] += (ACC)g;                                               \
+                        } else {                                                                   \
+                            if (fabsf(gg * w) < FLT_MIN) continue;                                 \
+                            size_t off = add_complex_at_fast##SUFFIX(data, P, i, j, k, dr * gg * w, \
+                                                                     di * gg * w);                 \
+                            norm[off / 2] += (ACC)(gg * w);                                        \
+                        }                                                                          \
                     }                                                                              \
             return;                                                                                \
         }                                                                                          \

@@ -24,6 +24,9 @@ extern "C" {
 #define OR_MODE_GAUSS2 0
 #define OR_MODE_TRILINEAR 1
 #define OR_MODE_GAUSSVAR 2
+#define OR_MODE_GAUSS5 3
+#define OR_MODE_NEAREST 4
+#define OR_MODE_GAUSS3 5
 
 /* ---- Util helpers (libEM/util.cpp:742-800, libEM/util.h:465-471,816-819) */
 void  or_fast_exp_table(float *out1000);

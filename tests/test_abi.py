@@ -43,7 +43,7 @@ def _create(**kw):
     (dict(nx=34, ny=34, nz=34), 3, "13-smooth"),
     (dict(sym=b"x5"), 2, "symmetry"),
     (dict(sym=b"icos2"), 2, "symmetry"),
-    (dict(mode=b"gauss_5"), 2, "mode"),
+    (dict(mode=b"gridding_5"), 2, "mode"),
     (dict(usessnr=1, corners=1), 1, "usessnr"),
 ])
 def test_create_validation(kw, code, word):

@@ -161,7 +161,8 @@ def test_batch_ctf(oracle):
 @pytest.mark.parametrize("sym,mode", [("c4", "gauss_2"), ("d2", "trilinear"), ("d7", "gauss_2"), ("d7", "trilinear"),
                                       ("c1", "trilinear"), ("tet", "gauss_2"), ("oct", "trilinear"),
                                       ("icos", "gauss_2"), ("h6:1:25.5:3", "trilinear"),
-                                      ("c1", "gauss_var"), ("d2", "gauss_var")])
+                                      ("c1", "gauss_var"), ("d2", "gauss_var"), ("c1", "gauss_5"),
+                                      ("d2", "gauss_3"), ("c4", "nearest_neighbor")])
 def test_batch_symmetry_modes(oracle, sym, mode):
     P, B = 48, 32
     ph = S.phantom(10, 10.0, seed=5)

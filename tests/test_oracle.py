@@ -393,14 +393,30 @@ def _brute_force(slice_, P, rot, syms, weight, mode):
                 rx, ry = f(x) / f(P), f(y) / f(P)
                 c = [f(f(rx * M[0, d]) + f(ry * M[1, d])) * f(P) for d in range(3)]
                 val = np.conj(S[-y, 0]) if (x == 0 and y < 0) else S[y % P, x]
-                if mode == "gauss_var":
-                    # 6 cells per axis from floor(p - 2.5), inside [-P/2, P/2]
-                    lo = [int(math.floor(float(q) - 2.5)) for q in c]
-                    if min(lo) < -h - 4 or max(lo) > h + 3:
+                wk = weight(x, y) if callable(weight) else weight
+                if mode == "nearest_neighbor":
+                    i, j, k = (int(math.floor(float(q) + 0.5)) for q in c)
+                    if abs(i) > h or abs(j) > h or abs(k) > h:
                         continue
-                    rng3 = [range(max(a, -h), min(a + 5, h) + 1) for a in lo]
-                    hv = 1.0 / (sum(float(q) ** 2 for q in c) / 4000.0 + 0.15)
-                    hv = max(hv, 0.1)
+                    V[i + h, j + h, k + h] += val * wk
+                    W[i + h, j + h, k + h] += wk
+                    A[i + h, j + h, k + h] += abs(val) * abs(wk)
+                    continue
+                if mode in ("gauss_var", "gauss_5", "gauss_3"):
+                    # 6 (3) cells per axis from floor(p - 2.5) (floor(p - 0.5)), inside [-P/2, P/2]
+                    sh, span, lo_, hi_ = (0.5, 2, 2, 1) if mode == "gauss_3" else (2.5, 5, 4, 3)
+                    lo = [int(math.floor(float(q) - sh)) for q in c]
+                    if min(lo) < -h - lo_ or max(lo) > h + hi_:
+                        continue
+                    rng3 = [range(max(a, -h), min(a + span, h) + 1) for a in lo]
+                    rad2 = sum(float(q) ** 2 for q in c)
+                    if mode == "gauss_var":
+                        hv = max(1.0 / (rad2 / 4000.0 + 0.15), 0.1)
+                    elif mode == "gauss_5":
+                        hv = math.pi ** 2 / 10.4
+                    else:
+                        hv = 32.0 / ((8.0 + math.sqrt(rad2)) * 6.4 / math.pi ** 2)
+                        wk = wk / (1 + 6 * math.exp(-hv) + 12 * math.exp(-2 * hv) + 8 * math.exp(-3 * hv))
                 else:
                     x0, y0, z0 = (int(math.floor(q)) for q in c)
                     if min(x0, y0, z0) < -h - 1 or max(x0, y0, z0) > h:
@@ -414,11 +430,11 @@ def _brute_force(slice_, P, rot, syms, weight, mode):
                             di, dj, dk = i - float(c[0]), j - float(c[1]), k - float(c[2])
                             if mode == "gauss_2":
                                 g = math.exp(-(di * di + dj * dj + dk * dk) * math.pi ** 2 / 4)
-                            elif mode == "gauss_var":
+                            elif mode in ("gauss_var", "gauss_5", "gauss_3"):
                                 g = math.exp(-(di * di + dj * dj + dk * dk) * hv)
                             else:
                                 g = (1 - abs(di)) * (1 - abs(dj)) * (1 - abs(dk))
-                            g *= weight(x, y) if callable(weight) else weight
+                            g *= wk
                             V[i + h, j + h, k + h] += val * g
                             W[i + h, j + h, k + h] += g
                             A[i + h, j + h, k + h] += abs(val) * abs(g)
@@ -460,7 +476,8 @@ def _brute_force(slice_, P, rot, syms, weight, mode):
 
 
 @pytest.mark.parametrize("sym,mode", [("c1", "gauss_2"), ("d2", "gauss_2"), ("c1", "trilinear"), ("c4", "trilinear"),
-                                      ("c1", "gauss_var"), ("d2", "gauss_var")])
+                                      ("c1", "gauss_var"), ("d2", "gauss_var"), ("c1", "gauss_5"),
+                                      ("c1", "gauss_3"), ("c4", "nearest_neighbor")])
 def test_insert_brute_force(oracle, sym, mode):
     P = 16
     rng = np.random.default_rng(8)
@@ -469,8 +486,9 @@ def test_insert_brute_force(oracle, sym, mode):
     v = oracle.Volume(P, sym, mode)
     v.insert(sl, rot, 0.8)
     data, norm, bound = _brute_force(sl, P, rot, oracle.sym_get(sym), 0.8, mode)
-    # gauss_2 / gauss_var use exp(-round(50 f)/50) from a table: <= 1.005% per term
-    rt = 2e-5 if mode == "trilinear" else 0.0102
+    # the Gaussian modes use exp(-round(50 f)/50) from a table: <= 1.005% per term
+    # (gauss_3's normalisation too: 1.005% of it)
+    rt = 2e-5 if mode in ("trilinear", "nearest_neighbor") else (0.0205 if mode == "gauss_3" else 0.0102)
     assert np.all(np.abs(v.norm - norm) <= rt * np.abs(norm) + 1e-6)
     assert np.all(np.abs(cplx(v.data) - data) <= rt * bound + 1e-5)
 
