@@ -379,6 +379,42 @@ def test_insert_remove_and_zero_weight():
     assert np.abs(n).max() <= 1e-3 * np.abs(rn).max()
 
 
+def test_reference_fourier_reconstructor_case(oracle):
+    """rt/pyem/test_reconstructor.py:94-117: three 32x32 uniform-noise images
+    inserted (raw, through preprocess_slice) at (az, alt, phi) = (2.56, 1.56,
+    3.56), (3.56, 2.56, 4.56), (4.56, 3.56, 5.56), gauss_2, c1, finish(True)"""
+    n = 32
+    imgs = np.random.default_rng(94).random((3, n, n)).astype(np.float32)
+    angles = [(2.56, 1.56, 3.56), (3.56, 2.56, 4.56), (4.56, 3.56, 5.56)]
+    r = recon(n)
+    v = oracle.Volume(n, "c1", "gauss_2")
+    for im, (az, alt, phi) in zip(imgs, angles):
+        t = Transform({"type": "eman", "az": az, "alt": alt, "phi": phi})
+        assert r.insert_slice(im, t, 1.0) == 0
+        v.insert(oracle.preprocess_slice(im), t.matrix, 1.0)
+    gd, gn = r.get_state()
+    assert_state_close(gd, gn, v.data, v.norm)
+    assert_map_close(r.finish(True, return_numpy=True), v.finish(True))
+
+
+@pytest.mark.parametrize("mode", ["gauss_2", "trilinear", "gauss_var", "gauss_5", "gauss_3", "nearest_neighbor"])
+def test_insert_remove_every_mode(mode):
+    """the reference's insert/remove invariant over the inserter modes
+    (rt/pyem/test_reconstructor.py:48-88, tolerance 1e-3): a slice inserted
+    with weight 1 and again with weight -1 leaves (numerically) nothing"""
+    n = 16
+    img = np.random.default_rng(48).random((n, n)).astype(np.float32)
+    for az, alt in ((0, 0), (35, 20), (120, 75)):
+        r = recon(n, "c1", mode)
+        t = Transform({"type": "eman", "az": az, "alt": alt, "phi": 0.0})
+        r.insert_slice(img, t, 1.0)
+        d1, n1 = r.get_state()
+        r.insert_slice(img, t, -1.0)
+        d, w = r.get_state()
+        assert np.abs(n1).max() > 0
+        assert np.abs(d).max() < 1e-3 and np.abs(w).max() < 1e-3, (az, alt, np.abs(d).max(), np.abs(w).max())
+
+
 def test_corners_and_sqrtnorm(oracle):
     P = 32
     ph = S.phantom(6, 6.0, seed=9)
@@ -599,6 +635,47 @@ def test_graph_batch_matches_eager(sym, mode):
             r.set_tuning(**dict(t, chunk=64, graph=0))  # eager work between the graph runs
             r.setup()
             r.insert_batch(dimg.value, list(mats), w, defocus=dfs, ctf=ctf, box=B)
+            r.sync()
+    finally:
+        hip.hipFree(dimg)
+
+
+def test_graph_batch_with_2d_transforms_and_ssnr():
+    """graph form with the 2-D resampling stage and class_ssnr weights, against
+    the eager form on the same device particles (two graph runs, eager work
+    between them)"""
+    import ctypes
+    P, B, n = 48, 32, 160
+    ph = S.phantom(12, 10.0, seed=51)
+    imgs, mats, _ = S.particles(ph, n, B, seed=52, snr=0.5)
+    xfs = _with_2d(mats, 53, True)
+    ss = np.random.default_rng(54).uniform(0.0, 2.0, (n, 13)).astype(np.float32)
+    hip = _hip_runtime()
+    src = np.ascontiguousarray(imgs, np.float32)
+    dimg = ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(dimg), src.nbytes) == 0
+    try:
+        assert hip.hipMemcpy(dimg, src.ctypes.data, src.nbytes, 1) == 0
+        r = recon(P, "d2", "gauss_2", usessnr=True)
+        t = r.get_tuning()
+        r.set_tuning(**dict(t, chunk=64, graph=0))
+        r.setup()
+        r.insert_batch(dimg.value, list(xfs), None, box=B, class_ssnr=ss)
+        ed, en = r.get_state()
+        # the eager device run against host particles (same kernels, staged copies)
+        h = recon(P, "d2", "gauss_2", usessnr=True)
+        h.insert_batch(imgs, list(xfs), None, class_ssnr=ss)
+        hd, hn = h.get_state()
+        assert_state_close(ed, en, hd, hn, tol=1e-5)
+        for k in range(2):
+            r.set_tuning(**dict(t, chunk=64, graph=1))
+            r.setup()
+            r.insert_batch(dimg.value, list(xfs), None, box=B, class_ssnr=ss)
+            gd, gn = r.get_state()
+            assert_state_close(gd, gn, ed, en, tol=1e-5)
+            r.set_tuning(**dict(t, chunk=64, graph=0))
+            r.setup()
+            r.insert_batch(dimg.value, list(xfs), None, box=B, class_ssnr=ss)
             r.sync()
     finally:
         hip.hipFree(dimg)
