@@ -1291,10 +1291,10 @@ int fh_agreement_batch(fh_reconstructor *h, const float *imgs, int n, int box, c
     if (rc) return rc;
     if (!out) return fail(h, FH_ERR_NULL, "NullPointerException: output is NULL");
     FH_TRY(h, hipSetDevice(h->device));
-    if (h->mode >= 2)
-        return fail(h, FH_ERR_INVALID_PARAM,
-                    "InvalidParameterException: the batched agreement gathers 2x2x2 stencils; use "
-                    "determine_slice_agreement for " + h->mode_name);
+    // the own-contribution gather knows 2x2x2 stencils only: the other modes
+    // take the reference's sequence per particle (remove, compare, re-add)
+    const bool seq = h->mode >= 2;
+    if (seq && (rc = ensure_partial(h))) return rc;
     std::vector<float> rot9, xf6;
     if ((rc = check_xforms(h, xforms, n, 1, rot9, &xf6))) return rc;
     if ((rc = check_xform_ctf(h, xf6, defocus, ctf))) return rc;
@@ -1321,6 +1321,13 @@ int fh_agreement_batch(fh_reconstructor *h, const float *imgs, int n, int box, c
         }
         if ((rc = prep_chunk(h, m, box, dimgs, defocus ? defocus + c0 : nullptr, ctf, xfh ? xfh + 6 * (size_t)c0 : nullptr)))
             return rc;
+        if (seq) {
+            for (int i = 0; i < m; i++)
+                if ((rc = agree_one(h, h->slices + (size_t)i * h->P * (h->H + 1), &rot9[9 * (size_t)(c0 + i)],
+                                    weights[c0 + i], sub, apix, &res[5 * (size_t)(c0 + i)])))
+                    return rc;
+            continue;
+        }
         FH_TRY(h, hipMemcpyAsync(h->rots, &rot9[9 * (size_t)c0], sizeof(float) * 9 * m, hipMemcpyHostToDevice, h->stream));
         FH_TRY(h, hipMemcpyAsync(h->wts, weights + c0, sizeof(float) * m, hipMemcpyHostToDevice, h->stream));
         FH_TRY(h, launch_compose(h->rots, h->wts, m, h->syms_d, h->nsym, h->t3d, h->wq, h->stream));
