@@ -23,6 +23,7 @@ SHAPES = [
     ("config 3 shape, gauss_2", 192, 288, 1.0, 60, "d7", "gauss_2", True, 0.05, 60),
     ("config 3 shape, trilinear", 192, 288, 1.0, 60, "d7", "trilinear", True, 0.05, 60),
     ("config 5 shape", 400, 600, 0.8, 40, "c1", "gauss_2", True, 0.05, 100),
+    ("config 1 shape, gauss_var", 64, 96, 1.5, 300, "c1", "gauss_var", False, 0.1, 50),
 ]
 
 
