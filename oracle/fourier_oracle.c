@@ -2,8 +2,10 @@
  * fourier_oracle.c -- TEST INFRASTRUCTURE ONLY.
  *
  * A plain-C restatement of the reference's direct Fourier inversion path
- * (EMAN2 libEM "fourier" Reconstructor with the gauss_2 and trilinear
- * inserters, its particle prep and finish()).  It is the checker the GPU
+ * (EMAN2 libEM "fourier" Reconstructor with the gauss_2, trilinear,
+ * gauss_var, gauss_5, gauss_3 and nearest_neighbor inserters, class_ssnr
+ * weighting, its particle prep with preprocess_slice's 2-D transform,
+ * slice agreement, projection() and finish()).  It is the checker the GPU
  * library is compared against, and the CPU baseline bench.py times.  Only
  * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it;
  * the shipped library never links or calls it.
@@ -28,6 +30,22 @@
  *                              scatter in numpy on tiny boxes
  *   finish                   - noiseless phantom reconstruction
  *   slice agreement          - self-consistency (absqual ~ 1 for noiseless)
+ *   tet / oct / icos / h     - rotation-angle classes of A4 / S4 / A5, the
+ *                              axes each table puts on z (and y for icos's
+ *                              2-fold), helical 0, +-daz, +-2 daz steps
+ *   wide inserters, ssnr     - the brute-force scatter with exact exp and
+ *                              per-radius weights
+ *   2-D transform            - closed forms (shift, half-pixel, mirror,
+ *                              scale, identity), Fourier shift theorem
+ *   projection               - inverse of an identity insertion, real-space
+ *                              round trip, circular shift, 0.005 threshold
+ *
+ * Parity unpinned: FFTW's FP32 rounding order and -ffast-math reassociation
+ * of the reference binary; the compare loop's read past the last slice row
+ * (taken as 0); the direction of icos's 234-degree offset (source reading
+ * only); projection()'s long double pi arithmetic (restated in long double
+ * here, double on the GPU); subnormal gauss_* weights, dropped here and on
+ * the GPU (flush-to-zero, the reference's -ffast-math build).
  */
 #include "fourier_oracle.h"
 
