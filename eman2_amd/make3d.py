@@ -117,3 +117,16 @@ def reconstruct(recon, imgs, xforms, weights=None, niter=2, keep=1.0, keepsig=Fa
     info = dict(norm=norm, reconstruct_absqual=absqual, reconstruct_qual=qual, qcutoff=qcutoff,
                 threed_ptcl_idxs=included, threed_excl_ptcl_idxs=excluded, ptcl_repr=ptcl)
     return out, info
+
+
+def reconstruct_mrc(recon, stack_path, xforms, weights=None, out_path=None, **kw):
+    """reconstruct() over a particle stack in an MRC file (memory-mapped,
+    one particle per section); the map goes to out_path when given, with
+    the stack's pixel size."""
+    from . import mrc
+    stack, h = mrc.read(stack_path)
+    out, info = reconstruct(recon, stack, xforms, weights, apix=kw.pop("apix", h["apix"]), **kw)
+    if out_path is not None:
+        vol = out.numpy() if hasattr(out, "numpy") else np.asarray(out)
+        mrc.write(out_path, vol, apix=h["apix"])
+    return out, info

@@ -71,3 +71,20 @@ def test_make3d_quality_cutoff():
     # qcutoff = the int(0.8 * len) -th largest relative quality: the rest are left out
     assert len(excl) - (n - nz) == nz - int(0.8 * nz)
     assert len(info["threed_ptcl_idxs"]) + len(excl) == n
+
+
+def test_make3d_from_mrc_stack(tmp_path):
+    """the same loop fed from an MRC particle stack, the map written back"""
+    from eman2_amd import Reconstructors, mrc
+    P, imgs, mats, w = _inputs(n=120)
+    stack = tmp_path / "ptcls.mrcs"
+    mrc.write(str(stack), imgs, apix=1.0)
+    r = Reconstructors.get("fourier_hip", dict(size=(P, P, P), sym="c1", mode="gauss_2"))
+    out_path = tmp_path / "map.mrc"
+    out, info = make3d.reconstruct_mrc(r, str(stack), mats, w, out_path=str(out_path), niter=1)
+    r2 = Reconstructors.get("fourier_hip", dict(size=(P, P, P), sym="c1", mode="gauss_2"))
+    ref, _ = make3d.reconstruct(r2, imgs, mats, w, niter=1)
+    vol, h = mrc.read(str(out_path))
+    g = ref.numpy() if hasattr(ref, "numpy") else np.asarray(ref)
+    assert h["nx"] == P and h["apix"] == 1.0
+    assert max_rel_err(np.asarray(vol), g) <= 1e-4
