@@ -213,15 +213,23 @@ bool sym_ops(const std::string &name, std::vector<float> &out)
 // m11 m13.
 bool xform2d_inverse(const float *m, float inv6[6])
 {
+    // fast path: no translation beyond ERR_LIMIT and determinant 1 (a pure
+    // rotation) make tmp the identity
+    if (fabsf(m[3]) < ERR_LIMIT && fabsf(m[7]) < ERR_LIMIT && fabsf(m[11]) < ERR_LIMIT && determinant(m) == 1.0f)
+        return false;
     const float det0 = determinant(m);
     const bool mirror = det0 < 0;
     const float scale = get_scale(m);
     float t[12];
     memcpy(t, m, sizeof(t));
-    float e[9];
-    euler_eman(0.0, 0.0, 0.0, e);
+    // the eman (0, 0, 0) rotation, evaluated once (the same trig as set_rotation)
+    static const std::vector<float> e0 = [] {
+        std::vector<float> e(9);
+        euler_eman(0.0, 0.0, 0.0, e.data());
+        return e;
+    }();
     for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) t[4 * i + j] = e[3 * i + j];
+        for (int j = 0; j < 3; j++) t[4 * i + j] = e0[3 * i + j];
     if (scale != 1.0f)
         for (int i = 0; i < 3; i++)
             for (int j = 0; j < 3; j++) t[4 * i + j] *= scale;
