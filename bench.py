@@ -169,6 +169,7 @@ def refine(a):
 
     from eman2_amd import Reconstructors
     from eman2_amd import synthetic as S
+    from eman2_amd.autotune import nsym_of
 
     dev = "cuda:0"
     P, B, n = a.pad, a.box, a.particles
@@ -213,9 +214,10 @@ def refine(a):
         print("[bench] refine step %d %s" % (i, {k: round(v[-1], 1) for k, v in stages.items()}), file=sys.stderr,
               flush=True)
     value = n * a.steps / total
-    nsym = {"c": 1, "d": 2}[a.sym[0].lower()] * int(a.sym[1:])
+    nsym = nsym_of(a.sym)
     H = P // 2
-    # SURVEY-style algorithmic bytes: two insertions, the agreement pass and finish
+    # algorithmic bytes: two insertions (raw read + 12 B per gridding contribution),
+    # the agreement pass (two scatters, the compare gather) and finish (normalise, 3-D FFT, map)
     b_ins = 4 * B * B + nsym * npix(P) * 96
     b_agree = 4 * B * B + 2 * nsym * npix(P) * 96 + 12 * (H + 2) * P * nsym
     b_fin = 20 * (H + 1) * P * P + 3 * 16 * (H + 1) * P * P + 4 * P ** 3
@@ -253,10 +255,11 @@ def main():
 
     from eman2_amd import Reconstructors
     from eman2_amd import synthetic as S
+    from eman2_amd.autotune import nsym_of
 
     dev = "cuda:0"
     P, B, n = a.pad, a.box, a.particles
-    nsym = {"c": 1, "d": 2}[a.sym[0].lower()] * int(a.sym[1:])
+    nsym = nsym_of(a.sym)
     print("[bench] generating %d particles %dpx on %s" % (n, B, dev), file=sys.stderr, flush=True)
     ph = S.phantom(200, 96.0 * B / 256.0, seed=2)
     imgs, mats, dfs = S.particles_device(ph, n, B, seed=3, apix=a.apix, snr=a.snr, ctf=CTF, device=dev)
@@ -320,7 +323,7 @@ def main():
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": traffic,
             "kernel": "k_scatter_insert", "avg_launch_ms": round(avg_ms, 3),
             "algorithmic_bytes_per_launch": int(per_launch_bytes)}
-    # whole-pipeline effective bandwidth with the 4*B^2 raw read (BASELINE.md formula)
+    # whole-pipeline effective bandwidth: the scatter-model bytes plus the 4*B^2 raw read
     pipe_bytes = (4 * B * B + bytes_per_particle) * a.steps * n
     pipeline = {"effective_GBs": round(pipe_bytes / elapsed / 1e9, 1),
                 "frac": round(pipe_bytes / elapsed / 1e9 / HBM_PEAK_GBS, 4)}
