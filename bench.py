@@ -243,12 +243,13 @@ def main():
     a = parse()
     if a.workload == "refine":
         if a.gpus != 1:
-            print("bench.py: this build measures one GPU only; --gpus %d is not supported" % a.gpus, file=sys.stderr)
+            print("bench.py: measures one GPU only; --gpus %d is not supported" % a.gpus, file=sys.stderr)
             sys.exit(2)
         return refine(a)
     if a.gpus != 1:
         if int(os.environ.get("RANK", "0")) == 0:
-            print("bench.py: this build measures one GPU only (the path shards by particle; see the design notes); "
+            print("bench.py: measures one GPU only (the path shards by particle, one volume replica per GPU "
+                  "summed by an all-reduce); "
                   "--gpus %d is not supported" % a.gpus, file=sys.stderr)
         sys.exit(2)
     import torch

@@ -404,7 +404,7 @@ class FourierHipReconstructor:
 
 
 class Reconstructors:
-    """Factory<Reconstructor> (libEM/emobject.h:724-889) with this build's
+    """Factory<Reconstructor> (libEM/emobject.h:724-889) with
     "fourier_hip" registered; ``add`` mirrors Factory<T>::add<>()."""
 
     _registry = {FourierHipReconstructor.NAME: FourierHipReconstructor}

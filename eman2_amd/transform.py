@@ -47,11 +47,10 @@ class Transform:
     """Minimal libEM ``Transform`` carrying a 3x4 float32 matrix.
 
     ``Transform({"type": "eman", "az": .., "alt": .., "phi": ..})`` mirrors the
-    reference constructor (libEM/transform.cpp:138-141, :250-283).  Only the
-    rotation part is used by the reconstructor; a non-zero translation,
-    scale or mirror is rejected by ``fourier_hip`` because the reference
-    applies those to the real-space image in ``preprocess_slice``
-    (libEM/reconstructor.cpp:1026-1035), which this build does not implement.
+    reference constructor (libEM/transform.cpp:138-141, :250-283).  The
+    insertion uses the rotation part; a translation, scale or x mirror is
+    applied to the real-space image by ``preprocess_slice``
+    (libEM/reconstructor.cpp:1026-1035), as in the reference.
     """
 
     def __init__(self, params=None):
