@@ -340,11 +340,185 @@ __global__ void __launch_bounds__(64) k_insert_plain_batch(const float2 *__restr
             }
 }
 
+// The wide modes with LDS accumulation: a workgroup takes an 8 x 8 tile of
+// one slice's pixels (4 threads per pixel, 54 of the 216 stencil cells each),
+// adds every contribution into an LDS window of 17^3 unfolded cells with LDS
+// float atomics, then adds each touched cell to the volume once through the
+// add_complex_at_fast fold (global atomics): about ten times fewer global
+// atomics than one per contribution.  Each contribution's value and weight
+// are computed exactly as in insert_wide.  Across an 8 x 8 tile a pixel
+// coordinate moves by at most 7 sqrt 2 (orthonormal slice axes), so the
+// stencil ranges of a tile span at most 10 + 6 = 16 cells per axis; a cell
+// outside the window would go straight to the volume.  The window (59 KB)
+// and the pixel records stay within 64 KB of LDS per workgroup.
+constexpr int WT_TILE = 8;
+constexpr int WT_WIN = 17;
+constexpr int WT_CELLS = WT_WIN * WT_WIN * WT_WIN;
+
+struct WidePix {
+    float xx, yy, zz, dre, dim, w, h, ww;
+    int lo[3], hi[3];
+};
+
+// stencil range (clamped to [-P/2, P/2]) and width of one pixel; false if the
+// reference's range check rejects it (insert_wide's first half)
+__device__ __forceinline__ bool wide_setup(int mode, float xx, float yy, float zz, float w, int H,
+                                           const float *__restrict__ etab, WidePix &px)
+{
+    px.xx = xx;
+    px.yy = yy;
+    px.zz = zz;
+    px.w = w;
+    px.ww = w;
+    px.h = 0.f;
+    if (mode == 4) {
+        const int c0 = (int)floor((double)(xx + 0.5f)), c1 = (int)floor((double)(yy + 0.5f));
+        const int c2 = (int)floor((double)(zz + 0.5f));
+        px.lo[0] = px.hi[0] = c0;
+        px.lo[1] = px.hi[1] = c1;
+        px.lo[2] = px.hi[2] = c2;
+        return true;
+    }
+    const double off = mode == 5 ? 0.5 : 2.5;
+    const int span = mode == 5 ? 2 : 5, lo = mode == 5 ? 2 : 4, hi = mode == 5 ? 1 : 3;
+    const float pp[3] = {xx, yy, zz};
+    for (int d = 0; d < 3; d++) {
+        const int a = (int)floor((double)pp[d] - off);
+        if (a < -H - lo || a > H + hi) return false;
+        px.lo[d] = max(a, -H);
+        px.hi[d] = min(a + span, H);
+        if (px.hi[d] < px.lo[d]) return false;  // an empty loop in the reference
+    }
+    if (mode == 2) {
+        float h = (float)(1.0f / ((double)((xx * xx + yy * yy + zz * zz) / 4000) + .15));
+        px.h = h < 0.1 ? (float)0.1 : h;
+    } else if (mode == 3) {
+        px.h = 1.0f / (float)(10.4 / (M_PI * M_PI));
+    } else {
+        const float I3G = (float)(6.4 / (M_PI * M_PI));
+        px.h = 32.0f / ((8.0f + sqrt_rn(xx * xx + yy * yy + zz * zz)) * I3G);
+        px.ww = w / (1.0f + 6.0f * fast_exp_full(-px.h, etab) + 12 * fast_exp_full(-px.h * 2.0f, etab) +
+                     8 * fast_exp_full(-px.h * 3.0f, etab));
+    }
+    return true;
+}
+
+// value and weight of cell (i, j, k) for the pixel (insert_wide's loop body);
+// false when the contribution is dropped
+__device__ __forceinline__ bool wide_term(int mode, const WidePix &px, int i, int j, int k,
+                                          const float *__restrict__ etab, float &vr, float &vi, float &g)
+{
+    if (mode == 4) {
+        vr = px.dre * px.w;
+        vi = px.dim * px.w;
+        g = px.w;
+        return true;
+    }
+    const float dx = (float)i - px.xx, dy = (float)j - px.yy, dz = (float)k - px.zz;
+    const float gg = fast_exp_full(-(dx * dx + dy * dy + dz * dz) * px.h, etab);
+    if (mode == 5) {
+        g = gg * px.ww;
+        if (fabsf(g) < FLT_MIN) return false;
+        vr = px.dre * g;
+        vi = px.dim * g;
+        return true;
+    }
+    if (fabsf(gg * px.ww) < FLT_MIN) return false;
+    vr = px.dre * gg * px.ww;
+    vi = px.dim * gg * px.ww;
+    g = gg * px.ww;
+    return true;
+}
+
+__global__ void __launch_bounds__(256) k_insert_wide_tiles(const float2 *__restrict__ slices, int nsym, int ntx,
+                                                           int nty, const float *__restrict__ t3d,
+                                                           const float *__restrict__ wq, InsertConst c, float2 *data,
+                                                           float *norm, const float *__restrict__ exptab,
+                                                           const float *__restrict__ rtab_g,
+                                                           const float *__restrict__ rw)
+{
+    extern __shared__ float win[];  // re, im, weight: 3 x WT_CELLS
+    __shared__ WidePix pix[WT_TILE * WT_TILE];
+    __shared__ int okp[WT_TILE * WT_TILE];
+    __shared__ int wlo[3], nvalid;
+    const int H = c.H, P = c.P;
+    const int tiles = ntx * nty;
+    const int q = blockIdx.x / tiles;
+    const int tt = blockIdx.x - q * tiles;
+    const int tx0 = (tt % ntx) * WT_TILE, ty0 = (tt / ntx) * WT_TILE - H;
+    const int p = q / nsym;
+    const float wgt = wq[q];
+    if (threadIdx.x < 3) wlo[threadIdx.x] = INT_MAX;
+    if (threadIdx.x == 0) nvalid = 0;
+    __syncthreads();
+    if (threadIdx.x < WT_TILE * WT_TILE) {
+        const int x = tx0 + (threadIdx.x % WT_TILE), y = ty0 + (threadIdx.x / WT_TILE);
+        bool ok = wgt != 0.0f && x <= H && y <= H - 1 && (c.corners || x * x + y * y <= H * H + H);
+        if (ok) {
+            const float weight = rw ? rw[(size_t)p * (H + 1) + hypot_int(x, y)] : wgt;
+            float xx, yy, zz;
+            pixel_coords(x, y, t3d + 9 * (size_t)q, rtab_g, H, c.Pf, xx, yy, zz);
+            WidePix px;
+            ok = wide_setup(c.mode, xx, yy, zz, weight, H, exptab, px);
+            if (ok) {
+                const float2 dt = slice_value(slices + (size_t)p * P * (H + 1), x, y, P, H);
+                px.dre = dt.x;
+                px.dim = dt.y;
+                pix[threadIdx.x] = px;
+                for (int d = 0; d < 3; d++) atomicMin(&wlo[d], px.lo[d]);
+                atomicAdd(&nvalid, 1);
+            }
+        }
+        okp[threadIdx.x] = ok;
+    }
+    __syncthreads();
+    if (nvalid == 0) return;  // block-uniform
+    for (int i = threadIdx.x; i < 3 * WT_CELLS; i += blockDim.x) win[i] = 0.f;
+    __syncthreads();
+    const int pi = threadIdx.x >> 2, part = threadIdx.x & 3;
+    if (okp[pi]) {
+        const WidePix px = pix[pi];
+        const int nx = px.hi[0] - px.lo[0] + 1, ny = px.hi[1] - px.lo[1] + 1;
+        const int ncell = nx * ny * (px.hi[2] - px.lo[2] + 1);
+        for (int ci = part; ci < ncell; ci += 4) {
+            const int kk = ci / (nx * ny), rem = ci - kk * nx * ny, jj = rem / nx, ii = rem - jj * nx;
+            const int i = px.lo[0] + ii, j = px.lo[1] + jj, k = px.lo[2] + kk;
+            float vr, vi, g;
+            if (!wide_term(c.mode, px, i, j, k, exptab, vr, vi, g)) continue;
+            const int a = i - wlo[0], b = j - wlo[1], e = k - wlo[2];
+            if ((unsigned)a < (unsigned)WT_WIN && (unsigned)b < (unsigned)WT_WIN && (unsigned)e < (unsigned)WT_WIN) {
+                const int cell = (e * WT_WIN + b) * WT_WIN + a;
+                atomicAdd(&win[cell], vr);
+                atomicAdd(&win[WT_CELLS + cell], vi);
+                atomicAdd(&win[2 * WT_CELLS + cell], g);
+            } else {
+                add_complex_atomic(data, norm, P, H, i, j, k, vr, vi, g);
+            }
+        }
+    }
+    __syncthreads();
+    for (int cell = threadIdx.x; cell < WT_CELLS; cell += blockDim.x) {
+        const float g = win[2 * WT_CELLS + cell];
+        if (g == 0.f) continue;
+        const int a = cell % WT_WIN, b = (cell / WT_WIN) % WT_WIN, e = cell / (WT_WIN * WT_WIN);
+        add_complex_atomic(data, norm, P, H, wlo[0] + a, wlo[1] + b, wlo[2] + e, win[cell], win[WT_CELLS + cell], g);
+    }
+}
+
 hipError_t launch_insert_plain_batch(const float2 *slices, int nslices, int nsym, const float *t3d, const float *wq,
                                      InsertConst c, float2 *data, float *norm, const float *exptab, const float *rtab,
                                      const float *rw, hipStream_t s)
 {
     if (nslices <= 0) return hipSuccess;
+    if (c.mode >= 2) {
+        const int ntx = (c.H + 1 + WT_TILE - 1) / WT_TILE, nty = (c.P + WT_TILE - 1) / WT_TILE;
+        const long long blocks = (long long)ntx * nty * nslices;
+        if (blocks < INT32_MAX) {
+            hipLaunchKernelGGL(k_insert_wide_tiles, dim3((unsigned)blocks), dim3(256), sizeof(float) * 3 * WT_CELLS, s,
+                               slices, nsym, ntx, nty, t3d, wq, c, data, norm, exptab, rtab, rw);
+            return hipGetLastError();
+        }
+    }
     const int nxb = (c.H + 1 + 63) / 64;
     hipLaunchKernelGGL(k_insert_plain_batch, dim3(nxb * nslices, c.P), dim3(64), 0, s, slices, nsym, nxb, t3d, wq, c,
                        data, norm, exptab, rtab, rw);
