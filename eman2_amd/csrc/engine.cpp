@@ -18,6 +18,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "fh_launch.hpp"
@@ -334,6 +335,9 @@ struct fh_reconstructor {
     fh_tuning tune{2, 1024, 0, 8, 0};
     fh_stats stats{};
     hipEvent_t ev[6] = {};
+    // pinned staging for large device-to-host copies (two buffers, one event each)
+    float *pin[2] = {nullptr, nullptr};
+    hipEvent_t pin_ev[2] = {};
 };
 
 #define FH_TRY(h, expr)                                                                    \
@@ -582,6 +586,10 @@ int fh_destroy(fh_reconstructor *h)
         free_volume(h);
         if (h->ximgs) (void)hipFree(h->ximgs);
         if (h->xfd) (void)hipFree(h->xfd);
+        for (int i = 0; i < 2; i++) {
+            if (h->pin[i]) (void)hipHostFree(h->pin[i]);
+            if (h->pin_ev[i]) (void)hipEventDestroy(h->pin_ev[i]);
+        }
         if (h->partial) (void)hipFree(h->partial);
         if (h->acc8) (void)hipFree(h->acc8);
         for (void *p : {(void *)h->exptab, (void *)h->rtab, (void *)h->tw, (void *)h->syms_d})
@@ -1359,6 +1367,58 @@ int fh_agreement_batch(fh_reconstructor *h, const float *imgs, int n, int box, c
     return FH_OK;
 }
 
+// Device-to-host copy of `bytes` into caller (pageable) memory: through two
+// pinned staging buffers, the DMA of one chunk running while several host
+// threads copy the previous chunk out; small copies go directly.
+constexpr size_t PIN_CHUNK = 64ull << 20;
+static void par_memcpy(char *dst, const char *src, size_t len)
+{
+    const int nt = (int)std::min<size_t>(8, std::max<size_t>(1, len >> 23));
+    if (nt == 1) {
+        memcpy(dst, src, len);
+        return;
+    }
+    std::vector<std::thread> th;
+    const size_t per = (len + nt - 1) / nt;
+    for (int t = 0; t < nt; t++) {
+        const size_t a = t * per, b = std::min(len, a + per);
+        if (a < b) th.emplace_back([=] { memcpy(dst + a, src + a, b - a); });
+    }
+    for (auto &x : th) x.join();
+}
+
+static hipError_t d2h_staged(fh_reconstructor *h, void *dst, const void *src, size_t bytes)
+{
+    hipError_t e;
+    if (bytes < (16ull << 20)) {
+        if ((e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->stream)) != hipSuccess) return e;
+        return hipStreamSynchronize(h->stream);
+    }
+    for (int i = 0; i < 2; i++) {
+        if (!h->pin[i] && (e = hipHostMalloc((void **)&h->pin[i], PIN_CHUNK, hipHostMallocDefault)) != hipSuccess)
+            return e;
+        if (!h->pin_ev[i] && (e = hipEventCreateWithFlags(&h->pin_ev[i], hipEventDisableTiming)) != hipSuccess)
+            return e;
+    }
+    const char *s = static_cast<const char *>(src);
+    char *d = static_cast<char *>(dst);
+    const size_t nchunk = (bytes + PIN_CHUNK - 1) / PIN_CHUNK;
+    for (size_t c = 0; c <= nchunk; c++) {
+        if (c < nchunk) {  // the DMA of chunk c into buffer c % 2 (free: chunk c-2 was copied out)
+            const size_t off = c * PIN_CHUNK, len = std::min(PIN_CHUNK, bytes - off);
+            if ((e = hipMemcpyAsync(h->pin[c & 1], s + off, len, hipMemcpyDeviceToHost, h->stream)) != hipSuccess)
+                return e;
+            if ((e = hipEventRecord(h->pin_ev[c & 1], h->stream)) != hipSuccess) return e;
+        }
+        if (c > 0) {  // chunk c-1 out of its buffer while chunk c is in flight
+            const size_t p = c - 1, off = p * PIN_CHUNK, len = std::min(PIN_CHUNK, bytes - off);
+            if ((e = hipEventSynchronize(h->pin_ev[p & 1])) != hipSuccess) return e;
+            par_memcpy(d + off, reinterpret_cast<const char *>(h->pin[p & 1]), len);
+        }
+    }
+    return hipSuccess;
+}
+
 int fh_finish(fh_reconstructor *h, int doift, float *out, float *norm_out)
 {
     if (!h) return FH_ERR_NULL;
@@ -1378,15 +1438,13 @@ int fh_finish(fh_reconstructor *h, int doift, float *out, float *norm_out)
         if (e == hipSuccess) {
             const float sc = 1.0f / (float)((size_t)P * P * P);
             e = launch_c2r_x(h->data, P, real, sc, h->plan, h->tw, h->stream);
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(out, real, sizeof(float) * (size_t)P * P * P, hipMemcpyDeviceToHost, h->stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+            if (e == hipSuccess) e = d2h_staged(h, out, real, sizeof(float) * (size_t)P * P * P);
             (void)hipFree(real);
         }
     } else if (e == hipSuccess) {
-        e = hipMemcpyAsync(out, h->data, sizeof(float2) * n, hipMemcpyDeviceToHost, h->stream);
+        e = d2h_staged(h, out, h->data, sizeof(float2) * n);
     }
-    if (e == hipSuccess && norm_out) e = hipMemcpyAsync(norm_out, h->norm, sizeof(float) * n, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess && norm_out) e = d2h_staged(h, norm_out, h->norm, sizeof(float) * n);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     (void)hipFree(planes);
     if (e != hipSuccess) return fail(h, FH_ERR_DEVICE, std::string("finish: ") + hipGetErrorString(e));
@@ -1401,8 +1459,8 @@ int fh_get_state(fh_reconstructor *h, float *data, float *weights)
     if (!h->ready) return fail(h, FH_ERR_STATE, "get_state() before setup()");
     FH_TRY(h, hipSetDevice(h->device));
     const size_t n = vol_elems(h->P);
-    if (data) FH_TRY(h, hipMemcpyAsync(data, h->data, sizeof(float2) * n, hipMemcpyDeviceToHost, h->stream));
-    if (weights) FH_TRY(h, hipMemcpyAsync(weights, h->norm, sizeof(float) * n, hipMemcpyDeviceToHost, h->stream));
+    if (data) FH_TRY(h, d2h_staged(h, data, h->data, sizeof(float2) * n));
+    if (weights) FH_TRY(h, d2h_staged(h, weights, h->norm, sizeof(float) * n));
     FH_TRY(h, hipStreamSynchronize(h->stream));
     return FH_OK;
 }
