@@ -357,6 +357,91 @@ static int fail(fh_reconstructor *h, int code, const std::string &msg)
 
 static size_t vol_elems(int P) { return (size_t)P * P * (P / 2 + 1); }
 
+// Device-to-host copy of `bytes` into caller (pageable) memory: through two
+// pinned staging buffers, the DMA of one chunk running while several host
+// threads copy the previous chunk out; small copies go directly.
+constexpr size_t PIN_CHUNK = 64ull << 20;
+static void par_memcpy(char *dst, const char *src, size_t len)
+{
+    const int nt = (int)std::min<size_t>(8, std::max<size_t>(1, len >> 23));
+    if (nt == 1) {
+        memcpy(dst, src, len);
+        return;
+    }
+    std::vector<std::thread> th;
+    const size_t per = (len + nt - 1) / nt;
+    for (int t = 0; t < nt; t++) {
+        const size_t a = t * per, b = std::min(len, a + per);
+        if (a < b) th.emplace_back([=] { memcpy(dst + a, src + a, b - a); });
+    }
+    for (auto &x : th) x.join();
+}
+
+// the two staging buffers and their events (recorded once, so that a first
+// wait on them returns at once)
+static hipError_t ensure_pinned(fh_reconstructor *h)
+{
+    hipError_t e;
+    for (int i = 0; i < 2; i++) {
+        if (!h->pin[i] && (e = hipHostMalloc((void **)&h->pin[i], PIN_CHUNK, hipHostMallocDefault)) != hipSuccess)
+            return e;
+        if (!h->pin_ev[i]) {
+            if ((e = hipEventCreateWithFlags(&h->pin_ev[i], hipEventDisableTiming)) != hipSuccess) return e;
+            if ((e = hipEventRecord(h->pin_ev[i], h->stream)) != hipSuccess) return e;
+        }
+    }
+    return hipSuccess;
+}
+
+static hipError_t d2h_staged(fh_reconstructor *h, void *dst, const void *src, size_t bytes)
+{
+    hipError_t e;
+    if (bytes < (16ull << 20)) {
+        if ((e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->stream)) != hipSuccess) return e;
+        return hipStreamSynchronize(h->stream);
+    }
+    if ((e = ensure_pinned(h)) != hipSuccess) return e;
+    const char *s = static_cast<const char *>(src);
+    char *d = static_cast<char *>(dst);
+    const size_t nchunk = (bytes + PIN_CHUNK - 1) / PIN_CHUNK;
+    for (size_t c = 0; c <= nchunk; c++) {
+        if (c < nchunk) {  // the DMA of chunk c into buffer c % 2 (free: chunk c-2 was copied out)
+            const size_t off = c * PIN_CHUNK, len = std::min(PIN_CHUNK, bytes - off);
+            if ((e = hipMemcpyAsync(h->pin[c & 1], s + off, len, hipMemcpyDeviceToHost, h->stream)) != hipSuccess)
+                return e;
+            if ((e = hipEventRecord(h->pin_ev[c & 1], h->stream)) != hipSuccess) return e;
+        }
+        if (c > 0) {  // chunk c-1 out of its buffer while chunk c is in flight
+            const size_t p = c - 1, off = p * PIN_CHUNK, len = std::min(PIN_CHUNK, bytes - off);
+            if ((e = hipEventSynchronize(h->pin_ev[p & 1])) != hipSuccess) return e;
+            par_memcpy(d + off, reinterpret_cast<const char *>(h->pin[p & 1]), len);
+        }
+    }
+    return hipSuccess;
+}
+
+// Host-to-device copy of `bytes` from caller (pageable) memory, the mirror
+// image: host threads copy chunk c into a pinned buffer (once the DMA that
+// last read that buffer is done) while the DMA of chunk c-1 runs.  Queued on
+// the handle's stream, so kernels queued after it see the data.
+static hipError_t h2d_staged(fh_reconstructor *h, void *dst, const void *src, size_t bytes)
+{
+    hipError_t e;
+    if (bytes < (16ull << 20)) return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream);
+    if ((e = ensure_pinned(h)) != hipSuccess) return e;
+    const char *s = static_cast<const char *>(src);
+    char *d = static_cast<char *>(dst);
+    const size_t nchunk = (bytes + PIN_CHUNK - 1) / PIN_CHUNK;
+    for (size_t c = 0; c < nchunk; c++) {
+        const size_t off = c * PIN_CHUNK, len = std::min(PIN_CHUNK, bytes - off);
+        if ((e = hipEventSynchronize(h->pin_ev[c & 1])) != hipSuccess) return e;  // buffer free
+        par_memcpy(reinterpret_cast<char *>(h->pin[c & 1]), s + off, len);
+        if ((e = hipMemcpyAsync(d + off, h->pin[c & 1], len, hipMemcpyHostToDevice, h->stream)) != hipSuccess) return e;
+        if ((e = hipEventRecord(h->pin_ev[c & 1], h->stream)) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 static void free_chunk(fh_reconstructor *h)
 {
     for (float **p : {&h->imgs, &h->means, &h->rots, &h->wts, &h->dfs, &h->t3d, &h->wq, &h->rw})
@@ -1038,7 +1123,7 @@ static int insert_batch_impl(fh_reconstructor *h, const float *imgs, int n, int 
         if (where == FH_DEVICE) {
             dimgs = src;
         } else {
-            FH_TRY(h, hipMemcpyAsync(h->imgs, src, sizeof(float) * (size_t)m * box * box, hipMemcpyHostToDevice, h->stream));
+            FH_TRY(h, h2d_staged(h, h->imgs, src, sizeof(float) * (size_t)m * box * box));
             dimgs = h->imgs;
         }
         rc = prep_chunk(h, m, box, dimgs, defocus ? defocus + c0 : nullptr, ctf, xfh ? xfh + 6 * (size_t)c0 : nullptr);
@@ -1141,7 +1226,7 @@ int fh_prep_batch(fh_reconstructor *h, const float *imgs, int n, int box, const 
         const float *src = imgs + (size_t)c0 * box * box;
         const float *dimgs = src;
         if (where != FH_DEVICE) {
-            FH_TRY(h, hipMemcpyAsync(h->imgs, src, sizeof(float) * (size_t)m * box * box, hipMemcpyHostToDevice, h->stream));
+            FH_TRY(h, h2d_staged(h, h->imgs, src, sizeof(float) * (size_t)m * box * box));
             dimgs = h->imgs;
         }
         rc = prep_chunk(h, m, box, dimgs, defocus ? defocus + c0 : nullptr, ctf);
@@ -1332,7 +1417,7 @@ int fh_agreement_batch(fh_reconstructor *h, const float *imgs, int n, int box, c
         const float *src = imgs + (size_t)c0 * box * box;
         const float *dimgs = src;
         if (where != FH_DEVICE) {
-            FH_TRY(h, hipMemcpyAsync(h->imgs, src, sizeof(float) * (size_t)m * box * box, hipMemcpyHostToDevice, h->stream));
+            FH_TRY(h, h2d_staged(h, h->imgs, src, sizeof(float) * (size_t)m * box * box));
             dimgs = h->imgs;
         }
         if ((rc = prep_chunk(h, m, box, dimgs, defocus ? defocus + c0 : nullptr, ctf, xfh ? xfh + 6 * (size_t)c0 : nullptr)))
@@ -1365,58 +1450,6 @@ int fh_agreement_batch(fh_reconstructor *h, const float *imgs, int n, int box, c
             memcpy(out, res.data(), sizeof(float) * 5 * n);
     }
     return FH_OK;
-}
-
-// Device-to-host copy of `bytes` into caller (pageable) memory: through two
-// pinned staging buffers, the DMA of one chunk running while several host
-// threads copy the previous chunk out; small copies go directly.
-constexpr size_t PIN_CHUNK = 64ull << 20;
-static void par_memcpy(char *dst, const char *src, size_t len)
-{
-    const int nt = (int)std::min<size_t>(8, std::max<size_t>(1, len >> 23));
-    if (nt == 1) {
-        memcpy(dst, src, len);
-        return;
-    }
-    std::vector<std::thread> th;
-    const size_t per = (len + nt - 1) / nt;
-    for (int t = 0; t < nt; t++) {
-        const size_t a = t * per, b = std::min(len, a + per);
-        if (a < b) th.emplace_back([=] { memcpy(dst + a, src + a, b - a); });
-    }
-    for (auto &x : th) x.join();
-}
-
-static hipError_t d2h_staged(fh_reconstructor *h, void *dst, const void *src, size_t bytes)
-{
-    hipError_t e;
-    if (bytes < (16ull << 20)) {
-        if ((e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->stream)) != hipSuccess) return e;
-        return hipStreamSynchronize(h->stream);
-    }
-    for (int i = 0; i < 2; i++) {
-        if (!h->pin[i] && (e = hipHostMalloc((void **)&h->pin[i], PIN_CHUNK, hipHostMallocDefault)) != hipSuccess)
-            return e;
-        if (!h->pin_ev[i] && (e = hipEventCreateWithFlags(&h->pin_ev[i], hipEventDisableTiming)) != hipSuccess)
-            return e;
-    }
-    const char *s = static_cast<const char *>(src);
-    char *d = static_cast<char *>(dst);
-    const size_t nchunk = (bytes + PIN_CHUNK - 1) / PIN_CHUNK;
-    for (size_t c = 0; c <= nchunk; c++) {
-        if (c < nchunk) {  // the DMA of chunk c into buffer c % 2 (free: chunk c-2 was copied out)
-            const size_t off = c * PIN_CHUNK, len = std::min(PIN_CHUNK, bytes - off);
-            if ((e = hipMemcpyAsync(h->pin[c & 1], s + off, len, hipMemcpyDeviceToHost, h->stream)) != hipSuccess)
-                return e;
-            if ((e = hipEventRecord(h->pin_ev[c & 1], h->stream)) != hipSuccess) return e;
-        }
-        if (c > 0) {  // chunk c-1 out of its buffer while chunk c is in flight
-            const size_t p = c - 1, off = p * PIN_CHUNK, len = std::min(PIN_CHUNK, bytes - off);
-            if ((e = hipEventSynchronize(h->pin_ev[p & 1])) != hipSuccess) return e;
-            par_memcpy(d + off, reinterpret_cast<const char *>(h->pin[p & 1]), len);
-        }
-    }
-    return hipSuccess;
 }
 
 int fh_finish(fh_reconstructor *h, int doift, float *out, float *norm_out)
