@@ -592,9 +592,19 @@ int fh_create(const fh_params *p, fh_reconstructor **out)
     else if (mode == "gauss_5") imode = 3;
     else if (mode == "nearest_neighbor") imode = 4;
     else if (mode == "gauss_3") imode = 5;
+    else if (mode == "kaiser_bessel") {
+        imode = 6;
+        // its window sqrt(1 - (r / (P/2 - 1) - 1)^2) of squared distances r <= 27 is NaN for
+        // P/2 - 1 < 13.5: refused here where the reference fills the volume with NaN
+        if (p->nx < 32) {
+            g_create_error = "InvalidParameterException: kaiser_bessel needs size >= 32";
+            return FH_ERR_INVALID_PARAM;
+        }
+    }
     else {
         g_create_error = "NotExistingObjectException: inserter mode '" + mode +
-                         "' (fourier_hip supports gauss_2, trilinear, gauss_var, gauss_5, gauss_3, nearest_neighbor)";
+                         "' (fourier_hip supports gauss_2, trilinear, gauss_var, gauss_5, gauss_3, nearest_neighbor, "
+                         "kaiser_bessel)";
         return FH_ERR_NOT_EXISTING;
     }
     if (p->usessnr && p->corners) {

@@ -30,7 +30,7 @@ struct InsertConst {
     int H;       // P/2
     float Pf;    // (float)P
     float h;     // 1.0f / EMConsts::I2G
-    int mode;    // 0 gauss_2, 1 trilinear, 2 gauss_var, 3 gauss_5, 4 nearest_neighbor, 5 gauss_3
+    int mode;    // 0 gauss_2, 1 trilinear, 2 gauss_var, 3 gauss_5, 4 nearest_neighbor, 5 gauss_3, 6 kaiser_bessel
     int corners;
 };
 

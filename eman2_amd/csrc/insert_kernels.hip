@@ -202,6 +202,25 @@ __device__ __forceinline__ float fast_exp_full(float f, const float *__restrict_
     return (f > 0 || f < -19.98) ? expf(f) : etab[(int)(-(double)f * 50.0 + 0.5)];
 }
 
+// e^-|x| I0(x), the exponentially scaled modified Bessel function of order 0
+// that kaiser_bessel takes from GSL (gsl_sf_bessel_i0_scaled): the
+// polynomial approximations of Abramowitz & Stegun 9.8.1 (|x| <= 3.75,
+// |error| < 1.6e-7 of I0) and 9.8.2 (x > 3.75, |error| < 1.9e-7 of
+// sqrt(x) e^-x I0(x)), evaluated in double.
+__host__ __device__ inline double bessel_i0_scaled(double x)
+{
+    const double ax = fabs(x);
+    if (ax <= 3.75) {
+        const double t = (ax / 3.75) * (ax / 3.75);
+        const double i0 = 1.0 + t * (3.5156229 + t * (3.0899424 + t * (1.2067492 + t * (0.2659732 +
+                          t * (0.0360768 + t * 0.0045813)))));
+        return i0 * exp(-ax);
+    }
+    const double u = 3.75 / ax;
+    return (0.39894228 + u * (0.01328592 + u * (0.00225319 + u * (-0.00157565 + u * (0.00916281 +
+            u * (-0.02057706 + u * (0.02635537 + u * (-0.01647633 + u * 0.00392377)))))))) / sqrt(ax);
+}
+
 // The inserters the brick scatter does not take, in the global-atomic form
 // (libEM/reconstructor_tools.cpp):
 //   mode 2 gauss_var (Mode6, :382-433): 6x6x6 stencil from floor(p - 2.5),
@@ -214,49 +233,7 @@ __device__ __forceinline__ float fast_exp_full(float f, const float *__restrict_
 // dropped (flush-to-zero: the reference's -ffast-math build and the float
 // atomics do not keep them).
 __device__ void insert_wide(int mode, float2 dt, float xx, float yy, float zz, float w, int P, int H,
-                            const float *__restrict__ etab, float2 *data, float *norm)
-{
-    if (mode == 4) {
-        const int x0 = (int)floor((double)(xx + 0.5f)), y0 = (int)floor((double)(yy + 0.5f));
-        const int z0 = (int)floor((double)(zz + 0.5f));
-        add_complex_atomic(data, norm, P, H, x0, y0, z0, dt.x * w, dt.y * w, w);
-        return;
-    }
-    const double off = mode == 5 ? 0.5 : 2.5;
-    const int span = mode == 5 ? 2 : 5, lo = mode == 5 ? 2 : 4, hi = mode == 5 ? 1 : 3;
-    int x0 = (int)floor((double)xx - off), y0 = (int)floor((double)yy - off), z0 = (int)floor((double)zz - off);
-    if (x0 < -H - lo || y0 < -H - lo || z0 < -H - lo || x0 > H + hi || y0 > H + hi || z0 > H + hi) return;
-    const int x1 = min(x0 + span, H), y1 = min(y0 + span, H), z1 = min(z0 + span, H);
-    x0 = max(x0, -H);
-    y0 = max(y0, -H);
-    z0 = max(z0, -H);
-    float h, ww = w;
-    if (mode == 2) {
-        h = (float)(1.0f / ((double)((xx * xx + yy * yy + zz * zz) / 4000) + .15));
-        h = h < 0.1 ? (float)0.1 : h;
-    } else if (mode == 3) {
-        h = 1.0f / (float)(10.4 / (M_PI * M_PI));  // 1.0f / EMConsts::I5G
-    } else {
-        const float I3G = (float)(6.4 / (M_PI * M_PI));
-        h = 32.0f / ((8.0f + sqrt_rn(xx * xx + yy * yy + zz * zz)) * I3G);  // Util::hypot3
-        ww = w / (1.0f + 6.0f * fast_exp_full(-h, etab) + 12 * fast_exp_full(-h * 2.0f, etab) +
-                  8 * fast_exp_full(-h * 3.0f, etab));
-    }
-    for (int k = z0; k <= z1; k++)
-        for (int j = y0; j <= y1; j++)
-            for (int i = x0; i <= x1; i++) {
-                const float dx = (float)i - xx, dy = (float)j - yy, dz = (float)k - zz;
-                const float gg = fast_exp_full(-(dx * dx + dy * dy + dz * dz) * h, etab);
-                if (mode == 5) {  // gg = fast_exp(-r h) * w; dt * gg; norm += gg
-                    const float g = gg * ww;
-                    if (fabsf(g) < FLT_MIN) continue;
-                    add_complex_atomic(data, norm, P, H, i, j, k, dt.x * g, dt.y * g, g);
-                } else {  // dt * gg * w; norm += gg * w
-                    if (fabsf(gg * ww) < FLT_MIN) continue;
-                    add_complex_atomic(data, norm, P, H, i, j, k, dt.x * gg * ww, dt.y * gg * ww, gg * ww);
-                }
-            }
-}
+                            const float *__restrict__ etab, float2 *data, float *norm);
 
 // ------------------------------------------------------------ plain form
 // Util::hypot_fast_int(x, y) (libEM/util.cpp:764-785): round(hypot(x, y)),
@@ -379,17 +356,21 @@ __device__ __forceinline__ bool wide_setup(int mode, float xx, float yy, float z
         px.lo[2] = px.hi[2] = c2;
         return true;
     }
+    // gauss_3: floor(p - .5) + [0, 2]; kaiser_bessel: floor(p - 2) + [0, 4]; else floor(p - 2.5) + [0, 5]
     const double off = mode == 5 ? 0.5 : 2.5;
-    const int span = mode == 5 ? 2 : 5, lo = mode == 5 ? 2 : 4, hi = mode == 5 ? 1 : 3;
+    const int span = mode == 5 ? 2 : mode == 6 ? 4 : 5, lo = mode == 5 ? 2 : mode == 6 ? 7 : 4;
+    const int hi = mode == 5 ? 1 : mode == 6 ? 6 : 3;
     const float pp[3] = {xx, yy, zz};
     for (int d = 0; d < 3; d++) {
-        const int a = (int)floor((double)pp[d] - off);
+        const int a = mode == 6 ? (int)floor((double)(pp[d] - 2.0f)) : (int)floor((double)pp[d] - off);
         if (a < -H - lo || a > H + hi) return false;
         px.lo[d] = max(a, -H);
         px.hi[d] = min(a + span, H);
         if (px.hi[d] < px.lo[d]) return false;  // an empty loop in the reference
     }
-    if (mode == 2) {
+    if (mode == 6) {
+        px.h = (float)(H - 1);  // nx2 - 1 of the kaiser_bessel radius scaling
+    } else if (mode == 2) {
         float h = (float)(1.0f / ((double)((xx * xx + yy * yy + zz * zz) / 4000) + .15));
         px.h = h < 0.1 ? (float)0.1 : h;
     } else if (mode == 3) {
@@ -415,6 +396,19 @@ __device__ __forceinline__ bool wide_term(int mode, const WidePix &px, int i, in
         return true;
     }
     const float dx = (float)i - px.xx, dy = (float)j - px.yy, dz = (float)k - px.zz;
+    if (mode == 6) {
+        // Mode9 (reconstructor_tools.cpp): kb = I0s(pi a sqrt(1 - (r / (nx2 - 1) - 1)^2)) / I0s(pi a),
+        // a = 15, r the squared distance; value dt * kb * w, weight w
+        const float r = dx * dx + dy * dy + dz * dz;
+        const float t = r / px.h - 1;
+        const float a = 15.0f;
+        const float kb = (float)(bessel_i0_scaled(M_PI * a * (double)sqrt_rn(1.0f - t * t)) /
+                                 bessel_i0_scaled(M_PI * a));
+        vr = px.dre * kb * px.w;
+        vi = px.dim * kb * px.w;
+        g = px.w;
+        return true;
+    }
     const float gg = fast_exp_full(-(dx * dx + dy * dy + dz * dz) * px.h, etab);
     if (mode == 5) {
         g = gg * px.ww;
@@ -428,6 +422,23 @@ __device__ __forceinline__ bool wide_term(int mode, const WidePix &px, int i, in
     vi = px.dim * gg * px.ww;
     g = gg * px.ww;
     return true;
+}
+
+// one pixel's stencil straight into the volume (the single-slice and
+// fallback form of the wide modes), the same terms as the LDS-window form
+__device__ void insert_wide(int mode, float2 dt, float xx, float yy, float zz, float w, int P, int H,
+                            const float *__restrict__ etab, float2 *data, float *norm)
+{
+    WidePix px;
+    if (!wide_setup(mode, xx, yy, zz, w, H, etab, px)) return;
+    px.dre = dt.x;
+    px.dim = dt.y;
+    for (int k = px.lo[2]; k <= px.hi[2]; k++)
+        for (int j = px.lo[1]; j <= px.hi[1]; j++)
+            for (int i = px.lo[0]; i <= px.hi[0]; i++) {
+                float vr, vi, g;
+                if (wide_term(mode, px, i, j, k, etab, vr, vi, g)) add_complex_atomic(data, norm, P, H, i, j, k, vr, vi, g);
+            }
 }
 
 __global__ void __launch_bounds__(256) k_insert_wide_tiles(const float2 *__restrict__ slices, int nsym, int ntx,

@@ -814,6 +814,23 @@ static void prep_particle_impl(const float *raw, int B, int P, const or_ctf *ctf
     free(img);
 }
 
+/* e^-|x| I0(x) (gsl_sf_bessel_i0_scaled, used by kaiser_bessel): Abramowitz &
+ * Stegun 9.8.1 / 9.8.2 polynomial approximations in double.  GSL is not in
+ * this image; the function is pinned against scipy.special.i0e. */
+double or_bessel_i0_scaled(double x)
+{
+    const double ax = fabs(x);
+    if (ax <= 3.75) {
+        const double t = (ax / 3.75) * (ax / 3.75);
+        const double i0 = 1.0 + t * (3.5156229 + t * (3.0899424 + t * (1.2067492 + t * (0.2659732 +
+                          t * (0.0360768 + t * 0.0045813)))));
+        return i0 * exp(-ax);
+    }
+    const double u = 3.75 / ax;
+    return (0.39894228 + u * (0.01328592 + u * (0.00225319 + u * (-0.00157565 + u * (0.00916281 +
+            u * (-0.02057706 + u * (0.02635537 + u * (-0.01647633 + u * 0.00392377)))))))) / sqrt(ax);
+}
+
 /* ======================================================================== */
 /* insertion                                                                 */
 /* ======================================================================== */
@@ -899,6 +916,35 @@ static inline void slice_get(const float *s, int P, int x, int y, float *re, flo
             int a = (int)floor(xx + 0.5f), b = (int)floor(yy + 0.5f), c = (int)floor(zz + 0.5f);  \
             size_t off = add_complex_at_fast##SUFFIX(data, P, a, b, c, dr * weight, di * weight);  \
             if (off != nxyz) norm[off / 2] += (ACC)weight;                                         \
+            return;                                                                                \
+        }                                                                                          \
+        if (mode == OR_MODE_KB) {                                                                  \
+            /* FourierInserter3DMode9 (kaiser_bessel): 5x5x5 from floor(p - 2), value          \
+             * dt kb w with kb = I0s(pi a sqrt(1 - (r / (nx2 - 1) - 1)^2)) / I0s(pi a),          \
+             * a = 15, r the squared distance; weight w */                                       \
+            int a0 = (int)floor(xx - 2), b0 = (int)floor(yy - 2), c0 = (int)floor(zz - 2);        \
+            if (a0 < -nx2 - 7 || b0 < -ny2 - 7 || c0 < -nz2 - 7 || a0 > nx2 + 6 ||                 \
+                b0 > ny2 + 6 || c0 > nz2 + 6)                                                      \
+                return;                                                                            \
+            int a1 = a0 + 4, b1 = b0 + 4, c1 = c0 + 4;                                             \
+            if (a0 < -nx2) a0 = -nx2;                                                              \
+            if (a1 > nx2) a1 = nx2;                                                                \
+            if (b0 < -ny2) b0 = -ny2;                                                              \
+            if (b1 > ny2) b1 = ny2;                                                                \
+            if (c0 < -nz2) c0 = -nz2;                                                              \
+            if (c1 > nz2) c1 = nz2;                                                                \
+            const float a = 15.0f;                                                                 \
+            for (int k = c0; k <= c1; k++)                                                         \
+                for (int j = b0; j <= b1; j++)                                                     \
+                    for (int i = a0; i <= a1; i++) {                                               \
+                        float r = hypot3sq((float)i - xx, (float)j - yy, (float)k - zz);           \
+                        float t = r / (float)(nx2 - 1) - 1;                                        \
+                        float kb = (float)(or_bessel_i0_scaled(M_PI * a * (double)sqrtf(1.0f - t * t)) / \
+                                           or_bessel_i0_scaled(M_PI * a));                         \
+                        size_t off = add_complex_at_fast##SUFFIX(data, P, i, j, k, dr * kb * weight, \
+                                                                 di * kb * weight);                \
+                        norm[off / 2] += (ACC)weight;                                              \
+                    }                                                                              \
             return;                                                                                \
         }                                                                                          \
         if (mode == OR_MODE_GAUSSVAR || mode == OR_MODE_GAUSS5 || mode == OR_MODE_GAUSS3) {        \

@@ -27,6 +27,7 @@ extern "C" {
 #define OR_MODE_GAUSS5 3
 #define OR_MODE_NEAREST 4
 #define OR_MODE_GAUSS3 5
+#define OR_MODE_KB 6
 
 /* ---- Util helpers (libEM/util.cpp:742-800, libEM/util.h:465-471,816-819) */
 void  or_fast_exp_table(float *out1000);
@@ -115,5 +116,7 @@ void or_setup_seed(int P, float *data, float *norm, const float *seed, float see
 }
 #endif
 void or_projection(int P, const float *data, const float *norm, const float euler[12], int ret_fourier, float *out);
+
+double or_bessel_i0_scaled(double x);
 
 #endif

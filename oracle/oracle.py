@@ -23,7 +23,8 @@ _LIBS = {}
 
 GAUSS2 = 0
 TRILINEAR = 1
-MODES = {"gauss_2": GAUSS2, "trilinear": TRILINEAR, "gauss_var": 2, "gauss_5": 3, "nearest_neighbor": 4, "gauss_3": 5}
+MODES = {"gauss_2": GAUSS2, "trilinear": TRILINEAR, "gauss_var": 2, "gauss_5": 3, "nearest_neighbor": 4, "gauss_3": 5,
+         "kaiser_bessel": 6}
 
 _fp = ctypes.POINTER(ctypes.c_float)
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -62,6 +63,7 @@ def lib(fast=False):
         "or_sym_get": (i, [ctypes.c_char_p, _fp]),
         "or_xform2d": (i, [_fp, i, i, _fp, _fp]),
         "or_projection": (v, [i, _fp, _fp, _fp, i, _fp]),
+        "or_bessel_i0_scaled": (d, [d]),
         "or_prep_particle_xform": (v, [_fp, i, i, ctypes.POINTER(CtfStruct), _fp, _fp]),
         "or_insert_slice_ssnr": (i, [i, _fp, _fp, _fp, _fp, _fp, i, _fp, i, i]),
         "or_edge_mean": (f, [_fp, i, i]),

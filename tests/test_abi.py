@@ -45,6 +45,7 @@ def _create(**kw):
     (dict(sym=b"icos2"), 2, "symmetry"),
     (dict(mode=b"gridding_5"), 2, "mode"),
     (dict(usessnr=1, corners=1), 1, "usessnr"),
+    (dict(nx=24, ny=24, nz=24, mode=b"kaiser_bessel"), 1, "kaiser_bessel"),
 ])
 def test_create_validation(kw, code, word):
     rc, msg = _create(**kw)

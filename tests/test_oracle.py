@@ -402,6 +402,21 @@ def _brute_force(slice_, P, rot, syms, weight, mode):
                     W[i + h, j + h, k + h] += wk
                     A[i + h, j + h, k + h] += abs(val) * abs(wk)
                     continue
+                if mode == "kaiser_bessel":
+                    import scipy.special as sp
+                    lo = [int(math.floor(float(q) - 2)) for q in c]
+                    if min(lo) < -h - 7 or max(lo) > h + 6:
+                        continue
+                    for k in range(max(lo[2], -h), min(lo[2] + 4, h) + 1):
+                        for j in range(max(lo[1], -h), min(lo[1] + 4, h) + 1):
+                            for i in range(max(lo[0], -h), min(lo[0] + 4, h) + 1):
+                                r = (i - float(c[0])) ** 2 + (j - float(c[1])) ** 2 + (k - float(c[2])) ** 2
+                                t = r / (h - 1) - 1
+                                kb = sp.i0e(math.pi * 15 * math.sqrt(1 - t * t)) / sp.i0e(math.pi * 15)
+                                V[i + h, j + h, k + h] += val * kb * wk
+                                W[i + h, j + h, k + h] += wk
+                                A[i + h, j + h, k + h] += abs(val) * abs(kb * wk)
+                    continue
                 if mode in ("gauss_var", "gauss_5", "gauss_3"):
                     # 6 (3) cells per axis from floor(p - 2.5) (floor(p - 0.5)), inside [-P/2, P/2]
                     sh, span, lo_, hi_ = (0.5, 2, 2, 1) if mode == "gauss_3" else (2.5, 5, 4, 3)
@@ -477,9 +492,10 @@ def _brute_force(slice_, P, rot, syms, weight, mode):
 
 @pytest.mark.parametrize("sym,mode", [("c1", "gauss_2"), ("d2", "gauss_2"), ("c1", "trilinear"), ("c4", "trilinear"),
                                       ("c1", "gauss_var"), ("d2", "gauss_var"), ("c1", "gauss_5"),
-                                      ("c1", "gauss_3"), ("c4", "nearest_neighbor")])
+                                      ("c1", "gauss_3"), ("c4", "nearest_neighbor"), ("d2", "kaiser_bessel")])
 def test_insert_brute_force(oracle, sym, mode):
-    P = 16
+    # kaiser_bessel's sqrt(1 - (r / (P/2 - 1) - 1)^2) needs P/2 - 1 >= 13.5 (NaN below, as in the reference)
+    P = 32 if mode == "kaiser_bessel" else 16
     rng = np.random.default_rng(8)
     sl = oracle.prep_particle(rng.standard_normal((12, 12)).astype(np.float32), P)
     rot = euler_eman_matrix(*(rng.random(3) * (360, 180, 360)))
@@ -488,7 +504,7 @@ def test_insert_brute_force(oracle, sym, mode):
     data, norm, bound = _brute_force(sl, P, rot, oracle.sym_get(sym), 0.8, mode)
     # the Gaussian modes use exp(-round(50 f)/50) from a table: <= 1.005% per term
     # (gauss_3's normalisation too: 1.005% of it)
-    rt = 2e-5 if mode in ("trilinear", "nearest_neighbor") else (0.0205 if mode == "gauss_3" else 0.0102)
+    rt = 2e-5 if mode in ("trilinear", "nearest_neighbor", "kaiser_bessel") else (0.0205 if mode == "gauss_3" else 0.0102)
     assert np.all(np.abs(v.norm - norm) <= rt * np.abs(norm) + 1e-6)
     assert np.all(np.abs(cplx(v.data) - data) <= rt * bound + 1e-5)
 
@@ -705,3 +721,12 @@ def test_projection_weight_threshold(oracle):
     assert np.all(v.projection(I, True) == 0)
     v.insert(sl, I, 0.002)
     assert np.abs(v.projection(I, True)).max() > 0
+
+
+def test_bessel_i0_scaled(oracle):
+    """kaiser_bessel's gsl_sf_bessel_i0_scaled (GSL is not in this image):
+    Abramowitz & Stegun 9.8.1/9.8.2 against scipy.special.i0e"""
+    import scipy.special as sp
+    xs = np.concatenate([np.linspace(0, 3.75, 40), np.linspace(3.7501, 60.0, 200), [47.12388980384690]])
+    for x in xs:
+        assert abs(oracle.lib().or_bessel_i0_scaled(float(x)) / sp.i0e(x) - 1) < 1e-6, x
