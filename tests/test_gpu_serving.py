@@ -95,3 +95,21 @@ def test_bad_job_fails_alone_and_cancel():
     assert st["state"] == "failed" and st["error"] == 2
     assert s.poll(queued)["state"] in ("cancelled", "done")
     s.close()
+
+
+def test_served_job_other_symmetry_and_mode():
+    """a served icos / gauss_var job is the direct API's result"""
+    from eman2_amd import Reconstructors
+    from eman2_amd.serving import Server
+    j = dict(make_jobs()[0], sym="icos", mode="gauss_var")
+    r = Reconstructors.get("fourier_hip", {"size": (j["size"],) * 3, "sym": "icos", "mode": "gauss_var"})
+    r.setup()
+    r.insert_batch(j["imgs"], j["xforms"], None, defocus=j["defocus"], ctf=CTF)
+    direct = r.finish(True, return_numpy=True)
+    s = Server(max_active=2)
+    jid = s.submit(**j)
+    s.drain()
+    assert s.poll(jid)["state"] == "done"
+    m = s.result(jid)
+    assert max_rel_err(m, direct) <= 1e-4 and fsc(m, direct).min() >= 0.9999
+    s.close()
