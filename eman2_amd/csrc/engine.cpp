@@ -990,10 +990,27 @@ int fh_preprocess_slice(fh_reconstructor *h, const float *img, float *slice_out)
 }
 
 // plain insertion of one prepared slice (h->slices + idx) with rotation rot9
+// one slice of a wide-stencil mode (gauss_var, gauss_5, gauss_3,
+// nearest_neighbor, kaiser_bessel): all its symmetry copies in one LDS-window
+// launch (t3d and weights staged in the chunk buffers, which hold >= nsym slices)
+static int wide_slice_insert(fh_reconstructor *h, const float2 *slice_d, const float *rot9, float weight,
+                             const float *rw_d, const InsertConst &c)
+{
+    std::vector<float> t(9 * (size_t)h->nsym), wv((size_t)h->nsym, weight);
+    for (int s = 0; s < h->nsym; s++) compose9(rot9, &h->syms[9 * s], &t[9 * (size_t)s]);
+    FH_TRY(h, hipMemcpyAsync(h->t3d, t.data(), sizeof(float) * t.size(), hipMemcpyHostToDevice, h->stream));
+    FH_TRY(h, hipMemcpyAsync(h->wq, wv.data(), sizeof(float) * wv.size(), hipMemcpyHostToDevice, h->stream));
+    FH_TRY(h, launch_insert_plain_batch(slice_d, h->nsym, h->nsym, h->t3d, h->wq, c, h->data, h->norm, h->exptab,
+                                        h->rtab, rw_d, h->stream));
+    FH_TRY(h, hipStreamSynchronize(h->stream));  // t and wv are read by the copies above
+    return FH_OK;
+}
+
 static int insert_slice_plain(fh_reconstructor *h, const float2 *slice_d, const float *rot9, float weight, bool sync_each,
                               const float *rw_d = nullptr)
 {
     const InsertConst c = iconst(h);
+    if (h->mode >= 2 && !sync_each) return wide_slice_insert(h, slice_d, rot9, weight, rw_d, c);
     for (int s = 0; s < h->nsym; s++) {
         float t[9];
         compose9(rot9, &h->syms[9 * s], t);
@@ -1146,7 +1163,8 @@ static int insert_batch_impl(fh_reconstructor *h, const float *imgs, int n, int 
         FH_TRY(h, launch_compose(h->rots, h->wts, m, h->syms_d, h->nsym, h->t3d, h->wq, h->stream));
         FH_TRY(h, hipEventRecord(h->ev[1], h->stream));
         if (h->mode >= 2) {
-            // wider or other stencils (gauss_var, gauss_5, gauss_3, nearest_neighbor): the global-atomic form
+            // wider or other stencils (gauss_var, gauss_5, gauss_3, nearest_neighbor, kaiser_bessel): the
+            // LDS-window tile form
             FH_TRY(h, launch_insert_plain_batch(h->slices, m * h->nsym, h->nsym, h->t3d, h->wq, c, h->data, h->norm,
                                                 h->exptab, h->rtab, rwh ? h->rw : nullptr, h->stream));
             FH_TRY(h, hipEventRecord(h->ev[5], h->stream));
@@ -1344,7 +1362,10 @@ static int agree_one(fh_reconstructor *h, const float2 *slice_d, const float *ro
     std::vector<float> t(9 * (size_t)h->nsym);
     for (int s = 0; s < h->nsym; s++) compose9(rot9, &h->syms[9 * s], &t[9 * (size_t)s]);
     FH_TRY(h, hipMemcpyAsync(h->t3d, t.data(), sizeof(float) * t.size(), hipMemcpyHostToDevice, h->stream));
-    if (sub)
+    if (sub && h->mode >= 2) {
+        int rc = wide_slice_insert(h, slice_d, rot9, -weight, nullptr, c);
+        if (rc) return rc;
+    } else if (sub)
         for (int s = 0; s < h->nsym; s++)
             FH_TRY(h, launch_insert_plain(slice_d, c, &t[9 * (size_t)s], -weight, h->data, h->norm, h->exptab, h->rtab, nullptr,
                                           h->stream));
@@ -1352,7 +1373,10 @@ static int agree_one(fh_reconstructor *h, const float2 *slice_d, const float *ro
                            h->partial, h->stream));
     std::vector<double> part((size_t)h->nsym * nb * 8);
     FH_TRY(h, hipMemcpyAsync(part.data(), h->partial, sizeof(double) * part.size(), hipMemcpyDeviceToHost, h->stream));
-    if (sub)
+    if (sub && h->mode >= 2) {
+        int rc = wide_slice_insert(h, slice_d, rot9, weight, nullptr, c);
+        if (rc) return rc;
+    } else if (sub)
         for (int s = 0; s < h->nsym; s++)
             FH_TRY(h, launch_insert_plain(slice_d, c, &t[9 * (size_t)s], weight, h->data, h->norm, h->exptab, h->rtab, nullptr,
                                           h->stream));

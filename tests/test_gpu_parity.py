@@ -176,7 +176,33 @@ def test_batch_symmetry_modes(oracle, sym, mode):
     assert_map_close(r.finish(True, return_numpy=True), v.finish(True))
 
 
-@pytest.mark.parametrize("sym,mode", [("d2", "gauss_2"), ("c1", "trilinear")])
+@pytest.mark.parametrize("sym,mode", [("d2", "gauss_3"), ("c4", "kaiser_bessel"), ("c3", "nearest_neighbor")])
+def test_wide_mode_single_slice_symmetry(oracle, sym, mode):
+    """insert_slice of a wide-stencil mode under a symmetry (every symmetry
+    copy of the slice in one launch), then determine_slice_agreement with the
+    particle's own slice removed and re-added (reconstructor.cpp:1090-1210)"""
+    from eman2_amd.reconstructor import EMData
+    P, B, n = 48, 32, 24
+    ph = S.phantom(10, 10.0, seed=11)
+    imgs, mats, _ = S.particles(ph, n, B, seed=12, snr=1.0)
+    slices = [oracle.prep_particle(im, P) for im in imgs]
+    r = recon(P, sym, mode)
+    v = oracle.Volume(P, sym, mode)
+    for sl, m in list(zip(slices, mats)) * 2:
+        assert r.insert_slice(EMData(sl, True, {"reconstruct_preproc": 1}), m, 0.75) == 0
+        v.insert(sl, m, 0.75)
+    gd, gn = r.get_state()
+    assert_state_close(gd, gn, v.data, v.norm)
+    got = r.determine_slice_agreement(EMData(slices[1], True, {"reconstruct_preproc": 1}), mats[1], 0.75, True)
+    ref = v.agreement(slices[1], mats[1], 0.75, sub=True, apix=1.0)
+    for k in ("reconstruct_norm", "reconstruct_absqual"):
+        assert abs(got[k] - ref[k]) <= 1e-4 * max(1.0, abs(ref[k])), (k, got[k], ref[k])
+    # the removed slice is back: the state is unchanged by the agreement
+    ad, an = r.get_state()
+    assert_state_close(ad, an, v.data, v.norm)
+
+
+@pytest.mark.parametrize("sym,mode", [("d2", "gauss_2"), ("c1", "trilinear"), ("d2", "gauss_var")])
 def test_usessnr_batch_and_slice(oracle, sym, mode):
     """usessnr: per-radius class_ssnr weights (reconstructor.cpp:1080-1088,
     1160-1184) in the batched scatter and the single-slice plain kernel."""
