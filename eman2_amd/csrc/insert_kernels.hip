@@ -332,6 +332,21 @@ constexpr int WT_TILE = 8;
 constexpr int WT_WIN = 17;
 constexpr int WT_CELLS = WT_WIN * WT_WIN * WT_WIN;
 
+// gridding_5 / gridding_7 kernels, 3x oversampled (reconstructor_tools.cpp:455, :817)
+#include "gridding_tables.inc"
+__constant__ float c_grid5[9 * 9 * 9] = {FH_GRIDDING5_VALUES};
+__constant__ float c_grid7[12 * 12 * 12] = {FH_GRIDDING7_VALUES};
+
+// Util::fast_floor (libEM/util.h:874-880): one less than floor at negative integers
+__device__ __forceinline__ int ref_fast_floor(float x) { return x < 0 ? (int)x - 1 : (int)x; }
+
+// table index of one axis: |fast_floor((i - p) * 3 + 0.5)|, the 0.5 added in double as in the reference
+__device__ __forceinline__ int grid_index(int i, float p, int n)
+{
+    const int t = abs(ref_fast_floor((float)((double)(((float)i - p) * 3.0f) + 0.5)));
+    return min(t, n - 1);  // <= n - 1 for every finite p (the stencil spans |i - p| <= n / 3 - 0.5)
+}
+
 struct WidePix {
     float xx, yy, zz, dre, dim, w, h, ww;
     int lo[3], hi[3];
@@ -354,6 +369,20 @@ __device__ __forceinline__ bool wide_setup(int mode, float xx, float yy, float z
         px.lo[0] = px.hi[0] = c0;
         px.lo[1] = px.hi[1] = c1;
         px.lo[2] = px.hi[2] = c2;
+        return true;
+    }
+    if (mode >= 7) {
+        // gridding_5: fast_floor(p - 1.5) + [0, 4]; gridding_7: fast_floor(p - 2.5) + [0, 6]
+        const double sh = mode == 8 ? 2.5 : 1.5;
+        const int span = mode == 8 ? 6 : 4, lo = mode == 8 ? 6 : 4, hi = mode == 8 ? 5 : 3;
+        const float pp[3] = {xx, yy, zz};
+        for (int d = 0; d < 3; d++) {
+            const int a = ref_fast_floor((float)((double)pp[d] - sh));
+            if (a < -H - lo || a > H + hi) return false;
+            px.lo[d] = max(a, -H);
+            px.hi[d] = min(a + span, H);
+            if (px.hi[d] < px.lo[d]) return false;
+        }
         return true;
     }
     // gauss_3: floor(p - .5) + [0, 2]; kaiser_bessel: floor(p - 2) + [0, 4]; else floor(p - 2.5) + [0, 5]
@@ -392,6 +421,16 @@ __device__ __forceinline__ bool wide_term(int mode, const WidePix &px, int i, in
     if (mode == 4) {
         vr = px.dre * px.w;
         vi = px.dim * px.w;
+        g = px.w;
+        return true;
+    }
+    if (mode >= 7) {
+        // Mode7 / Mode11: value dt gg w, weight w (reconstructor_tools.cpp:539-576, 964-1001)
+        const int n = mode == 8 ? 12 : 9;
+        const float *tab = mode == 8 ? c_grid7 : c_grid5;
+        const float gg = tab[(grid_index(i, px.xx, n) * n + grid_index(j, px.yy, n)) * n + grid_index(k, px.zz, n)];
+        vr = px.dre * gg * px.w;
+        vi = px.dim * gg * px.w;
         g = px.w;
         return true;
     }

@@ -3,7 +3,9 @@
  *
  * A plain-C restatement of the reference's direct Fourier inversion path
  * (EMAN2 libEM "fourier" Reconstructor with the gauss_2, trilinear,
- * gauss_var, gauss_5, gauss_3 and nearest_neighbor inserters, class_ssnr
+ * gauss_var, gauss_5, gauss_3, nearest_neighbor, kaiser_bessel, gridding_5
+ * and gridding_7 inserters (the last two with the reference's gridding
+ * kernel tables, taken over as data), class_ssnr
  * weighting, its particle prep with preprocess_slice's 2-D transform,
  * slice agreement, projection() and finish()).  It is the checker the GPU
  * library is compared against, and the CPU baseline bench.py times.  Only
@@ -35,6 +37,9 @@
  *                              2-fold), helical 0, +-daz, +-2 daz steps
  *   wide inserters, ssnr     - the brute-force scatter with exact exp and
  *                              per-radius weights
+ *   gridding_5 / gridding_7  - the brute-force scatter with the table at
+ *                              |fast_floor(3d + 0.5)|; the tables by axis-
+ *                              exchange symmetry, zero last planes, centres
  *   2-D transform            - closed forms (shift, half-pixel, mirror,
  *                              scale, identity), Fourier shift theorem
  *   projection               - inverse of an identity insertion, real-space
@@ -137,6 +142,11 @@ static inline int fast_floor(float x)
     if (x < 0) return ((int)x - 1);
     return (int)x;
 }
+
+/* the gridding_5 / gridding_7 kernels: reconstructor_tools.cpp:455, :817 (data) */
+#include "../eman2_amd/csrc/gridding_tables.inc"
+static const float or_grid5[9 * 9 * 9] = {FH_GRIDDING5_VALUES};
+static const float or_grid7[12 * 12 * 12] = {FH_GRIDDING7_VALUES};
 
 static inline void apply_precision(float *value, float precision)
 {
@@ -943,6 +953,39 @@ static inline void slice_get(const float *s, int P, int x, int y, float *re, flo
                                            or_bessel_i0_scaled(M_PI * a));                         \
                         size_t off = add_complex_at_fast##SUFFIX(data, P, i, j, k, dr * kb * weight, \
                                                                  di * kb * weight);                \
+                        norm[off / 2] += (ACC)weight;                                              \
+                    }                                                                              \
+            return;                                                                                \
+        }                                                                                          \
+        if (mode == OR_MODE_GRID5 || mode == OR_MODE_GRID7) {                                      \
+            /* FourierInserter3DMode7 gridding_5 (:539-576) and Mode11 gridding_7 (:964-1001):       \
+             * 5 or 7 cells from fast_floor(p - 1.5 or 2.5), value dt gg w with gg the 3x          \
+             * oversampled table at |fast_floor(3 d + 0.5)| per axis, weight w */                  \
+            const int g7 = mode == OR_MODE_GRID7;                                                  \
+            const double sh = g7 ? 2.5 : 1.5;                                                      \
+            const int span = g7 ? 6 : 4, lo = g7 ? 6 : 4, hi = g7 ? 5 : 3, tn = g7 ? 12 : 9;       \
+            const float *tab = g7 ? or_grid7 : or_grid5;                                           \
+            int a0 = fast_floor((float)(xx - sh)), b0 = fast_floor((float)(yy - sh));              \
+            int c0 = fast_floor((float)(zz - sh));                                                 \
+            if (a0 < -nx2 - lo || b0 < -ny2 - lo || c0 < -nz2 - lo || a0 > nx2 + hi ||             \
+                b0 > ny2 + hi || c0 > nz2 + hi)                                                    \
+                return;                                                                            \
+            int a1 = a0 + span, b1 = b0 + span, c1 = c0 + span;                                    \
+            if (a0 < -nx2) a0 = -nx2;                                                              \
+            if (a1 > nx2) a1 = nx2;                                                                \
+            if (b0 < -ny2) b0 = -ny2;                                                              \
+            if (b1 > ny2) b1 = ny2;                                                                \
+            if (c0 < -nz2) c0 = -nz2;                                                              \
+            if (c1 > nz2) c1 = nz2;                                                                \
+            for (int k = c0; k <= c1; k++)                                                         \
+                for (int j = b0; j <= b1; j++)                                                     \
+                    for (int i = a0; i <= a1; i++) {                                               \
+                        int ti = abs(fast_floor((float)((double)(((float)i - xx) * 3.0f) + 0.5)));  \
+                        int tj = abs(fast_floor((float)((double)(((float)j - yy) * 3.0f) + 0.5)));  \
+                        int tk = abs(fast_floor((float)((double)(((float)k - zz) * 3.0f) + 0.5)));  \
+                        float gg = tab[(ti * tn + tj) * tn + tk];                                  \
+                        size_t off = add_complex_at_fast##SUFFIX(data, P, i, j, k, dr * gg * weight, \
+                                                                 di * gg * weight);                \
                         norm[off / 2] += (ACC)weight;                                              \
                     }                                                                              \
             return;                                                                                \

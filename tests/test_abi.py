@@ -43,7 +43,7 @@ def _create(**kw):
     (dict(nx=34, ny=34, nz=34), 3, "13-smooth"),
     (dict(sym=b"x5"), 2, "symmetry"),
     (dict(sym=b"icos2"), 2, "symmetry"),
-    (dict(mode=b"gridding_5"), 2, "mode"),
+    (dict(mode=b"experimental"), 2, "mode"),
     (dict(usessnr=1, corners=1), 1, "usessnr"),
     (dict(nx=24, ny=24, nz=24, mode=b"kaiser_bessel"), 1, "kaiser_bessel"),
 ])

@@ -14,6 +14,7 @@ from eman2_amd.metrics import corr, fsc
 from eman2_amd.transform import Transform, euler_eman_matrix
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def tables():
@@ -372,6 +373,41 @@ def test_insert_known_answer_x0_plane(oracle):
     assert abs(D[0, P - 2, 0] - 2 * np.conj(F)) < 1e-6
 
 
+def ref_fast_floor(v):
+    """Util::fast_floor (libEM/util.h:874-880): one less than floor at
+    negative integers"""
+    return int(v) - 1 if v < 0 else int(v)
+
+
+def gridding_table(mode):
+    """the gridding_5 / gridding_7 kernel table as the build compiles it"""
+    import re
+    with open(os.path.join(ROOT, "eman2_amd", "csrc", "gridding_tables.inc")) as f:
+        src = f.read()
+    name, n = ("FH_GRIDDING5_VALUES", 9) if mode == "gridding_5" else ("FH_GRIDDING7_VALUES", 12)
+    body = src.split("#define " + name)[1].split("#define")[0]
+    return np.array([float(x) for x in re.findall(r"-?\d+\.\d+", body)], np.float32).reshape(n, n, n)
+
+
+@pytest.mark.parametrize("mode", ["gridding_5", "gridding_7"])
+def test_gridding_tables(mode):
+    """The two kernel tables (reconstructor_tools.cpp:455, :817): the full
+    9^3 / 12^3 values, symmetric under every exchange of axes up to the last
+    printed digit (a value moved or mistyped breaks that), the last plane
+    zero, and the centre values 0.3293228 / 0.3756136."""
+    t = gridding_table(mode)
+    n = 9 if mode == "gridding_5" else 12
+    assert t.shape == (n, n, n)
+    for perm in ((1, 0, 2), (2, 1, 0), (0, 2, 1)):
+        assert np.abs(t - t.transpose(perm)).max() <= 1.5e-7
+    assert not t[-1].any() and not t[:, -1].any() and not t[:, :, -1].any()
+    assert t[0, 0, 0] == np.float32(0.3293228 if n == 9 else 0.3756136)
+    # a smooth kernel: decreasing from the centre along each axis to its first zero
+    ax = t[:, 0, 0]
+    first_neg = int(np.argmax(ax < 0))
+    assert np.all(np.diff(ax[:first_neg + 1]) < 0)
+
+
 def _brute_force(slice_, P, rot, syms, weight, mode):
     """Independent formulation: accumulate every stencil contribution on the
     full (unfolded) frequency grid with the exact Gaussian, then fold onto
@@ -416,6 +452,23 @@ def _brute_force(slice_, P, rot, syms, weight, mode):
                                 V[i + h, j + h, k + h] += val * kb * wk
                                 W[i + h, j + h, k + h] += wk
                                 A[i + h, j + h, k + h] += abs(val) * abs(kb * wk)
+                    continue
+                if mode in ("gridding_5", "gridding_7"):
+                    # 5 (7) cells per axis from fast_floor(p - 1.5) (p - 2.5), the table at
+                    # |fast_floor(3 d + 0.5)| per axis; the weight is w whatever the table holds
+                    tab = gridding_table(mode)
+                    sh, span, lo_, hi_ = (1.5, 4, 4, 3) if mode == "gridding_5" else (2.5, 6, 6, 5)
+                    lo = [ref_fast_floor(float(q) - sh) for q in c]
+                    if min(lo) < -h - lo_ or max(lo) > h + hi_:
+                        continue
+                    for k in range(max(lo[2], -h), min(lo[2] + span, h) + 1):
+                        for j in range(max(lo[1], -h), min(lo[1] + span, h) + 1):
+                            for i in range(max(lo[0], -h), min(lo[0] + span, h) + 1):
+                                t = [abs(ref_fast_floor((a - float(q)) * 3 + 0.5)) for a, q in zip((i, j, k), c)]
+                                g = float(tab[t[0], t[1], t[2]])
+                                V[i + h, j + h, k + h] += val * g * wk
+                                W[i + h, j + h, k + h] += wk
+                                A[i + h, j + h, k + h] += abs(val) * abs(g * wk)
                     continue
                 if mode in ("gauss_var", "gauss_5", "gauss_3"):
                     # 6 (3) cells per axis from floor(p - 2.5) (floor(p - 0.5)), inside [-P/2, P/2]
@@ -492,7 +545,8 @@ def _brute_force(slice_, P, rot, syms, weight, mode):
 
 @pytest.mark.parametrize("sym,mode", [("c1", "gauss_2"), ("d2", "gauss_2"), ("c1", "trilinear"), ("c4", "trilinear"),
                                       ("c1", "gauss_var"), ("d2", "gauss_var"), ("c1", "gauss_5"),
-                                      ("c1", "gauss_3"), ("c4", "nearest_neighbor"), ("d2", "kaiser_bessel")])
+                                      ("c1", "gauss_3"), ("c4", "nearest_neighbor"), ("d2", "kaiser_bessel"),
+                                      ("c1", "gridding_5"), ("d2", "gridding_7")])
 def test_insert_brute_force(oracle, sym, mode):
     # kaiser_bessel's sqrt(1 - (r / (P/2 - 1) - 1)^2) needs P/2 - 1 >= 13.5 (NaN below, as in the reference)
     P = 32 if mode == "kaiser_bessel" else 16
@@ -504,7 +558,7 @@ def test_insert_brute_force(oracle, sym, mode):
     data, norm, bound = _brute_force(sl, P, rot, oracle.sym_get(sym), 0.8, mode)
     # the Gaussian modes use exp(-round(50 f)/50) from a table: <= 1.005% per term
     # (gauss_3's normalisation too: 1.005% of it)
-    rt = 2e-5 if mode in ("trilinear", "nearest_neighbor", "kaiser_bessel") else (0.0205 if mode == "gauss_3" else 0.0102)
+    rt = 2e-5 if mode in ("trilinear", "nearest_neighbor", "kaiser_bessel", "gridding_5", "gridding_7") else (0.0205 if mode == "gauss_3" else 0.0102)
     assert np.all(np.abs(v.norm - norm) <= rt * np.abs(norm) + 1e-6)
     assert np.all(np.abs(cplx(v.data) - data) <= rt * bound + 1e-5)
 
