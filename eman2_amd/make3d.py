@@ -12,6 +12,8 @@ reference multiplies each cached preprocessed slice by its norm; the batched
 form scales the raw particle instead, which is the same up to float
 rounding (edge-mean subtraction, padding and the FFT are linear).
 """
+import os
+
 import numpy as np
 
 
@@ -129,4 +131,22 @@ def reconstruct_mrc(recon, stack_path, xforms, weights=None, out_path=None, **kw
     if out_path is not None:
         vol = out.numpy() if hasattr(out, "numpy") else np.asarray(out)
         mrc.write(out_path, vol, apix=h["apix"])
+    return out, info
+
+
+def reconstruct_lst(recon, lst_path, out_path=None, apix=None, **kw):
+    """reconstruct() over the particles of an #LSX / #LST list (MRC stacks,
+    orientations from each record's xform.projection); the map goes to
+    out_path when given"""
+    from . import lst, mrc
+    imgs, xforms, recs = lst.load_particles(lst_path)
+    if apix is None:
+        first = recs[0][1] if recs else None
+        if first is not None and not os.path.isabs(first):
+            first = os.path.join(os.path.dirname(os.path.abspath(lst_path)), first)
+        apix = mrc.read_header(first)["apix"] if first else 1.0
+    out, info = reconstruct(recon, imgs, xforms, apix=apix, **kw)
+    if out_path is not None:
+        vol = out.numpy() if hasattr(out, "numpy") else np.asarray(out)
+        mrc.write(out_path, vol, apix=apix)
     return out, info
