@@ -603,10 +603,11 @@ int fh_create(const fh_params *p, fh_reconstructor **out)
     }
     else if (mode == "gridding_5") imode = 7;
     else if (mode == "gridding_7") imode = 8;
+    else if (mode == "kaiser_bessel_derived") imode = 9;
     else {
         g_create_error = "NotExistingObjectException: inserter mode '" + mode +
                          "' (fourier_hip supports gauss_2, trilinear, gauss_var, gauss_5, gauss_3, nearest_neighbor, "
-                         "kaiser_bessel, gridding_5, gridding_7)";
+                         "kaiser_bessel, kaiser_bessel_derived, gridding_5, gridding_7)";
         return FH_ERR_NOT_EXISTING;
     }
     if (p->usessnr && p->corners) {
@@ -993,7 +994,7 @@ int fh_preprocess_slice(fh_reconstructor *h, const float *img, float *slice_out)
 
 // plain insertion of one prepared slice (h->slices + idx) with rotation rot9
 // one slice of a wide-stencil mode (gauss_var, gauss_5, gauss_3,
-// nearest_neighbor, kaiser_bessel, gridding_5, gridding_7): all its symmetry copies in one LDS-window
+// nearest_neighbor, kaiser_bessel(_derived), gridding_5, gridding_7): all its symmetry copies in one LDS-window
 // launch (t3d and weights staged in the chunk buffers, which hold >= nsym slices)
 static int wide_slice_insert(fh_reconstructor *h, const float2 *slice_d, const float *rot9, float weight,
                              const float *rw_d, const InsertConst &c)
@@ -1166,7 +1167,7 @@ static int insert_batch_impl(fh_reconstructor *h, const float *imgs, int n, int 
         FH_TRY(h, hipEventRecord(h->ev[1], h->stream));
         if (h->mode >= 2) {
             // wider or other stencils (gauss_var, gauss_5, gauss_3, nearest_neighbor, kaiser_bessel,
-            // gridding_5, gridding_7): the LDS-window tile form
+            // kaiser_bessel_derived, gridding_5, gridding_7): the LDS-window tile form
             FH_TRY(h, launch_insert_plain_batch(h->slices, m * h->nsym, h->nsym, h->t3d, h->wq, c, h->data, h->norm,
                                                 h->exptab, h->rtab, rwh ? h->rw : nullptr, h->stream));
             FH_TRY(h, hipEventRecord(h->ev[5], h->stream));

@@ -31,7 +31,7 @@ struct InsertConst {
     float Pf;    // (float)P
     float h;     // 1.0f / EMConsts::I2G
     int mode;    // 0 gauss_2, 1 trilinear, 2 gauss_var, 3 gauss_5, 4 nearest_neighbor, 5 gauss_3, 6 kaiser_bessel,
-                 // 7 gridding_5, 8 gridding_7
+                 // 7 gridding_5, 8 gridding_7, 9 kaiser_bessel_derived
     int corners;
 };
 

@@ -371,6 +371,22 @@ __device__ __forceinline__ bool wide_setup(int mode, float xx, float yy, float z
         px.lo[2] = px.hi[2] = c2;
         return true;
     }
+    if (mode == 9) {
+        // kaiser_bessel_derived: floor(p - 4) + [0, 8]; h = ws[0..2] = I0s(0) / I0s(32 pi) (ws[3..4] = 1),
+        // ww = their float sum in the reference's order
+        const float pp[3] = {xx, yy, zz};
+        for (int d = 0; d < 3; d++) {
+            const int a = (int)floorf(pp[d] - 4.0f);
+            if (a < -H - 7 || a > H + 6) return false;
+            px.lo[d] = max(a, -H);
+            px.hi[d] = min(a + 8, H);
+            if (px.hi[d] < px.lo[d]) return false;
+        }
+        const float w0 = (float)(bessel_i0_scaled(0.0) / bessel_i0_scaled(M_PI * 32.0));
+        px.h = w0;
+        px.ww = w0 + w0 + w0 + 1.0f + 1.0f;
+        return true;
+    }
     if (mode >= 7) {
         // gridding_5: fast_floor(p - 1.5) + [0, 4]; gridding_7: fast_floor(p - 2.5) + [0, 6]
         const double sh = mode == 8 ? 2.5 : 1.5;
@@ -421,6 +437,20 @@ __device__ __forceinline__ bool wide_term(int mode, const WidePix &px, int i, in
     if (mode == 4) {
         vr = px.dre * px.w;
         vi = px.dim * px.w;
+        g = px.w;
+        return true;
+    }
+    if (mode == 9) {
+        // Mode10 (reconstructor_tools.cpp:759-812): kb = ws[0] + ... + ws[q], q <= min(r, 4) with r the
+        // squared distance; value dt sqrt(kb / sum ws) w, weight w
+        const float dx = (float)i - px.xx, dy = (float)j - px.yy, dz = (float)k - px.zz;
+        const float r = dx * dx + dy * dy + dz * dz;
+        const float lim = r < 4.0f ? r : 4.0f;
+        float kb = 0.0f;
+        for (int q = 0; q <= 4 && (float)q <= lim; q++) kb += q < 3 ? px.h : 1.0f;
+        const float dn = sqrt_rn(__fdiv_rn(kb, px.ww));
+        vr = px.dre * dn * px.w;
+        vi = px.dim * dn * px.w;
         g = px.w;
         return true;
     }

@@ -55,7 +55,8 @@ typedef struct {
     int nx, ny, nz;        /* "size": padded real-space box, nx==ny==nz, even */
     const char *sym;       /* "sym": c<n>, d<n>, h<n>[:1:daz:tz], tet, oct, icos (symmetry.cpp:75-142) */
     const char *mode;      /* "mode": gauss_2, trilinear, gauss_var, gauss_5, gauss_3, nearest_neighbor,
-                              kaiser_bessel, gridding_5, gridding_7 (reconstructor_tools.cpp:42-53) */
+                              kaiser_bessel, kaiser_bessel_derived, gridding_5, gridding_7
+                              (reconstructor_tools.cpp:42-53) */
     int corners;           /* "corners" (reconstructor.cpp:1175-1181) */
     int sqrtnorm;          /* "sqrtnorm" (reconstructor.cpp:1564) */
     int usessnr;           /* "usessnr": weight slices by their class_ssnr (not with corners) */

@@ -3,8 +3,8 @@
  *
  * A plain-C restatement of the reference's direct Fourier inversion path
  * (EMAN2 libEM "fourier" Reconstructor with the gauss_2, trilinear,
- * gauss_var, gauss_5, gauss_3, nearest_neighbor, kaiser_bessel, gridding_5
- * and gridding_7 inserters (the last two with the reference's gridding
+ * gauss_var, gauss_5, gauss_3, nearest_neighbor, kaiser_bessel,
+ * kaiser_bessel_derived, gridding_5 and gridding_7 inserters (the last two with the reference's gridding
  * kernel tables, taken over as data), class_ssnr
  * weighting, its particle prep with preprocess_slice's 2-D transform,
  * slice agreement, projection() and finish()).  It is the checker the GPU
@@ -40,6 +40,8 @@
  *   gridding_5 / gridding_7  - the brute-force scatter with the table at
  *                              |fast_floor(3d + 0.5)|; the tables by axis-
  *                              exchange symmetry, zero last planes, centres
+ *   kaiser_bessel_derived    - the brute-force scatter with its 1-D weights
+ *                              and the float squared distance
  *   2-D transform            - closed forms (shift, half-pixel, mirror,
  *                              scale, identity), Fourier shift theorem
  *   projection               - inverse of an identity insertion, real-space
@@ -953,6 +955,43 @@ static inline void slice_get(const float *s, int P, int x, int y, float *re, flo
                                            or_bessel_i0_scaled(M_PI * a));                         \
                         size_t off = add_complex_at_fast##SUFFIX(data, P, i, j, k, dr * kb * weight, \
                                                                  di * kb * weight);                \
+                        norm[off / 2] += (ACC)weight;                                              \
+                    }                                                                              \
+            return;                                                                                \
+        }                                                                                          \
+        if (mode == OR_MODE_KBD) {                                                                 \
+            /* FourierInserter3DMode10 (kaiser_bessel_derived, :759-812): 9 cells per axis from     \
+             * floor(p - 4); ws[q] = I0s(pi 32 sqrt(1 - ((4 + q) / 7 - 1)^2)) / I0s(pi 32) with     \
+             * the reference's integer division, value dt sqrt(sum_{q <= min(r, 4)} ws[q] / sum ws) w, \
+             * r the squared distance; weight w */                                                 \
+            int a0 = (int)floorf(xx - 4), b0 = (int)floorf(yy - 4), c0 = (int)floorf(zz - 4);     \
+            if (a0 < -nx2 - 7 || b0 < -ny2 - 7 || c0 < -nz2 - 7 || a0 > nx2 + 6 ||                 \
+                b0 > ny2 + 6 || c0 > nz2 + 6)                                                      \
+                return;                                                                            \
+            int a1 = a0 + 8, b1 = b0 + 8, c1 = c0 + 8;                                             \
+            if (a0 < -nx2) a0 = -nx2;                                                              \
+            if (a1 > nx2) a1 = nx2;                                                                \
+            if (b0 < -ny2) b0 = -ny2;                                                              \
+            if (b1 > ny2) b1 = ny2;                                                                \
+            if (c0 < -nz2) c0 = -nz2;                                                              \
+            if (c1 > nz2) c1 = nz2;                                                                \
+            float ws[5], wm = 0.0f;                                                                \
+            for (int q = 0; q <= 4; q++) {                                                         \
+                int u = (4 + q) / 7 - 1;                                                           \
+                double t = or_bessel_i0_scaled(M_PI * 32.0 * (double)sqrtf(1.0f - (float)(u * u))) / \
+                           or_bessel_i0_scaled(M_PI * 32.0);                                       \
+                ws[q] = (float)t;                                                                  \
+                wm += (float)t;                                                                    \
+            }                                                                                      \
+            for (int k = c0; k <= c1; k++)                                                         \
+                for (int j = b0; j <= b1; j++)                                                     \
+                    for (int i = a0; i <= a1; i++) {                                               \
+                        float r = hypot3sq((float)i - xx, (float)j - yy, (float)k - zz);           \
+                        float kb = 0.0f;                                                           \
+                        for (int q = 0; q <= (r < 4.0f ? r : 4.0f); q++) kb += ws[q];              \
+                        float dn = sqrtf(kb / wm);                                                 \
+                        size_t off = add_complex_at_fast##SUFFIX(data, P, i, j, k, dr * dn * weight, \
+                                                                 di * dn * weight);                \
                         norm[off / 2] += (ACC)weight;                                              \
                     }                                                                              \
             return;                                                                                \

@@ -30,6 +30,7 @@ extern "C" {
 #define OR_MODE_KB 6
 #define OR_MODE_GRID5 7
 #define OR_MODE_GRID7 8
+#define OR_MODE_KBD 9
 
 /* ---- Util helpers (libEM/util.cpp:742-800, libEM/util.h:465-471,816-819) */
 void  or_fast_exp_table(float *out1000);

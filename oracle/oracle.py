@@ -24,7 +24,8 @@ _LIBS = {}
 GAUSS2 = 0
 TRILINEAR = 1
 MODES = {"gauss_2": GAUSS2, "trilinear": TRILINEAR, "gauss_var": 2, "gauss_5": 3, "nearest_neighbor": 4, "gauss_3": 5,
-         "kaiser_bessel": 6, "gridding_5": 7, "gridding_7": 8}
+         "kaiser_bessel": 6, "gridding_5": 7, "gridding_7": 8,
+         "kaiser_bessel_derived": 9}
 
 _fp = ctypes.POINTER(ctypes.c_float)
 _dp = ctypes.POINTER(ctypes.c_double)

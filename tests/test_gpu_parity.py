@@ -163,7 +163,8 @@ def test_batch_ctf(oracle):
                                       ("icos", "gauss_2"), ("h6:1:25.5:3", "trilinear"),
                                       ("c1", "gauss_var"), ("d2", "gauss_var"), ("c1", "gauss_5"),
                                       ("d2", "gauss_3"), ("c4", "nearest_neighbor"), ("c1", "kaiser_bessel"),
-                                      ("c1", "gridding_5"), ("d2", "gridding_7"), ("icos", "gridding_5")])
+                                      ("c1", "gridding_5"), ("d2", "gridding_7"), ("icos", "gridding_5"),
+                                      ("c1", "kaiser_bessel_derived")])
 def test_batch_symmetry_modes(oracle, sym, mode):
     P, B = 48, 32
     ph = S.phantom(10, 10.0, seed=5)
@@ -182,7 +183,7 @@ def test_batch_symmetry_modes(oracle, sym, mode):
 
 
 @pytest.mark.parametrize("sym,mode", [("d2", "gauss_3"), ("c4", "kaiser_bessel"), ("c3", "nearest_neighbor"),
-                                      ("c1", "gridding_5"), ("d2", "gridding_7")])
+                                      ("c1", "gridding_5"), ("d2", "gridding_7"), ("d2", "kaiser_bessel_derived")])
 def test_wide_mode_single_slice_symmetry(oracle, sym, mode):
     """insert_slice of a wide-stencil mode under a symmetry (every symmetry
     copy of the slice in one launch), then determine_slice_agreement with the

@@ -453,6 +453,26 @@ def _brute_force(slice_, P, rot, syms, weight, mode):
                                 W[i + h, j + h, k + h] += wk
                                 A[i + h, j + h, k + h] += abs(val) * abs(kb * wk)
                     continue
+                if mode == "kaiser_bessel_derived":
+                    # 9 cells per axis from floor(p - 4); 1-D weights with the reference's integer
+                    # division (4 + q) / 7: I0e(0) / I0e(32 pi) for q < 3, 1 for q = 3, 4
+                    import scipy.special as sp
+                    ws = [1.0 / sp.i0e(32 * math.pi)] * 3 + [1.0, 1.0]
+                    lo = [int(math.floor(float(q) - 4)) for q in c]
+                    if min(lo) < -h - 7 or max(lo) > h + 6:
+                        continue
+                    for k in range(max(lo[2], -h), min(lo[2] + 8, h) + 1):
+                        for j in range(max(lo[1], -h), min(lo[1] + 8, h) + 1):
+                            for i in range(max(lo[0], -h), min(lo[0] + 8, h) + 1):
+                                # the step in r sits on integers, where the origin cell of every pixel
+                                # lies (x^2 + y^2): r in the reference's float arithmetic decides
+                                dd = [f(f(a) - q) for a, q in zip((i, j, k), c)]
+                                r = float(f(f(f(dd[0] * dd[0]) + f(dd[1] * dd[1])) + f(dd[2] * dd[2])))
+                                g = math.sqrt(sum(ws[:int(math.floor(min(r, 4.0))) + 1]) / sum(ws))
+                                V[i + h, j + h, k + h] += val * g * wk
+                                W[i + h, j + h, k + h] += wk
+                                A[i + h, j + h, k + h] += abs(val) * abs(g * wk)
+                    continue
                 if mode in ("gridding_5", "gridding_7"):
                     # 5 (7) cells per axis from fast_floor(p - 1.5) (p - 2.5), the table at
                     # |fast_floor(3 d + 0.5)| per axis; the weight is w whatever the table holds
@@ -546,7 +566,7 @@ def _brute_force(slice_, P, rot, syms, weight, mode):
 @pytest.mark.parametrize("sym,mode", [("c1", "gauss_2"), ("d2", "gauss_2"), ("c1", "trilinear"), ("c4", "trilinear"),
                                       ("c1", "gauss_var"), ("d2", "gauss_var"), ("c1", "gauss_5"),
                                       ("c1", "gauss_3"), ("c4", "nearest_neighbor"), ("d2", "kaiser_bessel"),
-                                      ("c1", "gridding_5"), ("d2", "gridding_7")])
+                                      ("c1", "gridding_5"), ("d2", "gridding_7"), ("c1", "kaiser_bessel_derived")])
 def test_insert_brute_force(oracle, sym, mode):
     # kaiser_bessel's sqrt(1 - (r / (P/2 - 1) - 1)^2) needs P/2 - 1 >= 13.5 (NaN below, as in the reference)
     P = 32 if mode == "kaiser_bessel" else 16
@@ -558,7 +578,8 @@ def test_insert_brute_force(oracle, sym, mode):
     data, norm, bound = _brute_force(sl, P, rot, oracle.sym_get(sym), 0.8, mode)
     # the Gaussian modes use exp(-round(50 f)/50) from a table: <= 1.005% per term
     # (gauss_3's normalisation too: 1.005% of it)
-    rt = 2e-5 if mode in ("trilinear", "nearest_neighbor", "kaiser_bessel", "gridding_5", "gridding_7") else (0.0205 if mode == "gauss_3" else 0.0102)
+    rt = 2e-5 if mode in ("trilinear", "nearest_neighbor", "kaiser_bessel", "gridding_5", "gridding_7",
+                          "kaiser_bessel_derived") else (0.0205 if mode == "gauss_3" else 0.0102)
     assert np.all(np.abs(v.norm - norm) <= rt * np.abs(norm) + 1e-6)
     assert np.all(np.abs(cplx(v.data) - data) <= rt * bound + 1e-5)
 
