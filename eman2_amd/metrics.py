@@ -1,7 +1,8 @@
 """Agreement metrics used by the parity tests and the benchmark report.
 
 * ``fsc``: Fourier shell correlation of two real cubic maps, shells of
-  width 1 voxel up to Nyquist.
+  width 1 voxel up to Nyquist.  A shell empty in both maps counts as 1 (they
+  agree); a shell empty in one map only counts as 0 (one map lost it).
 * ``max_rel_err``: max |a-b| / max |b| (error relative to the map's peak;
   a per-voxel ratio is ill-defined for sums that cancel to ~0).
 * ``weight_rel_err``: per-voxel relative error of the weight volume, whose
@@ -28,7 +29,9 @@ def fsc(a, b):
     db = np.bincount(shell[m], (np.abs(fb) ** 2)[m], nsh)
     with np.errstate(invalid="ignore", divide="ignore"):
         c = num / np.sqrt(da * db)
-    c[~np.isfinite(c)] = 1.0
+    both_empty = (da == 0) & (db == 0)
+    c[both_empty] = 1.0
+    c[~np.isfinite(c)] = 0.0
     return c
 
 

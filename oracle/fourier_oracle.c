@@ -19,7 +19,7 @@
  * and rounded to float, where the reference calls FFTW in single precision;
  * the FFT is pinned against numpy.fft and the reference's known answers.
  *
- * Parity pins (tests/test_oracle.py):
+ * Parity pins (tests/test_oracle.py, tests/test_oracle_pins.py):
  *   fast_exp / hypot tables  - closed forms + reference loop bounds (N_pix)
  *   FFT conventions          - rt/pyem/test_emdata.py:722-760 known answers,
  *                              numpy.fft.rfftn/irfftn
@@ -30,8 +30,16 @@
  *                              insert(w)+insert(-w)=0, linearity, Hermitian
  *                              symmetry of the finished map, brute-force
  *                              scatter in numpy on tiny boxes
- *   finish                   - noiseless phantom reconstruction
- *   slice agreement          - self-consistency (absqual ~ 1 for noiseless)
+ *   finish                   - noiseless phantom reconstruction;
+ *                              normalize_threed's pairing and x2 cells by the
+ *                              weights of the periodic P^3 grid (k and -k),
+ *                              gridded by an own loop (tests/test_oracle_pins.py)
+ *   slice agreement          - self-consistency (absqual ~ 1 for noiseless);
+ *                              closed forms on hand-built volumes (annulus
+ *                              counts for the 12 A band and its clamps,
+ *                              vweight / nval, qual at weight != 1), a numpy
+ *                              statement of the compare loop, sub = remove
+ *                              (tests/test_oracle_pins.py)
  *   tet / oct / icos / h     - rotation-angle classes of A4 / S4 / A5, the
  *                              axes each table puts on z (and y for icos's
  *                              2-fold), helical 0, +-daz, +-2 daz steps
