@@ -325,8 +325,12 @@ __global__ void __launch_bounds__(64) k_insert_plain_batch(const float2 *__restr
 // atomics than one per contribution.  Each contribution's value and weight
 // are computed exactly as in insert_wide.  Across an 8 x 8 tile a pixel
 // coordinate moves by at most 7 sqrt 2 (orthonormal slice axes), so the
-// stencil ranges of a tile span at most 10 + 6 = 16 cells per axis; a cell
-// outside the window would go straight to the volume.  The window (59 KB)
+// stencil origins of a tile span at most 10 cells per axis and the stencils
+// 10 + (n - 1) cells for an n-cell stencil: within the 17-cell window for
+// n <= 7 (gauss_var, gauss_5, gauss_3, kaiser_bessel, gridding_5,
+// gridding_7); kaiser_bessel_derived's 9-cell stencil can reach 18, and the
+// cells of a tile beyond the window go straight to the volume with global
+// atomics (same result, less of the window's saving).  The window (59 KB)
 // and the pixel records stay within 64 KB of LDS per workgroup.
 constexpr int WT_TILE = 8;
 constexpr int WT_WIN = 17;

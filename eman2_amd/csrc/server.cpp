@@ -51,10 +51,9 @@ struct fh_server {
 static long long job_footprint(const fh_job &j, int chunk)
 {
     const int P = j.params.nx;
-    int nsym = 1;
-    const char *s = j.params.sym ? j.params.sym : "c1";
-    if ((s[0] == 'c' || s[0] == 'C') && s[1]) nsym = atoi(s + 1);
-    else if ((s[0] == 'd' || s[0] == 'D') && s[1]) nsym = 2 * atoi(s + 1);
+    // the operator count from the library's own symmetry table (c, d, h, tet,
+    // oct, icos); an unknown name costs one operator and fails in fh_create
+    int nsym = fh_get_symmetries(j.params.sym ? j.params.sym : "c1", nullptr, 0);
     if (nsym < 1) nsym = 1;
     const int m = j.n < chunk ? (j.n > 0 ? j.n : 1) : chunk;
     long long fp = fh_footprint_bytes(P, nsym, m);

@@ -102,17 +102,28 @@ def write(path, records, comment=None):
             f.write(s.ljust(L - 1) + "\n")
 
 
+def resolve_image_path(lst_path, fname):
+    """Where a record's image file is: as written (absolute, or relative to
+    the current directory, which is how the reference opens it,
+    libEM/io/lstfastio.cpp:154-164 -- in an EMAN2 project sets/x.lst names
+    particles/y.hdf relative to the project root), else relative to the
+    list's own folder."""
+    if os.path.isabs(fname) or os.path.exists(fname):
+        return fname
+    alt = os.path.join(os.path.dirname(os.path.abspath(lst_path)), fname)
+    return alt if os.path.exists(alt) else fname
+
+
 def load_particles(path, key="xform.projection"):
     """(images (n, ny, nx) float32, 3x4 matrices (n, 3, 4), records): every
-    listed particle read from its MRC stack (paths relative to the list's
-    folder as well as absolute), and its Transform under key"""
+    listed particle read from its MRC stack (paths resolved by
+    resolve_image_path), and its Transform under key"""
     from . import mrc
     recs = read(path)
-    here = os.path.dirname(os.path.abspath(path))
     stacks = {}
     imgs, xfs = [], []
     for n, fname, d in recs:
-        p = fname if os.path.isabs(fname) else os.path.join(here, fname)
+        p = resolve_image_path(path, fname)
         if not p.lower().endswith((".mrc", ".mrcs", ".map")):
             raise ValueError("%s: only MRC stacks are read (record %d names %s)" % (path, n, fname))
         if p not in stacks:

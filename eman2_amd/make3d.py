@@ -141,9 +141,7 @@ def reconstruct_lst(recon, lst_path, out_path=None, apix=None, **kw):
     from . import lst, mrc
     imgs, xforms, recs = lst.load_particles(lst_path)
     if apix is None:
-        first = recs[0][1] if recs else None
-        if first is not None and not os.path.isabs(first):
-            first = os.path.join(os.path.dirname(os.path.abspath(lst_path)), first)
+        first = lst.resolve_image_path(lst_path, recs[0][1]) if recs else None
         apix = mrc.read_header(first)["apix"] if first else 1.0
     out, info = reconstruct(recon, imgs, xforms, apix=apix, **kw)
     if out_path is not None:

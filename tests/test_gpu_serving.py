@@ -113,3 +113,24 @@ def test_served_job_other_symmetry_and_mode():
     m = s.result(jid)
     assert max_rel_err(m, direct) <= 1e-4 and fsc(m, direct).min() >= 0.9999
     s.close()
+
+
+@pytest.mark.parametrize("sym,nsym", [("icos", 60), ("oct", 24), ("tet", 12), ("d7", 14), ("c4", 4)])
+def test_footprint_counts_every_symmetry_operator(sym, nsym):
+    """a job's admission footprint is the library's footprint for the number
+    of operators its symmetry really has, plus host staging and finish()"""
+    from eman2_amd import _lib
+    from eman2_amd.serving import Server
+    j = dict(make_jobs()[0], sym=sym)
+    chunk = 64
+    s = Server(max_active=1, chunk=chunk)
+    jid = s.submit(**j)
+    fp = s.poll(jid)["footprint_bytes"]
+    s.cancel(jid)
+    s.drain()
+    s.close()
+    P, B, m = j["size"], j["imgs"].shape[-1], min(chunk, len(j["xforms"]))
+    L = _lib.load()
+    assert L.fh_get_symmetries(sym.encode(), None, 0) == nsym
+    want = L.fh_footprint_bytes(P, nsym, m) + 4 * m * B * B + 4 * P ** 3 + 8 * P * P
+    assert fp == want, (fp, want)

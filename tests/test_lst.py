@@ -105,3 +105,29 @@ def test_reconstruct_from_list(tmp_path):
     lst.write(str(tmp_path / "bad.lst"), [(9, "s.mrcs", {"xform.projection": mats[0]})])
     with pytest.raises(IndexError):
         lst.load_particles(str(tmp_path / "bad.lst"))
+
+
+def test_image_paths_relative_to_working_directory(tmp_path, monkeypatch):
+    """a project's sets/x.lst names particles/y.mrcs relative to the project
+    root (the working directory), as the reference opens it; a name that
+    only resolves beside the list still loads"""
+    rng = np.random.default_rng(6)
+    (tmp_path / "particles").mkdir()
+    (tmp_path / "sets").mkdir()
+    stack = rng.standard_normal((4, 8, 8)).astype(np.float32)
+    mrc.write(str(tmp_path / "particles" / "y.mrcs"), stack, apix=2.1)
+    other = rng.standard_normal((4, 8, 8)).astype(np.float32)
+    mrc.write(str(tmp_path / "sets" / "y.mrcs"), other, apix=2.1)   # a decoy beside the list
+    m = np.eye(3, 4)
+    lst.write(str(tmp_path / "sets" / "x.lst"), [(2, "particles/y.mrcs", {"xform.projection": m}),
+                                                  (1, "particles/y.mrcs", {"xform.projection": m})])
+    monkeypatch.chdir(tmp_path)
+    imgs, _, _ = lst.load_particles("sets/x.lst")
+    np.testing.assert_array_equal(imgs, stack[[2, 1]])
+    lst.write(str(tmp_path / "sets" / "z.lst"), [(3, "y.mrcs", {"xform.projection": m})])
+    monkeypatch.chdir(tmp_path / "particles")       # y.mrcs here is the working directory's
+    imgs, _, _ = lst.load_particles(str(tmp_path / "sets" / "z.lst"))
+    np.testing.assert_array_equal(imgs, stack[[3]])
+    monkeypatch.chdir(tmp_path)                     # only beside the list
+    imgs, _, _ = lst.load_particles("sets/z.lst")
+    np.testing.assert_array_equal(imgs, other[[3]])
