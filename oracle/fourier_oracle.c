@@ -784,21 +784,33 @@ static int floordiv2(int a)
     return a >= 0 ? a / 2 : -((-a + 1) / 2);
 }
 
-static void prep_particle_impl(const float *raw, int B, int P, const or_ctf *ctf, const float *xf, float *out);
+static void prep_particle_impl(const float *raw, int B, int P, const or_ctf *ctf, const float *dfdf, const float *xf,
+                               float *out);
 
 void or_prep_particle(const float *raw, int B, int P, const or_ctf *ctf, float *out)
 {
-    prep_particle_impl(raw, B, P, ctf, NULL, out);
+    prep_particle_impl(raw, B, P, ctf, NULL, NULL, out);
 }
 
 /* the same with the particle's Transform given to preprocess_slice
  * (e2make3dpar.py:716): its 2-D part resamples the padded, flipped image */
 void or_prep_particle_xform(const float *raw, int B, int P, const or_ctf *ctf, const float xf[12], float *out)
 {
-    prep_particle_impl(raw, B, P, ctf, xf, out);
+    prep_particle_impl(raw, B, P, ctf, NULL, xf, out);
 }
 
-static void prep_particle_impl(const float *raw, int B, int P, const or_ctf *ctf, const float *xf, float *out)
+/* the two-sign flip of e2make3dpar.py:698-709 / e2spa_make3d.py:316-327: the
+ * padded image's FFT times CTF_SIGN at the particle's defocus, times CTF_SIGN
+ * at defocus + dfdf, back to real space; then preprocess_slice with xf (or
+ * none when xf is NULL) */
+void or_prep_particle_dfdf(const float *raw, int B, int P, const or_ctf *ctf, float dfdf, const float *xf,
+                           float *out)
+{
+    prep_particle_impl(raw, B, P, ctf, &dfdf, xf, out);
+}
+
+static void prep_particle_impl(const float *raw, int B, int P, const or_ctf *ctf, const float *dfdf, const float *xf,
+                               float *out)
 {
     /* programs/e2make3dpar.py:630-654 (edge-mean subtract, clip/pad),
      * :704-714 (CTF_SIGN flip: fft, mult, ift), :716 preprocess_slice */
@@ -818,6 +830,17 @@ static void prep_particle_impl(const float *raw, int B, int P, const or_ctf *ctf
             float a = F[i], b = F[i + 1], s = sg[i];
             F[i] = a * s - b * 0.0f;
             F[i + 1] = a * 0.0f + b * s;
+        }
+        if (dfdf) {
+            /* ctf1.defocus = ctf.defocus + dfdf, a second CTF_SIGN product */
+            or_ctf c2 = *ctf;
+            c2.defocus = (float)((double)ctf->defocus + (double)*dfdf);  /* the Python sum, stored as float */
+            or_ctf_sign_2d(&c2, P, sg);
+            for (size_t i = 0; i < (size_t)(P + 2) * P; i += 2) {
+                float a = F[i], b = F[i + 1], s = sg[i];
+                F[i] = a * s - b * 0.0f;
+                F[i + 1] = a * 0.0f + b * s;
+            }
         }
         or_fft_c2r_2d(F, P, P, pad);
         free(F);

@@ -83,6 +83,8 @@ void or_preprocess_slice(const float *img, int P, float *out /* (P+2)*P */);
 void or_prep_particle(const float *raw, int B, int P, const or_ctf *ctf_or_null,
                       float *out /* (P+2)*P */);
 void or_prep_particle_xform(const float *raw, int B, int P, const or_ctf *ctf, const float xf[12], float *out);
+void or_prep_particle_dfdf(const float *raw, int B, int P, const or_ctf *ctf, float dfdf, const float *xf_or_null,
+                           float *out);
 int or_xform2d(const float *img, int nx, int ny, const float t[12], float *out);
 
 /* ---- volume insertion (libEM/reconstructor.cpp:1122-1210,

@@ -66,6 +66,7 @@ def lib(fast=False):
         "or_projection": (v, [i, _fp, _fp, _fp, i, _fp]),
         "or_bessel_i0_scaled": (d, [d]),
         "or_prep_particle_xform": (v, [_fp, i, i, ctypes.POINTER(CtfStruct), _fp, _fp]),
+        "or_prep_particle_dfdf": (v, [_fp, i, i, ctypes.POINTER(CtfStruct), f, _fp, _fp]),
         "or_insert_slice_ssnr": (i, [i, _fp, _fp, _fp, _fp, _fp, i, _fp, i, i]),
         "or_edge_mean": (f, [_fp, i, i]),
         "or_clip_2d": (v, [_fp, i, i, i, i, i, i, _fp]),
@@ -265,13 +266,20 @@ def preprocess_slice(img):
     return out
 
 
-def prep_particle(raw, P, ctf=None, xform=None):
+def prep_particle(raw, P, ctf=None, xform=None, dfdf=None):
     """caller prep + preprocess_slice; xform: the particle's 3x4 Transform,
-    whose 2-D part (translation / scale / mirror) preprocess_slice applies"""
+    whose 2-D part (translation / scale / mirror) preprocess_slice applies;
+    dfdf (with ctf): the flip is the product of CTF_SIGN at the defocus and
+    at defocus + dfdf"""
     raw = f32(raw)
     B = raw.shape[0]
     out = np.zeros((P, P + 2), np.float32)
     cp = ctypes.byref(ctf_struct(ctf)) if ctf is not None else None
+    if dfdf is not None:
+        assert ctf is not None
+        xp = None if xform is None else _f(f32(xform).reshape(12).copy())
+        lib().or_prep_particle_dfdf(_f(raw), B, P, cp, float(dfdf), xp, _f(out))
+        return out
     if xform is None:
         lib().or_prep_particle(_f(raw), B, P, cp, _f(out))
     else:

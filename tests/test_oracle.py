@@ -334,6 +334,61 @@ def test_prep_particle_pad_and_ctf(oracle):
     assert np.abs(outc - ref * sg).max() < 1e-5 * np.abs(ref).max()
 
 
+def _pad_flip_numpy(oracle, raw, P, signs):
+    """the caller prep in numpy: edge mean, centre pad, FFT, times each sign
+    image in turn, inverse FFT (e2make3dpar.py:630-654, 698-714)"""
+    B = raw.shape[0]
+    em = oracle.edge_mean(raw)
+    pad = np.zeros((P, P), np.float64)
+    o = (P - B) // 2
+    pad[o:o + B, o:o + B] = raw - np.float32(em)
+    F = np.fft.rfft2(pad)
+    for sg in signs:
+        F = F * sg
+    return np.fft.irfft2(F, s=(P, P)).astype(np.float32)
+
+
+def test_prep_ctf_then_2d_transform_order(oracle):
+    """With a CTF and a translation the flip comes first, on the padded
+    image, and preprocess_slice resamples the flipped image: the numpy flip
+    followed by the (pinned) 2-D resampling and preprocess_slice.  The other
+    order (resample, then flip) differs by far more than the tolerance."""
+    B, P = 20, 32
+    raw = np.random.default_rng(23).standard_normal((B, B)).astype(np.float32)
+    ctf = dict(defocus=1.4, voltage=300.0, cs=2.7, ampcont=10.0, apix=1.0)
+    xf = _xf(2.5, -1.25, az=10, alt=20, phi=30)
+    sg = oracle.ctf_sign_2d(ctf, P)[:, 0::2]
+    flipped = _pad_flip_numpy(oracle, raw, P, [sg])
+    moved, _ = oracle.xform2d(flipped, xf)
+    want = cplx(oracle.preprocess_slice(moved))
+    got = cplx(oracle.prep_particle(raw, P, ctf, xform=xf))
+    assert np.abs(got - want).max() <= 2e-5 * np.abs(want).max()
+    other = cplx(oracle.prep_particle(raw, P, None, xform=xf)) * sg
+    assert np.abs(other - want).max() > 1e-2 * np.abs(want).max()
+
+
+def test_prep_dfdf_two_sign_flip(oracle):
+    """dfdf: the flip is CTF_SIGN(defocus) x CTF_SIGN(defocus + dfdf)
+    (e2make3dpar.py:698-709); dfdf = 0 cancels to no flip at all"""
+    B, P = 20, 32
+    raw = np.random.default_rng(24).standard_normal((B, B)).astype(np.float32)
+    ctf = dict(defocus=1.1, voltage=300.0, cs=2.7, ampcont=10.0, apix=1.2)
+    plain = cplx(oracle.prep_particle(raw, P))
+    zero = cplx(oracle.prep_particle(raw, P, ctf, dfdf=0.0))
+    assert np.abs(zero - plain).max() <= 2e-6 * np.abs(plain).max()
+    dfdf = 0.35
+    s1 = oracle.ctf_sign_2d(ctf, P)[:, 0::2]
+    s2 = oracle.ctf_sign_2d(dict(ctf, defocus=float(np.float32(1.1 + 0.35))), P)[:, 0::2]
+    assert (s1 * s2 < 0).any()
+    for xf in (None, _xf(-3.0, 1.5)):
+        flipped = _pad_flip_numpy(oracle, raw, P, [s1, s2])
+        if xf is not None:
+            flipped, _ = oracle.xform2d(flipped, xf)
+        want = cplx(oracle.preprocess_slice(flipped))
+        got = cplx(oracle.prep_particle(raw, P, ctf, xform=xf, dfdf=dfdf))
+        assert np.abs(got - want).max() <= 2e-5 * np.abs(want).max()
+
+
 # ------------------------------------------------------------------ insertion
 def test_insert_known_answer_identity(oracle):
     """single non-zero pixel at identity orientation, P=16 (exact coords)."""
