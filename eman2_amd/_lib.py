@@ -29,7 +29,8 @@ class Params(ctypes.Structure):
 
 
 class Ctf(ctypes.Structure):
-    _fields_ = [("voltage", _c_float), ("cs", _c_float), ("ampcont", _c_float), ("apix", _c_float)]
+    _fields_ = [("voltage", _c_float), ("cs", _c_float), ("ampcont", _c_float), ("apix", _c_float),
+                ("dfdf", ctypes.c_void_p)]
 
 
 class Tuning(ctypes.Structure):
@@ -83,6 +84,8 @@ EXPORTS = {
     "fh_prep_batch": (_c_int, [_vp, _vp, _c_int, _c_int, _vp, ctypes.POINTER(Ctf), _vp, _c_int]),
     "fh_insert_plain": (_c_int, [_vp, _vp, _c_int, _c_int, _vp, _vp, _vp, ctypes.POINTER(Ctf)]),
     "fh_determine_slice_agreement": (_c_int, [_vp, _vp, _c_int, _vp, _c_float, _c_int, _c_float, _vp]),
+    "fh_merge_add": (_c_int, [_vp, _vp, _vp, _c_int]),
+    "fh_merge_finish": (_c_int, [_vp, _vp]),
     "fh_agreement_batch": (_c_int, [_vp, _vp, _c_int, _c_int, _vp, _vp, _vp, ctypes.POINTER(Ctf), _c_int,
                                     _c_float, _vp, _c_int]),
     "fh_finish": (_c_int, [_vp, _c_int, _vp, _vp]),

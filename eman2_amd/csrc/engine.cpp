@@ -319,12 +319,14 @@ struct fh_reconstructor {
     // chunk staging
     int cap_n = 0, cap_B = 0;
     float *imgs = nullptr, *means = nullptr, *rots = nullptr, *wts = nullptr, *dfs = nullptr;
+    float *dfs2 = nullptr;      // second defocus (defocus + dfdf) of the two-sign flip
     float2 *slices = nullptr;
     float *t3d = nullptr, *wq = nullptr;
     float *rw = nullptr;        // class_ssnr radial weights, (P/2+1) per particle (usessnr)
     float *ximgs = nullptr;     // 2-D transformed padded particles, P x P each
+    float *ximgs2 = nullptr;    // second P x P staging: the flipped images' resampling (CTF with a 2-D part)
     float *xfd = nullptr;       // their inverse 2-D transforms, 6 per particle
-    int ximgs_cap = 0;
+    int ximgs_cap = 0, ximgs2_cap = 0;
     double *partial = nullptr;  // agreement partial sums
     double *acc8 = nullptr;     // batched agreement sums, 8 per particle
     int acc8_cap = 0;
@@ -444,7 +446,7 @@ static hipError_t h2d_staged(fh_reconstructor *h, void *dst, const void *src, si
 
 static void free_chunk(fh_reconstructor *h)
 {
-    for (float **p : {&h->imgs, &h->means, &h->rots, &h->wts, &h->dfs, &h->t3d, &h->wq, &h->rw})
+    for (float **p : {&h->imgs, &h->means, &h->rots, &h->wts, &h->dfs, &h->dfs2, &h->t3d, &h->wq, &h->rw})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
     if (h->slices) { (void)hipFree(h->slices); h->slices = nullptr; }
     h->cap_n = h->cap_B = 0;
@@ -481,6 +483,7 @@ static int ensure_chunk(fh_reconstructor *h, int n, int B)
     FH_TRY(h, hipMalloc(&h->rots, sizeof(float) * 9 * (size_t)n));
     FH_TRY(h, hipMalloc(&h->wts, sizeof(float) * n));
     FH_TRY(h, hipMalloc(&h->dfs, sizeof(float) * n));
+    FH_TRY(h, hipMalloc(&h->dfs2, sizeof(float) * n));
     FH_TRY(h, hipMalloc(&h->slices, sizeof(float2) * (size_t)n * P * (H + 1)));
     FH_TRY(h, hipMalloc(&h->t3d, sizeof(float) * 9 * ns));
     FH_TRY(h, hipMalloc(&h->wq, sizeof(float) * ns));
@@ -683,6 +686,7 @@ int fh_destroy(fh_reconstructor *h)
         free_bins(h);
         free_volume(h);
         if (h->ximgs) (void)hipFree(h->ximgs);
+        if (h->ximgs2) (void)hipFree(h->ximgs2);
         if (h->xfd) (void)hipFree(h->xfd);
         for (int i = 0; i < 2; i++) {
             if (h->pin[i]) (void)hipHostFree(h->pin[i]);
@@ -773,10 +777,31 @@ int fh_clear(fh_reconstructor *h)
 
 // prep kernels of n particles (box B) into h->slices; dfd: device defocus or null
 static hipError_t prep_kernels(fh_reconstructor *h, int n, int B, const float *dev_imgs, const float *dfd,
-                               const CtfConst &cc, const float *xfd = nullptr)
+                               const CtfConst &cc, const float *xfd = nullptr, const float *dfd2 = nullptr)
 {
     hipError_t e;
     if ((e = launch_edge_mean(dev_imgs, n, B, h->means, h->stream)) != hipSuccess) return e;
+    if (xfd && dfd) {
+        // the reference's order (e2make3dpar.py:698-716): the padded image is
+        // flipped first -- FFT, CTF_SIGN (times the second sign), inverse FFT
+        // -- and preprocess_slice then resamples the flipped image with the
+        // 2-D part: the whole prep with the flip, back to real space (phase
+        // origin to the centre, 1 / (P^2 scale)), the resampling, the prep
+        // again without flip
+        const int P = h->P;
+        if ((e = launch_prep_rows(dev_imgs, n, B, P, h->means, h->slices, h->plan, h->tw, h->stream)) != hipSuccess)
+            return e;
+        if ((e = launch_prep_cols(n, B, P, h->slices, dfd, dfd2, cc, h->scale, h->plan, h->tw, h->stream)) !=
+            hipSuccess)
+            return e;
+        const float inv = 1.0f / ((float)P * (float)P * h->scale);
+        if ((e = launch_ift2d_batch(h->slices, n, P, h->ximgs, inv, h->plan, h->tw, h->stream)) != hipSuccess) return e;
+        if ((e = launch_fill_f32(h->means, 0.f, n, h->stream)) != hipSuccess) return e;
+        if ((e = launch_xform2d(h->ximgs, n, P, P, h->means, xfd, h->ximgs2, h->stream)) != hipSuccess) return e;
+        if ((e = launch_prep_rows(h->ximgs2, n, P, P, h->means, h->slices, h->plan, h->tw, h->stream)) != hipSuccess)
+            return e;
+        return launch_prep_cols(n, P, P, h->slices, nullptr, nullptr, cc, h->scale, h->plan, h->tw, h->stream);
+    }
     if (xfd) {
         // pad + 2-D resampling into P x P images, then the prep of those (zero mean)
         if ((e = launch_xform2d(dev_imgs, n, B, h->P, h->means, xfd, h->ximgs, h->stream)) != hipSuccess) return e;
@@ -784,16 +809,24 @@ static hipError_t prep_kernels(fh_reconstructor *h, int n, int B, const float *d
         if ((e = launch_prep_rows(h->ximgs, n, h->P, h->P, h->means, h->slices, h->plan, h->tw, h->stream)) !=
             hipSuccess)
             return e;
-        return launch_prep_cols(n, h->P, h->P, h->slices, dfd, cc, h->scale, h->plan, h->tw, h->stream);
+        return launch_prep_cols(n, h->P, h->P, h->slices, nullptr, nullptr, cc, h->scale, h->plan, h->tw, h->stream);
     }
     if ((e = launch_prep_rows(dev_imgs, n, B, h->P, h->means, h->slices, h->plan, h->tw, h->stream)) != hipSuccess)
         return e;
-    return launch_prep_cols(n, B, h->P, h->slices, dfd, cc, h->scale, h->plan, h->tw, h->stream);
+    return launch_prep_cols(n, B, h->P, h->slices, dfd, dfd2, cc, h->scale, h->plan, h->tw, h->stream);
 }
 
-// staging of n 2-D transformed particles (P x P) and their inverse transforms
-static int ensure_ximgs(fh_reconstructor *h, int n)
+// staging of n 2-D transformed particles (P x P) and their inverse transforms;
+// two: also the second P x P buffer of the flip-then-resample order
+static int ensure_ximgs(fh_reconstructor *h, int n, bool two = false)
 {
+    if (two && n > h->ximgs2_cap) {
+        if (h->ximgs2) (void)hipFree(h->ximgs2);
+        h->ximgs2 = nullptr;
+        h->ximgs2_cap = 0;
+        FH_TRY(h, hipMalloc(&h->ximgs2, sizeof(float) * (size_t)n * h->P * h->P));
+        h->ximgs2_cap = n;
+    }
     if (n <= h->ximgs_cap) return FH_OK;
     if (h->ximgs) (void)hipFree(h->ximgs);
     if (h->xfd) (void)hipFree(h->xfd);
@@ -808,24 +841,30 @@ static int ensure_ximgs(fh_reconstructor *h, int n)
 
 // prep of n particles already in h->imgs (box B) into h->slices; xf_host:
 // their inverse 2-D transforms (n x 6) or null when all are identities
+// defocus2_host: the second defocus of the two-sign flip (or null)
 static int prep_chunk(fh_reconstructor *h, int n, int B, const float *dev_imgs, const float *defocus_host,
-                      const fh_ctf *ctf, const float *xf_host = nullptr)
+                      const fh_ctf *ctf, const float *xf_host = nullptr, const float *defocus2_host = nullptr)
 {
-    const float *dfd = nullptr;
+    const float *dfd = nullptr, *dfd2 = nullptr;
     CtfConst cc{};
     if (defocus_host && ctf) {
         FH_TRY(h, hipMemcpyAsync(h->dfs, defocus_host, sizeof(float) * n, hipMemcpyHostToDevice, h->stream));
         dfd = h->dfs;
         cc = ctf_constants(*ctf, h->P);
+        if (defocus2_host) {
+            FH_TRY(h, hipMemcpyAsync(h->dfs2, defocus2_host, sizeof(float) * n, hipMemcpyHostToDevice, h->stream));
+            dfd2 = h->dfs2;
+        }
     }
     const float *xfd = nullptr;
     if (xf_host) {
-        int rc = ensure_ximgs(h, n);
+        int rc = ensure_ximgs(h, n, dfd != nullptr);
         if (rc) return rc;
         FH_TRY(h, hipMemcpyAsync(h->xfd, xf_host, sizeof(float) * 6 * n, hipMemcpyHostToDevice, h->stream));
         xfd = h->xfd;
     }
-    FH_TRY(h, prep_kernels(h, n, B, dev_imgs, dfd, cc, xfd));
+    FH_TRY(h, prep_kernels(h, n, B, dev_imgs, dfd, cc, xfd, dfd2));
+    if (defocus2_host) FH_TRY(h, hipStreamSynchronize(h->stream));  // the caller's second-defocus array is transient
     return FH_OK;
 }
 
@@ -844,7 +883,7 @@ static int prep_single_raw(fh_reconstructor *h, const float *xf6)
         src = h->ximgs;
     }
     FH_TRY(h, launch_prep_rows(src, 1, P, P, h->means, h->slices, h->plan, h->tw, h->stream));
-    FH_TRY(h, launch_prep_cols(1, P, P, h->slices, nullptr, CtfConst{}, h->scale, h->plan, h->tw, h->stream));
+    FH_TRY(h, launch_prep_cols(1, P, P, h->slices, nullptr, nullptr, CtfConst{}, h->scale, h->plan, h->tw, h->stream));
     return FH_OK;
 }
 
@@ -857,11 +896,11 @@ static int prep_single_raw(fh_reconstructor *h, const float *xf6)
 // device.  The captured sequence holds kernels only (no memset nodes).
 static int insert_batch_graph(fh_reconstructor *h, const float *imgs, int n, int box, const std::vector<float> &rot9,
                               const float *weights, const float *defocus, const fh_ctf *ctf, int chunk,
-                              const float *rw_host, const float *xf_host)
+                              const float *rw_host, const float *xf_host, const float *df2_host)
 {
     int rc = ensure_partials(h, h->bins.item_cap);
     if (rc) return rc;
-    if (xf_host && (rc = ensure_ximgs(h, chunk))) return rc;
+    if (xf_host && (rc = ensure_ximgs(h, chunk, defocus && ctf))) return rc;
     const InsertConst c = iconst(h);
     fh_stats st{};
     const int wpb = h->tune.waves_per_block;
@@ -879,13 +918,17 @@ static int insert_batch_graph(fh_reconstructor *h, const float *imgs, int n, int
         if (xf_host)
             FH_TRY(h, hipMemcpyAsync(h->xfd, xf_host + 6 * (size_t)c0, sizeof(float) * 6 * m, hipMemcpyHostToDevice,
                                      h->stream));
-        const float *dfd = nullptr;
+        const float *dfd = nullptr, *dfd2 = nullptr;
         if (defocus && ctf) {
             FH_TRY(h, hipMemcpyAsync(h->dfs, defocus + c0, sizeof(float) * m, hipMemcpyHostToDevice, h->stream));
             dfd = h->dfs;
+            if (df2_host) {
+                FH_TRY(h, hipMemcpyAsync(h->dfs2, df2_host + c0, sizeof(float) * m, hipMemcpyHostToDevice, h->stream));
+                dfd2 = h->dfs2;
+            }
         }
         FH_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
-        hipError_t e = prep_kernels(h, m, box, dimgs, dfd, cc, xf_host ? h->xfd : nullptr);
+        hipError_t e = prep_kernels(h, m, box, dimgs, dfd, cc, xf_host ? h->xfd : nullptr, dfd2);
         if (e == hipSuccess) e = launch_compose(h->rots, h->wts, m, h->syms_d, h->nsym, h->t3d, h->wq, h->stream);
         if (e == hipSuccess)
             e = launch_bin(h->t3d, h->wq, m * h->nsym, c, h->geom, h->bins, h->tune.slices_per_item, h->stream);
@@ -955,16 +998,15 @@ static int check_xforms(fh_reconstructor *h, const float *xforms, int n, int nee
     return FH_OK;
 }
 
-// e2make3dpar.py:704-716 flips the CTF on the padded image before
-// preprocess_slice resamples it; the fused prep applies the flip in Fourier
-// space after the resampling, which equals it only for identity 2-D parts.
-static int check_xform_ctf(fh_reconstructor *h, const std::vector<float> &xf6, const float *defocus, const fh_ctf *ctf)
+// The second defocus of the two-sign flip (e2make3dpar.py:698-709,
+// e2spa_make3d.py:316-327): EMAN2Ctf's float defocus takes the Python sum
+// defocus + dfdf, i.e. the double sum rounded once.  Empty without dfdf.
+static void second_defocus(const float *defocus, const fh_ctf *ctf, int n, std::vector<float> &out)
 {
-    if (!xf6.empty() && defocus && ctf)
-        return fail(h, FH_ERR_INVALID_PARAM,
-                    "InvalidParameterException: CTF phase flip together with a 2-D translation/scale/mirror is not "
-                    "supported by fourier_hip (flip the particles first, as phase-flipped stacks are)");
-    return FH_OK;
+    out.clear();
+    if (!defocus || !ctf || !ctf->dfdf) return;
+    out.resize((size_t)std::max(n, 0));
+    for (int i = 0; i < n; i++) out[i] = (float)((double)defocus[i] + (double)ctf->dfdf[i]);
 }
 
 int fh_preprocess_slice_xform(fh_reconstructor *h, const float *img, const float xform[12], float *slice_out)
@@ -1127,19 +1169,23 @@ static int insert_batch_impl(fh_reconstructor *h, const float *imgs, int n, int 
     std::vector<float> rot9, xf6;
     rc = check_xforms(h, xforms, n, 1, rot9, &xf6);
     if (rc) return rc;
-    if ((rc = check_xform_ctf(h, xf6, defocus, ctf))) return rc;
     const float *xfh = xf6.empty() ? nullptr : xf6.data();
+    std::vector<float> df2;
+    second_defocus(defocus, ctf, n, df2);
+    const float *df2h = df2.empty() ? nullptr : df2.data();
+    // a CTF flip before a 2-D resampling stages two P x P images per particle
+    const long long xbytes = xfh ? ((defocus && ctf) ? 8LL : 4LL) * h->P * h->P : 0;
     // the brick list (at most 5 nb^2 entries per slice) is indexed with int
     const BrickGeom bg = brick_geom(h->P);
     const long long max_slices = std::max(1LL, (long long)INT32_MAX / (5LL * bg.nb * bg.nb) - 1);
-    const int chunk = (int)std::max(1LL, std::min({(long long)chunk_for(h->tune, h->P, box, xfh ? 4LL * h->P * h->P : 0),
+    const int chunk = (int)std::max(1LL, std::min({(long long)chunk_for(h->tune, h->P, box, xbytes),
                                                    (long long)std::max(n, 1), max_slices / h->nsym}));
     rc = ensure_chunk(h, chunk, box);
     if (rc) return rc;
     rc = ensure_bins(h, (long long)chunk * h->nsym);
     if (rc) return rc;
     if (h->tune.graph && where == FH_DEVICE && h->mode < 2)
-        return insert_batch_graph(h, imgs, n, box, rot9, weights, defocus, ctf, chunk, rwh, xfh);
+        return insert_batch_graph(h, imgs, n, box, rot9, weights, defocus, ctf, chunk, rwh, xfh, df2h);
     const InsertConst c = iconst(h);
     fh_stats st{};
     float ms;
@@ -1156,7 +1202,8 @@ static int insert_batch_impl(fh_reconstructor *h, const float *imgs, int n, int 
             FH_TRY(h, h2d_staged(h, h->imgs, src, sizeof(float) * (size_t)m * box * box));
             dimgs = h->imgs;
         }
-        rc = prep_chunk(h, m, box, dimgs, defocus ? defocus + c0 : nullptr, ctf, xfh ? xfh + 6 * (size_t)c0 : nullptr);
+        rc = prep_chunk(h, m, box, dimgs, defocus ? defocus + c0 : nullptr, ctf, xfh ? xfh + 6 * (size_t)c0 : nullptr,
+                        df2h ? df2h + c0 : nullptr);
         if (rc) return rc;
         FH_TRY(h, hipMemcpyAsync(h->rots, &rot9[9 * (size_t)c0], sizeof(float) * 9 * m, hipMemcpyHostToDevice, h->stream));
         FH_TRY(h, hipMemcpyAsync(h->wts, weights + c0, sizeof(float) * m, hipMemcpyHostToDevice, h->stream));
@@ -1252,6 +1299,8 @@ int fh_prep_batch(fh_reconstructor *h, const float *imgs, int n, int box, const 
     const int chunk = std::max(1, std::min(chunk_for(h->tune, P, box), std::max(n, 1)));
     int rc = ensure_chunk(h, chunk, box);
     if (rc) return rc;
+    std::vector<float> df2;
+    second_defocus(defocus, ctf, n, df2);
     for (int c0 = 0; c0 < n; c0 += chunk) {
         const int m = std::min(chunk, n - c0);
         const float *src = imgs + (size_t)c0 * box * box;
@@ -1260,7 +1309,8 @@ int fh_prep_batch(fh_reconstructor *h, const float *imgs, int n, int box, const 
             FH_TRY(h, h2d_staged(h, h->imgs, src, sizeof(float) * (size_t)m * box * box));
             dimgs = h->imgs;
         }
-        rc = prep_chunk(h, m, box, dimgs, defocus ? defocus + c0 : nullptr, ctf);
+        rc = prep_chunk(h, m, box, dimgs, defocus ? defocus + c0 : nullptr, ctf, nullptr,
+                        df2.empty() ? nullptr : df2.data() + c0);
         if (rc) return rc;
         FH_TRY(h, hipMemcpyAsync(slices_out + (size_t)c0 * P * (P + 2), h->slices, sizeof(float2) * (size_t)m * P * (H + 1),
                                  where == FH_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, h->stream));
@@ -1281,7 +1331,8 @@ int fh_insert_plain(fh_reconstructor *h, const float *imgs, int n, int box, cons
     std::vector<float> rot9, xf6;
     rc = check_xforms(h, xforms, n, 1, rot9, &xf6);
     if (rc) return rc;
-    if ((rc = check_xform_ctf(h, xf6, defocus, ctf))) return rc;
+    std::vector<float> df2;
+    second_defocus(defocus, ctf, n, df2);
     rc = ensure_chunk(h, std::max(1, h->cap_n), std::max(box, h->cap_B));
     if (rc) return rc;
     for (int i = 0; i < n; i++) {
@@ -1289,7 +1340,7 @@ int fh_insert_plain(fh_reconstructor *h, const float *imgs, int n, int box, cons
         // per-slice host->device copy (reconstructor.cpp:1074-1078)
         FH_TRY(h, hipMemcpy(h->imgs, imgs + (size_t)i * box * box, sizeof(float) * box * box, hipMemcpyHostToDevice));
         rc = prep_chunk(h, 1, box, h->imgs, defocus ? defocus + i : nullptr, ctf,
-                        xf6.empty() ? nullptr : &xf6[6 * (size_t)i]);
+                        xf6.empty() ? nullptr : &xf6[6 * (size_t)i], df2.empty() ? nullptr : &df2[i]);
         if (rc) return rc;
         // one launch per symmetry op, synchronised (cuda_reconstructor.cu:171-193)
         rc = insert_slice_plain(h, h->slices, &rot9[9 * (size_t)i], weights[i], true);
@@ -1435,9 +1486,11 @@ int fh_agreement_batch(fh_reconstructor *h, const float *imgs, int n, int box, c
     if (seq && (rc = ensure_partial(h))) return rc;
     std::vector<float> rot9, xf6;
     if ((rc = check_xforms(h, xforms, n, 1, rot9, &xf6))) return rc;
-    if ((rc = check_xform_ctf(h, xf6, defocus, ctf))) return rc;
     const float *xfh = xf6.empty() ? nullptr : xf6.data();
-    const int chunk = std::max(1, std::min(chunk_for(h->tune, h->P, box, xfh ? 4LL * h->P * h->P : 0), std::max(n, 1)));
+    std::vector<float> df2;
+    second_defocus(defocus, ctf, n, df2);
+    const long long xbytes = xfh ? ((defocus && ctf) ? 8LL : 4LL) * h->P * h->P : 0;
+    const int chunk = std::max(1, std::min(chunk_for(h->tune, h->P, box, xbytes), std::max(n, 1)));
     if ((rc = ensure_chunk(h, chunk, box))) return rc;
     if (!h->acc8 || h->acc8_cap < chunk) {
         if (h->acc8) (void)hipFree(h->acc8);
@@ -1457,7 +1510,8 @@ int fh_agreement_batch(fh_reconstructor *h, const float *imgs, int n, int box, c
             FH_TRY(h, h2d_staged(h, h->imgs, src, sizeof(float) * (size_t)m * box * box));
             dimgs = h->imgs;
         }
-        if ((rc = prep_chunk(h, m, box, dimgs, defocus ? defocus + c0 : nullptr, ctf, xfh ? xfh + 6 * (size_t)c0 : nullptr)))
+        if ((rc = prep_chunk(h, m, box, dimgs, defocus ? defocus + c0 : nullptr, ctf, xfh ? xfh + 6 * (size_t)c0 : nullptr,
+                             df2.empty() ? nullptr : df2.data() + c0)))
             return rc;
         if (seq) {
             for (int i = 0; i < m; i++)
@@ -1519,6 +1573,64 @@ int fh_finish(fh_reconstructor *h, int doift, float *out, float *norm_out)
     (void)hipFree(planes);
     if (e != hipSuccess) return fail(h, FH_ERR_DEVICE, std::string("finish: ") + hipGetErrorString(e));
     free_volume(h);  // ownership of the map passes to the caller (reconstructor.cpp:1633-1640)
+    return FH_OK;
+}
+
+int fh_merge_add(fh_reconstructor *h, const float *threed, const float *normv, int where)
+{
+    if (!h) return FH_ERR_NULL;
+    if (!threed || !normv) return fail(h, FH_ERR_NULL, "NullPointerException: task volume or normout is NULL");
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (!h->ready) return fail(h, FH_ERR_STATE, "merge_add() before setup()");
+    FH_TRY(h, hipSetDevice(h->device));
+    const size_t n = vol_elems(h->P);
+    const float2 *td = reinterpret_cast<const float2 *>(threed);
+    const float *nd = normv;
+    void *tmp = nullptr;
+    if (where != FH_DEVICE) {
+        // staged through one temporary device buffer (12 bytes per stored voxel)
+        hipError_t e = hipMalloc(&tmp, (sizeof(float2) + sizeof(float)) * n);
+        if (e != hipSuccess) return fail(h, FH_ERR_NOMEM, std::string("merge staging: ") + hipGetErrorString(e));
+        float2 *t2 = static_cast<float2 *>(tmp);
+        float *n2 = reinterpret_cast<float *>(t2 + n);
+        e = hipMemcpyAsync(t2, threed, sizeof(float2) * n, hipMemcpyHostToDevice, h->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(n2, normv, sizeof(float) * n, hipMemcpyHostToDevice, h->stream);
+        if (e != hipSuccess) {
+            (void)hipFree(tmp);
+            return fail(h, FH_ERR_DEVICE, std::string("merge staging: ") + hipGetErrorString(e));
+        }
+        td = t2;
+        nd = n2;
+    }
+    hipError_t e = launch_merge_add(h->data, h->norm, td, nd, n, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (tmp) (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(h, FH_ERR_DEVICE, std::string("merge_add: ") + hipGetErrorString(e));
+    return FH_OK;
+}
+
+int fh_merge_finish(fh_reconstructor *h, float *out)
+{
+    if (!h) return FH_ERR_NULL;
+    if (!out) return fail(h, FH_ERR_NULL, "NullPointerException: output buffer is NULL");
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (!h->ready) return fail(h, FH_ERR_STATE, "merge_finish() before setup()");
+    FH_TRY(h, hipSetDevice(h->device));
+    const int P = h->P;
+    const size_t n = vol_elems(P);
+    hipError_t e = launch_merge_scale(h->data, h->norm, n, h->stream);
+    if (e == hipSuccess) e = launch_fft3d_inverse_zy(h->data, P, h->plan, h->tw, h->stream);
+    float *real = nullptr;
+    if (e == hipSuccess) e = hipMalloc(&real, sizeof(float) * (size_t)P * P * P);
+    if (e == hipSuccess) {
+        const float sc = 1.0f / (float)((size_t)P * P * P);
+        e = launch_c2r_x(h->data, P, real, sc, h->plan, h->tw, h->stream);
+        if (e == hipSuccess) e = d2h_staged(h, out, real, sizeof(float) * (size_t)P * P * P);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        (void)hipFree(real);
+    }
+    if (e != hipSuccess) return fail(h, FH_ERR_DEVICE, std::string("merge_finish: ") + hipGetErrorString(e));
+    free_volume(h);
     return FH_OK;
 }
 

@@ -92,8 +92,10 @@ struct BrickGeom {
 hipError_t launch_edge_mean(const float *imgs, int n, int B, float *means, hipStream_t s);
 hipError_t launch_prep_rows(const float *imgs, int n, int B, int P, const float *means, float2 *slices,
                             const FftPlan &plan, const float2 *tw, hipStream_t s);
-hipError_t launch_prep_cols(int n, int B, int P, float2 *slices, const float *defocus, CtfConst cc,
-                            float scale, const FftPlan &plan, const float2 *tw, hipStream_t s);
+hipError_t launch_prep_cols(int n, int B, int P, float2 *slices, const float *defocus, const float *defocus2,
+                            CtfConst cc, float scale, const FftPlan &plan, const float2 *tw, hipStream_t s);
+hipError_t launch_ift2d_batch(float2 *data, int n, int P, float *out, float scale, const FftPlan &plan,
+                              const float2 *tw, hipStream_t s);
 // ---- finish (prep_kernels.hip) ----
 hipError_t launch_normalize(float2 *data, float *norm, int P, int sqrtnorm, float *plane_tmp,
                             hipStream_t s);
@@ -101,6 +103,9 @@ hipError_t launch_fft3d_inverse_zy(float2 *data, int P, const FftPlan &plan, con
                                    hipStream_t s);
 hipError_t launch_c2r_x(const float2 *data, int P, float *out, float scale, const FftPlan &plan,
                         const float2 *tw, hipStream_t s);
+hipError_t launch_merge_add(float2 *data, float *norm, const float2 *threed, const float *normv, size_t n,
+                            hipStream_t s);
+hipError_t launch_merge_scale(float2 *data, const float *norm, size_t n, hipStream_t s);
 hipError_t launch_fill_f32(float *p, float v, size_t n, hipStream_t s);
 hipError_t launch_fill_i32(int *p, int v, size_t n, hipStream_t s);
 hipError_t launch_seed_weights(float *norm, int P, float w, hipStream_t s);

@@ -288,9 +288,12 @@ __device__ __forceinline__ float ctf_sign(int x, int y, float defocus, const Ctf
     return cos(t) < 0 ? -1.0f : 1.0f;
 }
 
+// defocus2 (optional): a second defocus per particle; the flip is then the
+// product of the two signs (the dfdf flip of e2make3dpar.py:698-709)
 template <int T, int NF>
 __global__ void __launch_bounds__(256) k_prep_cols(int B, int Pr, int off, float2 *__restrict__ slices,
-                                                   const float *__restrict__ defocus, CtfConst cc, float scale,
+                                                   const float *__restrict__ defocus,
+                                                   const float *__restrict__ defocus2, CtfConst cc, float scale,
                                                    FftPlan plan, const float2 *__restrict__ twg)
 {
     const int P = NF ? NF : Pr;
@@ -325,6 +328,7 @@ __global__ void __launch_bounds__(256) k_prep_cols(int B, int Pr, int off, float
     if constexpr (NF != 0) fft_fixed<T, NF>(A, tw);
     else R = fft_tile<T>(A, Bf, plan, tw);
     const float df = defocus ? defocus[p] : 0.f;
+    const float df2 = defocus2 ? defocus2[p] : 0.f;
     // the sign depends on x^2 + y^2 only: rows y and P - y (signed y and -y,
     // 0 < y < P/2) share one evaluation
     for (int idx = threadIdx.x; idx < T * (H + 1); idx += blockDim.x) {
@@ -334,7 +338,8 @@ __global__ void __launch_bounds__(256) k_prep_cols(int B, int Pr, int off, float
         if (x > H) continue;
         // (v * (sg + 0i)) * scale with sg = +-1 equals v * (sg * scale) exactly
         // (up to the sign of a zero)
-        const float ss = defocus ? ctf_sign(x, y, df, cc) * scale : scale;
+        float ss = defocus ? ctf_sign(x, y, df, cc) * scale : scale;
+        if (defocus2) ss = ss * ctf_sign(x, y, df2, cc);
 #pragma unroll
         for (int m = 0; m < 2; m++) {
             const int row = m == 0 ? y : P - y;
@@ -345,8 +350,8 @@ __global__ void __launch_bounds__(256) k_prep_cols(int B, int Pr, int off, float
     }
 }
 
-hipError_t launch_prep_cols(int n, int B, int P, float2 *slices, const float *defocus, CtfConst cc,
-                            float scale, const FftPlan &plan, const float2 *tw, hipStream_t s)
+hipError_t launch_prep_cols(int n, int B, int P, float2 *slices, const float *defocus, const float *defocus2,
+                            CtfConst cc, float scale, const FftPlan &plan, const float2 *tw, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
     const int d = B - P;
@@ -358,16 +363,18 @@ hipError_t launch_prep_cols(int n, int B, int P, float2 *slices, const float *de
     dim3 grid((H + 1 + T - 1) / T, n);
 #define FH_COLS_CASE(TT, NN)                                                                                      \
     if (T == TT && P == NN) {                                                                                     \
-        hipLaunchKernelGGL((k_prep_cols<TT, NN>), grid, dim3(256), lds, s, B, P, off, slices, defocus, cc, scale, plan, \
-                           tw);                                                                                   \
+        hipLaunchKernelGGL((k_prep_cols<TT, NN>), grid, dim3(256), lds, s, B, P, off, slices, defocus, defocus2, cc, scale, \
+                           plan, tw);                                                                             \
         return hipGetLastError();                                                                                 \
     }
     FH_FIXED_PADS_COLS(FH_COLS_CASE)
 #undef FH_COLS_CASE
     if (T == 8)
-        hipLaunchKernelGGL((k_prep_cols<8, 0>), grid, dim3(256), lds, s, B, P, off, slices, defocus, cc, scale, plan, tw);
+        hipLaunchKernelGGL((k_prep_cols<8, 0>), grid, dim3(256), lds, s, B, P, off, slices, defocus, defocus2, cc, scale,
+                           plan, tw);
     else
-        hipLaunchKernelGGL((k_prep_cols<4, 0>), grid, dim3(256), lds, s, B, P, off, slices, defocus, cc, scale, plan, tw);
+        hipLaunchKernelGGL((k_prep_cols<4, 0>), grid, dim3(256), lds, s, B, P, off, slices, defocus, defocus2, cc, scale,
+                           plan, tw);
     return hipGetLastError();
 }
 
@@ -492,8 +499,9 @@ hipError_t launch_fft3d_inverse_zy(float2 *data, int P, const FftPlan &plan, con
 
 // c2r along x for row pairs, x 1/P^3 (do_ift_inplace, emdata_transform.cpp:392-425),
 // depad, and phase origin to centre (fftshift) on the write.
-// nz planes of P rows; shift: fftshift of the written volume (3-D finish),
-// else rows written in place (2-D projection).
+// nz planes of P rows; shift 1: fftshift of the written volume (3-D finish),
+// shift 2: fftshift of each plane (a batch of 2-D images), 0: rows written
+// in place (2-D projection).
 template <int T>
 __global__ void __launch_bounds__(256) k_c2r_x(const float2 *__restrict__ data, int P, int nz, int shift,
                                                float *__restrict__ out, float scale, FftPlan plan,
@@ -540,7 +548,7 @@ __global__ void __launch_bounds__(256) k_c2r_x(const float2 *__restrict__ data, 
             }
             const int y = (int)(r % P), zz = (int)(r / P);
             const int yc = y + H < P ? y + H : y + H - P;
-            const int zc = zz + H < P ? zz + H : zz + H - P;
+            const int zc = shift == 2 ? zz : (zz + H < P ? zz + H : zz + H - P);
             out[((size_t)zc * P + yc) * P + xc] = (q == 0 ? z.x : z.y) * scale;
         }
     }
@@ -582,6 +590,70 @@ hipError_t launch_ift2d(float2 *data, int P, float *out, const FftPlan &plan, co
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return c2r_rows(data, P, 1, 0, out, 1.0f / ((float)P * (float)P), plan, tw, s);
+}
+
+// do_ift of n complex images (P x (P/2+1) each, in place: overwritten) into n
+// real P x P images with scale, phase origin to the centre: the inverse of the
+// row and column passes (the particle's flip back to real space)
+hipError_t launch_ift2d_batch(float2 *data, int n, int P, float *out, float scale, const FftPlan &plan,
+                              const float2 *tw, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const int H = P / 2;
+    const int T = pick_T(P);
+    const size_t lds = fft_lds_bytes(P, T);
+    const size_t row = H + 1;
+    dim3 grid((H + 1 + T - 1) / T, n);
+    if (T == 8)
+        hipLaunchKernelGGL(k_fft_axis_inv<8>, grid, dim3(256), lds, s, data, P, row, (size_t)P * row, plan, tw);
+    else
+        hipLaunchKernelGGL(k_fft_axis_inv<4>, grid, dim3(256), lds, s, data, P, row, (size_t)P * row, plan, tw);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return c2r_rows(data, P, n, 2, out, scale, plan, tw, s);
+}
+
+// ---------------------------------------------------------------- task merge
+// e2spa_make3d.py:180-184: a task's normalised volume times its normout
+// ("math.multamplitude": both floats of a voxel times its weight), added to
+// the sum; the normout added to the summed weights
+__global__ void k_merge_add(float2 *__restrict__ data, float *__restrict__ norm, const float2 *__restrict__ threed,
+                            const float *__restrict__ normv, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float w = normv[i];
+        const float2 t = threed[i];
+        const float2 d = data[i];
+        data[i] = make_float2(d.x + t.x * w, d.y + t.y * w);
+        norm[i] = norm[i] + w;
+    }
+}
+
+// e2spa_make3d.py:186-187: the summed weights' reciprocal ("math.reciprocal",
+// 0 where the weight is 0), times the summed volume
+__global__ void k_merge_scale(float2 *__restrict__ data, const float *__restrict__ norm, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float w = norm[i];
+        const float r = w == 0.0f ? 0.0f : 1.0f / w;
+        const float2 d = data[i];
+        data[i] = make_float2(d.x * r, d.y * r);
+    }
+}
+
+hipError_t launch_merge_add(float2 *data, float *norm, const float2 *threed, const float *normv, size_t n,
+                            hipStream_t s)
+{
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_merge_add, dim3(blocks), dim3(256), 0, s, data, norm, threed, normv, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_merge_scale(float2 *data, const float *norm, size_t n, hipStream_t s)
+{
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_merge_scale, dim3(blocks), dim3(256), 0, s, data, norm, n);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------- fills

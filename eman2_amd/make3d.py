@@ -121,6 +121,40 @@ def reconstruct(recon, imgs, xforms, weights=None, niter=2, keep=1.0, keepsig=Fa
     return out, info
 
 
+def spa_make3d(make_recon, imgs, xforms, weights=None, ntasks=4, defocus=None, ctf=None):
+    """e2spa_make3d.py's split and merge (:154-192, Make3dTask :351-395):
+    particle i goes to task i % ntasks; each task inserts its particles of
+    weight > 0 into its own reconstructor (make_recon() -> a reconstructor
+    not yet set up) and ends with finish(False) and its normout; the tasks'
+    volumes times their normouts and the normouts are summed on a fresh
+    reconstructor, the sum multiplied by the reciprocal of the summed
+    normouts and inverse-transformed.  Returns the map (P^3, numpy)."""
+    imgs = np.ascontiguousarray(imgs, np.float32)
+    n = len(imgs)
+    xf = np.ascontiguousarray(xforms, np.float32).reshape(n, 3, 4)
+    w = np.ones(n, np.float32) if weights is None else np.ascontiguousarray(weights, np.float32)
+    parts = []
+    for t in range(ntasks):
+        idx = np.arange(t, n, ntasks)
+        idx = idx[w[idx] > 0]                     # reconstruct() skips wt <= 0 (e2spa_make3d.py:304-306)
+        r = make_recon()
+        r.setup()
+        if len(idx):
+            c = ctf
+            if ctf is not None and ctf.get("dfdf") is not None:
+                c = dict(ctf, dfdf=np.asarray(ctf["dfdf"], np.float32)[idx])
+            r.insert_batch(imgs[idx], xf[idx], w[idx],
+                           defocus=None if defocus is None else np.asarray(defocus, np.float32)[idx], ctf=c)
+        normout = np.zeros((r.P, r.P, r.P // 2 + 1), np.float32)
+        r.set_params({"normout": normout})
+        parts.append((r.finish(False, return_numpy=True), normout))
+    m = make_recon()
+    m.setup()
+    for threed, nv in parts:
+        m.merge_add(threed, nv)
+    return m.merge_finish(return_numpy=True)
+
+
 def reconstruct_mrc(recon, stack_path, xforms, weights=None, out_path=None, **kw):
     """reconstruct() over a particle stack in an MRC file (memory-mapped,
     one particle per section); the map goes to out_path when given, with

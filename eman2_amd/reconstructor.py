@@ -251,8 +251,9 @@ class FourierHipReconstructor:
                                                               _ptr(ss), int(ss.size)))
         return self._check(self._lib.fh_insert_slice(self._need(), _ptr(a), pre, _ptr(m), float(weight)))
 
-    def _batch_args(self, imgs, xforms, weights, defocus, ctf):
+    def _batch_args(self, imgs, xforms, weights, defocus, ctf, nprep=None):
         n = len(xforms)
+        nd = n if nprep is None else nprep
         if isinstance(xforms, np.ndarray) and xforms.ndim == 3 and xforms.shape[1:] == (3, 4):
             m = np.ascontiguousarray(xforms, np.float32).reshape(n, 12)
         elif isinstance(xforms, np.ndarray) and xforms.ndim == 2 and xforms.shape[1] == 12:
@@ -265,6 +266,13 @@ class FourierHipReconstructor:
         if ctf is not None:
             c = _lib.Ctf(float(ctf.get("voltage", 300.0)), float(ctf.get("cs", 2.7)),
                          float(ctf.get("ampcont", 10.0)), float(ctf["apix"]))
+            if ctf.get("dfdf") is not None:
+                # per-particle defocus offsets: the flip is sign(defocus) * sign(defocus + dfdf)
+                dd = np.ascontiguousarray(ctf["dfdf"], np.float32).reshape(-1)
+                if dd.shape[0] != nd:
+                    raise _lib.FourierHipError(1, "ctf['dfdf'] must hold one offset per particle")
+                c._keep = dd
+                c.dfdf = dd.ctypes.data
         return n, m, w, d, c
 
     def insert_batch(self, imgs, xforms, weights=None, defocus=None, ctf=None, box=None, class_ssnr=None):
@@ -298,7 +306,7 @@ class FourierHipReconstructor:
         """Batched caller prep + preprocess_slice: (n,box,box) -> (n,P,P+2)."""
         a = _arr(imgs)
         n = a.shape[0]
-        _, _, _, d, c = self._batch_args(None, [None] * 0, None, defocus, ctf)
+        _, _, _, d, c = self._batch_args(None, [None] * 0, None, defocus, ctf, nprep=n)
         out = np.zeros((n, self.P, self.P + 2), np.float32)
         self._check(self._lib.fh_prep_batch(self._need(), _ptr(a), n, a.shape[-1],
                                             None if d is None else _ptr(d),
@@ -376,6 +384,25 @@ class FourierHipReconstructor:
         if return_numpy:
             return out
         return EMData(out, complex_=not doift)
+
+    # ---------------------------------------------------------------- task merge
+    def merge_add(self, threed, normout):
+        """e2spa_make3d.py:180-184 on this (set up) reconstructor: a task's
+        finish(False) volume times its normout added to the sum, the normout
+        to the summed weights"""
+        t = _arr(threed.numpy() if isinstance(threed, EMData) else threed)
+        w = _arr(normout.numpy() if isinstance(normout, EMData) else normout)
+        P = self.P
+        if t.shape != (P, P, P + 2) or w.shape != (P, P, P // 2 + 1):
+            raise _lib.FourierHipError(3, "merge_add: volume (P, P, P+2) and normout (P, P, P/2+1) expected")
+        self._check(self._lib.fh_merge_add(self._need(), _ptr(t), _ptr(w), _lib.FH_HOST))
+
+    def merge_finish(self, return_numpy=False):
+        """e2spa_make3d.py:186-192: the merged map (P^3, phase origin centred)"""
+        P = self.P
+        out = np.zeros((P, P, P), np.float32)
+        self._check(self._lib.fh_merge_finish(self._need(), _ptr(out)))
+        return out if return_numpy else EMData(out)
 
     def get_state(self):
         P = self.P

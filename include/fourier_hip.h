@@ -65,12 +65,18 @@ typedef struct {
 } fh_params;
 
 /* CTF constants shared by a batch; defocus comes per particle.  Used for the
- * CTF_SIGN phase flip of e2make3dpar.py:704-714 (ctf.cpp:1076-1090). */
+ * CTF_SIGN phase flip of e2make3dpar.py:704-714 (ctf.cpp:1076-1090): the
+ * padded particle's FFT times CTF_SIGN at its defocus, back to real space,
+ * before preprocess_slice's 2-D resampling.  With dfdf (n values, host) the
+ * flip is the product of the signs at defocus and at defocus + dfdf, the
+ * defocus-offset flip of e2make3dpar.py:698-709 and e2spa_make3d.py:316-327
+ * (where dfdf is 0 the two signs cancel: no flip). */
 typedef struct {
-    float voltage;  /* kV */
-    float cs;       /* mm */
-    float ampcont;  /* % amplitude contrast */
-    float apix;     /* A/pixel */
+    float voltage;       /* kV */
+    float cs;            /* mm */
+    float ampcont;       /* % amplitude contrast */
+    float apix;          /* A/pixel */
+    const float *dfdf;   /* optional per-particle defocus offsets (um), or NULL */
 } fh_ctf;
 
 /* Tuning knobs of the batched insertion (picked by the autotuner).  The
@@ -181,6 +187,16 @@ int fh_determine_slice_agreement(fh_reconstructor *h, const float *data, int is_
 int fh_agreement_batch(fh_reconstructor *h, const float *imgs, int n, int box,
                        const float *xforms, const float *weights, const float *defocus,
                        const fh_ctf *ctf, int sub, float apix, float *out, int where);
+
+/* The task merge of e2spa_make3d.py:175-192: after fh_setup(h), each task's
+ * finish(doift=0) volume `threed` (P x P x (P+2) floats) times its normout
+ * (P x P x (P/2+1), fh_finish's norm_out) is added to h's volume and the
+ * normout to h's weights (host or device arrays per `where`); then
+ * fh_merge_finish multiplies the sum by the reciprocal of the summed weights
+ * (0 where 0), runs do_ift_inplace, depad and the phase origin to the centre
+ * into out (host, P^3 floats) and releases the volume, as fh_finish does. */
+int fh_merge_add(fh_reconstructor *h, const float *threed, const float *normv, int where);
+int fh_merge_finish(fh_reconstructor *h, float *out);
 
 /* FourierReconstructor::projection (reconstructor.cpp:1383-1445): the
  * Fourier slice of the accumulating volume at the rotation of xform (value

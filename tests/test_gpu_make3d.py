@@ -88,3 +88,25 @@ def test_make3d_from_mrc_stack(tmp_path):
     g = ref.numpy() if hasattr(ref, "numpy") else np.asarray(ref)
     assert h["nx"] == P and h["apix"] == 1.0
     assert max_rel_err(np.asarray(vol), g) <= 1e-4
+
+
+def test_spa_make3d_task_merge_equals_one_reconstructor():
+    """e2spa_make3d's split into tasks, each finished to its normalised
+    volume and normout, merged by summing volume x normout and normout, equals
+    one reconstructor over all particles (the weight fix-ups are linear)"""
+    from eman2_amd import Reconstructors
+    P, B, n = 48, 32, 400
+    ph = S.phantom(12, 9.0, seed=71)
+    ctf = dict(voltage=300.0, cs=2.7, ampcont=10.0, apix=1.0)
+    imgs, mats, dfs = S.particles(ph, n, B, seed=72, snr=0.5, ctf=dict(ctf, bfactor=80.0))
+    w = np.ones(n, np.float32)
+    w[::13] = 0.0
+    mk = lambda: Reconstructors.get("fourier_hip", dict(size=(P, P, P), sym="d2", mode="gauss_2"))  # noqa: E731
+    merged = make3d.spa_make3d(mk, imgs, mats, w, ntasks=3, defocus=dfs, ctf=ctf)
+    one = mk()
+    one.setup()
+    one.insert_batch(imgs, mats, w, defocus=dfs, ctf=ctf)
+    ref = one.finish(True, return_numpy=True)
+    e, f = max_rel_err(merged, ref), fsc(merged, ref).min()
+    print("spa merge: max_rel_err %.2e min FSC %.6f" % (e, f), flush=True)
+    assert e <= 1e-4 and f >= 0.9999

@@ -311,14 +311,21 @@ def test_2d_transforms_in_agreement(oracle):
             assert abs(got[i, j] - a[k]) <= 1e-4 * max(1.0, abs(a[k])), (i, k, got[i, j], a[k])
 
 
-def test_2d_transform_with_ctf_rejected():
-    from eman2_amd import _lib
-    P, B = 48, 32
+def test_2d_transform_with_ctf(oracle):
+    """a CTF flip with translations, scale and mirror at a pad of the runtime
+    FFT plan (48): the padded particle flipped first, then resampled
+    (e2make3dpar.py:698-716), as the oracle does"""
+    P, B, n = 48, 32, 16
     ph = S.phantom(5, 10.0, seed=14)
-    imgs, mats, dfs = S.particles(ph, 4, B, seed=15, snr=1.0, ctf=dict(CTF))
+    imgs, mats, dfs = S.particles(ph, n, B, seed=15, snr=1.0, ctf=dict(CTF))
+    xfs = _with_2d(mats, 16, True)
     r = recon(P)
-    with pytest.raises(_lib.FourierHipError, match="InvalidParameterException"):
-        r.insert_batch(imgs, list(_with_2d(mats, 16)), np.ones(4, np.float32), defocus=dfs, ctf=CTF)
+    r.insert_batch(imgs, list(xfs), np.ones(n, np.float32), defocus=dfs, ctf=CTF)
+    gd, gn = r.get_state()
+    v = oracle_volume(oracle, P, "c1", "gauss_2",
+                      [oracle.prep_particle(imgs[i], P, dict(CTF, defocus=float(dfs[i])), xform=xfs[i])
+                       for i in range(n)], xfs, np.ones(n, np.float32))
+    assert_state_close(gd, gn, v.data, v.norm)
 
 
 @pytest.mark.parametrize("P,B", [(56, 40), (88, 60), (104, 64), (168, 112)])

@@ -134,3 +134,32 @@ def test_footprint_counts_every_symmetry_operator(sym, nsym):
     assert L.fh_get_symmetries(sym.encode(), None, 0) == nsym
     want = L.fh_footprint_bytes(P, nsym, m) + 4 * m * B * B + 4 * P ** 3 + 8 * P * P
     assert fp == want, (fp, want)
+
+
+def test_alone_equals_mix_with_pad480_job():
+    """a box-320 / pad-480 job of the serving mix (compile-time FFT plans of
+    pads 192 and 480) gives the same map alone and beside smaller jobs"""
+    from eman2_amd.serving import Server
+    specs = [(480, 320, 120, "d2"), (192, 128, 300, "c4"), (96, 64, 400, "c1")]
+    jobs = []
+    for i, (P, B, n, sym) in enumerate(specs):
+        ph = S.phantom(30, B * 0.3, seed=300 + i)
+        imgs, mats, dfs = S.particles(ph, n, B, seed=400 + i, snr=0.1, ctf={})
+        jobs.append(dict(size=P, imgs=imgs, xforms=mats, defocus=dfs, ctf=CTF, sym=sym))
+    alone = []
+    s1 = Server(max_active=1, chunk=128)
+    for j in jobs:
+        jid = s1.submit(**j)
+        s1.drain()
+        assert s1.poll(jid)["state"] == "done"
+        alone.append(s1.result(jid).copy())
+    s1.close()
+    s3 = Server(max_active=3, chunk=64)
+    ids = [s3.submit(**j) for j in jobs]
+    s3.drain()
+    for jid, a in zip(ids, alone):
+        assert s3.poll(jid)["state"] == "done"
+        m = s3.result(jid)
+        print("job", jid, max_rel_err(m, a), fsc(m, a).min(), flush=True)
+        assert max_rel_err(m, a) <= 1e-4 and fsc(m, a).min() >= 0.9999
+    s3.close()
