@@ -1448,6 +1448,120 @@ static int ensure_partial(fh_reconstructor *h)
     return FH_OK;
 }
 
+// ---- allocation-free agreement ----
+// Scratch of fh_determine_slice_agreement_noalloc: the per-operator
+// rotations and weights, the compare kernel's per-block partial sums and
+// their eight totals, in one file-scope device array.  Each device holds its
+// own copy at its own address (hipGetSymbolAddress, resolved once per device
+// with that device current); host code uses only that address, never the
+// array's name.  Every call on a device shares the device's copy: calls that
+// overlap on one device race.  Nothing in it needs clearing: every slot a
+// call reads was written earlier in the same call.
+constexpr size_t NOALLOC_HEAD = 4096;             // t3d (64 x 9 floats), weights (64 floats), 8 doubles
+constexpr size_t NOALLOC_PART = 2u << 20;         // partial sums: 8 doubles per (block, operator)
+constexpr int NOALLOC_MAX_SYM = 64;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_HEAD + NOALLOC_PART];
+
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[64];
+
+// the address of `scratch` on h's device (current), resolved on first use
+static int scratch_base(fh_reconstructor *h, unsigned char **out)
+{
+    if (h->device < 0 || h->device >= 64) return fail(h, FH_ERR_INVALID_PARAM, "InvalidParameterException: device index");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[h->device]) {
+        void *ptr = nullptr;
+        FH_TRY(h, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return fail(h, FH_ERR_DEVICE, "device error: scratch symbol has no address");
+        g_scratch_ptr[h->device] = ptr;
+    }
+    *out = static_cast<unsigned char *>(g_scratch_ptr[h->device]);
+    return FH_OK;
+}
+
+// the eight sums over n partial records of 8 doubles, in one workgroup
+__global__ void __launch_bounds__(256) k_sum_partials8(const double *__restrict__ part, int n, double *__restrict__ out8)
+{
+    __shared__ double red[256];
+    const int k = threadIdx.x & 7, lane = threadIdx.x >> 3;  // 32 strided partial sums per component
+    double a = 0.0;
+    for (int i = lane; i < n; i += 32) a += part[(size_t)i * 8 + k];
+    red[threadIdx.x] = a;
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        double t = 0.0;
+        for (int j = 0; j < 32; j++) t += red[j * 8 + threadIdx.x];
+        out8[threadIdx.x] = t;
+    }
+}
+
+int fh_determine_slice_agreement_noalloc(fh_reconstructor *h, const float *slice_dev, const float xform[12],
+                                         float weight, int sub, float apix, float out[5])
+{
+    if (!h) return FH_ERR_NULL;
+    if (!slice_dev || !xform || !out) return fail(h, FH_ERR_NULL, "NullPointerException: slice, xform or out is NULL");
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (!h->ready) return fail(h, FH_ERR_STATE, "determine_slice_agreement() before setup()");
+    FH_TRY(h, hipSetDevice(h->device));
+    unsigned char *base = nullptr;
+    int rc = scratch_base(h, &base);
+    if (rc) return rc;
+    const int nb = agree_blocks(h->P), ns = h->nsym;
+    const size_t need = sizeof(double) * 8 * (size_t)nb * ns;
+    if (ns > NOALLOC_MAX_SYM || need > NOALLOC_PART)
+        return fail(h, FH_ERR_INVALID_PARAM,
+                    "InvalidParameterException: the allocation-free agreement holds at most 2 MiB of partial sums "
+                    "(8 doubles per compare block and symmetry operator) and 64 operators; this size and "
+                    "symmetry need " + std::to_string(need) + " bytes and " + std::to_string(ns) + " operators");
+    float *t3d_d = reinterpret_cast<float *>(base);
+    float *wq_d = reinterpret_cast<float *>(base + sizeof(float) * 9 * NOALLOC_MAX_SYM);
+    double *sum8_d = reinterpret_cast<double *>(base + sizeof(float) * 10 * NOALLOC_MAX_SYM);
+    double *part_d = reinterpret_cast<double *>(base + NOALLOC_HEAD);
+    const float2 *slice_d = reinterpret_cast<const float2 *>(slice_dev);
+    // host arrays live until the final synchronisation: the copies below read them
+    float rot9[9], t[9 * NOALLOC_MAX_SYM], wneg[NOALLOC_MAX_SYM], wpos[NOALLOC_MAX_SYM];
+    rotation_only(xform, rot9);
+    for (int s = 0; s < ns; s++) {
+        compose9(rot9, &h->syms[9 * s], &t[9 * s]);
+        wneg[s] = -weight;
+        wpos[s] = weight;
+    }
+    InsertConst c = iconst(h);
+    c.corners = 0;  // do_insert_slice_work default (reconstructor.h:540)
+    FH_TRY(h, hipMemcpyAsync(t3d_d, t, sizeof(float) * 9 * ns, hipMemcpyHostToDevice, h->stream));
+    FH_TRY(h, hipMemcpyAsync(wq_d, wneg, sizeof(float) * ns, hipMemcpyHostToDevice, h->stream));
+    if (sub) {
+        if (h->mode >= 2)
+            FH_TRY(h, launch_insert_plain_batch(slice_d, ns, ns, t3d_d, wq_d, c, h->data, h->norm, h->exptab, h->rtab,
+                                                nullptr, h->stream));
+        else
+            for (int s = 0; s < ns; s++)
+                FH_TRY(h, launch_insert_plain(slice_d, c, &t[9 * s], -weight, h->data, h->norm, h->exptab, h->rtab,
+                                              nullptr, h->stream));
+    }
+    FH_TRY(h, launch_agree(slice_d, ns, t3d_d, ns, c, agree_rmax(h->P, apix), h->data, h->norm, h->rtab, part_d,
+                           h->stream));
+    hipLaunchKernelGGL(k_sum_partials8, dim3(1), dim3(256), 0, h->stream, part_d, nb * ns, sum8_d);
+    FH_TRY(h, hipGetLastError());
+    double a[8];
+    FH_TRY(h, hipMemcpyAsync(a, sum8_d, sizeof(a), hipMemcpyDeviceToHost, h->stream));
+    if (sub) {
+        // the weights of the re-add, stream-ordered behind the removal's reads of wq_d
+        FH_TRY(h, hipMemcpyAsync(wq_d, wpos, sizeof(float) * ns, hipMemcpyHostToDevice, h->stream));
+        if (h->mode >= 2)
+            FH_TRY(h, launch_insert_plain_batch(slice_d, ns, ns, t3d_d, wq_d, c, h->data, h->norm, h->exptab, h->rtab,
+                                                nullptr, h->stream));
+        else
+            for (int s = 0; s < ns; s++)
+                FH_TRY(h, launch_insert_plain(slice_d, c, &t[9 * s], weight, h->data, h->norm, h->exptab, h->rtab,
+                                              nullptr, h->stream));
+    }
+    FH_TRY(h, hipStreamSynchronize(h->stream));
+    agree_outputs(a, weight, out);
+    return FH_OK;
+}
+
 int fh_determine_slice_agreement(fh_reconstructor *h, const float *data, int is_preprocessed, const float xform[12],
                                  float weight, int sub, float apix, float out[5])
 {

@@ -188,6 +188,21 @@ int fh_agreement_batch(fh_reconstructor *h, const float *imgs, int n, int box,
                        const float *xforms, const float *weights, const float *defocus,
                        const fh_ctf *ctf, int sub, float apix, float *out, int where);
 
+/* No counterpart in the reference: determine_slice_agreement for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same outputs as
+ * fh_determine_slice_agreement on an already preprocessed slice, which the
+ * caller passes in device memory (slice_dev: P x (P+2) floats); out (host)
+ * as there.  It calls no allocator: its working memory (rotations, the
+ * compare kernel's partial sums) is a fixed 2 MiB + 4 KiB device array per
+ * GPU, so it refuses (FH_ERR_INVALID_PARAM) sizes and symmetries needing
+ * more than 2 MiB of partial sums (8 doubles per 256 compare pixels and
+ * symmetry operator) or more than 64 operators.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU race. */
+int fh_determine_slice_agreement_noalloc(fh_reconstructor *h, const float *slice_dev,
+                                         const float xform[12], float weight, int sub,
+                                         float apix, float out[5]);
+
 /* The task merge of e2spa_make3d.py:175-192: after fh_setup(h), each task's
  * finish(doift=0) volume `threed` (P x P x (P+2) floats) times its normout
  * (P x P x (P/2+1), fh_finish's norm_out) is added to h's volume and the
