@@ -64,6 +64,10 @@ def parse():
     ap.add_argument("--serve-jobs", type=int, default=40,
                     help="jobs in the serving measurement (0 = skip); Poisson arrivals at --serve-rate")
     ap.add_argument("--serve-rate", type=float, default=4.0)
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the other measured shapes (100,000 x 192 px d7 / pad 288; the refine flow at pad 600)")
+    ap.add_argument("--refine-particles", type=int, default=50000,
+                    help="particles of the refine-flow line measured after the main one")
     ap.add_argument("--workload", default="insert", choices=["insert", "refine"],
                     help="insert: batched insertion (the metric); refine: insert, slice agreement of every "
                          "particle, re-insertion of img x reconstruct_norm, finish() (400 px -> 600 by default)")
@@ -115,7 +119,11 @@ def cpu_baseline(imgs_host, mats, dfs, P, sym, mode, apix, budget_s):
     from oracle import oracle as O
 
     L = O.lib(fast=True)
-    T = max(1, min(16, len(os.sched_getaffinity(0))))
+    # every core this process may run on, within the host's CPU share per GPU
+    # (OMP_NUM_THREADS, which the GPU host sets to its share)
+    aff = len(os.sched_getaffinity(0))
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    T = max(1, min(aff, share) if share > 0 else aff)
     B = imgs_host.shape[-1]
 
     def prep(i):
@@ -151,6 +159,7 @@ def cpu_baseline(imgs_host, mats, dfs, P, sym, mode, apix, budget_s):
     _, _, ti1 = O.insert_batch_mt(P, sl, mats[:k1], np.ones(k1, np.float32), sym, mode, nthreads=1)
     return {
         "value": n / (t_prep + t_ins), "unit": "particles/s", "cores": T, "kind": "port",
+        "affinity_cores": aff, "cpu_share": share or None,
         "sample": "%d particles of the same workload (prep %.2fs + insertion %.2fs, %d threads, "
                   "oracle -O3 -ffast-math, private volumes summed)" % (n, t_prep, t_ins, T),
         "prep_s": t_prep, "insert_s": t_ins, "insert_only_particles_per_s": n / t_ins,
@@ -159,7 +168,7 @@ def cpu_baseline(imgs_host, mats, dfs, P, sym, mode, apix, budget_s):
     }
 
 
-def refine(a):
+def refine(a, quiet=False):
     """Two-pass refinement flow (e2make3d.py:391-539): insert all particles,
     determine_slice_agreement(sub=True) for every particle, clear, re-insert
     each image multiplied by its reconstruct_norm, finish() with the 3-D
@@ -236,7 +245,47 @@ def refine(a):
                               "frac": round(job_bytes * a.steps / total / 1e9 / HBM_PEAK_GBS, 4),
                               "bytes_per_job": job_bytes},
     }
-    print(json.dumps(out), flush=True)
+    del imgs
+    torch.cuda.empty_cache()
+    if not quiet:
+        print(json.dumps(out), flush=True)
+    return out
+
+
+def measure_insert(P, B, n, sym, mode, steps, warmup, apix=1.0, snr=0.05, phantom=None):
+    """batched insertion throughput of one shape (particles HBM-resident,
+    CTF phase flip): setup() + insert_batch of all n particles per step"""
+    import torch
+
+    from eman2_amd import Reconstructors
+    from eman2_amd import synthetic as S
+
+    dev = "cuda:0"
+    print("[bench] shape %d x %dpx pad %d %s %s" % (n, B, P, sym, mode), file=sys.stderr, flush=True)
+    ph = phantom if phantom is not None else S.phantom(200, 96.0 * B / 256.0, seed=2)
+    imgs, mats, dfs = S.particles_device(ph, n, B, seed=4, apix=apix, snr=snr, ctf=CTF, device=dev)
+    torch.cuda.synchronize()
+    w = np.ones(n, np.float32)
+    ctf = dict(CTF, apix=apix)
+    r = Reconstructors.get("fourier_hip", {"size": (P, P, P), "sym": sym, "mode": mode})
+    ptr = int(imgs.data_ptr())
+    for _ in range(warmup):
+        r.setup()
+        r.insert_batch(ptr, mats, w, defocus=dfs, ctf=ctf, box=B)
+    r.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r.setup()
+        r.insert_batch(ptr, mats, w, defocus=dfs, ctf=ctf, box=B)
+    r.sync()
+    el = time.perf_counter() - t0
+    st = r.get_stats()
+    del imgs, r
+    torch.cuda.empty_cache()
+    return {"value": round(steps * n / el, 1), "unit": "particles/s", "ms_per_step": round(el / steps * 1e3, 2),
+            "steps": steps, "warmup": warmup, "stage_ms_last_step": {k: round(st[k], 2) for k in (
+                "prep_ms", "bin_ms", "insert_kernel_ms", "merge_ms", "total_ms")},
+            "config": "%d x %dpx particles, pad %d, %s, %s, CTF phase flip, HBM-resident" % (n, B, P, sym, mode)}
 
 
 def main():
@@ -376,7 +425,17 @@ def main():
                         "form": "per particle: host->device copy, prep launches, one insertion launch per "
                                 "symmetry op with global atomics, synchronise after each"}
         out["batched_vs_plain"] = round(value / pv, 2)
+        # the same per-particle launches with the particles already in HBM:
+        # launch and synchronisation cost without the PCIe copies
+        rp.setup()
+        t0 = time.perf_counter()
+        rp.insert_plain(ptr, mats[:k], weights[:k], defocus=dfs[:k], ctf=ctf, box=B)
+        tpd = time.perf_counter() - t0
+        out["plain_device"] = {"value": round(k / tpd, 1), "unit": "particles/s", "sample": k,
+                               "form": "as plain, particles HBM-resident (no per-particle copy)"}
+        out["batched_vs_plain_device"] = round(value / (k / tpd), 2)
         del rp
+    del r
     if a.serve_jobs > 0:
         print("[bench] serving %d jobs" % a.serve_jobs, file=sys.stderr, flush=True)
         del imgs
@@ -387,6 +446,26 @@ def main():
     if not a.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(host, mats, dfs, P, a.sym, a.mode, a.apix, a.cpu_seconds)
+    if not a.no_configs:
+        # the other shapes, measured in this run: the d7 shape (one subunit of
+        # 30 Gaussians replicated by the 14 operators) and the two-pass refine flow
+        from eman2_amd import _lib as LIB
+        from eman2_amd import synthetic as S
+        if "imgs" in locals():
+            del imgs
+        torch.cuda.empty_cache()
+        ops = np.zeros((14, 9), np.float32)
+        LIB.load().fh_get_symmetries(b"d7", ops.ctypes.data, 14)
+        d7 = np.zeros((14, 3, 4), np.float32)
+        d7[:, :, :3] = ops.reshape(14, 3, 3)
+        ph3 = S.symmetric_phantom(S.phantom(30, 60.0, seed=5), d7)
+        out["configs"] = {"d7_pad288": measure_insert(288, 192, 100000, "d7", "gauss_2", 2, 1, phantom=ph3)}
+        ra = argparse.Namespace(**dict(vars(a), box=400, pad=600, apix=0.8, particles=a.refine_particles,
+                                       steps=2, warmup=1, sym="c1", mode="gauss_2"))
+        rf = refine(ra, quiet=True)
+        out["configs"]["refine_pad600"] = {k: rf[k] for k in ("value", "unit", "ms_per_step", "config",
+                                                               "stage_ms_per_step", "agreement_particles_per_s",
+                                                               "pipeline_roofline")}
     if a.calibrate_copy:
         x = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device=dev).fill_(1.0)
         y = torch.empty_like(x)

@@ -1319,8 +1319,23 @@ int fh_prep_batch(fh_reconstructor *h, const float *imgs, int n, int box, const 
     return FH_OK;
 }
 
+static int insert_plain_impl(fh_reconstructor *h, const float *imgs, int n, int box, const float *xforms,
+                             const float *weights, const float *defocus, const fh_ctf *ctf, int where);
+
 int fh_insert_plain(fh_reconstructor *h, const float *imgs, int n, int box, const float *xforms, const float *weights,
                     const float *defocus, const fh_ctf *ctf)
+{
+    return insert_plain_impl(h, imgs, n, box, xforms, weights, defocus, ctf, FH_HOST);
+}
+
+int fh_insert_plain_where(fh_reconstructor *h, const float *imgs, int n, int box, const float *xforms,
+                          const float *weights, const float *defocus, const fh_ctf *ctf, int where)
+{
+    return insert_plain_impl(h, imgs, n, box, xforms, weights, defocus, ctf, where);
+}
+
+static int insert_plain_impl(fh_reconstructor *h, const float *imgs, int n, int box, const float *xforms,
+                             const float *weights, const float *defocus, const fh_ctf *ctf, int where)
 {
     if (!h) return FH_ERR_NULL;
     std::lock_guard<std::mutex> lk(h->mu);
@@ -1337,9 +1352,13 @@ int fh_insert_plain(fh_reconstructor *h, const float *imgs, int n, int box, cons
     if (rc) return rc;
     for (int i = 0; i < n; i++) {
         if (weights[i] == 0) continue;
-        // per-slice host->device copy (reconstructor.cpp:1074-1078)
-        FH_TRY(h, hipMemcpy(h->imgs, imgs + (size_t)i * box * box, sizeof(float) * box * box, hipMemcpyHostToDevice));
-        rc = prep_chunk(h, 1, box, h->imgs, defocus ? defocus + i : nullptr, ctf,
+        // per-slice host->device copy (reconstructor.cpp:1074-1078), unless the particle is on the device
+        const float *dimg = imgs + (size_t)i * box * box;
+        if (where != FH_DEVICE) {
+            FH_TRY(h, hipMemcpy(h->imgs, dimg, sizeof(float) * box * box, hipMemcpyHostToDevice));
+            dimg = h->imgs;
+        }
+        rc = prep_chunk(h, 1, box, dimg, defocus ? defocus + i : nullptr, ctf,
                         xf6.empty() ? nullptr : &xf6[6 * (size_t)i], df2.empty() ? nullptr : &df2[i]);
         if (rc) return rc;
         // one launch per symmetry op, synchronised (cuda_reconstructor.cu:171-193)

@@ -313,13 +313,18 @@ class FourierHipReconstructor:
                                             None if c is None else ctypes.byref(c), _ptr(out), _lib.FH_HOST))
         return out
 
-    def insert_plain(self, imgs, xforms, weights=None, defocus=None, ctf=None):
-        """Same contract as insert_batch, one particle per launch (baseline)."""
+    def insert_plain(self, imgs, xforms, weights=None, defocus=None, ctf=None, box=None):
+        """Same contract as insert_batch, one particle per launch (baseline);
+        imgs a numpy stack (copied per particle) or a device pointer with box."""
         n, m, w, d, c = self._batch_args(imgs, xforms, weights, defocus, ctf)
-        a = _arr(imgs)
-        return self._check(self._lib.fh_insert_plain(self._need(), _ptr(a), n, a.shape[-1], _ptr(m), _ptr(w),
-                                                     None if d is None else _ptr(d),
-                                                     None if c is None else ctypes.byref(c)))
+        if isinstance(imgs, int):
+            where, ptr, B = _lib.FH_DEVICE, ctypes.c_void_p(imgs), int(box)
+        else:
+            a = _arr(imgs)
+            where, ptr, B = _lib.FH_HOST, _ptr(a), a.shape[-1]
+        return self._check(self._lib.fh_insert_plain_where(self._need(), ptr, n, B, _ptr(m), _ptr(w),
+                                                           None if d is None else _ptr(d),
+                                                           None if c is None else ctypes.byref(c), where))
 
     def projection(self, xform, ret_fourier=0):
         """FourierReconstructor::projection (libEM/reconstructor.cpp:1383-1445):

@@ -172,6 +172,11 @@ int fh_prep_batch(fh_reconstructor *h, const float *imgs, int n, int box, const 
 int fh_insert_plain(fh_reconstructor *h, const float *imgs, int n, int box,
                     const float *xforms, const float *weights, const float *defocus,
                     const fh_ctf *ctf);
+/* The plain form with the particles where `where` says: FH_DEVICE skips the
+ * per-particle copy (launches and synchronisations only). */
+int fh_insert_plain_where(fh_reconstructor *h, const float *imgs, int n, int box,
+                          const float *xforms, const float *weights, const float *defocus,
+                          const fh_ctf *ctf, int where);
 
 /* FourierReconstructor::determine_slice_agreement (reconstructor.cpp:1212-1257)
  * for one slice: out = {reconstruct_norm, reconstruct_absqual,
