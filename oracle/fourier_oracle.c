@@ -1350,6 +1350,126 @@ static inline size_t vol_index_fast(int P, int x, int y, int z)
     return x * 2 + (y < 0 ? ny + y : y) * (size_t)nx + (z < 0 ? nz + z : z) * nxy;
 }
 
+/* ======================================================================== */
+/* fourier_iter                                                              */
+/* ======================================================================== */
+
+/* Util::trilinear_interpolate_complex (libEM/util.h:645-654) of the eight
+ * get_complex_at values around floor(p) */
+static void iter_trilinear(const float *ref, int P, float xx, float yy, float zz, float *re, float *im)
+{
+    const float xx0 = floorf(xx), yy0 = floorf(yy), zz0 = floorf(zz);
+    const int x0 = (int)xx0, y0 = (int)yy0, z0 = (int)zz0;
+    const float t = xx - xx0, u = yy - yy0, v = zz - zz0;
+    float pr[8], pi[8];
+    for (int c = 0; c < 8; c++) vol_get(ref, P, x0 + (c & 1), y0 + ((c >> 1) & 1), z0 + (c >> 2), &pr[c], &pi[c]);
+    const float w[8] = {(1 - t) * (1 - u) * (1 - v), t * (1 - u) * (1 - v), (1 - t) * u * (1 - v), t * u * (1 - v),
+                        (1 - t) * (1 - u) * v,       t * (1 - u) * v,       (1 - t) * u * v,       t * u * v};
+    float ar = w[0] * pr[0], ai = w[0] * pi[0];
+    for (int c = 1; c < 8; c++) {
+        ar = ar + w[c] * pr[c];
+        ai = ai + w[c] * pi[c];
+    }
+    *re = ar;
+    *im = ai;
+}
+
+/* FourierIterReconstructor::insert_slice (libEM/reconstructor.cpp:520-637).
+ * Without a reference volume: a 6x6x6 Gaussian stencil from floor(p - 2.5)
+ * with gauss_var's radius-dependent width; with one (setup_seed): the slice
+ * value moved by the reference's local difference, updt = dt + ref(i,j,k) -
+ * trilinear ref(p), under a Gaussian of h = 1/2 inside the sphere
+ * i^2 + j^2 + k^2 <= P^2/4.  Slice pixels to radius P/2 + 3; the x range is
+ * clamped to |x| <= P/2 - 1 (nx2 = nx/2 - 2), y and z to P/2.  Subnormal
+ * weights are dropped as in gauss_var. */
+void or_iter_insert(int P, float *data, float *norm, const float *slice, const float rot_in[12], const float *syms,
+                    int nsym, float weight, const float *ref)
+{
+    float rot[12];
+    memcpy(rot, rot_in, sizeof(rot));
+    or_rotation_only(rot);
+    const float inx = (float)(P + 2), iny = (float)P;
+    const int nx = P + 2, ny = P, nz = P;
+    const int nx2 = nx / 2 - 2, ny2 = ny / 2, nz2 = nz / 2;
+    const size_t ny2sq = (size_t)ny * ny / 4;
+    for (int s = 0; s < nsym; s++) {
+        float t3d[12];
+        or_transform_mult(rot, syms + 12 * s, t3d);
+        for (int y = (int)(-iny / 2); y < iny / 2; y++) {
+            for (int x = 0; x < inx / 2; x++) {
+                float rx = (float)x / (inx - 2.0f);
+                float ry = (float)y / iny;
+                int r = or_hypot_fast_int(x, y);
+                if ((float)r > iny / 2 + 3 && fabsf(inx - iny) < 3) continue;
+                float c[3];
+                or_vec_times_transform(rx, ry, 0.0f, t3d, c);
+                float xx = c[0] * (float)(nx - 2), yy = c[1] * (float)ny, zz = c[2] * (float)nz;
+                int x0 = (int)floor(xx - 2.5), y0 = (int)floor(yy - 2.5), z0 = (int)floor(zz - 2.5);
+                float dr, di;
+                slice_get(slice, P, x, y, &dr, &di);
+                if (x0 < -nx2 - 4 || y0 < -ny2 - 4 || z0 < -nz2 - 4 || x0 > nx2 + 3 || y0 > ny2 + 3 || z0 > nz2 + 3)
+                    continue;
+                int x1 = x0 + 5, y1 = y0 + 5, z1 = z0 + 5;
+                if (x0 < -nx2) x0 = -nx2;
+                if (x1 > nx2) x1 = nx2;
+                if (y0 < -ny2) y0 = -ny2;
+                if (y1 > ny2) y1 = ny2;
+                if (z0 < -nz2) z0 = -nz2;
+                if (z1 > nz2) z1 = nz2;
+                if (ref) {
+                    float nr, ni;
+                    iter_trilinear(ref, P, xx, yy, zz, &nr, &ni);
+                    const float h = 1.0f / 2.0f;
+                    for (int k = z0; k <= z1; k++)
+                        for (int j = y0; j <= y1; j++)
+                            for (int i = x0; i <= x1; i++) {
+                                if ((size_t)((long long)i * i + (long long)j * j + (long long)k * k) > ny2sq) continue;
+                                float tr, ti;
+                                vol_get(ref, P, i, j, k, &tr, &ti);
+                                const float ur = (dr + tr) - nr, ui = (di + ti) - ni;
+                                float rl = hypot3sq((float)i - xx, (float)j - yy, (float)k - zz);
+                                float gg = or_fast_exp(-rl * h);
+                                if (fabsf(gg * weight) < FLT_MIN) continue;
+                                size_t off = add_complex_at_fast_f32(data, P, i, j, k, ur * gg * weight, ui * gg * weight);
+                                norm[off / 2] += gg * weight;
+                            }
+                } else {
+                    float h = (float)(1.0f / ((double)(hypot3sq(xx, yy, zz) / 4000) + .15));
+                    h = h < 0.1 ? (float)0.1 : h;
+                    for (int k = z0; k <= z1; k++)
+                        for (int j = y0; j <= y1; j++)
+                            for (int i = x0; i <= x1; i++) {
+                                float rl = hypot3sq((float)i - xx, (float)j - yy, (float)k - zz);
+                                float gg = or_fast_exp(-rl * h);
+                                if (fabsf(gg * weight) < FLT_MIN) continue;
+                                size_t off = add_complex_at_fast_f32(data, P, i, j, k, dr * gg * weight, di * gg * weight);
+                                norm[off / 2] += gg * weight;
+                            }
+                }
+            }
+        }
+    }
+}
+
+/* FourierIterReconstructor::setup_seed with a real seed (reconstructor.cpp:
+ * 482-489): ref = do_fft(seed) (unnormalised r2c), then
+ * xform.phaseorigin.tocorner on the complex volume, which is
+ * center_origin_fft (processor.cpp:6742-6744, sparx/emdata_sparx.cpp:
+ * 8420-8460): each complex value times (-1)^(x + y + z) for even sizes */
+void or_iter_ref_from_map(int P, const float *map, float *ref)
+{
+    or_fft_r2c_3d(map, P, P, P, ref);
+    const int nxc = (P + 2) / 2;
+    for (int z = 0; z < P; z++)
+        for (int y = 0; y < P; y++)
+            for (int x = 0; x < nxc; x++) {
+                if (((x + y + z) & 1) == 0) continue;
+                size_t i = 2 * (size_t)x + (size_t)(P + 2) * (y + (size_t)P * z);
+                ref[i] = ref[i] * -1.0f;
+                ref[i + 1] = ref[i + 1] * -1.0f;
+            }
+}
+
 void or_determine_slice_agreement(int P, float *data, float *norm, float *slice, int mode, const float rot_in[12],
                                   const float *syms, int nsym, float weight, int sub, float apix,
                                   float out[5])

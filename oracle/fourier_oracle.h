@@ -87,6 +87,11 @@ void or_prep_particle_dfdf(const float *raw, int B, int P, const or_ctf *ctf, fl
                            float *out);
 int or_xform2d(const float *img, int nx, int ny, const float t[12], float *out);
 
+/* ---- fourier_iter (libEM/reconstructor.cpp:428-682) --------------------- */
+void or_iter_insert(int P, float *data, float *norm, const float *slice, const float rot[12], const float *syms,
+                    int nsym, float weight, const float *ref_or_null);
+void or_iter_ref_from_map(int P, const float *map, float *ref);
+
 /* ---- volume insertion (libEM/reconstructor.cpp:1122-1210,
  *      libEM/reconstructor_tools.cpp:128-239, libEM/emdata_core.h:368-413) */
 void or_insert_slice_work(int P, float *data, float *norm, const float *slice,

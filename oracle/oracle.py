@@ -25,7 +25,8 @@ GAUSS2 = 0
 TRILINEAR = 1
 MODES = {"gauss_2": GAUSS2, "trilinear": TRILINEAR, "gauss_var": 2, "gauss_5": 3, "nearest_neighbor": 4, "gauss_3": 5,
          "kaiser_bessel": 6, "gridding_5": 7, "gridding_7": 8,
-         "kaiser_bessel_derived": 9}
+         "kaiser_bessel_derived": 9,
+         "fourier_iter": 10}      # the fourier_iter reconstructor's own insertion (not an inserter mode)
 
 _fp = ctypes.POINTER(ctypes.c_float)
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -67,6 +68,8 @@ def lib(fast=False):
         "or_bessel_i0_scaled": (d, [d]),
         "or_prep_particle_xform": (v, [_fp, i, i, ctypes.POINTER(CtfStruct), _fp, _fp]),
         "or_prep_particle_dfdf": (v, [_fp, i, i, ctypes.POINTER(CtfStruct), f, _fp, _fp]),
+        "or_iter_insert": (v, [i, _fp, _fp, _fp, _fp, _fp, i, f, _fp]),
+        "or_iter_ref_from_map": (v, [i, _fp, _fp]),
         "or_insert_slice_ssnr": (i, [i, _fp, _fp, _fp, _fp, _fp, i, _fp, i, i]),
         "or_edge_mean": (f, [_fp, i, i]),
         "or_clip_2d": (v, [_fp, i, i, i, i, i, i, _fp]),
@@ -313,6 +316,12 @@ class Volume:
     def insert(self, slice_, rot, weight=1.0, corners=False):
         s = f32(slice_)
         r = f32(rot).reshape(12)
+        if self.mode == MODES["fourier_iter"]:
+            assert not self.fp64
+            ref = getattr(self, "ref", None)
+            lib().or_iter_insert(self.P, _f(self.data), _f(self.norm), _f(s), _f(r), _f(self.syms), len(self.syms),
+                                 float(weight), None if ref is None else _f(ref))
+            return
         if self.fp64:
             lib().or_insert_slice_work_f64(self.P, _d(self.data), _d(self.norm), _f(s), _f(r),
                                           _f(self.syms), len(self.syms), float(weight),
@@ -340,6 +349,22 @@ class Volume:
         lib().or_projection(P, _f(self.data), _f(self.norm), _f(f32(euler).reshape(12)), int(bool(ret_fourier)),
                             _f(out))
         return out
+
+    def setup_iter_ref(self, seed):
+        """fourier_iter's setup_seed: the volume cleared, the reference set to
+        the complex seed (P, P, P+2) as given, or to do_fft of a real P^3 map
+        with the phase origin moved to the corner"""
+        P = self.P
+        self.data[:] = 0
+        self.norm[:] = 0
+        seed = f32(seed)
+        if seed.shape == (P, P, P):
+            ref = np.zeros((P, P, P + 2), np.float32)
+            lib().or_iter_ref_from_map(P, _f(np.ascontiguousarray(seed)), _f(ref))
+            self.ref = ref
+        else:
+            assert seed.shape == (P, P, P + 2)
+            self.ref = np.ascontiguousarray(seed).copy()
 
     def setup_seed(self, seed, seed_weight):
         assert not self.fp64
