@@ -311,6 +311,7 @@ struct fh_reconstructor {
     bool ready = false;  // setup() called and finish() not yet
     float2 *data = nullptr;
     float *norm = nullptr;
+    float2 *ref = nullptr;      // fourier_iter: the reference volume of setup_seed (or null)
     float *exptab = nullptr, *rtab = nullptr, *syms_d = nullptr;
     float2 *tw = nullptr;
     FftPlan plan{};
@@ -465,6 +466,11 @@ static void free_bins(fh_reconstructor *h)
     h->bins_cap_slices = 0;
 }
 
+static void free_ref(fh_reconstructor *h)
+{
+    if (h->ref) { (void)hipFree(h->ref); h->ref = nullptr; }
+}
+
 static void free_volume(fh_reconstructor *h)
 {
     if (h->data) { (void)hipFree(h->data); h->data = nullptr; }
@@ -557,8 +563,10 @@ static InsertConst iconst(const fh_reconstructor *h)
     c.H = h->H;
     c.Pf = (float)h->P;
     c.h = 1.0f / (float)(4.0 / (M_PI * M_PI));  // 1.0f/EMConsts::I2G
-    c.mode = h->mode;
+    c.mode = h->mode == 10 && h->ref ? 11 : h->mode;
     c.corners = h->corners;
+    c.rcut = h->mode == 10 ? h->H + 3 : h->H;
+    c.ref = h->mode == 10 ? h->ref : nullptr;
     return c;
 }
 
@@ -603,6 +611,15 @@ int fh_create(const fh_params *p, fh_reconstructor **out)
             g_create_error = "InvalidParameterException: kaiser_bessel needs size >= 32";
             return FH_ERR_INVALID_PARAM;
         }
+    }
+    else if (mode == "fourier_iter") {
+        // the fourier_iter reconstructor (libEM/reconstructor.cpp:428-682): its own insertion,
+        // no inserter modes, class_ssnr, corners or sqrtnorm
+        if (p->usessnr || p->corners || p->sqrtnorm) {
+            g_create_error = "InvalidParameterException: fourier_iter takes no usessnr, corners or sqrtnorm";
+            return FH_ERR_INVALID_PARAM;
+        }
+        imode = 10;
     }
     else if (mode == "gridding_5") imode = 7;
     else if (mode == "gridding_7") imode = 8;
@@ -685,6 +702,7 @@ int fh_destroy(fh_reconstructor *h)
         free_chunk(h);
         free_bins(h);
         free_volume(h);
+        free_ref(h);
         if (h->ximgs) (void)hipFree(h->ximgs);
         if (h->ximgs2) (void)hipFree(h->ximgs2);
         if (h->xfd) (void)hipFree(h->xfd);
@@ -707,6 +725,7 @@ int fh_destroy(fh_reconstructor *h)
 static int setup_locked(fh_reconstructor *h)
 {
     FH_TRY(h, hipSetDevice(h->device));
+    free_ref(h);  // fourier_iter's setup() drops the reference volume (reconstructor.cpp:462-464)
     const size_t n = vol_elems(h->P);
     if (!h->data) {
         hipError_t e = hipMalloc(&h->data, sizeof(float2) * n);
@@ -734,11 +753,46 @@ int fh_setup(fh_reconstructor *h)
     return setup_locked(h);
 }
 
+// fourier_iter's setup_seed with a complex seed (reconstructor.cpp:482-486):
+// the volume cleared and the seed kept as the reference ("we assume tocorner
+// was called properly"); the seed weight is not used
+static int iter_seed_complex(fh_reconstructor *h, const float *seed)
+{
+    int rc = setup_locked(h);
+    if (rc) return rc;
+    const size_t n = vol_elems(h->P);
+    FH_TRY(h, hipMalloc(&h->ref, sizeof(float2) * n));
+    FH_TRY(h, hipMemcpyAsync(h->ref, seed, sizeof(float2) * n, hipMemcpyHostToDevice, h->stream));
+    FH_TRY(h, hipStreamSynchronize(h->stream));
+    return FH_OK;
+}
+
+int fh_setup_seed_map(fh_reconstructor *h, const float *map)
+{
+    if (!h) return FH_ERR_NULL;
+    if (!map) return fail(h, FH_ERR_NULL, "NullPointerException: seed map is NULL");
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (h->mode != 10)
+        return fail(h, FH_ERR_INVALID_PARAM, "InvalidParameterException: a real seed map seeds fourier_iter only");
+    int rc = setup_locked(h);
+    if (rc) return rc;
+    const int P = h->P;
+    // the map goes through the particle staging (P images of P x P) and the prep's FFT passes
+    if ((rc = ensure_chunk(h, std::max(P, h->cap_n), std::max(P, h->cap_B)))) return rc;
+    FH_TRY(h, hipMalloc(&h->ref, sizeof(float2) * vol_elems(P)));
+    FH_TRY(h, h2d_staged(h, h->imgs, map, sizeof(float) * (size_t)P * P * P));
+    FH_TRY(h, launch_fill_f32(h->means, 0.f, P, h->stream));
+    FH_TRY(h, launch_fft3d_forward_corner(h->imgs, P, h->means, h->ref, h->plan, h->tw, h->stream));
+    FH_TRY(h, hipStreamSynchronize(h->stream));
+    return FH_OK;
+}
+
 int fh_setup_seed(fh_reconstructor *h, const float *seed, float seed_weight)
 {
     if (!h) return FH_ERR_NULL;
     if (!seed) return fail(h, FH_ERR_NULL, "NullPointerException: seed volume is NULL");
     std::lock_guard<std::mutex> lk(h->mu);
+    if (h->mode == 10) return iter_seed_complex(h, seed);
     int rc = setup_locked(h);
     if (rc) return rc;
     const size_t n = vol_elems(h->P);
@@ -754,6 +808,7 @@ int fh_setup_seedandweights(fh_reconstructor *h, const float *seed, const float 
     if (!h) return FH_ERR_NULL;
     if (!seed || !weights) return fail(h, FH_ERR_NULL, "NullPointerException: seed or weight volume is NULL");
     std::lock_guard<std::mutex> lk(h->mu);
+    if (h->mode == 10) return iter_seed_complex(h, seed);  // fourier_iter ignores the weights (reconstructor.cpp:491-499)
     int rc = setup_locked(h);
     if (rc) return rc;
     const size_t n = vol_elems(h->P);
@@ -770,6 +825,7 @@ int fh_clear(fh_reconstructor *h)
     if (!h->ready) return fail(h, FH_ERR_STATE, "clear() before setup()");
     const size_t n = vol_elems(h->P);
     FH_TRY(h, hipSetDevice(h->device));
+    free_ref(h);  // fourier_iter's clear() is setup() (reconstructor.cpp:502-504)
     FH_TRY(h, launch_fill_f32((float *)h->data, 0.f, 2 * n, h->stream));
     FH_TRY(h, launch_fill_f32(h->norm, 0.f, n, h->stream));
     return FH_OK;
@@ -1090,7 +1146,7 @@ static int insert_slice_impl(fh_reconstructor *h, const float *data, int is_prep
     // usessnr: weight -1 (the SSNR flag) with class_ssnr, else 0 (reconstructor.cpp:1082-1088)
     const bool use_ssnr = h->usessnr && ssnr;
     if (h->usessnr) weight = ssnr ? -1.0f : 0.0f;
-    if (weight == 0) return FH_SKIPPED;
+    if (weight == 0) return h->mode == 10 ? FH_OK : FH_SKIPPED;  // fourier_iter adds the zeros and returns 0
     FH_TRY(h, hipSetDevice(h->device));
     const int P = h->P, H = h->H;
     std::vector<float> rot9, xf6;
@@ -1374,6 +1430,8 @@ int fh_projection(fh_reconstructor *h, const float xform[12], int ret_fourier, f
     if (!xform || !out) return fail(h, FH_ERR_NULL, "NullPointerException: projection output is NULL");
     std::lock_guard<std::mutex> lk(h->mu);
     if (!h->ready) return fail(h, FH_ERR_STATE, "projection() before setup() or after finish()");
+    if (h->mode == 10)
+        return fail(h, FH_ERR_INVALID_PARAM, "InvalidParameterException: fourier_iter has no projection()");
     FH_TRY(h, hipSetDevice(h->device));
     const int P = h->P, H = h->H;
     // only the rotation enters the slice (reconstructor.cpp:1398-1401); the
@@ -1522,6 +1580,7 @@ int fh_determine_slice_agreement_noalloc(fh_reconstructor *h, const float *slice
     if (!slice_dev || !xform || !out) return fail(h, FH_ERR_NULL, "NullPointerException: slice, xform or out is NULL");
     std::lock_guard<std::mutex> lk(h->mu);
     if (!h->ready) return fail(h, FH_ERR_STATE, "determine_slice_agreement() before setup()");
+    if (h->mode == 10) return FH_OK;  // fourier_iter: returns without comparing (reconstructor.cpp:639)
     FH_TRY(h, hipSetDevice(h->device));
     unsigned char *base = nullptr;
     int rc = scratch_base(h, &base);
@@ -1588,6 +1647,7 @@ int fh_determine_slice_agreement(fh_reconstructor *h, const float *data, int is_
     if (!data || !xform || !out) return fail(h, FH_ERR_NULL, "NullPointerException: EMData pointer (input image) is NULL");
     std::lock_guard<std::mutex> lk(h->mu);
     if (!h->ready) return fail(h, FH_ERR_STATE, "determine_slice_agreement() before setup()");
+    if (h->mode == 10) return FH_OK;  // fourier_iter: returns without comparing (reconstructor.cpp:639)
     FH_TRY(h, hipSetDevice(h->device));
     const int P = h->P, H = h->H;
     std::vector<float> rot9, xf6;
@@ -1612,6 +1672,7 @@ int fh_agreement_batch(fh_reconstructor *h, const float *imgs, int n, int box, c
     int rc = check_batch_args(h, imgs, n, box, xforms, weights);
     if (rc) return rc;
     if (!out) return fail(h, FH_ERR_NULL, "NullPointerException: output is NULL");
+    if (h->mode == 10) return FH_OK;  // fourier_iter: no comparison (reconstructor.cpp:639); out untouched
     FH_TRY(h, hipSetDevice(h->device));
     // the own-contribution gather knows 2x2x2 stencils only: the other modes
     // take the reference's sequence per particle (remove, compare, re-add)

@@ -31,8 +31,11 @@ struct InsertConst {
     float Pf;    // (float)P
     float h;     // 1.0f / EMConsts::I2G
     int mode;    // 0 gauss_2, 1 trilinear, 2 gauss_var, 3 gauss_5, 4 nearest_neighbor, 5 gauss_3, 6 kaiser_bessel,
-                 // 7 gridding_5, 8 gridding_7, 9 kaiser_bessel_derived
+                 // 7 gridding_5, 8 gridding_7, 9 kaiser_bessel_derived; 10 fourier_iter, 11 fourier_iter with
+                 // a reference volume
     int corners;
+    int rcut;            // slice pixels to round(hypot(x, y)) <= rcut (P/2; P/2 + 3 for fourier_iter)
+    const float2 *ref;   // fourier_iter's reference volume (mode 11), else null
 };
 
 // Row-major 3x3 rotation passed by value.
@@ -103,6 +106,8 @@ hipError_t launch_fft3d_inverse_zy(float2 *data, int P, const FftPlan &plan, con
                                    hipStream_t s);
 hipError_t launch_c2r_x(const float2 *data, int P, float *out, float scale, const FftPlan &plan,
                         const float2 *tw, hipStream_t s);
+hipError_t launch_fft3d_forward_corner(const float *vol, int P, const float *zeros, float2 *out, const FftPlan &plan,
+                                       const float2 *tw, hipStream_t s);
 hipError_t launch_merge_add(float2 *data, float *norm, const float2 *threed, const float *normv, size_t n,
                             hipStream_t s);
 hipError_t launch_merge_scale(float2 *data, const float *norm, size_t n, hipStream_t s);

@@ -228,12 +228,17 @@ __host__ __device__ inline double bessel_i0_scaled(double x)
 //   mode 3 gauss_5 (Mode5): the same stencil with h = 1 / I5G;
 //   mode 4 nearest_neighbor (Mode1, :113-126): the voxel floor(p + 0.5);
 //   mode 5 gauss_3 (Mode3): 3x3x3 from floor(p - 0.5), h = 32 / ((8 + |p|) I3G),
-//     weight fast_exp(-r h) w / (1 + 6 e^-h + 12 e^-2h + 8 e^-3h).
+//     weight fast_exp(-r h) w / (1 + 6 e^-h + 12 e^-2h + 8 e^-3h);
+//   mode 10 fourier_iter (FourierIterReconstructor::insert_slice,
+//     libEM/reconstructor.cpp:520-637): gauss_var's stencil and width with x
+//     clamped to |x| <= P/2 - 1 (nx2 = nx/2 - 2); mode 11 the same
+//     reconstructor with a reference volume: value dt + ref(i,j,k) -
+//     trilinear ref(p), h = 1/2, cells inside i^2 + j^2 + k^2 <= P^2/4.
 // Stencils are clamped to [-P/2, P/2] per axis; subnormal weights are
 // dropped (flush-to-zero: the reference's -ffast-math build and the float
 // atomics do not keep them).
 __device__ void insert_wide(int mode, float2 dt, float xx, float yy, float zz, float w, int P, int H,
-                            const float *__restrict__ etab, float2 *data, float *norm);
+                            const float *__restrict__ etab, float2 *data, float *norm, const float2 *ref);
 
 // ------------------------------------------------------------ plain form
 // Util::hypot_fast_int(x, y) (libEM/util.cpp:764-785): round(hypot(x, y)),
@@ -257,13 +262,13 @@ __global__ void __launch_bounds__(256) k_insert_plain(const float2 *__restrict__
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = (int)blockIdx.y - H;
     if (x > H) return;
-    if (!c.corners && x * x + y * y > H * H + H) return;  // hypot_fast_int(x,y) > P/2
+    if (!c.corners && x * x + y * y > c.rcut * c.rcut + c.rcut) return;  // hypot_fast_int(x,y) > rcut
     // class_ssnr weighting (reconstructor.cpp:1170-1184): the weight of radius r
     if (rw) weight = rw[hypot_int(x, y)];
     float xx, yy, zz;
     pixel_coords(x, y, m.m, rtab_g, H, c.Pf, xx, yy, zz);
     if (c.mode >= 2) {
-        insert_wide(c.mode, slice_value(S, x, y, P, H), xx, yy, zz, weight, P, H, exptab, data, norm);
+        insert_wide(c.mode, slice_value(S, x, y, P, H), xx, yy, zz, weight, P, H, exptab, data, norm, c.ref);
         return;
     }
     const int x0 = (int)floorf(xx), y0 = (int)floorf(yy), z0 = (int)floorf(zz);
@@ -295,7 +300,7 @@ __global__ void __launch_bounds__(64) k_insert_plain_batch(const float2 *__restr
     const int y = (int)blockIdx.y - H;
     const float wgt = wq[q];
     if (x > H || wgt == 0.0f) return;
-    if (!c.corners && x * x + y * y > H * H + H) return;  // hypot_fast_int(x,y) > P/2
+    if (!c.corners && x * x + y * y > c.rcut * c.rcut + c.rcut) return;  // hypot_fast_int(x,y) > rcut
     const int p = q / nsym;
     const float weight = rw ? rw[(size_t)p * (H + 1) + hypot_int(x, y)] : wgt;
     const float2 *S = slices + (size_t)p * P * (H + 1);
@@ -304,7 +309,7 @@ __global__ void __launch_bounds__(64) k_insert_plain_batch(const float2 *__restr
     pixel_coords(x, y, m, rtab_g, H, c.Pf, xx, yy, zz);
     const float2 dt = slice_value(S, x, y, P, H);
     if (c.mode >= 2) {
-        insert_wide(c.mode, dt, xx, yy, zz, weight, P, H, exptab, data, norm);
+        insert_wide(c.mode, dt, xx, yy, zz, weight, P, H, exptab, data, norm, c.ref);
         return;
     }
     const int x0 = (int)floorf(xx), y0 = (int)floorf(yy), z0 = (int)floorf(zz);
@@ -353,13 +358,30 @@ __device__ __forceinline__ int grid_index(int i, float p, int n)
 
 struct WidePix {
     float xx, yy, zz, dre, dim, w, h, ww;
+    float nre, nim;        // fourier_iter with a reference: the reference trilinear at p
     int lo[3], hi[3];
+    int H;
+    const float2 *ref;
 };
+
+// EMData::get_complex_at(x, y, z) of a reference volume in the reconstructor's
+// layout (libEM/emdata_core.cpp:135-144): 0 outside, conjugate for x < 0
+__device__ __forceinline__ float2 ref_get(const float2 *__restrict__ ref, int x, int y, int z, int H)
+{
+    const int P = 2 * H;
+    if (abs(x) >= H + 1 || abs(y) > H || abs(z) > H) return make_float2(0.f, 0.f);
+    if (x < 0) {
+        const float2 v = ref[cidx(-x, y <= 0 ? -y : P - y, z <= 0 ? -z : P - z, P, H)];
+        return make_float2(v.x, -v.y);
+    }
+    return ref[cidx(x, y < 0 ? P + y : y, z < 0 ? P + z : z, P, H)];
+}
 
 // stencil range (clamped to [-P/2, P/2]) and width of one pixel; false if the
 // reference's range check rejects it (insert_wide's first half)
 __device__ __forceinline__ bool wide_setup(int mode, float xx, float yy, float zz, float w, int H,
-                                           const float *__restrict__ etab, WidePix &px)
+                                           const float *__restrict__ etab, WidePix &px,
+                                           const float2 *ref = nullptr)
 {
     px.xx = xx;
     px.yy = yy;
@@ -367,6 +389,49 @@ __device__ __forceinline__ bool wide_setup(int mode, float xx, float yy, float z
     px.w = w;
     px.ww = w;
     px.h = 0.f;
+    px.H = H;
+    px.ref = ref;
+    px.nre = px.nim = 0.f;
+    if (mode >= 10) {
+        // FourierIterReconstructor::insert_slice: floor(p - 2.5) + [0, 5]; rejected when the
+        // origin is below -n2 - 4 or above n2 + 3, n2 = P/2 - 1 for x and P/2 for y, z; clamped to [-n2, n2]
+        const float pp[3] = {xx, yy, zz};
+        for (int d = 0; d < 3; d++) {
+            const int n2 = d == 0 ? H - 1 : H;
+            const int a = (int)floor((double)pp[d] - 2.5);
+            if (a < -n2 - 4 || a > n2 + 3) return false;
+            px.lo[d] = max(a, -n2);
+            px.hi[d] = min(a + 5, n2);
+            if (px.hi[d] < px.lo[d]) return false;
+        }
+        if (mode == 10) {
+            float h = (float)(1.0f / ((double)((xx * xx + yy * yy + zz * zz) / 4000) + .15));
+            px.h = h < 0.1 ? (float)0.1 : h;
+            return true;
+        }
+        // Util::trilinear_interpolate_complex (libEM/util.h:645-654) at floor(p)
+        px.h = 1.0f / 2.0f;
+        const float xx0 = floorf(xx), yy0 = floorf(yy), zz0 = floorf(zz);
+        const int x0 = (int)xx0, y0 = (int)yy0, z0 = (int)zz0;
+        const float t = xx - xx0, u = yy - yy0, v = zz - zz0;
+        float ar = 0.f, ai = 0.f;
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const float2 q = ref_get(ref, x0 + (c & 1), y0 + ((c >> 1) & 1), z0 + (c >> 2), H);
+            const float a = (c & 1) ? t : 1 - t, b = (c & 2) ? u : 1 - u, e = (c & 4) ? v : 1 - v;
+            const float wc = a * b * e;
+            if (c == 0) {
+                ar = wc * q.x;
+                ai = wc * q.y;
+            } else {
+                ar = ar + wc * q.x;
+                ai = ai + wc * q.y;
+            }
+        }
+        px.nre = ar;
+        px.nim = ai;
+        return true;
+    }
     if (mode == 4) {
         const int c0 = (int)floor((double)(xx + 0.5f)), c1 = (int)floor((double)(yy + 0.5f));
         const int c2 = (int)floor((double)(zz + 0.5f));
@@ -391,7 +456,7 @@ __device__ __forceinline__ bool wide_setup(int mode, float xx, float yy, float z
         px.ww = w0 + w0 + w0 + 1.0f + 1.0f;
         return true;
     }
-    if (mode >= 7) {
+    if (mode == 7 || mode == 8) {
         // gridding_5: fast_floor(p - 1.5) + [0, 4]; gridding_7: fast_floor(p - 2.5) + [0, 6]
         const double sh = mode == 8 ? 2.5 : 1.5;
         const int span = mode == 8 ? 6 : 4, lo = mode == 8 ? 6 : 4, hi = mode == 8 ? 5 : 3;
@@ -444,6 +509,20 @@ __device__ __forceinline__ bool wide_term(int mode, const WidePix &px, int i, in
         g = px.w;
         return true;
     }
+    if (mode == 11) {
+        // updt = dt + ref(i,j,k) - ref~(p) under fast_exp(-r / 2), inside the sphere of radius P/2
+        if ((long long)i * i + (long long)j * j + (long long)k * k > (long long)(2 * px.H) * (2 * px.H) / 4)
+            return false;
+        const float2 nt = ref_get(px.ref, i, j, k, px.H);
+        const float ur = (px.dre + nt.x) - px.nre, ui = (px.dim + nt.y) - px.nim;
+        const float dx = (float)i - px.xx, dy = (float)j - px.yy, dz = (float)k - px.zz;
+        const float gg = fast_exp_full(-(dx * dx + dy * dy + dz * dz) * px.h, etab);
+        if (fabsf(gg * px.w) < FLT_MIN) return false;
+        vr = ur * gg * px.w;
+        vi = ui * gg * px.w;
+        g = gg * px.w;
+        return true;
+    }
     if (mode == 9) {
         // Mode10 (reconstructor_tools.cpp:759-812): kb = ws[0] + ... + ws[q], q <= min(r, 4) with r the
         // squared distance; value dt sqrt(kb / sum ws) w, weight w
@@ -458,7 +537,7 @@ __device__ __forceinline__ bool wide_term(int mode, const WidePix &px, int i, in
         g = px.w;
         return true;
     }
-    if (mode >= 7) {
+    if (mode == 7 || mode == 8) {
         // Mode7 / Mode11: value dt gg w, weight w (reconstructor_tools.cpp:539-576, 964-1001)
         const int n = mode == 8 ? 12 : 9;
         const float *tab = mode == 8 ? c_grid7 : c_grid5;
@@ -500,10 +579,10 @@ __device__ __forceinline__ bool wide_term(int mode, const WidePix &px, int i, in
 // one pixel's stencil straight into the volume (the single-slice and
 // fallback form of the wide modes), the same terms as the LDS-window form
 __device__ void insert_wide(int mode, float2 dt, float xx, float yy, float zz, float w, int P, int H,
-                            const float *__restrict__ etab, float2 *data, float *norm)
+                            const float *__restrict__ etab, float2 *data, float *norm, const float2 *ref)
 {
     WidePix px;
-    if (!wide_setup(mode, xx, yy, zz, w, H, etab, px)) return;
+    if (!wide_setup(mode, xx, yy, zz, w, H, etab, px, ref)) return;
     px.dre = dt.x;
     px.dim = dt.y;
     for (int k = px.lo[2]; k <= px.hi[2]; k++)
@@ -537,13 +616,13 @@ __global__ void __launch_bounds__(256) k_insert_wide_tiles(const float2 *__restr
     __syncthreads();
     if (threadIdx.x < WT_TILE * WT_TILE) {
         const int x = tx0 + (threadIdx.x % WT_TILE), y = ty0 + (threadIdx.x / WT_TILE);
-        bool ok = wgt != 0.0f && x <= H && y <= H - 1 && (c.corners || x * x + y * y <= H * H + H);
+        bool ok = wgt != 0.0f && x <= H && y <= H - 1 && (c.corners || x * x + y * y <= c.rcut * c.rcut + c.rcut);
         if (ok) {
             const float weight = rw ? rw[(size_t)p * (H + 1) + hypot_int(x, y)] : wgt;
             float xx, yy, zz;
             pixel_coords(x, y, t3d + 9 * (size_t)q, rtab_g, H, c.Pf, xx, yy, zz);
             WidePix px;
-            ok = wide_setup(c.mode, xx, yy, zz, weight, H, exptab, px);
+            ok = wide_setup(c.mode, xx, yy, zz, weight, H, exptab, px, c.ref);
             if (ok) {
                 const float2 dt = slice_value(slices + (size_t)p * P * (H + 1), x, y, P, H);
                 px.dre = dt.x;

@@ -444,7 +444,9 @@ hipError_t launch_normalize(float2 *data, float *norm, int P, int sqrtnorm, floa
 // ---------------------------------------------------------------- 3-D inverse
 // In-place inverse complex transform of length P along z (pass 0) or y
 // (pass 1) for tiles of T consecutive x; inverse = conj o forward o conj.
-template <int T>
+// FWD: the forward transform instead (no conjugation), each output n times
+// (-1)^n: the axis's phase origin moved to the corner
+template <int T, int FWD = 0>
 __global__ void __launch_bounds__(256) k_fft_axis_inv(float2 *__restrict__ data, int P, size_t axis_stride,
                                                       size_t outer_stride, FftPlan plan,
                                                       const float2 *__restrict__ twg)
@@ -462,7 +464,7 @@ __global__ void __launch_bounds__(256) k_fft_axis_inv(float2 *__restrict__ data,
         const int t = idx - n * T;
         const int x = x0 + t;
         float2 v = make_float2(0.f, 0.f);
-        if (x <= H) v = conjf2(base[(size_t)n * axis_stride + x]);
+        if (x <= H) v = FWD ? base[(size_t)n * axis_stride + x] : conjf2(base[(size_t)n * axis_stride + x]);
         A[n * T + t] = v;
     }
     __syncthreads();
@@ -471,7 +473,10 @@ __global__ void __launch_bounds__(256) k_fft_axis_inv(float2 *__restrict__ data,
         const int n = idx / T;
         const int t = idx - n * T;
         const int x = x0 + t;
-        if (x <= H) base[(size_t)n * axis_stride + x] = conjf2(R[n * T + t]);
+        if (x <= H) {
+            const float2 r = R[n * T + t];
+            base[(size_t)n * axis_stride + x] = FWD ? ((n & 1) ? make_float2(-r.x, -r.y) : r) : conjf2(r);
+        }
     }
 }
 
@@ -611,6 +616,31 @@ hipError_t launch_ift2d_batch(float2 *data, int n, int P, float *out, float scal
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return c2r_rows(data, P, n, 2, out, scale, plan, tw, s);
+}
+
+// ---------------------------------------------------------------- forward 3-D
+// do_fft of a real P^3 volume with the phase origin moved to the corner (the
+// Fourier-space xform.phaseorigin.tocorner, center_origin_fft): the row and
+// column passes of the particle prep over the P z planes (their load index
+// applies the x and y shift), then the z transform, each output times
+// (-1)^z.  vol: P^3 floats, out: P x P x (P/2+1) complex.
+hipError_t launch_fft3d_forward_corner(const float *vol, int P, const float *zeros, float2 *out, const FftPlan &plan,
+                                       const float2 *tw, hipStream_t s)
+{
+    hipError_t e = launch_prep_rows(vol, P, P, P, zeros, out, plan, tw, s);
+    if (e != hipSuccess) return e;
+    e = launch_prep_cols(P, P, P, out, nullptr, nullptr, CtfConst{}, 1.0f, plan, tw, s);
+    if (e != hipSuccess) return e;
+    const int H = P / 2;
+    const int T = pick_T(P);
+    const size_t lds = fft_lds_bytes(P, T);
+    const size_t row = H + 1;
+    dim3 grid((H + 1 + T - 1) / T, P);
+    if (T == 8)
+        hipLaunchKernelGGL((k_fft_axis_inv<8, 1>), grid, dim3(256), lds, s, out, P, (size_t)P * row, row, plan, tw);
+    else
+        hipLaunchKernelGGL((k_fft_axis_inv<4, 1>), grid, dim3(256), lds, s, out, P, (size_t)P * row, row, plan, tw);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------- task merge

@@ -12,6 +12,7 @@ reference multiplies each cached preprocessed slice by its norm; the batched
 form scales the raw particle instead, which is the same up to float
 rounding (edge-mean subtraction, padding and the FFT are linear).
 """
+import math
 import os
 
 import numpy as np
@@ -119,6 +120,51 @@ def reconstruct(recon, imgs, xforms, weights=None, niter=2, keep=1.0, keepsig=Fa
     info = dict(norm=norm, reconstruct_absqual=absqual, reconstruct_qual=qual, qcutoff=qcutoff,
                 threed_ptcl_idxs=included, threed_excl_ptcl_idxs=excluded, ptcl_repr=ptcl)
     return out, info
+
+
+def threshold_compress(vol):
+    """threshold.compress at the map's mean with its standard deviation as
+    the range (e2make3dpar.py:386-390; CollapseProcessor, processor.h:
+    3274-3317 -- clamponly is declared but not used by process_pixel):
+    x > v + r -> x - r, x < v - r -> x + r, else v.  mean and sigma as
+    EMData::update_stat (emdata.cpp:3154-3204): double sums, the n - 1
+    variance, stored as float."""
+    a = np.asarray(vol, np.float32)
+    d = a.astype(np.float64).ravel()
+    n = d.size
+    s1, s2 = d.sum(), (d * d).sum()
+    v = np.float32(s1 / n)
+    var = (s2 - s1 * s1 / n) / (n - 1)
+    r = np.float32(math.sqrt(var) if var >= 0 else 0.0)
+    return np.where(a > v + r, a - r, np.where(a < v - r, a + r, v)).astype(np.float32)
+
+
+def reconstruct_iterative(recon, imgs, xforms, weights=None, niter=4, seed=None, defocus=None, ctf=None):
+    """e2make3dpar.py --iterative (:241-246, 353-392): niter passes of a
+    fourier_iter reconstructor, each inserting every particle and ending
+    with finish(True); the first starts from setup() (or the seed map), each
+    later one from setup_seed(previous map, 1.0), the previous map first
+    compressed about its mean by its standard deviation.  Returns the last
+    map (numpy, P^3)."""
+    imgs = np.ascontiguousarray(imgs, np.float32)
+    n = len(imgs)
+    xf = np.ascontiguousarray(xforms, np.float32).reshape(n, 3, 4)
+    w = np.ones(n, np.float32) if weights is None else np.ascontiguousarray(weights, np.float32)
+    out = None
+    for it in range(niter):
+        if it == 0:
+            if seed is not None:
+                recon.setup_seed(seed, 1.0)
+            else:
+                recon.setup()
+        else:
+            recon.setup_seed(out, 1.0)
+        recon.insert_batch(imgs, xf, w, defocus=defocus, ctf=ctf)
+        out = recon.finish(True)
+        out = out.numpy() if hasattr(out, "numpy") else np.asarray(out)
+        if it < niter - 1:
+            out = threshold_compress(out)
+    return out
 
 
 def spa_make3d(make_recon, imgs, xforms, weights=None, ntasks=4, defocus=None, ctf=None):

@@ -435,11 +435,62 @@ class FourierHipReconstructor:
         self._check(self._lib.fh_sync(self._need()))
 
 
+class FourierIterHipReconstructor(FourierHipReconstructor):
+    """The "fourier_iter_hip" Reconstructor: FourierIterReconstructor
+    (libEM/reconstructor.cpp:428-682) on an MI355X.  Its own 6x6x6 Gaussian
+    insertion (no inserter modes); setup_seed with the previous map makes
+    that map the reference volume of the next pass (e2make3dpar.py
+    --iterative); determine_slice_agreement returns without comparing."""
+
+    NAME = "fourier_iter_hip"
+    PARAM_TYPES = {
+        "size": "INTARRAY", "sym": "STRING", "verbose": "BOOL", "quiet": "BOOL", "savenorm": "STRING",
+        "normout": "EMDATA", "pad": "INT", "device": "INT", "waves_per_block": "INT", "slices_per_item": "INT",
+        "chunk": "INT", "blocks_per_cu": "INT",
+    }
+
+    def __init__(self, params=None):
+        super().__init__(params)
+
+    def set_params(self, params):
+        super().set_params(params)
+        self.params["mode"] = "fourier_iter"
+
+    def _create(self):
+        self.params["mode"] = "fourier_iter"
+        super()._create()
+
+    def get_desc(self):
+        return "Iterative direct Fourier inversion (fourier_iter) on MI355X"
+
+    def setup_seed(self, seed, seed_weight=1.0):
+        """a real P^3 map (the previous pass's finish(True)) becomes the
+        reference after do_fft and the phase origin moved to the corner, on
+        the GPU; a complex (P, P, P+2) volume is taken as it is
+        (reconstructor.cpp:482-489).  The seed weight is not used."""
+        a = _arr(seed)
+        P = self.P
+        if a.shape == (P, P, P) and not (isinstance(seed, EMData) and seed.is_complex()):
+            self._check(self._lib.fh_setup_seed_map(self._need(), _ptr(a)))
+        else:
+            self._check(self._lib.fh_setup_seed(self._need(), _ptr(a), float(seed_weight)))
+
+    def setup_seedandweights(self, seed, weights):
+        self.setup_seed(seed, 1.0)
+
+    def determine_slice_agreement(self, img, xform, weight=1.0, sub=True):
+        """returns 0 without comparing (reconstructor.cpp:639): no attributes set"""
+        self._need()
+        return {}
+
+
 class Reconstructors:
     """Factory<Reconstructor> (libEM/emobject.h:724-889) with
-    "fourier_hip" registered; ``add`` mirrors Factory<T>::add<>()."""
+    "fourier_hip" and "fourier_iter_hip" registered; ``add`` mirrors
+    Factory<T>::add<>()."""
 
-    _registry = {FourierHipReconstructor.NAME: FourierHipReconstructor}
+    _registry = {FourierHipReconstructor.NAME: FourierHipReconstructor,
+                 FourierIterHipReconstructor.NAME: FourierIterHipReconstructor}
 
     @classmethod
     def get(cls, name, params=None):

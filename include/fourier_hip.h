@@ -208,6 +208,17 @@ int fh_determine_slice_agreement_noalloc(fh_reconstructor *h, const float *slice
                                          const float xform[12], float weight, int sub,
                                          float apix, float out[5]);
 
+/* fourier_iter (FourierIterReconstructor, reconstructor.cpp:428-682) is the
+ * handle created with mode "fourier_iter": its own 6x6x6 Gaussian insertion;
+ * fh_setup_seed / fh_setup_seedandweights keep a complex seed (P x P x (P+2))
+ * as its reference volume (weights ignored), which moves every inserted value
+ * by the reference's local difference; determine_slice_agreement returns
+ * without comparing; finish() as fourier's.  fh_setup_seed_map sets the
+ * reference from a real P^3 map (host): do_fft with the phase origin moved
+ * to the corner, on the GPU (e2make3dpar.py:370-372 seeds each iteration
+ * with the previous map). */
+int fh_setup_seed_map(fh_reconstructor *h, const float *map);
+
 /* The task merge of e2spa_make3d.py:175-192: after fh_setup(h), each task's
  * finish(doift=0) volume `threed` (P x P x (P+2) floats) times its normout
  * (P x P x (P/2+1), fh_finish's norm_out) is added to h's volume and the
