@@ -312,6 +312,7 @@ struct fh_reconstructor {
     float2 *data = nullptr;
     float *norm = nullptr;
     float2 *ref = nullptr;      // fourier_iter: the reference volume of setup_seed (or null)
+    float2 *pvol = nullptr;     // gauss_fft projector: the map's transform (phase origin at the corner)
     float *exptab = nullptr, *rtab = nullptr, *syms_d = nullptr;
     float2 *tw = nullptr;
     FftPlan plan{};
@@ -703,6 +704,7 @@ int fh_destroy(fh_reconstructor *h)
         free_bins(h);
         free_volume(h);
         free_ref(h);
+        if (h->pvol) (void)hipFree(h->pvol);
         if (h->ximgs) (void)hipFree(h->ximgs);
         if (h->ximgs2) (void)hipFree(h->ximgs2);
         if (h->xfd) (void)hipFree(h->xfd);
@@ -1825,6 +1827,124 @@ int fh_merge_finish(fh_reconstructor *h, float *out)
     }
     if (e != hipSuccess) return fail(h, FH_ERR_DEVICE, std::string("merge_finish: ") + hipGetErrorString(e));
     free_volume(h);
+    return FH_OK;
+}
+
+// ------------------------------------------------------------ gauss_fft projector
+int fh_gauss_fft_volume(fh_reconstructor *h, const float *map)
+{
+    if (!h) return FH_ERR_NULL;
+    if (!map) return fail(h, FH_ERR_NULL, "NullPointerException: volume is NULL");
+    std::lock_guard<std::mutex> lk(h->mu);
+    FH_TRY(h, hipSetDevice(h->device));
+    const int P = h->P;
+    int rc = ensure_chunk(h, std::max(P, h->cap_n), std::max(P, h->cap_B));
+    if (rc) return rc;
+    if (!h->pvol) FH_TRY(h, hipMalloc(&h->pvol, sizeof(float2) * vol_elems(P)));
+    // xform.phaseorigin.tocorner and do_fft (projector.cpp:77-79) on the GPU; the
+    // Fourier origin shift to the centre is a re-indexing, done by the reads
+    FH_TRY(h, h2d_staged(h, h->imgs, map, sizeof(float) * (size_t)P * P * P));
+    FH_TRY(h, launch_fill_f32(h->means, 0.f, P, h->stream));
+    FH_TRY(h, launch_fft3d_forward_corner(h->imgs, P, h->means, h->pvol, h->plan, h->tw, h->stream));
+    FH_TRY(h, hipStreamSynchronize(h->stream));
+    return FH_OK;
+}
+
+int fh_gauss_fft_project(fh_reconstructor *h, int n, const float *xforms, int mode, int returnfft, float *out)
+{
+    if (!h) return FH_ERR_NULL;
+    if (!xforms || !out) return fail(h, FH_ERR_NULL, "NullPointerException: transforms or output is NULL");
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (!h->pvol) return fail(h, FH_ERR_STATE, "gauss_fft projection before its volume was set");
+    if (mode < 0 || mode > 2)
+        return fail(h, FH_ERR_INVALID_PARAM, "InvalidParameterException: gauss_fft modes 0, 1 and 2 are supported");
+    if (n < 0) return fail(h, FH_ERR_INVALID_PARAM, "InvalidParameterException: negative projection count");
+    FH_TRY(h, hipSetDevice(h->device));
+    const int P = h->P, H = h->H;
+    // per projection: inverse of the rotation part (get_rotation_transform, invert:
+    // transform.cpp:711-719, 1324-1356), the scale and the translation (get_trans)
+    std::vector<float> rec(12 * (size_t)std::max(n, 1)), xf6;
+    bool any_shift = false;
+    for (int q = 0; q < n; q++) {
+        const float *t = xforms + 12 * (size_t)q;
+        if (determinant(t) < 0)
+            return fail(h, FH_ERR_INVALID_PARAM,
+                        "InvalidParameterException: gauss_fft with a mirrored Transform is not supported by fourier_hip");
+        float r9[9];
+        rotation_only(t, r9);
+        const double m00 = r9[0], m01 = r9[1], m02 = r9[2], m10 = r9[3], m11 = r9[4], m12 = r9[5];
+        const double m20 = r9[6], m21 = r9[7], m22 = r9[8];
+        const double cof00 = m11 * m22 - m12 * m21, cof11 = m22 * m00 - m20 * m02, cof22 = m00 * m11 - m01 * m10;
+        const double cof01 = m10 * m22 - m20 * m12, cof02 = m10 * m21 - m20 * m11, cof12 = m00 * m21 - m01 * m20;
+        const double cof10 = m01 * m22 - m02 * m21, cof20 = m01 * m12 - m02 * m11, cof21 = m00 * m12 - m10 * m02;
+        const double det = m00 * cof00 + m02 * cof02 - m01 * cof01;
+        float *o = &rec[12 * (size_t)q];
+        o[0] = (float)(cof00 / det);
+        o[1] = -(float)(cof10 / det);
+        o[2] = (float)(cof20 / det);
+        o[3] = -(float)(cof01 / det);
+        o[4] = (float)(cof11 / det);
+        o[5] = -(float)(cof21 / det);
+        o[6] = (float)(cof02 / det);
+        o[7] = -(float)(cof12 / det);
+        o[8] = (float)(cof22 / det);
+        o[9] = get_scale(t);
+        float tx = t[3], ty = t[7];
+        apply_precision(tx, ERR_LIMIT);
+        apply_precision(ty, ERR_LIMIT);
+        o[10] = tx;
+        o[11] = ty;
+        any_shift = any_shift || tx != 0.0f || ty != 0.0f;
+    }
+    const float gw = mode == 2 ? (float)(4.0 / (M_PI * M_PI)) : 1.0f;  // EMConsts::I2G for mode 2
+    const int chunk = std::max(1, std::min(n, 4096));
+    int rc = ensure_chunk(h, std::max(chunk, h->cap_n), std::max(P, h->cap_B));
+    if (rc) return rc;
+    if (!returnfft && (rc = ensure_ximgs(h, chunk))) return rc;
+    float *tmp = nullptr;  // the projections' records on the device
+    FH_TRY(h, hipMalloc(&tmp, sizeof(float) * 12 * (size_t)chunk));
+    float *rec_d = tmp;
+    hipError_t e = hipSuccess;
+    for (int c0 = 0; c0 < n && e == hipSuccess; c0 += chunk) {
+        const int m = std::min(chunk, n - c0);
+        e = hipMemcpyAsync(rec_d, &rec[12 * (size_t)c0], sizeof(float) * 12 * m, hipMemcpyHostToDevice, h->stream);
+        if (e == hipSuccess) e = launch_gauss_fft(h->pvol, P, m, mode, gw, rec_d, returnfft ? 1 : 0, h->slices, h->stream);
+        if (e != hipSuccess) break;
+        if (returnfft) {
+            e = hipMemcpyAsync(out + (size_t)c0 * P * (P + 2), h->slices, sizeof(float2) * (size_t)m * P * (H + 1),
+                               hipMemcpyDeviceToHost, h->stream);
+        } else {
+            // do_ift (x 1/P^2) written with the phase origin moved to the centre
+            e = launch_ift2d_batch(h->slices, m, P, h->ximgs, 1.0f / ((float)P * (float)P), h->plan, h->tw, h->stream);
+            float *img = h->ximgs;
+            if (e == hipSuccess && any_shift) {
+                // translate(): the "xform" processor's real-space resampling by the translation
+                xf6.assign(6 * (size_t)m, 0.f);
+                for (int q = 0; q < m; q++) {
+                    const float *o = &rec[12 * (size_t)(c0 + q)];
+                    float tr[12] = {1, 0, 0, o[10], 0, 1, 0, o[11], 0, 0, 1, 0};
+                    float inv[6] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f};
+                    xform2d_inverse(tr, inv);
+                    memcpy(&xf6[6 * (size_t)q], inv, sizeof(inv));
+                }
+                if ((rc = ensure_ximgs(h, chunk, true))) {
+                    (void)hipFree(tmp);
+                    return rc;
+                }
+                e = hipMemcpyAsync(h->xfd, xf6.data(), sizeof(float) * 6 * m, hipMemcpyHostToDevice, h->stream);
+                if (e == hipSuccess) e = launch_fill_f32(h->means, 0.f, m, h->stream);
+                if (e == hipSuccess) e = launch_xform2d(h->ximgs, m, P, P, h->means, h->xfd, h->ximgs2, h->stream);
+                img = h->ximgs2;
+            }
+            if (e == hipSuccess) e = launch_gausskernelfix2d(img, P, m, gw, h->stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(out + (size_t)c0 * P * P, img, sizeof(float) * (size_t)m * P * P, hipMemcpyDeviceToHost,
+                                   h->stream);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    }
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(h, FH_ERR_DEVICE, std::string("gauss_fft: ") + hipGetErrorString(e));
     return FH_OK;
 }
 

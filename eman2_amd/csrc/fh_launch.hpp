@@ -108,6 +108,10 @@ hipError_t launch_c2r_x(const float2 *data, int P, float *out, float scale, cons
                         const float2 *tw, hipStream_t s);
 hipError_t launch_fft3d_forward_corner(const float *vol, int P, const float *zeros, float2 *out, const FftPlan &plan,
                                        const float2 *tw, hipStream_t s);
+// gauss_fft projector (modes 0-2) of n projections (rec: 12 floats each) from the map's transform pv
+hipError_t launch_gauss_fft(const float2 *pv, int P, int n, int mode, float gw, const float *rec, int ramp,
+                            float2 *out, hipStream_t s);
+hipError_t launch_gausskernelfix2d(float *img, int P, int n, float gauss_width, hipStream_t s);
 hipError_t launch_merge_add(float2 *data, float *norm, const float2 *threed, const float *normv, size_t n,
                             hipStream_t s);
 hipError_t launch_merge_scale(float2 *data, const float *norm, size_t n, hipStream_t s);

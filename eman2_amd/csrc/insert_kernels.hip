@@ -1223,6 +1223,176 @@ hipError_t launch_project(const float2 *data, const float *norm, InsertConst c, 
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------ gauss_fft projector
+// GaussFFTProjector::project3d (libEM/projector.cpp:69-212) and interp_ft_3d
+// (:216-295) for modes 0-2 on the map's transform kept in the volume layout
+// (phase origin at the corner, rows and planes in wrap order).  The
+// reference interpolates the complex volume with its Fourier origin moved
+// to the centre in y and z and reads it by linear float index
+// 2 x + y (P+2) + z (P+2) P; pv_lin maps such an index (0 outside the array)
+// back to the wrap-ordered storage.  Per projection q: rec[12 q ..] =
+// inverse rotation rows (9), scale, tx, ty.  Output element (x, r) of
+// projection q is the slice pixel (x, row (r + P/2) mod P of the centred
+// slice): the Fourier origin back in the corner.
+__device__ __forceinline__ float pv_lin(const float2 *__restrict__ pv, long long L, int P, int H)
+{
+    const int nx = P + 2;
+    if (L < 0 || L >= (long long)nx * P * P) return 0.f;
+    const int xr = (int)(L % nx);
+    const long long q = L / nx;
+    const int y = (int)(q % P), z = (int)(q / P);
+    const float2 v = pv[cidx(xr >> 1, y + H < P ? y + H : y + H - P, z + H < P ? z + H : z + H - P, P, H)];
+    return (xr & 1) ? v.y : v.x;
+}
+
+__device__ __forceinline__ float agauss_f(float a, float dx, float dy, float dz, float d)
+{
+    return a * expf(-(dx * dx + dy * dy + dz * dz) / d);  // Util::agauss (util.h:912-915)
+}
+
+__device__ bool gfft_interp(int mode, const float2 *__restrict__ pv, int P, int H, float x, float y, float z,
+                            float &d0, float &d1, float gw)
+{
+    const int nx = P + 2, ny = P, nz = P;
+    if (mode == 0) {
+        const int x0 = (int)floor(x), y0 = (int)floor(y), z0 = (int)floor(z);
+        float a0 = 0.f, a1 = 0.f, nrm = 0.f;
+        for (int ix = x0; ix <= x0 + 1; ix++)
+            for (int iy = y0; iy <= y0 + 1; iy++)
+                for (int iz = z0; iz <= z0 + 1; iz++) {
+                    const float w = (float)((1.0 - (double)fabsf((float)ix - x)) * (1.0 - (double)fabsf((float)iy - y)) *
+                                            (1.0 - (double)fabsf((float)iz - z)));
+                    const long long L = 2LL * ix + (long long)iy * nx + (long long)iz * nx * ny;
+                    a0 += w * pv_lin(pv, L, P, H);
+                    a1 += w * pv_lin(pv, L + 1, P, H);
+                    nrm += w;
+                }
+        d0 = a0 / nrm;
+        d1 = a1 / nrm;
+        return true;
+    }
+    if (mode == 1) {
+        const int x0 = 2 * (int)floor(x + 0.5f), y0 = (int)floor(y + 0.5f), z0 = (int)floor(z + 0.5f);
+        const long long L = (long long)x0 + (long long)y0 * nx + (long long)z0 * nx * ny;
+        d0 = pv_lin(pv, L, P, H);
+        d1 = pv_lin(pv, L + 1, P, H);
+        return true;
+    }
+    const int x0 = (int)floor(x), y0 = (int)floor(y), z0 = (int)floor(z);
+    const float dx = x - x0, dy = y - y0, dz = z - z0;
+    if (!(x0 <= nx / 2 - 2 && y0 <= ny - 1 && z0 <= nz - 1)) return false;
+    const long long i0 = (long long)x0 * 2 + (long long)y0 * nx + (long long)z0 * nx * ny;
+    const float n = agauss_f(1, dx, dy, dz, gw) + agauss_f(1, 1 - dx, dy, dz, gw) + agauss_f(1, dx, 1 - dy, dz, gw) +
+                    agauss_f(1, 1 - dx, 1 - dy, dz, gw) + agauss_f(1, dx, dy, 1 - dz, gw) +
+                    agauss_f(1, 1 - dx, dy, 1 - dz, gw) + agauss_f(1, dx, 1 - dy, 1 - dz, gw) +
+                    agauss_f(1, 1 - dx, 1 - dy, 1 - dz, gw);
+    float d[2];
+    for (int c = 0; c < 2; c++) {
+        const long long i = i0 + c;
+        const long long nxy = (long long)nx * ny;
+        // the reference divides its last term only by n (operator precedence)
+        d[c] = agauss_f(pv_lin(pv, i, P, H), dx, dy, dz, gw) + agauss_f(pv_lin(pv, i + 2, P, H), 1 - dx, dy, dz, gw) +
+               agauss_f(pv_lin(pv, i + nx, P, H), dx, 1 - dy, dz, gw) +
+               agauss_f(pv_lin(pv, i + nx + 2, P, H), 1 - dx, 1 - dy, dz, gw) +
+               agauss_f(pv_lin(pv, i + nxy, P, H), dx, dy, 1 - dz, gw) +
+               agauss_f(pv_lin(pv, i + 2 + nxy, P, H), 1 - dx, dy, 1 - dz, gw) +
+               agauss_f(pv_lin(pv, i + nx + nxy, P, H), dx, 1 - dy, 1 - dz, gw) +
+               agauss_f(pv_lin(pv, i + 2 + nx + nxy, P, H), 1 - dx, 1 - dy, 1 - dz, gw) / n;
+    }
+    d0 = d[0];
+    d1 = d[1];
+    return true;
+}
+
+// ramp: the translation's phase ramp (the "xform" processor's complex
+// branch, as in k_project), for the Fourier-slice output
+__global__ void __launch_bounds__(256) k_gauss_fft(const float2 *__restrict__ pv, int P, int mode, float gw,
+                                                   const float *__restrict__ rec, int ramp, float2 *__restrict__ out)
+{
+    const int H = P / 2;
+    const int q = blockIdx.y;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= P * (H + 1)) return;
+    const int x = idx % (H + 1), r = idx / (H + 1);
+    const int yc = r + H < P ? r + H : r + H - P;  // the centred slice's row
+    const float *m = rec + 12 * (size_t)q;
+    float re = 0.f, im = 0.f;
+    const int yy_i = yc - P / 2;
+    if ((long long)x * x + (long long)yy_i * yy_i < (long long)(P / 2 - 1) * (P / 2 - 1)) {
+        const float cx = (float)x, cy = (float)yy_i, cz = 0.0f;
+        float xx = m[0] * cx + m[1] * cy + m[2] * cz + 0.0f;
+        float yy = m[3] * cx + m[4] * cy + m[5] * cz + 0.0f;
+        float zz = m[6] * cx + m[7] * cy + m[8] * cz + 0.0f;
+        int cc = 1;
+        if (xx < 0) {
+            xx = -xx;
+            yy = -yy;
+            zz = -zz;
+            cc = -1;
+        }
+        const float scale = m[9];
+        if (scale != 1.0f) {
+            xx *= scale;
+            yy *= scale;
+            zz *= scale;
+        }
+        yy += (float)(P / 2);
+        zz += (float)(P / 2);
+        if (!(yy < 0 || xx < 0 || zz < 0)) {
+            float d0, d1;
+            if (gfft_interp(mode, pv, P, H, xx, yy, zz, d0, d1, gw)) {
+                re = d0;
+                im = d1 * (float)cc;
+            }
+        }
+    }
+    const float xshift = m[10], yshift = m[11];
+    if (ramp && (xshift != 0.0f || yshift != 0.0f)) {
+        const double pi = 3.141592653589793238462643383279502884;
+        const int ky = r >= H + 1 ? r - P : r;
+        const float phaseY = (float)(-2 * pi * ky * (double)yshift / P);
+        const float phase = (float)((double)phaseY - 2 * pi * x * (double)xshift / P);
+        const double cp = cos((double)phase), sp = sin((double)phase);
+        const float rb = (float)((double)re * cp - (double)im * sp);
+        const float ib = (float)((double)re * sp + (double)im * cp);
+        re = rb;
+        im = ib;
+    }
+    out[(size_t)q * P * (H + 1) + idx] = make_float2(re, im);
+}
+
+// math.gausskernelfix on n P x P images (MaskGaussInvProcessor,
+// processor.h:4131-4176): each pixel over exp(-dist sv), dist the squared
+// distance from (P/2, P/2), sv = gauss_width / P^2
+__global__ void k_gausskernelfix2d(float *__restrict__ img, int P, int n, float sv)
+{
+    const size_t tot = (size_t)n * P * P;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+        const int x = (int)(i % P), y = (int)((i / P) % P);
+        const float xc = (float)(P / 2), yc = (float)(P / 2);
+        const float dist = ((float)x - xc) * ((float)x - xc) + ((float)y - yc) * ((float)y - yc) + 0.0f;
+        img[i] = img[i] / expf(-dist * sv);
+    }
+}
+
+hipError_t launch_gauss_fft(const float2 *pv, int P, int n, int mode, float gw, const float *rec, int ramp,
+                            float2 *out, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const int m = P * (P / 2 + 1);
+    hipLaunchKernelGGL(k_gauss_fft, dim3((m + 255) / 256, n), dim3(256), 0, s, pv, P, mode, gw, rec, ramp, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_gausskernelfix2d(float *img, int P, int n, float gauss_width, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const float sv = gauss_width / (float)(P * P);
+    const unsigned blocks = (unsigned)std::min<size_t>(((size_t)n * P * P + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_gausskernelfix2d, dim3(blocks), dim3(256), 0, s, img, P, n, sv);
+    return hipGetLastError();
+}
+
 hipError_t launch_agree(const float2 *slices, int nsym, const float *t3d, int nslices, InsertConst c, float rmax,
                         const float2 *data, const float *norm, const float *rtab, double *partial, hipStream_t s)
 {

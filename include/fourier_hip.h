@@ -219,6 +219,18 @@ int fh_determine_slice_agreement_noalloc(fh_reconstructor *h, const float *slice
  * with the previous map). */
 int fh_setup_seed_map(fh_reconstructor *h, const float *map);
 
+/* The gauss_fft projector (GaussFFTProjector::project3d, projector.cpp:
+ * 69-295; e2make3dpar.py:716-721 projects the reference with it), modes 0
+ * (trilinear, the default), 1 (nearest) and 2 (Gaussian): fh_gauss_fft_volume
+ * keeps the transform of a real P^3 map (host) on the handle's device;
+ * fh_gauss_fft_project makes n projections at the Transforms xforms (n x 12,
+ * host): returnfft -> n x P x (P+2) Fourier slices with the translation's
+ * phase ramp, else n x P x P images (inverse FFT, phase origin centred, the
+ * translation resampled, math.gausskernelfix).  Mirrored Transforms are
+ * refused. */
+int fh_gauss_fft_volume(fh_reconstructor *h, const float *map);
+int fh_gauss_fft_project(fh_reconstructor *h, int n, const float *xforms, int mode, int returnfft, float *out);
+
 /* The task merge of e2spa_make3d.py:175-192: after fh_setup(h), each task's
  * finish(doift=0) volume `threed` (P x P x (P+2) floats) times its normout
  * (P x P x (P/2+1), fh_finish's norm_out) is added to h's volume and the

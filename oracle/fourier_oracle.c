@@ -1374,6 +1374,200 @@ static void iter_trilinear(const float *ref, int P, float xx, float yy, float zz
     *im = ai;
 }
 
+/* ======================================================================== */
+/* gauss_fft projector                                                       */
+/* ======================================================================== */
+
+/* Transform::invert of a rotation (libEM/transform.cpp:1324-1356): cofactors
+ * in double, rounded to float */
+static void invert9(const float m12[12], float r[9])
+{
+    double m00 = m12[0], m01 = m12[1], m02 = m12[2], m10 = m12[4], m11 = m12[5], m12_ = m12[6];
+    double m20 = m12[8], m21 = m12[9], m22 = m12[10];
+    double cof00 = m11 * m22 - m12_ * m21, cof11 = m22 * m00 - m20 * m02, cof22 = m00 * m11 - m01 * m10;
+    double cof01 = m10 * m22 - m20 * m12_, cof02 = m10 * m21 - m20 * m11, cof12 = m00 * m21 - m01 * m20;
+    double cof10 = m01 * m22 - m02 * m21, cof20 = m01 * m12_ - m02 * m11, cof21 = m00 * m12_ - m10 * m02;
+    double det = m00 * cof00 + m02 * cof02 - m01 * cof01;
+    r[0] = (float)(cof00 / det);
+    r[1] = -(float)(cof10 / det);
+    r[2] = (float)(cof20 / det);
+    r[3] = -(float)(cof01 / det);
+    r[4] = (float)(cof11 / det);
+    r[5] = -(float)(cof21 / det);
+    r[6] = (float)(cof02 / det);
+    r[7] = -(float)(cof12 / det);
+    r[8] = (float)(cof22 / det);
+}
+
+/* GaussFFTProjector::interp_ft_3d (libEM/projector.cpp:216-295), modes 0
+ * (trilinear, normalised by its weights), 1 (nearest) and 2 (agauss with
+ * gauss width gw; the reference's "/ n" applies to its last term only) on
+ * the centred complex volume f ((P+2) x P x P floats); out-of-volume reads
+ * (a scaled transform only) are taken as 0 */
+static float fget(const float *f, int P, int i)
+{
+    return (i >= 0 && (size_t)i < (size_t)(P + 2) * P * P) ? f[i] : 0.0f;
+}
+static float agauss(float a, float dx, float dy, float dz, float d)
+{
+    return a * expf(-(dx * dx + dy * dy + dz * dz) / d); /* Util::agauss, util.h:912-915 */
+}
+static int interp_ft_3d(int mode, const float *f, int P, float x, float y, float z, float *data, float gw)
+{
+    const int nx = P + 2, ny = P, nz = P;
+    if (mode == 0) {
+        int x0 = (int)floor(x), y0 = (int)floor(y), z0 = (int)floor(z);
+        float d0 = 0, d1 = 0, nrm = 0;
+        for (int ix = x0; ix <= x0 + 1; ix++)
+            for (int iy = y0; iy <= y0 + 1; iy++)
+                for (int iz = z0; iz <= z0 + 1; iz++) {
+                    float w = (float)((1.0 - fabsf((float)ix - x)) * (1.0 - fabsf((float)iy - y)) *
+                                      (1.0 - fabsf((float)iz - z)));
+                    d0 += w * fget(f, P, 2 * ix + iy * nx + iz * nx * ny);
+                    d1 += w * fget(f, P, 2 * ix + iy * nx + iz * nx * ny + 1);
+                    nrm += w;
+                }
+        data[0] = d0 / nrm;
+        data[1] = d1 / nrm;
+        return 1;
+    }
+    if (mode == 1) {
+        int x0 = 2 * (int)floor(x + 0.5f), y0 = (int)floor(y + 0.5f), z0 = (int)floor(z + 0.5f);
+        data[0] = fget(f, P, x0 + y0 * nx + z0 * nx * ny);
+        data[1] = fget(f, P, x0 + y0 * nx + z0 * nx * ny + 1);
+        return 1;
+    }
+    int x0 = (int)floor(x), y0 = (int)floor(y), z0 = (int)floor(z);
+    float dx = x - x0, dy = y - y0, dz = z - z0;
+    if (x0 <= nx / 2 - 2 && y0 <= ny - 1 && z0 <= nz - 1) {
+        int i = (int)(x0 * 2 + y0 * nx + z0 * nx * ny);
+        float n = agauss(1, dx, dy, dz, gw) + agauss(1, 1 - dx, dy, dz, gw) + agauss(1, dx, 1 - dy, dz, gw) +
+                  agauss(1, 1 - dx, 1 - dy, dz, gw) + agauss(1, dx, dy, 1 - dz, gw) +
+                  agauss(1, 1 - dx, dy, 1 - dz, gw) + agauss(1, dx, 1 - dy, 1 - dz, gw) +
+                  agauss(1, 1 - dx, 1 - dy, 1 - dz, gw);
+        for (int c = 0; c < 2; c++, i++)
+            data[c] = agauss(fget(f, P, i), dx, dy, dz, gw) + agauss(fget(f, P, i + 2), 1 - dx, dy, dz, gw) +
+                      agauss(fget(f, P, i + nx), dx, 1 - dy, dz, gw) +
+                      agauss(fget(f, P, i + nx + 2), 1 - dx, 1 - dy, dz, gw) +
+                      agauss(fget(f, P, i + nx * ny), dx, dy, 1 - dz, gw) +
+                      agauss(fget(f, P, i + 2 + nx * ny), 1 - dx, dy, 1 - dz, gw) +
+                      agauss(fget(f, P, i + nx + nx * ny), dx, 1 - dy, 1 - dz, gw) +
+                      agauss(fget(f, P, i + 2 + nx + nx * ny), 1 - dx, 1 - dy, 1 - dz, gw) / n;
+        return 1;
+    }
+    return 0;
+}
+
+/* GaussFFTProjector::project3d (libEM/projector.cpp:69-212) of a real P^3
+ * map, modes 0-2: the map's FFT with the phase origin at the corner and the
+ * Fourier origin moved to the centre in y and z; each slice pixel (x, y -
+ * P/2) inside radius P/2 - 1 takes the interpolated value at inverse(R) .
+ * (x, y - P/2, 0) (conjugated when that lands at x < 0; times the
+ * Transform's scale); returnfft: the slice with its Fourier origin back in
+ * the corner and the translation's phase ramp ((P, P+2) floats), else the
+ * real image: inverse FFT, phase origin to the centre, the translation
+ * resampled, math.gausskernelfix with the mode's width ((P, P) floats).
+ * Mirror transforms are not handled here (return -1). */
+int or_gauss_fft_project(int P, const float *map, const float t[12], int mode, int returnfft, float *out)
+{
+    const int nx = P + 2, ny = P, nz = P, H = P / 2;
+    if (get_determinant(t) < 0) return -1;
+    /* xform.phaseorigin.tocorner, do_fft, xform.fourierorigin.tocenter */
+    float *m = (float *)malloc(sizeof(float) * P * P * P);
+    memcpy(m, map, sizeof(float) * P * P * P);
+    or_phaseorigin_tocorner(m, P, P, P);
+    float *w = (float *)malloc(sizeof(float) * nx * ny * nz);
+    or_fft_r2c_3d(m, P, P, P, w);
+    float *f = (float *)malloc(sizeof(float) * nx * ny * nz);
+    for (int z = 0; z < nz; z++)
+        for (int y = 0; y < ny; y++)
+            memcpy(f + (size_t)nx * (((y + H) % ny) + (size_t)ny * ((z + H) % nz)), w + (size_t)nx * (y + (size_t)ny * z),
+                   sizeof(float) * nx);
+    float rot[12], r[9];
+    memcpy(rot, t, sizeof(rot));
+    or_rotation_only(rot); /* get_rotation_transform (transform.cpp:711-719) */
+    invert9(rot, r);
+    const float scale = get_scale(t);
+    float gw = 1;
+    if (mode == 2) gw = OR_I2G;
+    float *tmp = (float *)calloc((size_t)nx * ny, sizeof(float));
+    for (int y = 0; y < ny; y++)
+        for (int x = 0; x < nx / 2; x++) {
+            int ii = x * 2 + y * nx;
+            if (hypot((double)x, (double)(y - ny / 2)) >= ny / 2 - 1) continue;
+            float cx = (float)x, cy = (float)(y - ny / 2), cz = 0.0f;
+            float xx = r[0] * cx + r[1] * cy + r[2] * cz + 0.0f; /* Transform::transform (transform.h:440-447) */
+            float yy = r[3] * cx + r[4] * cy + r[5] * cz + 0.0f;
+            float zz = r[6] * cx + r[7] * cy + r[8] * cz + 0.0f;
+            int cc = 1;
+            if (xx < 0) {
+                xx = -xx;
+                yy = -yy;
+                zz = -zz;
+                cc = -1;
+            }
+            if (scale != 1.0) {
+                xx *= scale;
+                yy *= scale;
+                zz *= scale;
+            }
+            yy += ny / 2;
+            zz += nz / 2;
+            if (yy < 0 || xx < 0 || zz < 0) continue;
+            if (interp_ft_3d(mode, f, P, xx, yy, zz, tmp + ii, gw)) tmp[ii + 1] *= cc;
+            else tmp[ii] = tmp[ii + 1] = 0;
+        }
+    /* xform.fourierorigin.tocorner: rows back to wrap order */
+    float *ret = (float *)calloc((size_t)nx * ny, sizeof(float));
+    for (int y = 0; y < ny; y++) memcpy(ret + (size_t)nx * ((y + H) % ny), tmp + (size_t)nx * y, sizeof(float) * nx);
+    /* the translation (Transform::get_trans, transform.cpp:1077-1092) */
+    float tx = t[3], ty = t[7];
+    apply_precision(&tx, OR_ERR_LIMIT);
+    apply_precision(&ty, OR_ERR_LIMIT);
+    if (returnfft) {
+        /* "xform" with (tx, ty) on the complex slice: the phase ramp of TransformProcessor's complex branch */
+        if (tx != 0.0f || ty != 0.0f) {
+            const long double pi = 3.141592653589793238462643383279502884197L;
+            for (int kyN = 0; kyN < ny; kyN++) {
+                int kyNew = kyN >= nx / 2 ? kyN - ny : kyN;
+                float phaseY = (float)(-2 * pi * kyNew * ty / ny);
+                for (int kxN = 0; kxN < nx / 2; kxN++) {
+                    float tempR = ret[2 * kxN + kyN * nx], tempI = ret[2 * kxN + 1 + kyN * nx];
+                    float phase = (float)(phaseY - 2 * pi * kxN * tx / ny);
+                    out[2 * kxN + nx * kyN] = (float)(tempR * cos((double)phase) - tempI * sin((double)phase)) / 1.0f;
+                    out[2 * kxN + 1 + nx * kyN] = (float)(tempR * sin((double)phase) + tempI * cos((double)phase)) / 1.0f;
+                }
+            }
+        } else {
+            memcpy(out, ret, sizeof(float) * nx * ny);
+        }
+    } else {
+        /* do_ift, xform.phaseorigin.tocenter, translate, math.gausskernelfix */
+        float *img = (float *)malloc(sizeof(float) * P * P);
+        or_fft_c2r_2d(ret, P, P, img);
+        or_phaseorigin_tocenter(img, P, P, 1);
+        if (tx != 0.0f || ty != 0.0f) {
+            float tr[12] = {1, 0, 0, tx, 0, 1, 0, ty, 0, 0, 1, 0};
+            or_xform2d(img, P, P, tr, out);
+            memcpy(img, out, sizeof(float) * P * P);
+        }
+        const float sv = gw / (float)(ny * ny); /* MaskGaussInvProcessor (processor.h:4131-4176) */
+        const float xc = (float)(P / 2), yc = (float)(P / 2);
+        for (int y = 0; y < P; y++)
+            for (int x = 0; x < P; x++) {
+                float dist = ((float)x - xc) * ((float)x - xc) + ((float)y - yc) * ((float)y - yc) + 0.0f;
+                out[y * P + x] = img[y * P + x] / expf(-dist * sv);
+            }
+        free(img);
+    }
+    free(ret);
+    free(tmp);
+    free(f);
+    free(w);
+    free(m);
+    return 0;
+}
+
 /* FourierIterReconstructor::insert_slice (libEM/reconstructor.cpp:520-637).
  * Without a reference volume: a 6x6x6 Gaussian stencil from floor(p - 2.5)
  * with gauss_var's radius-dependent width; with one (setup_seed): the slice

@@ -92,6 +92,9 @@ void or_iter_insert(int P, float *data, float *norm, const float *slice, const f
                     int nsym, float weight, const float *ref_or_null);
 void or_iter_ref_from_map(int P, const float *map, float *ref);
 
+/* ---- gauss_fft projector (libEM/projector.cpp:69-295), modes 0-2 -------- */
+int or_gauss_fft_project(int P, const float *map, const float t[12], int mode, int returnfft, float *out);
+
 /* ---- volume insertion (libEM/reconstructor.cpp:1122-1210,
  *      libEM/reconstructor_tools.cpp:128-239, libEM/emdata_core.h:368-413) */
 void or_insert_slice_work(int P, float *data, float *norm, const float *slice,

@@ -69,6 +69,7 @@ def lib(fast=False):
         "or_prep_particle_xform": (v, [_fp, i, i, ctypes.POINTER(CtfStruct), _fp, _fp]),
         "or_prep_particle_dfdf": (v, [_fp, i, i, ctypes.POINTER(CtfStruct), f, _fp, _fp]),
         "or_iter_insert": (v, [i, _fp, _fp, _fp, _fp, _fp, i, f, _fp]),
+        "or_gauss_fft_project": (i, [i, _fp, _fp, i, i, _fp]),
         "or_iter_ref_from_map": (v, [i, _fp, _fp]),
         "or_insert_slice_ssnr": (i, [i, _fp, _fp, _fp, _fp, _fp, i, _fp, i, i]),
         "or_edge_mean": (f, [_fp, i, i]),
@@ -287,6 +288,17 @@ def prep_particle(raw, P, ctf=None, xform=None, dfdf=None):
         lib().or_prep_particle(_f(raw), B, P, cp, _f(out))
     else:
         lib().or_prep_particle_xform(_f(raw), B, P, cp, _f(f32(xform).reshape(12)), _f(out))
+    return out
+
+
+def gauss_fft_project(vol, xform, mode=0, returnfft=1):
+    """GaussFFTProjector::project3d of a real P^3 map: the (P, P+2) Fourier
+    slice (returnfft) or the (P, P) image"""
+    vol = np.ascontiguousarray(f32(vol))
+    P = vol.shape[0]
+    out = np.zeros((P, P + 2) if returnfft else (P, P), np.float32)
+    rc = lib().or_gauss_fft_project(P, _f(vol), _f(f32(xform).reshape(12).copy()), int(mode), int(returnfft), _f(out))
+    assert rc == 0
     return out
 
 
