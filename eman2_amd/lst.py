@@ -9,7 +9,8 @@ spaces.  The JSON dict overrides the particle's header; a Transform is
 {"__class__": "Transform", "matrix": "[m00,m01,...,m23]"}, the twelve 3x4
 values printed with %1.6g (EMAN2jsondb.py:192-206, transform.cpp:169-178).
 
-Particles listed here are read from MRC stacks (one particle per section);
+Particles listed here are read from MRC stacks (one particle per section)
+or EMAN2 HDF5 stacks (eman2_amd/hdf.py);
 other image formats are not read.
 """
 import json
@@ -124,14 +125,23 @@ def load_particles(path, key="xform.projection"):
     imgs, xfs = [], []
     for n, fname, d in recs:
         p = resolve_image_path(path, fname)
-        if not p.lower().endswith((".mrc", ".mrcs", ".map")):
-            raise ValueError("%s: only MRC stacks are read (record %d names %s)" % (path, n, fname))
-        if p not in stacks:
-            stacks[p] = mrc.read(p)[0]
-        st = stacks[p]
-        if not 0 <= n < st.shape[0]:
-            raise IndexError("%s: particle %d of %s, which holds %d" % (path, n, fname, st.shape[0]))
-        imgs.append(np.asarray(st[n], np.float32))
+        if p.lower().endswith((".hdf", ".h5")):
+            if p not in stacks:
+                from . import hdf
+                stacks[p] = hdf.HdfStack(p)
+            st = stacks[p]
+            if not 0 <= n < len(st):
+                raise IndexError("%s: particle %d of %s, which holds %d" % (path, n, fname, len(st)))
+            imgs.append(np.asarray(st.read_image(n), np.float32))
+        elif p.lower().endswith((".mrc", ".mrcs", ".map")):
+            if p not in stacks:
+                stacks[p] = mrc.read(p)[0]
+            st = stacks[p]
+            if not 0 <= n < st.shape[0]:
+                raise IndexError("%s: particle %d of %s, which holds %d" % (path, n, fname, st.shape[0]))
+            imgs.append(np.asarray(st[n], np.float32))
+        else:
+            raise ValueError("%s: MRC and HDF5 stacks are read (record %d names %s)" % (path, n, fname))
         if key not in d:
             raise KeyError("%s: record for particle %d of %s has no %s" % (path, n, fname, key))
         xfs.append(np.asarray(d[key], np.float32).reshape(3, 4))

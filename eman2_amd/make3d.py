@@ -222,9 +222,44 @@ def reconstruct_lst(recon, lst_path, out_path=None, apix=None, **kw):
     imgs, xforms, recs = lst.load_particles(lst_path)
     if apix is None:
         first = lst.resolve_image_path(lst_path, recs[0][1]) if recs else None
-        apix = mrc.read_header(first)["apix"] if first else 1.0
+        if first is None:
+            apix = 1.0
+        elif first.lower().endswith((".hdf", ".h5")):
+            from . import hdf
+            apix = float(hdf.HdfStack(first).header(recs[0][0]).get("apix_x", 1.0))
+        else:
+            apix = mrc.read_header(first)["apix"]
     out, info = reconstruct(recon, imgs, xforms, apix=apix, **kw)
     if out_path is not None:
-        vol = out.numpy() if hasattr(out, "numpy") else np.asarray(out)
-        mrc.write(out_path, vol, apix=apix)
+        write_map(out_path, out, apix)
+    return out, info
+
+
+def write_map(path, vol, apix, compressbits=10):
+    """the finished map to disk: an HDF5 path as e2make3dpar.py:494 writes it
+    (write_compressed with compressbits, 10 by default; negative for plain
+    float), anything else as MRC2014"""
+    from . import mrc
+    v = vol.numpy() if hasattr(vol, "numpy") else np.asarray(vol, np.float32)
+    if path.lower().endswith((".hdf", ".h5")):
+        from . import hdf
+        hdf.write_compressed(path, v, bits=compressbits,
+                             header={"apix_x": float(apix), "apix_y": float(apix), "apix_z": float(apix)})
+    else:
+        mrc.write(path, v, apix=apix)
+
+
+def reconstruct_hdf(recon, stack_path, key="xform.projection", out_path=None, compressbits=10, **kw):
+    """reconstruct() over an EMAN2 HDF5 particle stack whose images carry
+    their orientation under key (and apix_x); the map goes to out_path when
+    given (write_map)"""
+    from . import hdf
+    imgs, heads = hdf.read_stack(stack_path)
+    if any(key not in h for h in heads):
+        raise KeyError("%s: an image has no %s" % (stack_path, key))
+    xforms = np.stack([np.asarray(h[key], np.float32).reshape(3, 4) for h in heads])
+    apix = kw.pop("apix", float(heads[0].get("apix_x", 1.0)) if heads else 1.0)
+    out, info = reconstruct(recon, imgs, xforms, apix=apix, **kw)
+    if out_path is not None:
+        write_map(out_path, out, apix, compressbits)
     return out, info
