@@ -27,6 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+ENGINE_CLOCK_HZ = 2.4e9  # MI355X peak engine clock
 CTF = dict(voltage=300.0, cs=2.7, ampcont=10.0, bfactor=80.0)
 
 
@@ -351,10 +352,14 @@ def main():
     elapsed = time.perf_counter() - t0
     value = a.steps * n / elapsed
 
-    # roofline of the dominant kernel (k_scatter_insert): scatter model, one
-    # 12-byte (complex + weight) update per gridding contribution, 8 per
-    # pixel per symmetry copy
-    bytes_per_particle = nsym * npix(P) * 8 * 12
+    # roofline of the dominant kernel (k_scatter_insert; k_scatter_wide for
+    # the wide stencils): scatter model, one 12-byte (complex + weight) update
+    # per gridding contribution, 8 per pixel per symmetry copy (S^3 for a
+    # stencil of S cells per axis)
+    wide = {"gauss_var": 6, "gauss_5": 6, "gauss_3": 3, "kaiser_bessel": 5, "gridding_5": 5, "gridding_7": 7,
+            "kaiser_bessel_derived": 9}.get(a.mode)
+    kname = "k_scatter_wide" if wide else "k_scatter_insert"
+    bytes_per_particle = nsym * npix(P) * (wide ** 3 if wide else 8) * 12
     launches = sum(s["chunks"] for s in stats)
     avg_ms = kern_ms / max(1, launches)
     per_launch_bytes = bytes_per_particle * (a.steps * n) / max(1, launches)
@@ -367,12 +372,30 @@ def main():
     if os.path.exists(tf):
         with open(tf) as f:
             t = json.load(f)
-        if t.get("P") == P and t.get("sym") == a.sym and t.get("box") == B:
+        if t.get("P") == P and t.get("sym") == a.sym and t.get("box") == B and t.get("mode", "gauss_2") == a.mode:
             traffic = round(t["hbm_bytes_per_particle"] * (a.steps * n) / max(1, launches))
     roof = {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": traffic,
-            "kernel": "k_scatter_insert", "avg_launch_ms": round(avg_ms, 3),
+            "kernel": kname, "avg_launch_ms": round(avg_ms, 3),
             "algorithmic_bytes_per_launch": int(per_launch_bytes)}
+    if wide and achieved:
+        # the wide-stencil scatter accumulates in LDS and is VALU-issue bound: its roofline is the
+        # CUs' VALU issue rate (one wave64 instruction per CU per clock) over the VALU wave
+        # instructions per contribution that the PMC run of this shape counted
+        contrib = nsym * npix(P) * wide ** 3 * (a.steps * n) / max(1, launches)
+        gcs = contrib / (avg_ms * 1e-3) / 1e9
+        peak = None
+        pf = os.path.join(ROOT, "profiles", "scatter_wide_pmc.json")
+        if os.path.exists(pf):
+            with open(pf) as f:
+                t = json.load(f)
+            if t.get("P") == P and t.get("sym") == a.sym and t.get("box") == B and t.get("mode") == a.mode:
+                cus = torch.cuda.get_device_properties(0).multi_processor_count
+                peak = cus * ENGINE_CLOCK_HZ / t["valu_insts_per_contribution"] / 1e9
+        roof = {"bound": "valu", "achieved": round(gcs, 1), "peak": round(peak, 1) if peak else None,
+                "unit": "G contributions/s", "frac": round(gcs / peak, 4) if peak else None, "traffic": None,
+                "kernel": kname, "avg_launch_ms": round(avg_ms, 3), "contributions_per_launch": int(contrib),
+                "scatter_model_GBs": round(achieved, 1)}
     # whole-pipeline effective bandwidth: the scatter-model bytes plus the 4*B^2 raw read
     pipe_bytes = (4 * B * B + bytes_per_particle) * a.steps * n
     pipeline = {"effective_GBs": round(pipe_bytes / elapsed / 1e9, 1),
@@ -460,6 +483,8 @@ def main():
         d7[:, :, :3] = ops.reshape(14, 3, 3)
         ph3 = S.symmetric_phantom(S.phantom(30, 60.0, seed=5), d7)
         out["configs"] = {"d7_pad288": measure_insert(288, 192, 100000, "d7", "gauss_2", 2, 1, phantom=ph3)}
+        # gauss_var, e2make3dpar's default inserter (6^3 stencils, the wide-stencil brick scatter)
+        out["configs"]["gauss_var_pad384"] = measure_insert(384, 256, 50000, "c1", "gauss_var", 2, 1)
         ra = argparse.Namespace(**dict(vars(a), box=400, pad=600, apix=0.8, particles=a.refine_particles,
                                        steps=2, warmup=1, sym="c1", mode="gauss_2"))
         rf = refine(ra, quiet=True)

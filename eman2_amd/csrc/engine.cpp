@@ -505,6 +505,7 @@ static int ensure_chunk(fh_reconstructor *h, int n, int B)
 // Larger chunks put more slices into each work item, so fewer partial
 // bricks are written and merged per particle.
 constexpr long long AUTO_CHUNK_BYTES = 48LL << 30;
+constexpr size_t WIDE_LDS_PER_CU = 160u << 10;  // LDS of one gfx950 compute unit
 static int chunk_for(const fh_tuning &t, int P, int box, long long extra = 0)
 {
     if (t.chunk > 0) return t.chunk;
@@ -512,10 +513,38 @@ static int chunk_for(const fh_tuning &t, int P, int box, long long extra = 0)
     return (int)std::max(256LL, std::min(65536LL, AUTO_CHUNK_BYTES / per));
 }
 
-static BrickGeom brick_geom(int P)
+// The brick grid of an inserter mode: gauss_2 and trilinear (stencil origin
+// floor(p), origins -P/2-1 .. P/2, 8^3 origins per brick), or for the wide
+// stencils of S cells per axis 4^3 origins per brick over the origins from
+// the lowest one whose stencil reaches a cell in [-P/2, P/2] and that
+// wide_setup accepts, up to P/2 (higher origins reach no cell).
+static BrickGeom brick_geom(int P, int mode)
 {
     BrickGeom g;
-    g.nb = (P + 2 + BRICK - 1) / BRICK;  // origins -P/2-1 .. P/2
+    const int H = P / 2, S = wide_cells(mode);
+    if (S == 0) {
+        g.bs = BRICK;
+        g.halo = HALO;
+        g.ow = -H - 1;
+        g.po = (float)(-H - 1);
+    } else {
+        int ow;
+        float off;  // origin(p) = floor(p - off)
+        switch (mode) {
+        case 5: ow = -H - 2; off = 0.5f; break;  // gauss_3
+        case 6: ow = -H - 4; off = 2.0f; break;  // kaiser_bessel
+        case 7: ow = -H - 4; off = 1.5f; break;  // gridding_5
+        case 8: ow = -H - 6; off = 2.5f; break;  // gridding_7
+        case 9: ow = -H - 7; off = 4.0f; break;  // kaiser_bessel_derived
+        default: ow = -H - 4; off = 2.5f; break; // gauss_var, gauss_5, fourier_iter
+        }
+        g.bs = WBS;
+        g.halo = WBS + S - 1;
+        g.ow = ow;
+        g.po = (float)ow + off;
+    }
+    g.nb = (H - g.ow + 1 + g.bs - 1) / g.bs;
+    g.part_floats = 3 * g.halo * g.halo * g.halo;
     g.nbricks = g.nb * g.nb * g.nb;
     g.nkeys = 3 * g.nbricks;
     return g;
@@ -525,7 +554,7 @@ static int ensure_bins(fh_reconstructor *h, long long nslices)
 {
     if (nslices <= h->bins_cap_slices) return FH_OK;
     free_bins(h);
-    const BrickGeom g = brick_geom(h->P);
+    const BrickGeom g = brick_geom(h->P, h->mode);
     h->geom = g;
     BinBuffers &b = h->bins;
     // at most 5 bricks per brick column per slice (4 shown, see k_bin_cols)
@@ -544,7 +573,8 @@ static int ensure_bins(fh_reconstructor *h, long long nslices)
     return FH_OK;
 }
 
-// Partial bricks for `items` work items (grown on demand, 6 KiB each).
+// Partial bricks for `items` work items (grown on demand; 8.5 KiB each for
+// gauss_2 and gauss_var).
 static int ensure_partials(fh_reconstructor *h, long long items)
 {
     BinBuffers &b = h->bins;
@@ -552,7 +582,7 @@ static int ensure_partials(fh_reconstructor *h, long long items)
     if (b.part) { (void)hipFree(b.part); b.part = nullptr; }
     b.part_cap = 0;
     const long long cap = items + items / 4 + 1024;
-    FH_TRY(h, hipMalloc(&b.part, sizeof(float) * PART_FLOATS * (size_t)cap));
+    FH_TRY(h, hipMalloc(&b.part, sizeof(float) * h->geom.part_floats * (size_t)cap));
     b.part_cap = cap;
     return FH_OK;
 }
@@ -572,6 +602,37 @@ static InsertConst iconst(const fh_reconstructor *h)
 }
 
 // ====================================================================== C ABI
+// InsertConst::mode of an inserter name (0 for gauss_2 and unknown names)
+static int inserter_index(const char *m)
+{
+    static const char *names[] = {"gauss_2", "trilinear", "gauss_var", "gauss_5", "nearest_neighbor", "gauss_3",
+                                  "kaiser_bessel", "gridding_5", "gridding_7", "kaiser_bessel_derived",
+                                  "fourier_iter"};
+    for (int i = 0; i < 11; i++)
+        if (!strcmp(m, names[i])) return i;
+    return 0;
+}
+
+// Device bytes of a handle of size P and inserter `mode` (a mode name; unknown
+// names count as gauss_2) after setup, with batches of `chunk` particles.
+long long footprint_bytes_for_mode(int P, int nsym, int chunk, const char *mode_name)
+{
+    const int mode = mode_name ? inserter_index(mode_name) : 0;
+    if (chunk <= 0) chunk = chunk_for(fh_tuning{1, 1, 0, 1, 0}, P, P);
+    const long long v = 12LL * P * P * (P / 2 + 1);
+    const long long sl = 8LL * chunk * P * (P / 2 + 1) + 4LL * chunk * P * P;
+    const BrickGeom g = brick_geom(P, mode);
+    const long long nslices = (long long)chunk * nsym;
+    const long long list = 5LL * g.nb * g.nb * nslices;
+    const long long items = list / 64 + g.nkeys + 1;
+    const long long bins = 4LL * list + 16LL * items + 16LL * g.nkeys + 3LL * (long long)sizeof(SliceRec) * nslices;
+    // partial bricks: about 1.1 (P/bs)^2 (slice, brick) pairs per slice, one
+    // item per 64 pairs of a key plus one per key the slices reach
+    const double pb = (double)P / g.bs;
+    const long long part_items = (long long)(1.1 * pb * pb * nslices / 64) + g.nkeys / 2;
+    return v + sl + bins + 4LL * g.part_floats * part_items;
+}
+
 extern "C" {
 
 const char *fh_last_error(const fh_reconstructor *h) { return h ? h->err.c_str() : g_create_error.c_str(); }
@@ -1234,7 +1295,7 @@ static int insert_batch_impl(fh_reconstructor *h, const float *imgs, int n, int 
     // a CTF flip before a 2-D resampling stages two P x P images per particle
     const long long xbytes = xfh ? ((defocus && ctf) ? 8LL : 4LL) * h->P * h->P : 0;
     // the brick list (at most 5 nb^2 entries per slice) is indexed with int
-    const BrickGeom bg = brick_geom(h->P);
+    const BrickGeom bg = brick_geom(h->P, h->mode);
     const long long max_slices = std::max(1LL, (long long)INT32_MAX / (5LL * bg.nb * bg.nb) - 1);
     const int chunk = (int)std::max(1LL, std::min({(long long)chunk_for(h->tune, h->P, box, xbytes),
                                                    (long long)std::max(n, 1), max_slices / h->nsym}));
@@ -1270,9 +1331,8 @@ static int insert_batch_impl(fh_reconstructor *h, const float *imgs, int n, int 
                                      hipMemcpyHostToDevice, h->stream));
         FH_TRY(h, launch_compose(h->rots, h->wts, m, h->syms_d, h->nsym, h->t3d, h->wq, h->stream));
         FH_TRY(h, hipEventRecord(h->ev[1], h->stream));
-        if (h->mode >= 2) {
-            // wider or other stencils (gauss_var, gauss_5, gauss_3, nearest_neighbor, kaiser_bessel,
-            // kaiser_bessel_derived, gridding_5, gridding_7): the LDS-window tile form
+        if (h->mode >= 2 && !wide_cells(h->mode)) {
+            // nearest_neighbor (one cell per pixel): the LDS-window tile form
             FH_TRY(h, launch_insert_plain_batch(h->slices, m * h->nsym, h->nsym, h->t3d, h->wq, c, h->data, h->norm,
                                                 h->exptab, h->rtab, rwh ? h->rw : nullptr, h->stream));
             FH_TRY(h, hipEventRecord(h->ev[5], h->stream));
@@ -1298,7 +1358,15 @@ static int insert_batch_impl(fh_reconstructor *h, const float *imgs, int n, int 
         rc = ensure_partials(h, ctl[1]);
         if (rc) return rc;
         FH_TRY(h, hipEventRecord(h->ev[3], h->stream));
-        if (ctl[1] > 0) {
+        if (ctl[1] > 0 && wide_cells(h->mode)) {
+            // the wide stencils: one wave per item, 4 per workgroup (2 for 9-cell stencils), as
+            // many workgroups as fit the LDS of every compute unit
+            const int ww = wide_cells(h->mode) > 7 ? 2 : 4;
+            const int per_cu = std::max(1, (int)(WIDE_LDS_PER_CU / scatter_wide_lds_bytes(h->mode, ww)));
+            FH_TRY(h, launch_scatter_wide(h->slices, m * h->nsym, h->nsym, h->t3d, h->wq, c, h->geom, h->bins,
+                                          h->exptab, h->rtab, rwh ? h->rw : nullptr,
+                                          std::min(h->cus * per_cu, (ctl[1] + ww - 1) / ww), ww, h->stream));
+        } else if (ctl[1] > 0) {
             FH_TRY(h, launch_scatter_insert(h->slices, m * h->nsym, h->nsym, h->t3d, h->wq, c, h->geom, h->bins,
                                             h->exptab, h->rtab, rwh ? h->rw : nullptr,
                                             std::min(grid, (ctl[1] + wpb - 1) / wpb), wpb, h->stream));
@@ -2012,18 +2080,7 @@ int fh_get_symmetries(const char *sym, float *out9, int max_ops)
 
 long long fh_footprint_bytes(int P, int nsym, int chunk)
 {
-    if (chunk <= 0) chunk = chunk_for(fh_tuning{1, 1, 0, 1, 0}, P, P);
-    const long long v = 12LL * P * P * (P / 2 + 1);
-    const long long sl = 8LL * chunk * P * (P / 2 + 1) + 4LL * chunk * P * P;
-    const BrickGeom g = brick_geom(P);
-    const long long nslices = (long long)chunk * nsym;
-    const long long list = 5LL * g.nb * g.nb * nslices;
-    const long long items = list / 64 + g.nkeys + 1;
-    const long long bins = 4LL * list + 16LL * items + 16LL * g.nkeys + 3LL * (long long)sizeof(SliceRec) * nslices;
-    // partial bricks: about 1.1 (P/8)^2 (slice, brick) pairs per slice, one
-    // item per 64 pairs of a key plus one per key the slices reach
-    const long long part_items = (long long)(1.1 * (P / 8.0) * (P / 8.0) * nslices / 64) + g.nkeys / 2;
-    return v + sl + bins + 4LL * PART_FLOATS * part_items;
+    return footprint_bytes_for_mode(P, nsym, chunk, nullptr);
 }
 
 }  // extern "C"

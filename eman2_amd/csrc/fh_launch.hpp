@@ -74,7 +74,7 @@ struct BinBuffers {
                       // [4..6] slices per dominant axis
     SliceRec *recs;   // 3 segments of rec_cap: the slices of each dominant axis
     long long rec_cap;
-    float *part;      // partial bricks, PART_FLOATS per work item
+    float *part;      // partial bricks, BrickGeom::part_floats per work item
     long long list_cap;
     long long item_cap;
     long long part_cap;  // work items the partial buffer holds
@@ -85,11 +85,32 @@ constexpr int HALO = BRICK + 1;                    // cells a brick's origins re
 constexpr int HALO_CELLS = HALO * HALO * HALO;     // cells of one partial brick
 constexpr int PART_FLOATS = 3 * HALO_CELLS;        // re, im, weight of one partial brick
 
+// Bricks over the stencil origins: brick b owns origins [ow + bs*b, ow + bs*(b+1))
+// per axis, i.e. the pixel positions p with origin(p) in that range, which is
+// p in [po + bs*b, po + bs*(b+1)) (po = ow + the origin's offset: origin(p) =
+// floor(p - off)); its partial covers the halo^3 cells [ow + bs*b, ... + halo).
+//   gauss_2 / trilinear: origin floor(p), bs 8, halo 9, ow = po = -P/2-1.
+//   wide stencils of S cells per axis: bs 4, halo 3 + S (wide_geom).
 struct BrickGeom {
-    int nb;       // bricks per axis over the stencil origins [-H-1, H]
-    int nbricks;  // nb^3
-    int nkeys;    // 3 * nbricks
+    int nb;           // bricks per axis
+    int nbricks;      // nb^3
+    int nkeys;        // 3 * nbricks
+    int bs;           // brick edge, in stencil origins
+    int halo;         // cells per axis of a partial brick
+    int ow;           // lowest stencil origin of brick 0
+    int part_floats;  // 3 * halo^3: re, im, weight of one partial brick
+    float po;         // p-space start of brick 0
 };
+
+constexpr int WBS = 4;  // brick edge of the wide-stencil scatter
+
+// stencil cells per axis of the inserters the wide-stencil scatter takes
+// (0: not taken); modes as InsertConst::mode
+__host__ __device__ constexpr int wide_cells(int mode)
+{
+    return (mode == 2 || mode == 3 || mode == 10 || mode == 11) ? 6 : mode == 5 ? 3 : (mode == 6 || mode == 7) ? 5
+         : mode == 8 ? 7 : mode == 9 ? 9 : 0;
+}
 
 // ---- prep (prep_kernels.hip) ----
 hipError_t launch_edge_mean(const float *imgs, int n, int B, float *means, hipStream_t s);
@@ -150,6 +171,11 @@ hipError_t launch_scatter_insert(const float2 *slices, int nslices, int nsym, co
                                  int waves_per_block, hipStream_t s);
 hipError_t launch_merge_partials(BrickGeom g, BinBuffers b, InsertConst c, float2 *data, float *norm,
                                  hipStream_t s);
+// the wide-stencil modes (wide_cells(mode) > 0) on the bricks of wide_geom
+size_t scatter_wide_lds_bytes(int mode, int waves_per_block);
+hipError_t launch_scatter_wide(const float2 *slices, int nslices, int nsym, const float *t3d, const float *wq,
+                               InsertConst c, BrickGeom g, BinBuffers b, const float *exptab, const float *rtab,
+                               const float *rw, int grid, int waves_per_block, hipStream_t s);
 int agree_blocks(int P);
 hipError_t launch_agree_batch(const float2 *slices, int nsym, const float *t3d, const float *wq, int nslices,
                               InsertConst c, float rmax, int sub, const float2 *data, const float *norm,

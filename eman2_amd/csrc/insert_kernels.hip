@@ -359,7 +359,8 @@ __device__ __forceinline__ int grid_index(int i, float p, int n)
 struct WidePix {
     float xx, yy, zz, dre, dim, w, h, ww;
     float nre, nim;        // fourier_iter with a reference: the reference trilinear at p
-    int lo[3], hi[3];
+    int lo[3], hi[3];      // stencil cells, clamped to the volume
+    int a[3];              // stencil origin before the clamp
     int H;
     const float2 *ref;
 };
@@ -400,6 +401,7 @@ __device__ __forceinline__ bool wide_setup(int mode, float xx, float yy, float z
             const int n2 = d == 0 ? H - 1 : H;
             const int a = (int)floor((double)pp[d] - 2.5);
             if (a < -n2 - 4 || a > n2 + 3) return false;
+            px.a[d] = a;
             px.lo[d] = max(a, -n2);
             px.hi[d] = min(a + 5, n2);
             if (px.hi[d] < px.lo[d]) return false;
@@ -435,9 +437,9 @@ __device__ __forceinline__ bool wide_setup(int mode, float xx, float yy, float z
     if (mode == 4) {
         const int c0 = (int)floor((double)(xx + 0.5f)), c1 = (int)floor((double)(yy + 0.5f));
         const int c2 = (int)floor((double)(zz + 0.5f));
-        px.lo[0] = px.hi[0] = c0;
-        px.lo[1] = px.hi[1] = c1;
-        px.lo[2] = px.hi[2] = c2;
+        px.lo[0] = px.hi[0] = px.a[0] = c0;
+        px.lo[1] = px.hi[1] = px.a[1] = c1;
+        px.lo[2] = px.hi[2] = px.a[2] = c2;
         return true;
     }
     if (mode == 9) {
@@ -447,6 +449,7 @@ __device__ __forceinline__ bool wide_setup(int mode, float xx, float yy, float z
         for (int d = 0; d < 3; d++) {
             const int a = (int)floorf(pp[d] - 4.0f);
             if (a < -H - 7 || a > H + 6) return false;
+            px.a[d] = a;
             px.lo[d] = max(a, -H);
             px.hi[d] = min(a + 8, H);
             if (px.hi[d] < px.lo[d]) return false;
@@ -464,6 +467,7 @@ __device__ __forceinline__ bool wide_setup(int mode, float xx, float yy, float z
         for (int d = 0; d < 3; d++) {
             const int a = ref_fast_floor((float)((double)pp[d] - sh));
             if (a < -H - lo || a > H + hi) return false;
+            px.a[d] = a;
             px.lo[d] = max(a, -H);
             px.hi[d] = min(a + span, H);
             if (px.hi[d] < px.lo[d]) return false;
@@ -478,6 +482,7 @@ __device__ __forceinline__ bool wide_setup(int mode, float xx, float yy, float z
     for (int d = 0; d < 3; d++) {
         const int a = mode == 6 ? (int)floor((double)(pp[d] - 2.0f)) : (int)floor((double)pp[d] - off);
         if (a < -H - lo || a > H + hi) return false;
+        px.a[d] = a;
         px.lo[d] = max(a, -H);
         px.hi[d] = min(a + span, H);
         if (px.hi[d] < px.lo[d]) return false;  // an empty loop in the reference
@@ -1403,10 +1408,11 @@ hipError_t launch_agree(const float2 *slices, int nsym, const float *t3d, int ns
 }
 
 // ------------------------------------------------------------ binning
-// Brick b = (bx, by, bz) owns the stencil origins [lo, lo + BS) per axis,
-// lo = -H - 1 + BS*b: a pixel is added by the one brick holding its origin
-// floor(p), i.e. p in [lo, lo+BS) on every axis, and its 8 corners land in
-// the brick's (BS+1)^3 cells [lo, lo+BS]^3.  Pixel positions lie on the
+// Brick b = (bx, by, bz) owns the stencil origins [lo, lo + bs) per axis,
+// lo = ow + bs*b: a pixel is added by the one brick holding its stencil
+// origin, i.e. p in [po + bs*b, po + bs*(b+1)) on every axis (for gauss_2
+// the origin is floor(p), po = ow = -H-1, and the 8 corners land in the
+// brick's (bs+1)^3 cells [lo, lo+bs]^3).  Pixel positions lie on the
 // plane n.p = 0 (n = u x v of the slice's t3d rows), so per brick column
 // along the dominant normal axis at most 4 bricks qualify.  A slice is
 // binned under key 3 * brick + (its dominant normal axis), so that every
@@ -1461,11 +1467,11 @@ __global__ void __launch_bounds__(256) k_slice_recs(const float *__restrict__ t3
 
 // Does the slice's pixel domain (half-disk x>=0 of radius sqrt(H^2+H), or
 // the half-square with corners) come within reach of the brick?
-__device__ __forceinline__ bool brick_in_domain(const SliceRec &f, const int bc[3], int H, int BS, int corners,
-                                                double rr)
+__device__ __forceinline__ bool brick_in_domain(const SliceRec &f, const int bc[3], int H, int BS, double po,
+                                                int corners, double rr)
 {
     double c[3];
-    for (int d = 0; d < 3; d++) c[d] = -H - 1 + BS * bc[d] + 0.5 * BS;
+    for (int d = 0; d < 3; d++) c[d] = po + BS * bc[d] + 0.5 * BS;
     const double s = c[0] * (double)f.u[0] + c[1] * (double)f.u[1] + c[2] * (double)f.u[2];
     const double t = c[0] * (double)f.v[0] + c[1] * (double)f.v[1] + c[2] * (double)f.v[2];
     const double rh = 0.8660254037844386 * BS + 0.01;
@@ -1481,7 +1487,8 @@ template <int MODE>
 __global__ void __launch_bounds__(256) k_bin_cols(InsertConst c, BrickGeom g, BinBuffers bb)
 {
     extern __shared__ int colcnt[];
-    const int nb = g.nb, BS = BRICK, H = c.H;
+    const int nb = g.nb, BS = g.bs, H = c.H;
+    const double po = g.po;
     const int col = blockIdx.x, A = blockIdx.y;
     const int c1 = col % nb, c2 = col / nb;
     const int b1 = A == 0 ? 1 : 0, b2 = A == 2 ? 1 : 2;
@@ -1494,9 +1501,9 @@ __global__ void __launch_bounds__(256) k_bin_cols(InsertConst c, BrickGeom g, Bi
     }
     __syncthreads();
     const double dlt = 1e-3;
-    const double L1 = -H - 1 + BS * c1 - dlt, U1 = -H - 1 + BS * c1 + BS + dlt;
-    const double L2 = -H - 1 + BS * c2 - dlt, U2 = -H - 1 + BS * c2 + BS + dlt;
-    const double rr = sqrt((double)H * H + H) + 0.8660254037844386 * BS + 0.01;
+    const double L1 = po + BS * c1 - dlt, U1 = po + BS * c1 + BS + dlt;
+    const double L2 = po + BS * c2 - dlt, U2 = po + BS * c2 + BS + dlt;
+    const double rr = sqrt((double)c.rcut * c.rcut + c.rcut) + 0.8660254037844386 * BS + 0.01;
     const int n = bb.ctl[4 + A];
     const SliceRec *recs = bb.recs + (size_t)A * bb.rec_cap;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
@@ -1504,13 +1511,13 @@ __global__ void __launch_bounds__(256) k_bin_cols(InsertConst c, BrickGeom g, Bi
         const double pa0 = f.r1 * L1 + f.r2 * L2, pa1 = f.r1 * U1 + f.r2 * L2;
         const double pa2 = f.r1 * L1 + f.r2 * U2, pa3 = f.r1 * U1 + f.r2 * U2;
         const double amin = fmin(fmin(pa0, pa1), fmin(pa2, pa3)), amax = fmax(fmax(pa0, pa1), fmax(pa2, pa3));
-        int lo = (int)floor((amin - dlt + H + 1) / BS);
-        int hi = (int)floor((amax + dlt + H + 1) / BS);
+        int lo = (int)floor((amin - dlt - po) / BS);
+        int hi = (int)floor((amax + dlt - po) / BS);
         if (lo < 0) lo = 0;
         if (hi > nb - 1) hi = nb - 1;
         for (int ca = lo; ca <= hi; ca++) {
             bc[A] = ca;
-            if (!brick_in_domain(f, bc, H, BS, c.corners, rr)) continue;
+            if (!brick_in_domain(f, bc, H, BS, po, c.corners, rr)) continue;
             const int pos = atomicAdd(&colcnt[ca], 1);
             if (MODE == 1) {
                 if (pos < bb.list_cap) bb.list[pos] = f.q;
@@ -2138,28 +2145,302 @@ hipError_t launch_scatter_insert(const float2 *slices, int nslices, int nsym, co
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------ wide-stencil brick scatter
+// The inserters with stencils of S = 3 to 9 cells per axis (gauss_var,
+// gauss_5, gauss_3, kaiser_bessel, kaiser_bessel_derived, gridding_5,
+// gridding_7 and fourier_iter's Gaussian) on bricks of 4^3 stencil origins,
+// binned like gauss_2's on the origin grid of the mode.  A work item's wave
+// owns the brick's (3 + S)^3 cells in LDS.  For each slice of the item it
+// tests the 8 x 8 pixels around the projection of the brick's centre (a
+// pixel whose origin lies in the brick is within 2 sqrt 3 of that centre in
+// p, so within 7 pixels per axis in the slice), then adds the stencils of the
+// pixels whose origin lies in the brick one pixel after another, all 64 lanes
+// over the pixel's S^3 cells.  The cells of one stencil are distinct and the
+// pixels of a wave follow one another in program order, so plain LDS
+// read-modify-writes suffice: no atomics and no election.  Each contribution
+// is computed by wide_setup / wide_term, as in every other form.
+#define WR_CAND 3.4691f  // 2 sqrt 3 + margin for the FP32 position and origin rounding
+
+__device__ __forceinline__ float bcast_f(float v, int j)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_scatter_wide(const float2 *__restrict__ slices, int nslices, int nsym,
+                                                      const float *__restrict__ t3d, const float *__restrict__ wq,
+                                                      InsertConst c, BrickGeom g, BinBuffers bb,
+                                                      const float *__restrict__ exptab_g,
+                                                      const float *__restrict__ rtab, const float *__restrict__ rw)
+{
+    constexpr int S = wide_cells(MODE), HL = WBS + S - 1, HC = HL * HL * HL;
+    constexpr int NC = S * S * S, NIT = (NC + 63) / 64;
+    // the Gaussian 6^3 stencils (gauss_var, gauss_5, fourier_iter without a reference): lanes 0..53
+    // keep one x offset oi = lane % 6 in every pass and take the (oj, ok) pair lane / 6 + 9 t in
+    // pass t, so the x distance is computed once per pixel
+    constexpr bool G6 = MODE == 2 || MODE == 3 || MODE == 10;
+    extern __shared__ float lds_w[];
+    float *etab = lds_w;  // the 1000 fast_exp entries
+    float *acc = lds_w + 1024 + (size_t)(threadIdx.x >> 6) * 3 * HC;  // re, im, w over [z][y][x] of the cells
+    const int lane = threadIdx.x & 63;
+    for (int i = threadIdx.x; i < 1000; i += blockDim.x) etab[i] = exptab_g[i];
+    __syncthreads();
+    const int H = c.H, P = c.P, nb = g.nb;
+    // this lane's stencil cell in pass t: offsets oi | oj << 4 | ok << 8 (-1 past S^3) and its LDS offset
+    int oc[NIT], off[NIT];
+    float fo_j[NIT], fo_k[NIT];
+    const float fo_i = (float)(lane % S);
+#pragma unroll
+    for (int t = 0; t < NIT; t++) {
+        int oi, oj, ok;
+        bool in;
+        if (G6) {
+            const int pr = lane / 6 + 9 * t;
+            oi = lane % 6;
+            oj = pr % 6;
+            ok = pr / 6;
+            in = lane < 54;
+        } else {
+            const int ci = lane + 64 * t;
+            oi = ci % S;
+            oj = (ci / S) % S;
+            ok = ci / (S * S);
+            in = ci < NC;
+        }
+        oc[t] = in ? (oi | (oj << 4) | (ok << 8)) : -1;
+        off[t] = in ? (ok * HL + oj) * HL + oi : 0;
+        fo_j[t] = (float)oj;
+        fo_k[t] = (float)ok;
+    }
+    const int total = bb.ctl[1];
+    for (;;) {
+        int it = 0;
+        if (lane == 0) it = atomicAdd(&bb.ctl[0], 1);
+        it = __shfl(it, 0, 64);
+        if (it >= total) break;
+        const WorkItem w = bb.items[it];
+        const int brick = w.key / 3;
+        const int bc[3] = {brick % nb, (brick / nb) % nb, brick / (nb * nb)};
+        int lo[3];
+        float pc[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            lo[d] = g.ow + WBS * bc[d];
+            pc[d] = g.po + (float)(WBS * bc[d]) + 0.5f * WBS;  // exact: half-integers of magnitude < 2^20
+        }
+        for (int i = lane; i < 3 * HC; i += 64) acc[i] = 0.f;
+        wave_sync_lds();
+        int qbuf = -1;
+        for (int s = 0; s < w.count; s++) {
+            if ((s & 63) == 0) qbuf = s + lane < w.count ? bb.list[w.start + s + lane] : -1;
+            const int q = __builtin_amdgcn_readlane(qbuf, s & 63);
+            if (q < 0 || q >= nslices) continue;  // never written list entries are skipped
+            const float wgt = wq[q];
+            if (wgt == 0.0f) continue;
+            float mm[6];
+#pragma unroll
+            for (int d = 0; d < 6; d++) mm[d] = t3d[9 * (size_t)q + d];
+            const float sc = pc[0] * mm[0] + pc[1] * mm[1] + pc[2] * mm[2];
+            const float tc = pc[0] * mm[3] + pc[1] * mm[4] + pc[2] * mm[5];
+            const int x = (int)ceilf(sc - WR_CAND) + (lane & 7), y = (int)ceilf(tc - WR_CAND) + (lane >> 3);
+            bool ok = x >= 0 && x <= H && y >= -H && y <= H - 1 &&
+                      (c.corners || x * x + y * y <= c.rcut * c.rcut + c.rcut);  // hypot_fast_int(x, y) <= rcut
+            WidePix px;
+            int pk = 0;
+            if (ok) {
+                const int p = q / nsym;
+                const float weight = rw ? rw[(size_t)p * (H + 1) + hypot_int(x, y)] : wgt;
+                float xx, yy, zz;
+                pixel_coords(x, y, mm, rtab, H, c.Pf, xx, yy, zz);
+                ok = wide_setup(MODE, xx, yy, zz, weight, H, etab, px, c.ref);
+                if (ok) {
+#pragma unroll
+                    for (int d = 0; d < 3; d++) ok = ok && (unsigned)(px.a[d] - lo[d]) < (unsigned)WBS;
+                }
+                if (ok) {
+                    const float2 dt = slice_value(slices + (size_t)p * P * (H + 1), x, y, P, H);
+                    px.dre = dt.x;
+                    px.dim = dt.y;
+#pragma unroll
+                    for (int d = 0; d < 3; d++)
+                        pk |= ((px.a[d] - lo[d]) | ((px.lo[d] - px.a[d]) << 2) | ((px.hi[d] - px.a[d]) << 6))
+                              << (10 * d);
+                }
+            }
+            unsigned long long hits = __ballot(ok);
+            while (hits) {
+                const int j = (int)__builtin_ctzll(hits);
+                hits &= hits - 1;
+                WidePix u;
+                u.xx = bcast_f(px.xx, j);
+                u.yy = bcast_f(px.yy, j);
+                u.zz = bcast_f(px.zz, j);
+                u.w = bcast_f(px.w, j);
+                u.ww = bcast_f(px.ww, j);
+                u.h = bcast_f(px.h, j);
+                u.dre = bcast_f(px.dre, j);
+                u.dim = bcast_f(px.dim, j);
+                u.nre = MODE == 11 ? bcast_f(px.nre, j) : 0.f;
+                u.nim = MODE == 11 ? bcast_f(px.nim, j) : 0.f;
+                u.H = H;
+                u.ref = c.ref;
+                const int pkj = __builtin_amdgcn_readlane(pk, j);
+                bool full = true;
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    const int f = pkj >> (10 * d);
+                    u.a[d] = lo[d] + (f & 3);
+                    u.lo[d] = u.a[d] + ((f >> 2) & 15);
+                    u.hi[d] = u.a[d] + ((f >> 6) & 15);
+                    full = full && u.lo[d] == u.a[d] && u.hi[d] == u.a[d] + S - 1;
+                }
+                const int base = ((u.a[2] - lo[2]) * HL + (u.a[1] - lo[1])) * HL + (u.a[0] - lo[0]);
+                // the pixel's cells are distinct: every lane reads its cells first (lanes past the
+                // stencil read cell `base`, in bounds) and the lanes with a contribution write them
+                // back after the terms are computed
+                float ar[NIT], ai[NIT], aw[NIT];
+#pragma unroll
+                for (int t = 0; t < NIT; t++) {
+                    ar[t] = acc[base + off[t]];
+                    ai[t] = acc[HC + base + off[t]];
+                    aw[t] = acc[2 * HC + base + off[t]];
+                }
+                float vr[NIT], vi[NIT], vg[NIT];
+                bool on[NIT];
+#pragma unroll
+                for (int t = 0; t < NIT; t++) {
+                    const int o = oc[t];
+                    const int i = u.a[0] + (o & 15), jj = u.a[1] + ((o >> 4) & 15), k = u.a[2] + ((o >> 8) & 15);
+                    on[t] = o >= 0 && (full || (i >= u.lo[0] && i <= u.hi[0] && jj >= u.lo[1] && jj <= u.hi[1] &&
+                                                k >= u.lo[2] && k <= u.hi[2]));
+                    if (!G6 && on[t]) on[t] = wide_term(MODE, u, i, jj, k, etab, vr[t], vi[t], vg[t]);
+                }
+                if (G6) {
+                    // wide_term's Gaussian, pass by pass: (float)i = (float)a + (float)o exactly (small
+                    // integers), r = hypot3sq in the reference's order, fast_exp(-r h): the table entry
+                    // (int)(-(double)f * 50 + 0.5) -- f * 50 is exact in double, so one fma -- for
+                    // f >= -19.98, expf below (f = -(r h) <= 0 since h > 0).  All passes' table reads go
+                    // out before the first is used; expf runs only when a lane of the wave needs it.
+                    const float dx = ((float)u.a[0] + fo_i) - u.xx;
+                    const float dx2 = dx * dx;
+                    float fv[NIT], tv[NIT];
+                    bool big[NIT], anyb = false;
+#pragma unroll
+                    for (int t = 0; t < NIT; t++) {
+                        const float dy = ((float)u.a[1] + fo_j[t]) - u.yy, dz = ((float)u.a[2] + fo_k[t]) - u.zz;
+                        const float f = -(dx2 + dy * dy + dz * dz) * u.h;
+                        fv[t] = f;
+                        big[t] = f < -19.98f;  // the same floats as (double)f < -19.98
+                        anyb = anyb || big[t];
+                        const int gi = big[t] ? 0 : (int)__builtin_fma(-(double)f, 50.0, 0.5);
+                        tv[t] = etab[gi];
+                    }
+                    if (__ballot(anyb)) {
+#pragma unroll
+                        for (int t = 0; t < NIT; t++)
+                            if (big[t]) {
+                                tv[t] = expf(fv[t]);
+                                on[t] = on[t] && !(fabsf(tv[t] * u.ww) < FLT_MIN);
+                            }
+                    }
+                    if (u.ww == 1.0f) {
+                        // weight 1 (wave-uniform): gg * 1 = gg exactly, and a table entry is at least
+                        // exp(-19.98) > FLT_MIN, so only the expf terms above can be dropped
+#pragma unroll
+                        for (int t = 0; t < NIT; t++) {
+                            vg[t] = tv[t];
+                            vr[t] = u.dre * tv[t];
+                            vi[t] = u.dim * tv[t];
+                        }
+                    } else {
+#pragma unroll
+                        for (int t = 0; t < NIT; t++) {
+                            const float gg = tv[t];
+                            vg[t] = gg * u.ww;
+                            on[t] = on[t] && !(fabsf(vg[t]) < FLT_MIN);
+                            vr[t] = u.dre * gg * u.ww;
+                            vi[t] = u.dim * gg * u.ww;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < NIT; t++)
+                    if (on[t]) {
+                        acc[base + off[t]] = ar[t] + vr[t];
+                        acc[HC + base + off[t]] = ai[t] + vi[t];
+                        acc[2 * HC + base + off[t]] = aw[t] + vg[t];
+                    }
+            }
+        }
+        wave_sync_lds();
+        float *out = bb.part + (size_t)it * g.part_floats;
+        for (int i = lane; i < 3 * HC; i += 64) out[i] = acc[i];
+        wave_sync_lds();
+    }
+}
+
+size_t scatter_wide_lds_bytes(int mode, int waves_per_block)
+{
+    const int HL = WBS + wide_cells(mode) - 1;
+    return sizeof(float) * (1024 + (size_t)waves_per_block * 3 * HL * HL * HL);
+}
+
+hipError_t launch_scatter_wide(const float2 *slices, int nslices, int nsym, const float *t3d, const float *wq,
+                               InsertConst c, BrickGeom g, BinBuffers b, const float *exptab, const float *rtab,
+                               const float *rw, int grid, int waves_per_block, hipStream_t s)
+{
+    const int S = wide_cells(c.mode);
+    if (S == 0 || waves_per_block < 1 || waves_per_block > 4 || g.bs != WBS || g.halo != WBS + S - 1)
+        return hipErrorInvalidValue;
+    const size_t lds = scatter_wide_lds_bytes(c.mode, waves_per_block);
+    if (lds > 65536) return hipErrorInvalidValue;
+    const dim3 gr(grid), bl(64 * waves_per_block);
+#define FH_WIDE(M) \
+    case M: hipLaunchKernelGGL(k_scatter_wide<M>, gr, bl, lds, s, slices, nslices, nsym, t3d, wq, c, g, b, exptab, rtab, rw); break;
+    switch (c.mode) {
+        FH_WIDE(2)
+        FH_WIDE(3)
+        FH_WIDE(5)
+        FH_WIDE(6)
+        FH_WIDE(7)
+        FH_WIDE(8)
+        FH_WIDE(9)
+        FH_WIDE(10)
+        FH_WIDE(11)
+    default:
+        return hipErrorInvalidValue;
+    }
+#undef FH_WIDE
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------ merge of partial bricks
 // Sum over the work items of the bricks whose cells include unfolded cell
-// (i, j, k): the brick owning origin (i, j, k), and on each axis where the
-// cell is a brick's first, also the previous brick (its cell BRICK there).
+// (i, j, k): brick b covers cells [ow + BS*b, ow + BS*b + HL) per axis, so a
+// cell lies in the bricks floor(u / BS) down to ceil((u - HL + 1) / BS),
+// u = cell - ow (for gauss_2, BS 8 and HL 9: the brick owning the origin and,
+// on each axis where the cell is a brick's first, also the previous brick).
+template <int BS, int HL>
 __device__ __forceinline__ void partial_sum(const float *__restrict__ part, const int *__restrict__ item_off, int nb,
-                                            int H, int i, int j, int k, float &re, float &im, float &w)
+                                            int ow, int pf, int i, int j, int k, float &re, float &im, float &w)
 {
-    const int ui = i + H + 1, uj = j + H + 1, uk = k + H + 1;  // >= 1 for cells >= -H
-    const int bi = ui / BRICK, bj = uj / BRICK, bk = uk / BRICK;
-    const int li = ui - BRICK * bi, lj = uj - BRICK * bj, lk = uk - BRICK * bk;
+    constexpr int HC = HL * HL * HL;
+    const int ui = i - ow, uj = j - ow, uk = k - ow;  // >= 0 for cells >= -H
+    const int bi1 = min(ui / BS, nb - 1), bj1 = min(uj / BS, nb - 1), bk1 = min(uk / BS, nb - 1);
+    const int bi0 = ui < HL ? 0 : (ui - HL + BS) / BS, bj0 = uj < HL ? 0 : (uj - HL + BS) / BS;
+    const int bk0 = uk < HL ? 0 : (uk - HL + BS) / BS;
     re = im = w = 0.f;
-    for (int ek = 0; ek <= (lk == 0 ? 1 : 0); ek++)
-        for (int ej = 0; ej <= (lj == 0 ? 1 : 0); ej++)
-            for (int ei = 0; ei <= (li == 0 ? 1 : 0); ei++) {
-                const int b = ((bk - ek) * nb + (bj - ej)) * nb + (bi - ei);
-                const int cc = ((ek ? BRICK : lk) * HALO + (ej ? BRICK : lj)) * HALO + (ei ? BRICK : li);
+    for (int bk = bk1; bk >= bk0; bk--)
+        for (int bj = bj1; bj >= bj0; bj--)
+            for (int bi = bi1; bi >= bi0; bi--) {
+                const int b = (bk * nb + bj) * nb + bi;
+                const int cc = ((uk - BS * bk) * HL + (uj - BS * bj)) * HL + (ui - BS * bi);
                 const int i0 = item_off[3 * b], i1 = item_off[3 * b + 3];
                 for (int it = i0; it < i1; it++) {
-                    const float *p = part + (size_t)it * PART_FLOATS + cc;
+                    const float *p = part + (size_t)it * pf + cc;
                     re += p[0];
-                    im += p[HALO_CELLS];
-                    w += p[2 * HALO_CELLS];
+                    im += p[HC];
+                    w += p[2 * HC];
                 }
             }
 }
@@ -2171,10 +2452,11 @@ __device__ __forceinline__ void partial_sum(const float *__restrict__ part, cons
 // conjugate copies of the double insertion (value only) and, at x = P/2,
 // the x = -P/2 cells added twice with their weight once; at (0,0,0) and
 // (P/2,0,0) the real part and weight with the imaginary part set to 0.
+template <int BS, int HL>
 __global__ void __launch_bounds__(256) k_merge_partials(const float *__restrict__ part, const int *__restrict__ item_off,
                                                         BrickGeom g, InsertConst c, float2 *data, float *norm)
 {
-    const int H = c.H, P = c.P, nb = g.nb;
+    const int H = c.H, P = c.P, nb = g.nb, ow = g.ow, pf = g.part_floats;
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long long)P * P * (H + 1)) return;
     const int xs = (int)(idx % (H + 1));
@@ -2182,10 +2464,10 @@ __global__ void __launch_bounds__(256) k_merge_partials(const float *__restrict_
     const int ys = (int)(rest % P), zs = (int)(rest / P);
     float r, i, w;
     if (ys == 0 && zs == 0 && (xs == 0 || xs == H)) {
-        partial_sum(part, item_off, nb, H, xs, 0, 0, r, i, w);
+        partial_sum<BS, HL>(part, item_off, nb, ow, pf, xs, 0, 0, r, i, w);
         if (xs == H) {
             float r2, i2, w2;
-            partial_sum(part, item_off, nb, H, -H, 0, 0, r2, i2, w2);
+            partial_sum<BS, HL>(part, item_off, nb, ow, pf, -H, 0, 0, r2, i2, w2);
             r += r2;
             i += i2;
             w += w2;
@@ -2210,7 +2492,7 @@ __global__ void __launch_bounds__(256) k_merge_partials(const float *__restrict_
     float dre = 0.f, dim = 0.f, dw = 0.f;
     for (int a = 0; a < njd; a++)
         for (int b = 0; b < nkd; b++) {
-            partial_sum(part, item_off, nb, H, xs, jd[a], kd[b], r, i, w);
+            partial_sum<BS, HL>(part, item_off, nb, ow, pf, xs, jd[a], kd[b], r, i, w);
             dre += r;
             dim += i;
             dw += w;
@@ -2218,17 +2500,17 @@ __global__ void __launch_bounds__(256) k_merge_partials(const float *__restrict_
     for (int a = 0; a < njn; a++)
         for (int b = 0; b < nkn; b++) {
             if (xs == 0 || xs == H) {
-                partial_sum(part, item_off, nb, H, xs, jn[a], kn[b], r, i, w);
+                partial_sum<BS, HL>(part, item_off, nb, ow, pf, xs, jn[a], kn[b], r, i, w);
                 dre += r;
                 dim -= i;
                 if (xs == H) {
-                    partial_sum(part, item_off, nb, H, -H, jn[a], kn[b], r, i, w);
+                    partial_sum<BS, HL>(part, item_off, nb, ow, pf, -H, jn[a], kn[b], r, i, w);
                     dre += 2.f * r;
                     dim -= 2.f * i;
                     dw += w;
                 }
             } else {
-                partial_sum(part, item_off, nb, H, -xs, jn[a], kn[b], r, i, w);
+                partial_sum<BS, HL>(part, item_off, nb, ow, pf, -xs, jn[a], kn[b], r, i, w);
                 dre += r;
                 dim -= i;
                 dw += w;
@@ -2245,8 +2527,21 @@ __global__ void __launch_bounds__(256) k_merge_partials(const float *__restrict_
 hipError_t launch_merge_partials(BrickGeom g, BinBuffers b, InsertConst c, float2 *data, float *norm, hipStream_t s)
 {
     const long long n = (long long)c.P * c.P * (c.H + 1);
-    hipLaunchKernelGGL(k_merge_partials, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, b.part, b.item_off, g, c,
-                       data, norm);
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if (g.bs == BRICK && g.halo == HALO)
+        hipLaunchKernelGGL((k_merge_partials<BRICK, HALO>), grid, dim3(256), 0, s, b.part, b.item_off, g, c, data, norm);
+    else if (g.bs == WBS && g.halo == WBS + 2)
+        hipLaunchKernelGGL((k_merge_partials<WBS, WBS + 2>), grid, dim3(256), 0, s, b.part, b.item_off, g, c, data, norm);
+    else if (g.bs == WBS && g.halo == WBS + 4)
+        hipLaunchKernelGGL((k_merge_partials<WBS, WBS + 4>), grid, dim3(256), 0, s, b.part, b.item_off, g, c, data, norm);
+    else if (g.bs == WBS && g.halo == WBS + 5)
+        hipLaunchKernelGGL((k_merge_partials<WBS, WBS + 5>), grid, dim3(256), 0, s, b.part, b.item_off, g, c, data, norm);
+    else if (g.bs == WBS && g.halo == WBS + 6)
+        hipLaunchKernelGGL((k_merge_partials<WBS, WBS + 6>), grid, dim3(256), 0, s, b.part, b.item_off, g, c, data, norm);
+    else if (g.bs == WBS && g.halo == WBS + 8)
+        hipLaunchKernelGGL((k_merge_partials<WBS, WBS + 8>), grid, dim3(256), 0, s, b.part, b.item_off, g, c, data, norm);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

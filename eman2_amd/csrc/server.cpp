@@ -48,6 +48,9 @@ struct fh_server {
     }
 };
 
+// device bytes of a handle by inserter mode name (engine.cpp)
+long long footprint_bytes_for_mode(int P, int nsym, int chunk, const char *mode_name);
+
 static long long job_footprint(const fh_job &j, int chunk)
 {
     const int P = j.params.nx;
@@ -56,7 +59,7 @@ static long long job_footprint(const fh_job &j, int chunk)
     int nsym = fh_get_symmetries(j.params.sym ? j.params.sym : "c1", nullptr, 0);
     if (nsym < 1) nsym = 1;
     const int m = j.n < chunk ? (j.n > 0 ? j.n : 1) : chunk;
-    long long fp = fh_footprint_bytes(P, nsym, m);
+    long long fp = footprint_bytes_for_mode(P, nsym, m, j.params.mode);
     fp += 4LL * m * j.box * j.box;       // staging of host particles
     fp += 4LL * P * P * P + 8LL * P * P;  // finish(): real map + conjugate planes
     return fp;
