@@ -269,7 +269,7 @@ void compose9(const float *a, const float *b, float *r)
 // CTF constants (libEM/ctf.h:291-296, libEM/ctf.cpp:671-675, :985-992)
 CtfConst ctf_constants(const fh_ctf &c, int ny)
 {
-    CtfConst k;
+    CtfConst k{};
     float lambda = 12.2639f / sqrtf(c.voltage * 1000.0f + 0.97845f * c.voltage * c.voltage);
     float phase;
     if (c.ampcont > -100.0 && c.ampcont <= 100.0) phase = (float)asin(c.ampcont / 100.0);

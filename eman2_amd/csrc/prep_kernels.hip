@@ -153,15 +153,17 @@ __global__ void __launch_bounds__(256) k_prep_rows_pipe(const float *__restrict_
                                                         const float *__restrict__ means, float2 *__restrict__ slices,
                                                         const float2 *__restrict__ twg)
 {
-    constexpr int P = NF, H = P / 2, K = (T * P + 255) / 256;
+    constexpr int P = NF, H = P / 2, KX = (P + 255) / 256;
     extern __shared__ float2 smem[];
     float2 *tw = smem;
     float2 *A = tw + P;
     load_twiddles(tw, twg, P);
     const int nrb = (B + 2 * T - 1) / (2 * T);
     const int ntiles = nrb * n;
-    // raw pixels of a tile; out-of-image slots hold the mean (zero after the subtraction)
-    float2 v[K];
+    // raw pixels of a tile: thread x-slot k holds padded column xs = tid + 256 k of all 2T rows, so
+    // each load instruction reads 64 consecutive pixels of one row; out-of-image slots hold the mean
+    // (zero after the subtraction)
+    float v[KX][2 * T];
     float vmean = 0.f;
     auto fetch = [&](int tile) {
         const int p = tile / nrb, b0 = (tile - p * nrb) * 2 * T;
@@ -169,17 +171,13 @@ __global__ void __launch_bounds__(256) k_prep_rows_pipe(const float *__restrict_
         const float *img = imgs + (size_t)p * B * B;
         vmean = mean;
 #pragma unroll
-        for (int k = 0; k < K; k++) {
-            const int idx = threadIdx.x + 256 * k;
-            const int t = idx % T, xs = idx / T;
+        for (int k = 0; k < KX; k++) {
+            const int xs = threadIdx.x + 256 * k;
             const int xp = xs + H < P ? xs + H : xs + H - P;
             const int bx = xp - off;
-            const int r0 = b0 + 2 * t;
-            const bool in = idx < T * P && bx >= 0 && bx < B;
-            float re = mean, im = mean;
-            if (in && r0 < B) re = img[(size_t)r0 * B + bx];
-            if (in && r0 + 1 < B) im = img[(size_t)(r0 + 1) * B + bx];
-            v[k] = make_float2(re, im);
+            const bool in = xs < P && bx >= 0 && bx < B;
+#pragma unroll
+            for (int e = 0; e < 2 * T; e++) v[k][e] = in && b0 + e < B ? img[(size_t)(b0 + e) * B + bx] : mean;
         }
     };
     int tile = blockIdx.x;
@@ -188,9 +186,16 @@ __global__ void __launch_bounds__(256) k_prep_rows_pipe(const float *__restrict_
         const int p = tile / nrb, b0 = (tile - p * nrb) * 2 * T;
         const float mean = vmean;
 #pragma unroll
-        for (int k = 0; k < K; k++) {
-            const int idx = threadIdx.x + 256 * k;
-            if (idx < T * P) A[idx] = make_float2(v[k].x - mean, v[k].y - mean);
+        for (int k = 0; k < KX; k++) {
+            const int xs = threadIdx.x + 256 * k;
+            if (xs < P) {
+                // transform t takes rows b0 + 2t (real part) and b0 + 2t + 1 (imaginary part)
+                float4 *dst = reinterpret_cast<float4 *>(A + xs * T);
+#pragma unroll
+                for (int t2 = 0; t2 < T / 2; t2++)
+                    dst[t2] = make_float4(v[k][4 * t2] - mean, v[k][4 * t2 + 1] - mean, v[k][4 * t2 + 2] - mean,
+                                          v[k][4 * t2 + 3] - mean);
+            }
         }
         __syncthreads();
         if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
@@ -269,15 +274,14 @@ hipError_t launch_prep_rows(const float *imgs, int n, int B, int P, const float 
 
 // ---------------------------------------------------------------- column pass
 // CTF_SIGN of EMAN2Ctf::compute_2d_complex (libEM/ctf.cpp:1076-1090) at
-// Fourier pixel (x, y), y signed.  Evaluated with the reference's float
+// the Fourier pixel of radius hyp = Util::hypot_fast(x, y).  Evaluated with the reference's float
 // operation order (no contraction); the sign of cos is taken in double.
 // cos(t) < 0 exactly when floor(t / pi - 1/2) is even; away from the
 // zeros of cos (|frac| > 1e-9, far above the ~1e-13 error of u) the parity
 // decides, next to them cos itself is evaluated.
-__device__ __forceinline__ float ctf_sign(int x, int y, float defocus, const CtfConst &cc)
+__device__ __forceinline__ float ctf_sign(float hyp, float defocus, const CtfConst &cc)
 {
 #pragma clang fp contract(off)
-    const float hyp = sqrt_int_rn(x * x + y * y);  // Util::hypot_fast
     const float s = hyp * cc.ds;
     const float gam = -cc.g1 * s * s * s * s + cc.g2 * defocus * s * s;
     const double t = (double)(gam - cc.acac);
@@ -338,8 +342,9 @@ __global__ void __launch_bounds__(256) k_prep_cols(int B, int Pr, int off, float
         if (x > H) continue;
         // (v * (sg + 0i)) * scale with sg = +-1 equals v * (sg * scale) exactly
         // (up to the sign of a zero)
-        float ss = defocus ? ctf_sign(x, y, df, cc) * scale : scale;
-        if (defocus2) ss = ss * ctf_sign(x, y, df2, cc);
+        const float hyp = defocus ? sqrt_int_rn(x * x + y * y) : 0.f;  // Util::hypot_fast(x, y)
+        float ss = defocus ? ctf_sign(hyp, df, cc) * scale : scale;
+        if (defocus2) ss = ss * ctf_sign(hyp, df2, cc);
 #pragma unroll
         for (int m = 0; m < 2; m++) {
             const int row = m == 0 ? y : P - y;
@@ -363,8 +368,8 @@ hipError_t launch_prep_cols(int n, int B, int P, float2 *slices, const float *de
     dim3 grid((H + 1 + T - 1) / T, n);
 #define FH_COLS_CASE(TT, NN)                                                                                      \
     if (T == TT && P == NN) {                                                                                     \
-        hipLaunchKernelGGL((k_prep_cols<TT, NN>), grid, dim3(256), lds, s, B, P, off, slices, defocus, defocus2, cc, scale, \
-                           plan, tw);                                                                             \
+        hipLaunchKernelGGL((k_prep_cols<TT, NN>), grid, dim3(256), lds, s, B, P, off, slices, defocus, defocus2, cc, \
+                           scale, plan, tw);                                                                      \
         return hipGetLastError();                                                                                 \
     }
     FH_FIXED_PADS_COLS(FH_COLS_CASE)
