@@ -85,6 +85,8 @@ EXPORTS = {
     "fh_insert_plain": (_c_int, [_vp, _vp, _c_int, _c_int, _vp, _vp, _vp, ctypes.POINTER(Ctf)]),
     "fh_insert_plain_where": (_c_int, [_vp, _vp, _c_int, _c_int, _vp, _vp, _vp, ctypes.POINTER(Ctf), _c_int]),
     "fh_determine_slice_agreement": (_c_int, [_vp, _vp, _c_int, _vp, _c_float, _c_int, _c_float, _vp]),
+    "fh_determine_slice_agreement_ssnr": (_c_int, [_vp, _vp, _c_int, _vp, _c_float, _c_int, _c_float, _vp, _c_int,
+                                                   _vp]),
     "fh_setup_seed_map": (_c_int, [_vp, _vp]),
     "fh_gauss_fft_volume": (_c_int, [_vp, _vp]),
     "fh_gauss_fft_project": (_c_int, [_vp, _c_int, _vp, _c_int, _c_int, _vp]),

@@ -325,6 +325,7 @@ struct fh_reconstructor {
     float2 *slices = nullptr;
     float *t3d = nullptr, *wq = nullptr;
     float *rw = nullptr;        // class_ssnr radial weights, (P/2+1) per particle (usessnr)
+    float *rw1 = nullptr;       // one slice's class_ssnr radial weights (agreement in the reference's order)
     float *ximgs = nullptr;     // 2-D transformed padded particles, P x P each
     float *ximgs2 = nullptr;    // second P x P staging: the flipped images' resampling (CTF with a 2-D part)
     float *xfd = nullptr;       // their inverse 2-D transforms, 6 per particle
@@ -774,6 +775,7 @@ int fh_destroy(fh_reconstructor *h)
             if (h->pin_ev[i]) (void)hipEventDestroy(h->pin_ev[i]);
         }
         if (h->partial) (void)hipFree(h->partial);
+        if (h->rw1) (void)hipFree(h->rw1);
         if (h->acc8) (void)hipFree(h->acc8);
         for (void *p : {(void *)h->exptab, (void *)h->rtab, (void *)h->tw, (void *)h->syms_d})
             if (p) (void)hipFree(p);
@@ -1554,8 +1556,12 @@ static void agree_outputs(const double *a, float weight, float out[5])
 
 // One slice already in slice_d: optional subtraction, compare, re-add
 // (libEM/reconstructor.cpp:1212-1257) -> out[5] (reconstructor.cpp:1365-1378).
+// rw_d (device, P/2+1 radial weights): the reference's own first step for a
+// slice carrying class_ssnr -- do_insert_slice_work with -weight takes its
+// weight < 0 SSNR branch and adds the slice with those weights
+// (reconstructor.cpp:1160-1184,1236) -- in place of the removal
 static int agree_one(fh_reconstructor *h, const float2 *slice_d, const float *rot9, float weight, int sub, float apix,
-                     float out[5])
+                     float out[5], const float *rw_d = nullptr)
 {
     InsertConst c = iconst(h);
     c.corners = 0;  // do_insert_slice_work default (reconstructor.h:540)
@@ -1564,11 +1570,11 @@ static int agree_one(fh_reconstructor *h, const float2 *slice_d, const float *ro
     for (int s = 0; s < h->nsym; s++) compose9(rot9, &h->syms[9 * s], &t[9 * (size_t)s]);
     FH_TRY(h, hipMemcpyAsync(h->t3d, t.data(), sizeof(float) * t.size(), hipMemcpyHostToDevice, h->stream));
     if (sub && h->mode >= 2) {
-        int rc = wide_slice_insert(h, slice_d, rot9, -weight, nullptr, c);
+        int rc = wide_slice_insert(h, slice_d, rot9, -weight, rw_d, c);
         if (rc) return rc;
     } else if (sub)
         for (int s = 0; s < h->nsym; s++)
-            FH_TRY(h, launch_insert_plain(slice_d, c, &t[9 * (size_t)s], -weight, h->data, h->norm, h->exptab, h->rtab, nullptr,
+            FH_TRY(h, launch_insert_plain(slice_d, c, &t[9 * (size_t)s], -weight, h->data, h->norm, h->exptab, h->rtab, rw_d,
                                           h->stream));
     FH_TRY(h, launch_agree(slice_d, h->nsym, h->t3d, h->nsym, c, agree_rmax(h->P, apix), h->data, h->norm, h->rtab,
                            h->partial, h->stream));
@@ -1732,6 +1738,41 @@ int fh_determine_slice_agreement(fh_reconstructor *h, const float *data, int is_
         if ((rc = prep_single_raw(h, xf6.empty() ? nullptr : xf6.data()))) return rc;
     }
     return agree_one(h, h->slices, rot9.data(), weight, sub, apix, out);
+}
+
+int fh_determine_slice_agreement_ssnr(fh_reconstructor *h, const float *data, int is_preprocessed, const float xform[12],
+                                      float weight, int sub, float apix, const float *class_ssnr, int ssnr_len,
+                                      float out[5])
+{
+    if (!h) return FH_ERR_NULL;
+    if (!data || !xform || !out) return fail(h, FH_ERR_NULL, "NullPointerException: EMData pointer (input image) is NULL");
+    if (sub && !class_ssnr)
+        return fail(h, FH_ERR_NOT_EXISTING, "NotExistingObjectException: The requested key does not exist: class_ssnr");
+    if (sub && ssnr_len < 1) return fail(h, FH_ERR_INVALID_PARAM, "InvalidParameterException: empty class_ssnr");
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (!h->ready) return fail(h, FH_ERR_STATE, "determine_slice_agreement() before setup()");
+    if (h->mode == 10) return FH_OK;  // fourier_iter: returns without comparing (reconstructor.cpp:639)
+    FH_TRY(h, hipSetDevice(h->device));
+    const int P = h->P, H = h->H;
+    std::vector<float> rot9, xf6;
+    int rc = check_xforms(h, xform, 1, !is_preprocessed, rot9, &xf6);
+    if (rc) return rc;
+    if ((rc = ensure_chunk(h, std::max(1, h->cap_n), std::max(P, h->cap_B)))) return rc;
+    if ((rc = ensure_partial(h))) return rc;
+    if (is_preprocessed) {
+        FH_TRY(h, hipMemcpyAsync(h->slices, data, sizeof(float2) * P * (H + 1), hipMemcpyHostToDevice, h->stream));
+    } else {
+        FH_TRY(h, hipMemcpyAsync(h->imgs, data, sizeof(float) * P * P, hipMemcpyHostToDevice, h->stream));
+        if ((rc = prep_single_raw(h, xf6.empty() ? nullptr : xf6.data()))) return rc;
+    }
+    std::vector<float> rw(H + 1);
+    if (sub) {
+        if (!h->rw1) FH_TRY(h, hipMalloc(&h->rw1, sizeof(float) * (H + 1)));
+        ssnr_radial(P, class_ssnr, ssnr_len, rw.data());
+        FH_TRY(h, hipMemcpyAsync(h->rw1, rw.data(), sizeof(float) * (H + 1), hipMemcpyHostToDevice, h->stream));
+    }
+    // agree_one synchronises the stream before it returns: rw outlives the copy
+    return agree_one(h, h->slices, rot9.data(), weight, sub, apix, out, sub ? h->rw1 : nullptr);
 }
 
 int fh_agreement_batch(fh_reconstructor *h, const float *imgs, int n, int box, const float *xforms, const float *weights,

@@ -336,14 +336,28 @@ class FourierHipReconstructor:
         self._check(self._lib.fh_projection(self._need(), _ptr(m), int(bool(ret_fourier)), _ptr(out)))
         return EMData(out, bool(ret_fourier))
 
-    def determine_slice_agreement(self, img, xform, weight=1.0, sub=True):
+    def determine_slice_agreement(self, img, xform, weight=1.0, sub=True, reference_sub=False):
+        """sub: the slice's own contribution is removed before the compare and
+        added back after it.  reference_sub: the reference's own sequence
+        instead (reconstructor.cpp:1212-1257) -- its removal call passes
+        -weight, the SSNR flag of do_insert_slice_work, so the slice is added
+        with its class_ssnr weights, compared, and added again with +weight
+        (the volume keeps both); needs the slice's class_ssnr attribute
+        (NotExistingObjectException without it, as in the reference)."""
         pre = int(isinstance(img, EMData) and bool(img.get_attr_default("reconstruct_preproc", 0)))
         a = _arr(img)
         m = _xform12(xform)
         out = np.zeros(5, np.float32)
         apix = float(img.get_attr_default("apix_x", 1.0)) if isinstance(img, EMData) else 1.0
-        self._check(self._lib.fh_determine_slice_agreement(self._need(), _ptr(a), pre, _ptr(m), float(weight),
-                                                           int(bool(sub)), apix, _ptr(out)))
+        if sub and reference_sub:
+            has = isinstance(img, EMData) and img.has_attr("class_ssnr")
+            ss = np.ascontiguousarray(img.get_attr("class_ssnr"), np.float32).ravel() if has else None
+            self._check(self._lib.fh_determine_slice_agreement_ssnr(
+                self._need(), _ptr(a), pre, _ptr(m), float(weight), 1, apix, None if ss is None else _ptr(ss),
+                0 if ss is None else int(ss.size), _ptr(out)))
+        else:
+            self._check(self._lib.fh_determine_slice_agreement(self._need(), _ptr(a), pre, _ptr(m), float(weight),
+                                                               int(bool(sub)), apix, _ptr(out)))
         keys = ("reconstruct_norm", "reconstruct_absqual", "reconstruct_absqual_lowres", "reconstruct_qual",
                 "reconstruct_weight")
         res = dict(zip(keys, (float(v) for v in out)))
@@ -478,7 +492,7 @@ class FourierIterHipReconstructor(FourierHipReconstructor):
     def setup_seedandweights(self, seed, weights):
         self.setup_seed(seed, 1.0)
 
-    def determine_slice_agreement(self, img, xform, weight=1.0, sub=True):
+    def determine_slice_agreement(self, img, xform, weight=1.0, sub=True, reference_sub=False):
         """returns 0 without comparing (reconstructor.cpp:639): no attributes set"""
         self._need()
         return {}

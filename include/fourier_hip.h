@@ -185,6 +185,21 @@ int fh_determine_slice_agreement(fh_reconstructor *h, const float *data, int is_
                                  const float xform[12], float weight, int sub, float apix,
                                  float out[5]);
 
+/* determine_slice_agreement in the reference's own order for a slice that
+ * carries class_ssnr (reconstructor.cpp:1212-1257): with sub, the first
+ * do_insert_slice_work(-weight) takes its weight < 0 SSNR branch
+ * (reconstructor.cpp:1160-1184) and ADDS the slice with the per-radius
+ * weights max(0, class_ssnr[int(r * 2 (len-1) / P)]) instead of removing it,
+ * the compare follows, and the slice is added again with +weight: the
+ * volume keeps both copies.  sub without class_ssnr fails with
+ * FH_ERR_NOT_EXISTING, the reference's NotExistingObjectException.
+ * fh_determine_slice_agreement (sub) removes the slice's own contribution
+ * instead -- what the reference code intends and what the refine flow on
+ * raw particles needs.  sub = 0 is fh_determine_slice_agreement. */
+int fh_determine_slice_agreement_ssnr(fh_reconstructor *h, const float *data, int is_preprocessed,
+                                      const float xform[12], float weight, int sub, float apix,
+                                      const float *class_ssnr, int ssnr_len, float out[5]);
+
 /* Batched agreement of n raw particles against the current volume, each
  * compared as if its own contribution were removed (sub) -- the
  * e2make3d.py:447-491 reweighting pass.  out is n x 5 (host or device per

@@ -209,6 +209,40 @@ def test_wide_mode_single_slice_symmetry(oracle, sym, mode):
     assert_state_close(ad, an, v.data, v.norm)
 
 
+@pytest.mark.parametrize("mode", ["gauss_2", "gauss_var"])
+def test_agreement_in_the_reference_order(oracle, mode):
+    """determine_slice_agreement(sub=True) in the reference's own order
+    (reconstructor.cpp:1212-1257): its removal call passes -weight, the SSNR
+    flag of do_insert_slice_work (reconstructor.cpp:1160-1184), so the slice
+    is added with its class_ssnr weights, compared, and added again with
+    +weight; without class_ssnr the reference throws
+    NotExistingObjectException"""
+    from eman2_amd._lib import FourierHipError
+    from eman2_amd.reconstructor import EMData
+    P, B, n = 48, 32, 12
+    ph = S.phantom(10, 10.0, seed=21)
+    imgs, mats, _ = S.particles(ph, n, B, seed=22, snr=1.0)
+    slices = [oracle.prep_particle(im, P) for im in imgs]
+    r = recon(P, "c1", mode)
+    v = oracle.Volume(P, "c1", mode)
+    for sl, m in zip(slices, mats):
+        assert r.insert_slice(EMData(sl, True, {"reconstruct_preproc": 1}), m, 1.0) == 0
+        v.insert(sl, m, 1.0)
+    q = np.random.default_rng(23).uniform(-0.2, 2.0, 17).astype(np.float32)
+    e = EMData(slices[3], True, {"reconstruct_preproc": 1, "class_ssnr": q})
+    got = r.determine_slice_agreement(e, mats[3], 0.75, True, reference_sub=True)
+    v.insert_ssnr(slices[3], mats[3], q)
+    ref = v.agreement(slices[3], mats[3], 0.75, sub=False, apix=1.0)
+    v.insert(slices[3], mats[3], 0.75)
+    for k in ("reconstruct_norm", "reconstruct_absqual", "reconstruct_absqual_lowres", "reconstruct_weight"):
+        assert abs(got[k] - ref[k]) <= 1e-4 * max(1.0, abs(ref[k])), (k, got[k], ref[k])
+    gd, gn = r.get_state()
+    assert_state_close(gd, gn, v.data, v.norm)
+    with pytest.raises(FourierHipError, match="NotExistingObjectException"):
+        r.determine_slice_agreement(EMData(slices[3], True, {"reconstruct_preproc": 1}), mats[3], 0.75, True,
+                                    reference_sub=True)
+
+
 @pytest.mark.parametrize("sym,mode", [("d2", "gauss_2"), ("c1", "trilinear"), ("d2", "gauss_var")])
 def test_usessnr_batch_and_slice(oracle, sym, mode):
     """usessnr: per-radius class_ssnr weights (reconstructor.cpp:1080-1088,
