@@ -61,11 +61,12 @@ def quality_cutoff(absqual, qlist, keep, keepsig, keepabs):
 
 
 def reconstruct(recon, imgs, xforms, weights=None, niter=2, keep=1.0, keepsig=False, keepabs=False,
-                start=None, startweight=0.0, apix=1.0, defocus=None, ctf=None):
+                start=None, startweight=0.0, apix=1.0, defocus=None, ctf=None, progress=None):
     """e2make3d.py reconstruct() over numpy particles (n, box, box) and their
     3x4 Transforms.  Returns (map, info): the finished map and per-particle
     norm / qualities plus the included / excluded indices of the last pass
-    and ptcl_repr (the summed weight of the inserted particles)."""
+    and ptcl_repr (the summed weight of the inserted particles).
+    progress(fraction) is called after each pass (E2progress)."""
     imgs = np.ascontiguousarray(imgs, np.float32)
     n = len(imgs)
     xf = np.ascontiguousarray(xforms, np.float32).reshape(n, 3, 4)
@@ -117,9 +118,74 @@ def reconstruct(recon, imgs, xforms, weights=None, niter=2, keep=1.0, keepsig=Fa
                 qcutoff = quality_cutoff(absqual, qlist, keep, keepsig, keepabs)
         else:
             out = recon.finish(True)
+        if progress is not None:
+            progress((it + 1) / niter)
     info = dict(norm=norm, reconstruct_absqual=absqual, reconstruct_qual=qual, qcutoff=qcutoff,
                 threed_ptcl_idxs=included, threed_excl_ptcl_idxs=excluded, ptcl_repr=ptcl)
     return out, info
+
+
+def fill_orientations(xform, fillangle, pad):
+    """e2make3dpar.py --fillangle (reconstruct(), :666-677, 737-747): the
+    orientations a particle is inserted at and their weight factors.  The
+    step is atan2(1, P/2) degrees (P = max(pad)); fillangle / step is cut
+    to at most 9 and at most 1 means no filling (the particle's own
+    Transform, factor 1); otherwise a den x den grid of (alt, az) offsets
+    from -fillangle/2 with step fillangle/(den-1) - 1e-5, each the eman
+    rotation (az + daz, alt + dalt, phi 0) with factor
+    exp(-(dalt^2 + daz^2) / (fillangle/4)^2).  Returns [(3x4 float32, factor)]:
+    the fill rotations carry the particle's own scale, x mirror and
+    translation, which preprocess_slice applies to the image before the
+    insertion, as the reference preprocesses it with the particle's
+    Transform and inserts at the new ones."""
+    from .transform import Transform, euler_eman_matrix
+    m0 = np.asarray(xform, np.float32).reshape(3, 4)
+    P = max(pad) if np.ndim(pad) else pad
+    astep = math.atan2(1.0, P / 2.0) * 180.0 / math.pi
+    den = math.floor(fillangle / astep)
+    if den > 9:
+        den = 9
+    if den <= 1:
+        return [(m0.copy(), 1.0)]
+    astep = fillangle / (den - 1) - .00001
+    t = Transform(m0)
+    rot = t.get_rotation("eman")
+    scale, mirror = t.get_scale_and_mirror()
+    out = []
+    for dalt in np.arange(-fillangle / 2.0, fillangle / 2.0, astep):
+        for daz in np.arange(-fillangle / 2.0, fillangle / 2.0, astep):
+            wm = math.exp(-(dalt ** 2 + daz ** 2) / (fillangle / 4.0) ** 2)
+            m = euler_eman_matrix(rot["az"] + daz, rot["alt"] + dalt, 0.0)
+            if scale != 1.0:
+                m[:, :3] *= np.float32(scale)
+            if mirror:
+                m[0, :3] *= -1
+            m[:, 3] = m0[:, 3]
+            out.append((m, wm))
+    return out
+
+
+def insert_with_fill(recon, imgs, xforms, weights=None, fillangle=0.0, pad=None, defocus=None, ctf=None,
+                     progress=None):
+    """e2make3dpar.py reconstruct()'s insertion with --fillangle (:737-747):
+    every particle inserted at each of its fill_orientations() with its
+    weight times the factor -- batched as one insert_batch per grid offset
+    (every particle has the same den x den grid).  pad defaults to the
+    reconstructor's size.  progress(fraction) is called after each batch
+    (E2progress)."""
+    imgs = np.ascontiguousarray(imgs, np.float32)
+    n = len(imgs)
+    xf = np.ascontiguousarray(xforms, np.float32).reshape(n, 3, 4)
+    w = np.ones(n, np.float32) if weights is None else np.ascontiguousarray(weights, np.float32)
+    pad = recon.P if pad is None else pad
+    fills = [fill_orientations(xf[i], fillangle, pad) for i in range(n)]
+    k = len(fills[0]) if n else 0
+    for j in range(k):
+        xj = np.stack([f[j][0] for f in fills])
+        wj = (w * np.array([f[j][1] for f in fills], np.float64)).astype(np.float32)
+        recon.insert_batch(imgs, xj, wj, defocus=defocus, ctf=ctf)
+        if progress is not None:
+            progress((j + 1) / k)
 
 
 def threshold_compress(vol):
@@ -139,7 +205,8 @@ def threshold_compress(vol):
     return np.where(a > v + r, a - r, np.where(a < v - r, a + r, v)).astype(np.float32)
 
 
-def reconstruct_iterative(recon, imgs, xforms, weights=None, niter=4, seed=None, defocus=None, ctf=None):
+def reconstruct_iterative(recon, imgs, xforms, weights=None, niter=4, seed=None, defocus=None, ctf=None,
+                          progress=None):
     """e2make3dpar.py --iterative (:241-246, 353-392): niter passes of a
     fourier_iter reconstructor, each inserting every particle and ending
     with finish(True); the first starts from setup() (or the seed map), each
@@ -164,10 +231,12 @@ def reconstruct_iterative(recon, imgs, xforms, weights=None, niter=4, seed=None,
         out = out.numpy() if hasattr(out, "numpy") else np.asarray(out)
         if it < niter - 1:
             out = threshold_compress(out)
+        if progress is not None:
+            progress((it + 1) / niter)
     return out
 
 
-def spa_make3d(make_recon, imgs, xforms, weights=None, ntasks=4, defocus=None, ctf=None):
+def spa_make3d(make_recon, imgs, xforms, weights=None, ntasks=4, defocus=None, ctf=None, progress=None):
     """e2spa_make3d.py's split and merge (:154-192, Make3dTask :351-395):
     particle i goes to task i % ntasks; each task inserts its particles of
     weight > 0 into its own reconstructor (make_recon() -> a reconstructor
@@ -194,6 +263,8 @@ def spa_make3d(make_recon, imgs, xforms, weights=None, ntasks=4, defocus=None, c
         normout = np.zeros((r.P, r.P, r.P // 2 + 1), np.float32)
         r.set_params({"normout": normout})
         parts.append((r.finish(False, return_numpy=True), normout))
+        if progress is not None:
+            progress((t + 1) / ntasks)
     m = make_recon()
     m.setup()
     for threed, nv in parts:

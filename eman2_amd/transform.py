@@ -11,6 +11,19 @@ import math
 import numpy as np
 
 _DEG2RAD = math.pi / 180.0  # EMConsts::deg2rad (libEM/emobject.cpp:55-56)
+_ERR_LIMIT = 0.000001         # Transform::ERR_LIMIT (libEM/transform.cpp:57)
+
+
+def _apply_precision(v, precision=_ERR_LIMIT):
+    """Util::apply_precision (libEM/util.h:1309-1314): a value within
+    `precision` of an integer becomes that integer."""
+    c = float(np.ceil(np.float32(v)))
+    f = float(math.floor(v))
+    if abs(v - c) < precision:
+        return c
+    if abs(v - f) < precision:
+        return f
+    return v
 
 
 def euler_eman_matrix(az, alt, phi):
@@ -82,6 +95,58 @@ class Transform:
 
     def get_matrix(self):
         return self.matrix.reshape(12).tolist()
+
+    def get_determinant(self):
+        """Transform::get_determinant (libEM/transform.cpp:1310-1322): the
+        3x3 determinant in double, stored as float, snapped to an integer
+        within ERR_LIMIT (Util::apply_precision)."""
+        m = self.matrix
+        d = float(m[0, 0]) * (float(m[1, 1]) * float(m[2, 2]) - float(m[2, 1]) * float(m[1, 2]))
+        d -= float(m[0, 1]) * (float(m[1, 0]) * float(m[2, 2]) - float(m[2, 0]) * float(m[1, 2]))
+        d += float(m[0, 2]) * (float(m[1, 0]) * float(m[2, 1]) - float(m[2, 0]) * float(m[1, 1]))
+        return _apply_precision(float(np.float32(d)))
+
+    def get_scale_and_mirror(self):
+        """Transform::get_scale_and_mirror (libEM/transform.cpp:1291-1308):
+        the x mirror is a negative determinant, the scale its cube root."""
+        det = self.get_determinant()
+        mirror = det < 0
+        if mirror:
+            det = -det
+        if det != 1:
+            scale = float(np.float32(det ** (1.0 / 3.0)))
+            if scale - int(scale) < _ERR_LIMIT:
+                scale = float(int(scale))
+        else:
+            scale = 1.0
+        return _apply_precision(scale), mirror
+
+    def get_rotation(self, euler_type="eman"):
+        """Transform::get_rotation("eman") (libEM/transform.cpp:852-898):
+        az, alt, phi in degrees; alt 0 or 180 puts the whole in-plane angle
+        into phi."""
+        if str(euler_type).lower() != "eman":
+            raise ValueError("only the 'eman' Euler convention is supported, got %r" % euler_type)
+        scale, mirror = self.get_scale_and_mirror()
+        if scale == 0:
+            raise ValueError("UnexpectedBehaviorException: The determinant of the Transform is 0")
+        m = self.matrix.astype(np.float64)
+        cosalt = m[2, 2] / scale
+        xs = -1.0 if mirror else 1.0
+        if cosalt >= 1:
+            az, alt, phi = 0.0, 0.0, math.degrees(math.atan2(xs * m[0, 1], xs * m[0, 0]))
+        elif cosalt <= -1:
+            az, alt, phi = 0.0, 180.0, math.degrees(math.atan2(-xs * m[0, 1], xs * m[0, 0]))
+        else:
+            az = math.degrees(math.atan2(scale * m[2, 0], -scale * m[2, 1]))
+            if m[2, 2] == 0.0:
+                alt = 90.0
+            else:
+                alt = math.degrees(math.atan(math.sqrt(m[2, 0] * m[2, 0] + m[2, 1] * m[2, 1]) / abs(m[2, 2])))
+            if m[2, 2] * scale < 0:
+                alt = 180.0 - alt
+            phi = math.degrees(math.atan2(xs * m[0, 2], m[1, 2]))
+        return {"type": "eman", "az": az, "alt": alt, "phi": phi}
 
     def is_pure_rotation(self, tol=1e-6):
         m = self.matrix.astype(np.float64)

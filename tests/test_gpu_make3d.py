@@ -110,3 +110,38 @@ def test_spa_make3d_task_merge_equals_one_reconstructor():
     e, f = max_rel_err(merged, ref), fsc(merged, ref).min()
     print("spa merge: max_rel_err %.2e min FSC %.6f" % (e, f), flush=True)
     assert e <= 1e-4 and f >= 0.9999
+
+
+@pytest.mark.parametrize("mode", ["gauss_2", "gauss_var"])
+def test_fillangle_insertion_matches_oracle(oracle, mode):
+    """e2make3dpar.py --fillangle (:737-747): each particle, preprocessed
+    with its own Transform (here with a translation), inserted at the den x
+    den (alt, az) grid of fill_orientations with phi 0 and the Gaussian
+    factor -- against the oracle's prep with the particle's Transform and
+    insertion at each fill rotation"""
+    from eman2_amd import Reconstructors
+    from eman2_amd.transform import euler_eman_matrix
+    P, B, n = 48, 32, 10
+    ph = S.phantom(10, 10.0, seed=71)
+    imgs, mats, _ = S.particles(ph, n, B, seed=72, snr=1.0)
+    rng = np.random.default_rng(73)
+    xf = np.array(mats, np.float32).reshape(n, 3, 4).copy()
+    xf[:, :2, 3] = rng.uniform(-2.0, 2.0, (n, 2))
+    w = rng.uniform(0.5, 1.5, n).astype(np.float32)
+    fa = 6.0   # P = 48: step atan2(1, 24) = 2.39 deg, den = 2
+    print("fillangle %s P=%d n=%d" % (mode, P, n), flush=True)
+    r = Reconstructors.get("fourier_hip", dict(size=(P, P, P), sym="c1", mode=mode))
+    r.setup()
+    make3d.insert_with_fill(r, imgs, xf, w, fillangle=fa)
+    gd, gn = r.get_state()
+    v = oracle.Volume(P, "c1", mode)
+    for i in range(n):
+        s = oracle.prep_particle(imgs[i], P, xform=xf[i])
+        for mk, wm in make3d.fill_orientations(xf[i], fa, P):
+            rot = mk.copy()
+            rot[:, 3] = 0.0   # the insertion takes the rotation; the translation went to the prep
+            v.insert(s, rot, float(np.float32(w[i] * wm)))
+    assert len(make3d.fill_orientations(xf[0], fa, P)) == 4
+    from eman2_amd.metrics import weight_rel_err
+    assert weight_rel_err(gn, v.norm) <= 1e-4
+    assert max_rel_err(gd, v.data) <= 1e-4

@@ -82,3 +82,78 @@ def test_reconstruct_iterative_seeds_each_pass_with_the_compressed_map():
     np.testing.assert_array_equal(r.calls[2][1], threshold_compress(first))
     assert r.calls[2][2] == 1.0
     assert out[0, 0, 0] == 3.0     # the last pass's map, not compressed
+
+
+def test_get_rotation_recovers_the_matrix():
+    """Transform.get_rotation("eman") (transform.cpp:852-898): the angles it
+    returns rebuild the same rotation, with and without an x mirror or a
+    scale, and at alt 0 / 180 (the whole in-plane angle goes to phi)"""
+    from eman2_amd.transform import Transform, euler_eman_matrix
+    rng = np.random.default_rng(3)
+    cases = [tuple(rng.random(3) * (360, 180, 360)) for _ in range(20)] + [(35.0, 0.0, 20.0), (35.0, 180.0, 20.0)]
+    for i, (az, alt, phi) in enumerate(cases):
+        mirror, scale = i % 3 == 1, 1.0 if i % 4 else 1.25
+        t = Transform({"type": "eman", "az": az, "alt": alt, "phi": phi, "mirror": mirror, "scale": scale})
+        s, m = t.get_scale_and_mirror()
+        assert m == mirror and abs(s - scale) < 1e-6
+        r = t.get_rotation("eman")
+        back = euler_eman_matrix(r["az"], r["alt"], r["phi"])[:, :3] * np.float32(scale)
+        if mirror:
+            back[0] *= -1
+        np.testing.assert_allclose(back, t.matrix[:, :3], atol=2e-6)
+    r0 = Transform({"type": "eman", "az": 35.0, "alt": 0.0, "phi": 20.0}).get_rotation()
+    assert r0["az"] == 0.0 and r0["alt"] == 0.0 and abs(r0["phi"] - 55.0) < 1e-4
+
+
+def test_fill_orientations_grid_and_weights():
+    """e2make3dpar.py --fillangle (:666-677, 737-747): step atan2(1, P/2) deg,
+    den = floor(fillangle / step) capped at 9, no filling for den <= 1; a den x
+    den grid of (alt, az) offsets with factor exp(-(da^2 + dz^2) / (fa/4)^2),
+    phi 0, the particle's own translation kept for preprocess_slice"""
+    import math
+
+    from eman2_amd.make3d import fill_orientations
+    from eman2_amd.transform import Transform, euler_eman_matrix
+    m = euler_eman_matrix(30.0, 40.0, 50.0)
+    m[:, 3] = (2.5, -1.5, 0.0)
+    step = math.degrees(math.atan2(1.0, 192.0))               # P = 384
+    assert len(fill_orientations(m, 1.9 * step, 384)) == 1   # den 1: no filling
+    np.testing.assert_array_equal(fill_orientations(m, 1.9 * step, 384)[0][0], m)
+    assert len(fill_orientations(m, 100.0, 384)) == 81       # den capped at 9
+    f = fill_orientations(m, 1.0, 384)                        # den 3, offsets -0.5, ~0, ~0.5
+    assert len(f) == 9
+    ws = sorted(w for _, w in f)
+    np.testing.assert_allclose(ws[-1], 1.0, atol=1e-8)                      # the centre
+    np.testing.assert_allclose(ws[0], math.exp(-8.0), rtol=1e-3)            # a corner: 2 (0.5)^2 / 0.25^2
+    for mk, _ in f:
+        r = Transform(mk).get_rotation("eman")
+        assert abs(r["phi"]) < 1e-3 or abs(abs(r["phi"]) - 360.0) < 1e-3
+        assert abs(((r["az"] - 30.0 + 180.0) % 360.0) - 180.0) <= 0.5 + 1e-4
+        assert abs(r["alt"] - 40.0) <= 0.5 + 1e-4
+        np.testing.assert_array_equal(mk[:, 3], m[:, 3])
+
+
+def test_insert_with_fill_batches_by_offset():
+    from eman2_amd.make3d import insert_with_fill
+
+    class Rec:
+        P = 96
+
+        def __init__(self):
+            self.calls = []
+
+        def insert_batch(self, imgs, xf, w, defocus=None, ctf=None):
+            self.calls.append((len(imgs), np.array(xf), np.array(w)))
+
+    from eman2_amd.transform import euler_eman_matrix
+    xf = np.stack([euler_eman_matrix(10.0 * i, 30.0, 5.0) for i in range(4)])
+    r = Rec()
+    seen = []
+    insert_with_fill(r, np.zeros((4, 8, 8), np.float32), xf, np.full(4, 2.0, np.float32), fillangle=3.0,
+                     progress=seen.append)
+    # P = 96: step atan2(1, 48) = 1.19 deg, den = floor(3 / 1.19) = 2: a 2 x 2 grid, one batch per offset
+    assert len(r.calls) == 4 and all(c[0] == 4 for c in r.calls)
+    assert seen[-1] == 1.0 and len(seen) == len(r.calls)
+    for _, xk, wk in r.calls:
+        assert xk.shape == (4, 3, 4)
+        np.testing.assert_allclose(wk, 2.0 * np.exp(-2 * 1.5 ** 2 / 0.75 ** 2), rtol=3e-4)  # offsets -1.5, 1.49999
