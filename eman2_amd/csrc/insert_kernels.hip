@@ -2338,29 +2338,15 @@ __global__ void __launch_bounds__(256) k_scatter_wide(const float2 *__restrict__
                     if (__ballot(anyb)) {
 #pragma unroll
                         for (int t = 0; t < NIT; t++)
-                            if (big[t]) {
-                                tv[t] = expf(fv[t]);
-                                on[t] = on[t] && !(fabsf(tv[t] * u.ww) < FLT_MIN);
-                            }
+                            if (big[t]) tv[t] = expf(fv[t]);
                     }
-                    if (u.ww == 1.0f) {
-                        // weight 1 (wave-uniform): gg * 1 = gg exactly, and a table entry is at least
-                        // exp(-19.98) > FLT_MIN, so only the expf terms above can be dropped
 #pragma unroll
-                        for (int t = 0; t < NIT; t++) {
-                            vg[t] = tv[t];
-                            vr[t] = u.dre * tv[t];
-                            vi[t] = u.dim * tv[t];
-                        }
-                    } else {
-#pragma unroll
-                        for (int t = 0; t < NIT; t++) {
-                            const float gg = tv[t];
-                            vg[t] = gg * u.ww;
-                            on[t] = on[t] && !(fabsf(vg[t]) < FLT_MIN);
-                            vr[t] = u.dre * gg * u.ww;
-                            vi[t] = u.dim * gg * u.ww;
-                        }
+                    for (int t = 0; t < NIT; t++) {
+                        const float gg = tv[t];
+                        vg[t] = gg * u.ww;
+                        vr[t] = u.dre * gg * u.ww;
+                        vi[t] = u.dim * gg * u.ww;
+                        on[t] = on[t] && !(fabsf(vg[t]) < FLT_MIN);
                     }
                 }
 #pragma unroll
