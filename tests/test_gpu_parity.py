@@ -514,6 +514,25 @@ def test_corners_and_sqrtnorm(oracle):
     assert_map_close(r.finish(True, return_numpy=True), v.finish(True, sqrtnorm=True))
 
 
+@pytest.mark.parametrize("mode", ["gauss_var", "gauss_3", "gridding_7", "kaiser_bessel_derived"])
+def test_corners_wide_modes_batch(oracle, mode):
+    """corners (reconstructor.cpp:1172-1176: the whole half-square of slice
+    pixels, no radius cut) through the wide-stencil brick scatter: its
+    binning takes the half-square domain and its candidate windows the
+    pixels beyond the P/2 circle"""
+    P = 32
+    ph = S.phantom(6, 6.0, seed=13)
+    imgs, mats, _ = S.particles(ph, 24, 24, seed=14, snr=2.0)
+    w = np.linspace(0.5, 1.5, 24).astype(np.float32)
+    r = recon(P, "d2", mode, corners=True, slices_per_item=4)
+    r.insert_batch(imgs, list(mats), w)
+    gd, gn = r.get_state()
+    v = oracle.Volume(P, "d2", mode)
+    for im, m, wi in zip(imgs, mats, w):
+        v.insert(oracle.prep_particle(im, P), m, float(wi), corners=True)
+    assert_state_close(gd, gn, v.data, v.norm)
+
+
 def test_finish_complex_and_normout(oracle):
     P = 32
     ph = S.phantom(6, 6.0, seed=11)
