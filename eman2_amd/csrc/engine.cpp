@@ -546,6 +546,7 @@ static BrickGeom brick_geom(int P, int mode)
     }
     g.nb = (H - g.ow + 1 + g.bs - 1) / g.bs;
     g.part_floats = 3 * g.halo * g.halo * g.halo;
+    g.per_brick = S != 0;
     g.nbricks = g.nb * g.nb * g.nb;
     g.nkeys = 3 * g.nbricks;
     return g;

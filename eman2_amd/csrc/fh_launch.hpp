@@ -100,6 +100,7 @@ struct BrickGeom {
     int ow;           // lowest stencil origin of brick 0
     int part_floats;  // 3 * halo^3: re, im, weight of one partial brick
     float po;         // p-space start of brick 0
+    int per_brick;    // 1: work items span a brick's three axis keys (the wide-stencil scatter)
 };
 
 constexpr int WBS = 4;  // brick edge of the wide-stencil scatter

@@ -1536,8 +1536,20 @@ __global__ void __launch_bounds__(256) k_bin_cols(InsertConst c, BrickGeom g, Bi
 
 // Exclusive scans of the key counts (-> list offsets) and of the work items
 // per key (ceil(count / K) -> item offsets) in three passes: per-tile sums,
-// a scan of the tile sums with the totals, and the per-key offsets.
+// a scan of the tile sums with the totals, and the per-key offsets.  With
+// per_brick the items of a brick span its three axis keys (whose lists are
+// consecutive): key 3b holds ceil((c[3b] + c[3b+1] + c[3b+2]) / K) items,
+// keys 3b+1 and 3b+2 none.
 constexpr int SCAN_TILE = 1024;  // keys per tile: 256 threads x 4
+
+__device__ __forceinline__ int key_items(const int *__restrict__ counts, int k, int nkeys, int ck, int K,
+                                         int per_brick)
+{
+    if (!per_brick) return (ck + K - 1) / K;
+    if (k % 3) return 0;
+    const int c = ck + (k + 1 < nkeys ? counts[k + 1] : 0) + (k + 2 < nkeys ? counts[k + 2] : 0);
+    return (c + K - 1) / K;
+}
 
 __device__ __forceinline__ int4 load_counts4(const int *counts, int k, int n)
 {
@@ -1584,8 +1596,12 @@ __global__ void __launch_bounds__(256) k_scan_tiles(BrickGeom g, BinBuffers bb, 
 {
     const int k = blockIdx.x * SCAN_TILE + threadIdx.x * 4;
     const int4 c = load_counts4(bb.counts, k, g.nkeys);
+    const int cv[4] = {c.x, c.y, c.z, c.w};
     int a = c.x + c.y + c.z + c.w;
-    int b = (c.x + K - 1) / K + (c.y + K - 1) / K + (c.z + K - 1) / K + (c.w + K - 1) / K;
+    int b = 0;
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+        if (k + t < g.nkeys) b += key_items(bb.counts, k + t, g.nkeys, cv[t], K, g.per_brick);
     int ta, tb;
     block_scan2(a, b, ta, tb);
     if (threadIdx.x == 0) {
@@ -1640,8 +1656,11 @@ __global__ void __launch_bounds__(256) k_scan_apply(BrickGeom g, BinBuffers bb, 
     const int k = blockIdx.x * SCAN_TILE + threadIdx.x * 4;
     const int4 c = load_counts4(bb.counts, k, g.nkeys);
     const int cv[4] = {c.x, c.y, c.z, c.w};
+    int iv[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) iv[t] = k + t < g.nkeys ? key_items(bb.counts, k + t, g.nkeys, cv[t], K, g.per_brick) : 0;
     int a = c.x + c.y + c.z + c.w;
-    int b = (c.x + K - 1) / K + (c.y + K - 1) / K + (c.z + K - 1) / K + (c.w + K - 1) / K;
+    int b = iv[0] + iv[1] + iv[2] + iv[3];
     int ta, tb;
     block_scan2(a, b, ta, tb);
     a += bb.tile[2 * blockIdx.x];
@@ -1653,7 +1672,7 @@ __global__ void __launch_bounds__(256) k_scan_apply(BrickGeom g, BinBuffers bb, 
             bb.item_off[k + t] = b;
         }
         a += cv[t];
-        b += (cv[t] + K - 1) / K;
+        b += iv[t];
     }
 }
 
@@ -1663,7 +1682,10 @@ __global__ void k_bin_items(BrickGeom g, BinBuffers bb, int K)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= g.nkeys) return;
-    const int cnt = bb.counts[b];
+    if (g.per_brick && b % 3) return;  // the brick's items are listed under key 3 * brick
+    const int cnt = g.per_brick ? bb.counts[b] + (b + 1 < g.nkeys ? bb.counts[b + 1] : 0) +
+                                      (b + 2 < g.nkeys ? bb.counts[b + 2] : 0)
+                                : bb.counts[b];
     const int base = bb.item_off[b];
     const int off = bb.offsets[b];
     for (int i = 0; i * K < cnt; i++) {
