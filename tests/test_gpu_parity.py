@@ -514,6 +514,23 @@ def test_corners_and_sqrtnorm(oracle):
     assert_map_close(r.finish(True, return_numpy=True), v.finish(True, sqrtnorm=True))
 
 
+@pytest.mark.parametrize("P,sym,mode", [(50, "c4", "gauss_var"), (90, "c1", "gauss_var"), (50, "c1", "gridding_5"),
+                                        (90, "d2", "gauss_3")])
+def test_wide_modes_odd_half_size(oracle, P, sym, mode):
+    """the wide-stencil brick grid when P/2 is odd (P = 50, 90): the brick
+    origins start at -P/2 - 4 (-P/2 - 2 for gauss_3) and the last brick is
+    partly past P/2"""
+    ph = S.phantom(8, 0.2 * P, seed=P)
+    imgs, mats, _ = S.particles(ph, 20, int(P * 0.66) // 2 * 2, seed=P + 1, snr=1.0)
+    w = np.linspace(0.5, 1.5, 20).astype(np.float32)
+    r = recon(P, sym, mode, slices_per_item=8)
+    r.insert_batch(imgs, list(mats), w)
+    gd, gn = r.get_state()
+    v = oracle_volume(oracle, P, sym, mode, [oracle.prep_particle(im, P) for im in imgs], mats, w)
+    assert_state_close(gd, gn, v.data, v.norm)
+    assert_map_close(r.finish(True, return_numpy=True), v.finish(True))
+
+
 @pytest.mark.parametrize("mode", ["gauss_var", "gauss_3", "gridding_7", "kaiser_bessel_derived"])
 def test_corners_wide_modes_batch(oracle, mode):
     """corners (reconstructor.cpp:1172-1176: the whole half-square of slice
