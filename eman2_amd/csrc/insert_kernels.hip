@@ -884,18 +884,37 @@ __device__ void own_contribution(const float2 *S, const float *t3dp, int nsym, i
 }
 
 // Tiles of compare pixels: each workgroup takes a 32 x 16 tile of one
-// (particle, symmetry op) slice and stages the slice around it (3-pixel
-// halo) in LDS -- pixel positions, values and the insertion loop's pixel
+// (particle, symmetry op) slice and stages the slice around it (2-pixel
+// halo: the sources of a compare pixel's nearest cell lie within 2 pixels
+// of it) in LDS -- pixel positions, values and the insertion loop's pixel
 // filters -- so that the own-contribution gather of the slice's own plane
 // reads LDS; the other symmetry planes, the conjugate cells and windows
 // that leave the patch take the global gather.  Same arithmetic and order
 // as gather_cell.
-constexpr int AT_X = 32, AT_Y = 16, AT_HALO = 3;
+constexpr int AT_X = 32, AT_Y = 16, AT_HALO = 2;
 constexpr int AT_PX = AT_X + 2 * AT_HALO, AT_PY = AT_Y + 2 * AT_HALO;
 constexpr int AT_PATCH = AT_PX * AT_PY;
+// Mirror patch of the x = 0 tiles: pixels x in [-4, 2], y in
+// [-ty0 - AT_Y - 1, -ty0 + 2], the sources of the mirrored cells of compare
+// pixels x < 3 (around (-x, -y)).
+constexpr int AT_MX = 7, AT_MY = AT_Y + 4, AT_MPATCH = AT_MX * AT_MY;
+
+// Patch entries carry the floors of their position packed as
+// (x0 + 512) | (y0 + 512) << 10 | (z0 + 512) << 20 in the .w bits, or
+// AP_NOKEY for pixels the insertion loop skips (radius cut, origin range).
+// For a cell key kc, kc - key has every 10-bit digit in {0, 1} exactly when
+// the cell is a corner of the pixel's 2x2x2 stencil; the digits are
+// differences of at most a few units between neighbouring pixels, so the
+// packed difference is unambiguous, and AP_NOKEY makes it negative.
+constexpr int AP_NOKEY = (int)0x80000000u;
+constexpr unsigned AP_DIGITS = 0x100401u;
+__device__ __forceinline__ int ap_key(int x0, int y0, int z0)
+{
+    return (x0 + 512) | ((y0 + 512) << 10) | ((z0 + 512) << 20);
+}
 
 struct AgreePatch {
-    const float4 *pos;   // xx, yy, zz, usable (1.0f) -- radius cut and origin range of the insertion loop
+    const float4 *pos;   // xx, yy, zz, packed floors (AP_NOKEY: skipped by the insertion loop)
     const float2 *val;   // slice_value
     int x0, y0;          // pixel of patch entry 0
 };
@@ -921,7 +940,7 @@ __device__ __forceinline__ void gather_cell_patch(const float2 *__restrict__ S, 
         for (int x = xa; x <= xb; x++) {
             const int e = (y - pt.y0) * AT_PX + (x - pt.x0);
             const float4 q = pt.pos[e];
-            if (q.w == 0.f) continue;  // radius cut / origin out of range
+            if (__float_as_int(q.w) == AP_NOKEY) continue;  // radius cut / origin out of range
             const int x0 = (int)floorf(q.x), y0 = (int)floorf(q.y), z0 = (int)floorf(q.z);
             if ((unsigned)(ci - x0) > 1u || (unsigned)(cj - y0) > 1u || (unsigned)(ck - z0) > 1u) continue;
             const float gg = corner_weight(c.mode, ci, cj, ck, q.x, q.y, q.z, c.h, weight, etab);
@@ -930,6 +949,46 @@ __device__ __forceinline__ void gather_cell_patch(const float2 *__restrict__ S, 
             im += dt.y * gg;
             w += gg;
         }
+}
+
+// Own-plane contributions to unfolded cell (ci, cj, ck), the nearest cell of
+// patch entry e (row pitch PITCH), or its mirror -(ci, cj, ck) gathered
+// around the mirrored pixel.  A stencil that holds the cell starts within 1 of it on every
+// axis, and the cell lies within 0.5 of entry e's position, so the source
+// pixel sits less than 1.5 from that position on every axis: only the
+// in-plane offsets (ox, oy) in [-2, 2]^2 flagged in omask (bit
+// (oy + 2) * 5 + ox + 2, uniform over the slice) can hold it.  Same terms, in
+// the same y-then-x order, as gather_cell_patch.
+template <int PITCH>
+__device__ __forceinline__ void gather_own_offsets(const float4 *pos, const float2 *val, int e, unsigned omask, int ci,
+                                                   int cj, int ck, const InsertConst &c, float weight,
+                                                   const float *etab, float &re, float &im, float &w)
+{
+    const int kc = ap_key(ci, cj, ck);
+    for (unsigned mm = omask; mm != 0u; mm &= mm - 1u) {  // set bits in increasing order
+        const int k = __builtin_ctz(mm);
+        const int o = e + (k / 5 - 2) * PITCH + (k % 5 - 2);
+        const float4 q = pos[o];
+        if (((unsigned)(kc - __float_as_int(q.w)) & ~AP_DIGITS) != 0u) continue;
+        const float gg = corner_weight(c.mode, ci, cj, ck, q.x, q.y, q.z, c.h, weight, etab);
+        const float2 dt = val[o];
+        re += dt.x * gg;
+        im += dt.y * gg;
+        w += gg;
+    }
+}
+
+// Offsets (ox, oy) in [-2, 2]^2 whose step ox * row0 + oy * row1 of slice
+// rows m stays under 1.5 (plus rounding slack) on every axis.
+__device__ __forceinline__ unsigned own_offset_mask(const float *m)
+{
+    unsigned mask = 0;
+    for (int k = 0; k < 25; k++) {
+        const float ox = (float)(k % 5 - 2), oy = (float)(k / 5 - 2);
+        const float d0 = ox * m[0] + oy * m[3], d1 = ox * m[1] + oy * m[4], d2 = ox * m[2] + oy * m[5];
+        if (fmaxf(fabsf(d0), fmaxf(fabsf(d1), fabsf(d2))) < 1.51f) mask |= 1u << k;
+    }
+    return __builtin_amdgcn_readfirstlane(mask);
 }
 
 // own_contribution() with the slice's own plane gathered from the patch.
@@ -1005,7 +1064,7 @@ __device__ __forceinline__ double wave_sum_f64(double v)
 // One workgroup per (32 x 16 compare-pixel tile, particle x symmetry op
 // slice); eight double sums per particle accumulated with float64 atomics
 // into acc[p][8].
-__global__ void __launch_bounds__(256) k_agree_batch(const float2 *__restrict__ slices, int nsym,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_agree_batch(const float2 *__restrict__ slices, int nsym,
                                                      const float *__restrict__ t3d, const float *__restrict__ wq,
                                                      InsertConst c, float rmax, int sub,
                                                      const float2 *__restrict__ data, const float *__restrict__ norm,
@@ -1021,48 +1080,73 @@ __global__ void __launch_bounds__(256) k_agree_batch(const float2 *__restrict__ 
     const float *t3dp = t3d + 9 * (size_t)p * nsym;
     const float *m = t3d + 9 * (size_t)q;
     const float weight = wq[q];
-    // LDS: patch positions, patch values, fast_exp and fl(i/P) tables
+    // LDS: patch positions and values, mirror patch positions and values,
+    // fast_exp and fl(i/P) tables
     extern __shared__ float4 agree_lds[];
     float4 *ppos = agree_lds;
     float2 *pval = reinterpret_cast<float2 *>(ppos + AT_PATCH);
-    float *etab = reinterpret_cast<float *>(pval + AT_PATCH), *rtab = etab + EXPN;
+    float *etab = reinterpret_cast<float *>(ppos + AT_PATCH + AT_PATCH / 2 + AT_MPATCH + AT_MPATCH / 2);
+    float *rtab = etab + EXPN;
     for (int i = threadIdx.x; i < EXPN; i += blockDim.x) etab[i] = exptab[i];
     for (int i = threadIdx.x; i < P + 2; i += blockDim.x) rtab[i] = rtab_g[i];
     __syncthreads();
     AgreePatch pt{ppos, pval, tx0 - AT_HALO, ty0 - AT_HALO};
-    for (int e = threadIdx.x; e < AT_PATCH; e += blockDim.x) {
-        const int x = pt.x0 + e % AT_PX, y = pt.y0 + e / AT_PX;
-        float4 pos = make_float4(0.f, 0.f, 0.f, 0.f);
-        float2 val = make_float2(0.f, 0.f);
+    __shared__ unsigned short dq[AT_X * AT_Y];  // queued compare pixels (tile index)
+    __shared__ int dn;
+    if (threadIdx.x == 0) dn = 0;
+    // packed floor keys need |x0|, |y0|, |z0| <= 511
+    const bool fastkeys = sub && H <= 510;
+    const unsigned omask = fastkeys ? own_offset_mask(m) : 0u;
+    auto stage = [&](int x, int y, float4 &pos, float2 &val) {
+        pos = make_float4(0.f, 0.f, 0.f, __int_as_float(AP_NOKEY));
+        val = make_float2(0.f, 0.f);
         // the insertion loop's pixels: x in [0, P/2], y in [-P/2, P/2), radius cut (corners off here)
         if (x >= 0 && x <= H && y >= -H && y <= H - 1 && x * x + y * y <= H * H + H) {
             float xx, yy, zz;
             pixel_coords(x, y, m, rtab, H, c.Pf, xx, yy, zz);
             const int x0 = (int)floorf(xx), y0 = (int)floorf(yy), z0 = (int)floorf(zz);
             const bool in = !(x0 < -H - 1 || y0 < -H - 1 || z0 < -H - 1 || x0 > H || y0 > H || z0 > H);
-            pos = make_float4(xx, yy, zz, in ? 1.f : 0.f);
+            pos = make_float4(xx, yy, zz, __int_as_float(in ? ap_key(x0, y0, z0) : AP_NOKEY));
             if (in) val = slice_value(S, x, y, P, H);
         }
-        ppos[e] = pos;
-        pval[e] = val;
-    }
+    };
+    for (int e = threadIdx.x; e < AT_PATCH; e += blockDim.x)
+        stage(pt.x0 + e % AT_PX, pt.y0 + e / AT_PX, ppos[e], pval[e]);
+    float4 *mpos = ppos + AT_PATCH + AT_PATCH / 2;  // after pval
+    float2 *mval = reinterpret_cast<float2 *>(mpos + AT_MPATCH);
+    const int my0 = -ty0 - AT_Y - 1;
+    const bool mirror = fastkeys && tx0 == 0;
+    if (mirror)
+        for (int e = threadIdx.x; e < AT_MPATCH; e += blockDim.x)
+            stage(e % AT_MX - 4, my0 + e / AT_MX, mpos[e], mval[e]);
     __syncthreads();
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int it = 0; it < AT_Y * AT_X / 256; it++) {
-        const int x = tx0 + (threadIdx.x % AT_X), y = ty0 + (int)(threadIdx.x / AT_X) + it * (256 / AT_X);
-        if (x >= W || y >= H || (x == 0 && y == 0)) continue;
+    // One compare pixel.  Outside the general pass a pixel whose voxel needs
+    // the general own-contribution enumeration returns false untouched, to be
+    // queued: the few such pixels of a tile (x < 3, voxels on the x = 0,
+    // x = P/2 and y, z = P/2 planes) then share waves instead of holding back
+    // every wave they fall in.
+    auto pixel = [&](int x, int y, bool general) -> bool {
         const float rx = rtab[x + H], ry = rtab[y + H];
         const float c0 = rx * m[0] + ry * m[3];
         const float c1 = rx * m[1] + ry * m[4];
         const float c2 = rx * m[2] + ry * m[5];
-        if (fabsf(c0) > 0.5f || fabsf(c1) >= 0.5f || fabsf(c2) >= 0.5f) continue;
+        if (fabsf(c0) > 0.5f || fabsf(c1) >= 0.5f || fabsf(c2) >= 0.5f) return true;
         const float xx = c0 * c.Pf, yy = c1 * c.Pf, zz = c2 * c.Pf;
+        const int x0 = round_ref(xx), y0 = round_ref(yy), z0 = round_ref(zz);
+        // Common voxel: |x0|, |y0|, |z0| < P/2 leave one direct and one
+        // mirrored source cell, +-(x0, y0, z0) (on x0 = 0 the mirrored one
+        // through the double insertion's conjugate copy, value only).  From
+        // x >= 3 on, the own plane (pixels x >= 0) cannot reach the mirrored
+        // cell, whose in-plane distance exceeds sqrt(3) + sqrt(3)/2; below,
+        // its sources lie around (-x, -y), in the mirror patch.
+        const bool common = fastkeys && (x >= 3 || mirror) && abs(x0) < H && abs(y0) < H && abs(z0) < H;
+        if (sub && !general && !common) return false;
         const float *Sf = reinterpret_cast<const float *>(S);
         const int slen = (P + 2) * P;
         const int fidx = 2 * x + (P + 2) * (y < 0 ? P + y : y);
         const float d20 = fidx < slen ? Sf[fidx] : 0.f;
         const float d21 = fidx + 1 < slen ? Sf[fidx + 1] : 0.f;
-        const int x0 = round_ref(xx), y0 = round_ref(yy), z0 = round_ref(zz);
         int xs, ysn, zsn;
         if (x0 < 0) {
             xs = -x0;
@@ -1084,7 +1168,32 @@ __global__ void __launch_bounds__(256) k_agree_batch(const float2 *__restrict__ 
         float nv = norm[ci];
         if (sub) {
             float ore, oim, ow;
-            own_contribution_tile(S, t3dp, nsym, own, pt, xs, ysn, zsn, c, weight, etab, rtab, ore, oim, ow);
+            if (general) {
+                own_contribution_tile(S, t3dp, nsym, own, pt, xs, ysn, zsn, c, weight, etab, rtab, ore, oim, ow);
+            } else {
+                const int e = (y - pt.y0) * AT_PX + (x - pt.x0);
+                float r = 0.f, i = 0.f, w = 0.f, r2 = 0.f, i2 = 0.f, w2 = 0.f;
+                for (int sop = 0; sop < nsym; sop++) {
+                    if (sop == own) {
+                        gather_own_offsets<AT_PX>(ppos, pval, e, omask, x0, y0, z0, c, weight, etab, r, i, w);
+                        if (x < 3)  // mirror patch entry of (-x, -y)
+                            gather_own_offsets<AT_MX>(mpos, mval, 4 - x + (-y - my0) * AT_MX, omask, -x0, -y0, -z0,
+                                                      c, weight, etab, r2, i2, w2);
+                    } else {
+                        gather_cell(S, t3dp + 9 * sop, x0, y0, z0, c, weight, etab, rtab, r, i, w);
+                        gather_cell(S, t3dp + 9 * sop, -x0, -y0, -z0, c, weight, etab, rtab, r2, i2, w2);
+                    }
+                }
+                if (x0 >= 0) {  // direct cell (x0, y0, z0), mirrored -(x0, y0, z0)
+                    ore = r + r2;
+                    oim = i - i2;
+                    ow = x0 == 0 ? w : w + w2;
+                } else {
+                    ore = r2 + r;
+                    oim = i2 - i;
+                    ow = w2 + w;
+                }
+            }
             if (inb) {
                 vr = vr - ore;
                 vi = vi - oim;
@@ -1109,6 +1218,18 @@ __global__ void __launch_bounds__(256) k_agree_batch(const float2 *__restrict__ 
             acc[3] += pw2;
             acc[7] += 1.0;
         }
+        return true;
+    };
+    for (int it = 0; it < AT_Y * AT_X / 256; it++) {
+        const int l = threadIdx.x + it * 256;
+        const int x = tx0 + l % AT_X, y = ty0 + l / AT_X;
+        if (x >= W || y >= H || (x == 0 && y == 0)) continue;
+        if (!pixel(x, y, false)) dq[atomicAdd(&dn, 1)] = (unsigned short)l;
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < dn; j += blockDim.x) {
+        const int l = dq[j];
+        pixel(tx0 + l % AT_X, ty0 + l / AT_X, true);
     }
     __shared__ double red[4][8];
 #pragma unroll
@@ -1129,7 +1250,7 @@ hipError_t launch_agree_batch(const float2 *slices, int nsym, const float *t3d, 
     if (nslices <= 0) return hipSuccess;
     c.corners = 0;
     const int ntx = (c.P / 2 + 2 + AT_X - 1) / AT_X, nty = (c.P + AT_Y - 1) / AT_Y;
-    const size_t lds = (sizeof(float4) + sizeof(float2)) * AT_PATCH + sizeof(float) * (EXPN + c.P + 2);
+    const size_t lds = (sizeof(float4) + sizeof(float2)) * (AT_PATCH + AT_MPATCH) + sizeof(float) * (EXPN + c.P + 2);
     hipLaunchKernelGGL(k_agree_batch, dim3(ntx * nty, nslices), dim3(256), lds, s, slices, nsym, t3d, wq, c, rmax, sub,
                        data, norm, exptab, rtab, acc);
     return hipGetLastError();

@@ -673,6 +673,27 @@ def test_agreement_batch_matches_sequential(oracle):
             assert abs(got[i, j] - o[k]) <= 1e-4 * max(1.0, abs(o[k])), (i, k, got[i, j], o[k])
 
 
+@pytest.mark.parametrize("sym,mode,P", [("c4", "gauss_2", 48), ("d2", "trilinear", 32), ("c1", "trilinear", 48)])
+def test_agreement_batch_symmetry_and_modes(oracle, sym, mode, P):
+    """The batched agreement's own-contribution gather under symmetry (the
+    other copies' cells read from global memory) and with trilinear weights,
+    against the oracle's remove / compare / re-add, sub on and off."""
+    B, n = 2 * P // 3, 600
+    ph = S.phantom(B // 2 - 2, B / 4.0, seed=47)
+    imgs, mats, _ = S.particles(ph, n, B, seed=48, snr=0.5)
+    w = np.ones(n, np.float32)
+    r = recon(P, sym, mode)
+    r.insert_batch(imgs, list(mats), w)
+    slices = [oracle.prep_particle(im, P) for im in imgs]
+    v = oracle_volume(oracle, P, sym, mode, slices, mats, w)
+    for sub in (True, False):
+        got = r.agreement_batch(imgs[:10], mats[:10], w[:10], sub=sub, apix=1.0)
+        for i in range(10):
+            o = v.agreement(slices[i], mats[i], 1.0, sub=sub, apix=1.0)
+            for j, k in enumerate(AGREE_KEYS):
+                assert abs(got[i, j] - o[k]) <= 1e-4 * max(1.0, abs(o[k])), (sub, i, k, got[i, j], o[k])
+
+
 def _hip_runtime():
     """The HIP runtime libfourier_hip.so is linked against (already loaded)."""
     import ctypes
