@@ -323,6 +323,8 @@ struct fh_reconstructor {
     float *imgs = nullptr, *means = nullptr, *rots = nullptr, *wts = nullptr, *dfs = nullptr;
     float *dfs2 = nullptr;      // second defocus (defocus + dfdf) of the two-sign flip
     float2 *slices = nullptr;
+    float2 *inter = nullptr;  // column-first prep intermediate of inter_n particles
+    int inter_n = 0;
     float *t3d = nullptr, *wq = nullptr;
     float *rw = nullptr;        // class_ssnr radial weights, (P/2+1) per particle (usessnr)
     float *rw1 = nullptr;       // one slice's class_ssnr radial weights (agreement in the reference's order)
@@ -452,6 +454,8 @@ static void free_chunk(fh_reconstructor *h)
     for (float **p : {&h->imgs, &h->means, &h->rots, &h->wts, &h->dfs, &h->dfs2, &h->t3d, &h->wq, &h->rw})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
     if (h->slices) { (void)hipFree(h->slices); h->slices = nullptr; }
+    if (h->inter) { (void)hipFree(h->inter); h->inter = nullptr; }
+    h->inter_n = 0;
     h->cap_n = h->cap_B = 0;
 }
 
@@ -480,6 +484,14 @@ static void free_volume(fh_reconstructor *h)
     h->ready = false;
 }
 
+// Particles of box P the column-first prep intermediate holds (the prep runs
+// in sub-batches of at most this many).
+constexpr int PREP_INTER_PARTICLES = 4096;
+static size_t prep_inter_bytes(int P, int n)
+{
+    return sizeof(float2) * (size_t)n * (P / 2 + 1) * prep_inter_pitch(P);
+}
+
 static int ensure_chunk(fh_reconstructor *h, int n, int B)
 {
     if (n <= h->cap_n && B <= h->cap_B) return FH_OK;
@@ -493,6 +505,10 @@ static int ensure_chunk(fh_reconstructor *h, int n, int B)
     FH_TRY(h, hipMalloc(&h->dfs, sizeof(float) * n));
     FH_TRY(h, hipMalloc(&h->dfs2, sizeof(float) * n));
     FH_TRY(h, hipMalloc(&h->slices, sizeof(float2) * (size_t)n * P * (H + 1)));
+    if (prep_column_first(P)) {
+        h->inter_n = std::min(n, PREP_INTER_PARTICLES);
+        FH_TRY(h, hipMalloc(&h->inter, prep_inter_bytes(P, h->inter_n)));
+    }
     FH_TRY(h, hipMalloc(&h->t3d, sizeof(float) * 9 * ns));
     FH_TRY(h, hipMalloc(&h->wq, sizeof(float) * ns));
     if (h->usessnr) FH_TRY(h, hipMalloc(&h->rw, sizeof(float) * (size_t)n * (H + 1)));
@@ -632,7 +648,8 @@ long long footprint_bytes_for_mode(int P, int nsym, int chunk, const char *mode_
     // item per 64 pairs of a key plus one per key the slices reach
     const double pb = (double)P / g.bs;
     const long long part_items = (long long)(1.1 * pb * pb * nslices / 64) + g.nkeys / 2;
-    return v + sl + bins + 4LL * g.part_floats * part_items;
+    const long long inter = prep_column_first(P) ? (long long)prep_inter_bytes(P, std::min(chunk, PREP_INTER_PARTICLES)) : 0;
+    return v + sl + inter + bins + 4LL * g.part_floats * part_items;
 }
 
 extern "C" {
@@ -911,31 +928,25 @@ static hipError_t prep_kernels(fh_reconstructor *h, int n, int B, const float *d
         // origin to the centre, 1 / (P^2 scale)), the resampling, the prep
         // again without flip
         const int P = h->P;
-        if ((e = launch_prep_rows(dev_imgs, n, B, P, h->means, h->slices, h->plan, h->tw, h->stream)) != hipSuccess)
-            return e;
-        if ((e = launch_prep_cols(n, B, P, h->slices, dfd, dfd2, cc, h->scale, h->plan, h->tw, h->stream)) !=
-            hipSuccess)
+        if ((e = launch_prep2d(dev_imgs, n, B, P, h->means, h->slices, h->inter, h->inter_n, dfd, dfd2, cc, h->scale,
+                               h->plan, h->tw, h->stream)) != hipSuccess)
             return e;
         const float inv = 1.0f / ((float)P * (float)P * h->scale);
         if ((e = launch_ift2d_batch(h->slices, n, P, h->ximgs, inv, h->plan, h->tw, h->stream)) != hipSuccess) return e;
         if ((e = launch_fill_f32(h->means, 0.f, n, h->stream)) != hipSuccess) return e;
         if ((e = launch_xform2d(h->ximgs, n, P, P, h->means, xfd, h->ximgs2, h->stream)) != hipSuccess) return e;
-        if ((e = launch_prep_rows(h->ximgs2, n, P, P, h->means, h->slices, h->plan, h->tw, h->stream)) != hipSuccess)
-            return e;
-        return launch_prep_cols(n, P, P, h->slices, nullptr, nullptr, cc, h->scale, h->plan, h->tw, h->stream);
+        return launch_prep2d(h->ximgs2, n, P, P, h->means, h->slices, h->inter, h->inter_n, nullptr, nullptr, cc,
+                             h->scale, h->plan, h->tw, h->stream);
     }
     if (xfd) {
         // pad + 2-D resampling into P x P images, then the prep of those (zero mean)
         if ((e = launch_xform2d(dev_imgs, n, B, h->P, h->means, xfd, h->ximgs, h->stream)) != hipSuccess) return e;
         if ((e = launch_fill_f32(h->means, 0.f, n, h->stream)) != hipSuccess) return e;
-        if ((e = launch_prep_rows(h->ximgs, n, h->P, h->P, h->means, h->slices, h->plan, h->tw, h->stream)) !=
-            hipSuccess)
-            return e;
-        return launch_prep_cols(n, h->P, h->P, h->slices, nullptr, nullptr, cc, h->scale, h->plan, h->tw, h->stream);
+        return launch_prep2d(h->ximgs, n, h->P, h->P, h->means, h->slices, h->inter, h->inter_n, nullptr, nullptr, cc,
+                             h->scale, h->plan, h->tw, h->stream);
     }
-    if ((e = launch_prep_rows(dev_imgs, n, B, h->P, h->means, h->slices, h->plan, h->tw, h->stream)) != hipSuccess)
-        return e;
-    return launch_prep_cols(n, B, h->P, h->slices, dfd, dfd2, cc, h->scale, h->plan, h->tw, h->stream);
+    return launch_prep2d(dev_imgs, n, B, h->P, h->means, h->slices, h->inter, h->inter_n, dfd, dfd2, cc, h->scale,
+                         h->plan, h->tw, h->stream);
 }
 
 // staging of n 2-D transformed particles (P x P) and their inverse transforms;

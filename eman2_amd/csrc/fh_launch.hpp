@@ -121,6 +121,11 @@ hipError_t launch_prep_cols(int n, int B, int P, float2 *slices, const float *de
                             CtfConst cc, float scale, const FftPlan &plan, const float2 *tw, hipStream_t s);
 hipError_t launch_ift2d_batch(float2 *data, int n, int P, float *out, float scale, const FftPlan &plan,
                               const float2 *tw, hipStream_t s);
+int prep_inter_pitch(int B);
+bool prep_column_first(int P);
+hipError_t launch_prep2d(const float *imgs, int n, int B, int P, const float *means, float2 *slices,
+                         float2 *inter, int inter_n, const float *defocus, const float *defocus2, CtfConst cc,
+                         float scale, const FftPlan &plan, const float2 *tw, hipStream_t s);
 // ---- finish (prep_kernels.hip) ----
 hipError_t launch_normalize(float2 *data, float *norm, int P, int sqrtnorm, float *plane_tmp,
                             hipStream_t s);

@@ -383,6 +383,228 @@ hipError_t launch_prep_cols(int n, int B, int P, float2 *slices, const float *de
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------- column-first prep
+// The compile-time pads take the 2-D r2c transform columns first, so that
+// the larger second pass reads and writes whole rows.
+// Pass 1 (k_prep_ycols): 2T image columns per block, packed pairwise into T
+// complex transforms along y (padded row ys = (b + off + P/2) mod P of image
+// row b, edge mean subtracted, zero outside the image); the half spectrum
+// ky = 0..P/2 of each real column is split out of the pair and stored to the
+// intermediate I[p][ky][c] (row pitch IP, c = image column), 2T columns of a
+// row per store run.
+// Pass 2 (k_prep_xrows): T intermediate rows per block, transformed along x
+// (padded column xs = (c + off + P/2) mod P); slice row ky takes
+// kx = 0..P/2 and, for 0 < ky < P/2, row P - ky takes the conjugates of
+// kx' = (P - kx) mod P (F(kx, -ky) = conj F(-kx, ky) for a real image), both
+// times the CTF sign and scale of (kx, |ky|), evaluated once for the pair.
+// The same output as k_prep_rows + k_prep_cols up to float rounding.
+// Persistent blocks: the tiles of XCD x (blocks b = x mod 8) are the
+// contiguous range [x ntiles / 8, (x + 1) ntiles / 8), so the tiles of a
+// particle that share cache lines meet in the same L2; each block reads the
+// next tile into registers while it transforms and stores the current one.
+struct TileWalk {
+    int t, end, step;
+    __device__ TileWalk(int ntiles)
+    {
+        const int G8 = gridDim.x / 8, x = blockIdx.x % 8, lb = blockIdx.x / 8;
+        const int lo = (int)((long long)ntiles * x / 8), hi = (int)((long long)ntiles * (x + 1) / 8);
+        t = lo + lb;
+        end = hi;
+        step = G8;
+    }
+};
+
+template <int T, int NF>
+__global__ void __launch_bounds__(256) k_prep_ycols(const float *__restrict__ imgs, int B, int off, int n,
+                                                    const float *__restrict__ means, float2 *__restrict__ inter,
+                                                    int IP, const float2 *__restrict__ twg)
+{
+    constexpr int P = NF, H = P / 2, E = (T * P + 255) / 256;
+    extern __shared__ float2 smem[];
+    float2 *tw = smem;
+    float2 *A = tw + P;
+    load_twiddles(tw, twg, P);
+    const int ng = (B + 2 * T - 1) / (2 * T);
+    TileWalk w(ng * n);
+    // element e of the thread: (ys, t) = ((tid + 256 e) / T, (tid + 256 e) % T), image columns c, c + 1
+    // of padded row ys; slots outside the image hold the mean (zero after the subtraction)
+    float2 v[E];
+    float vmean = 0.f;
+    auto fetch = [&](int tile) {
+        const int p = tile / ng, c0 = (tile - p * ng) * 2 * T;
+        const float mean = means[p];
+        const float *img = imgs + (size_t)p * B * B;
+        vmean = mean;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int idx = threadIdx.x + 256 * e;
+            const int t = idx % T, ys = idx / T;
+            int b = ys - off - H;  // image row of padded row ys, in [-P, P/2)
+            if (b < 0) b += P;
+            const int c = c0 + 2 * t;
+            const bool in = idx < T * P && b < B;
+            v[e].x = in && c < B ? img[(size_t)b * B + c] : mean;
+            v[e].y = in && c + 1 < B ? img[(size_t)b * B + c + 1] : mean;
+        }
+    };
+    if (w.t < w.end) fetch(w.t);
+    for (; w.t < w.end; w.t += w.step) {
+        const int p = w.t / ng, c0 = (w.t - p * ng) * 2 * T;
+        const float mean = vmean;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int idx = threadIdx.x + 256 * e;
+            if (E * 256 == T * P || idx < T * P) A[idx] = make_float2(v[e].x - mean, v[e].y - mean);  // A[ys T + t]
+        }
+        __syncthreads();
+        if (w.t + w.step < w.end) fetch(w.t + w.step);
+        fft_fixed<T, NF>(A, tw);
+        float2 *I = inter + (size_t)p * (H + 1) * IP;
+        for (int idx = threadIdx.x; idx < T * (H + 1); idx += 256) {
+            const int t = idx % T;
+            const int k = idx / T;
+            const int c = c0 + 2 * t;
+            if (c >= B) continue;
+            const float2 z1 = A[k * T + t];
+            const float2 z2 = A[(k == 0 ? 0 : P - k) * T + t];
+            // columns c and c + 1 (for odd B the last pair's second column is
+            // zero and lands in the row padding, IP > B)
+            *reinterpret_cast<float4 *>(I + (size_t)k * IP + c) = make_float4(
+                0.5f * (z1.x + z2.x), 0.5f * (z1.y - z2.y), 0.5f * (z1.y + z2.y), -0.5f * (z1.x - z2.x));
+        }
+        __syncthreads();  // the tile is read before the next one is stored
+    }
+}
+
+template <int T, int NF>
+__global__ void __launch_bounds__(256) k_prep_xrows(int B, int off, int n, const float2 *__restrict__ inter, int IP,
+                                                    float2 *__restrict__ slices, const float *__restrict__ defocus,
+                                                    const float *__restrict__ defocus2, CtfConst cc, float scale,
+                                                    const float2 *__restrict__ twg)
+{
+    constexpr int P = NF, H = P / 2, E = (T * P + 255) / 256, NG = (H + 1 + T - 1) / T;
+    extern __shared__ float2 smem[];
+    float2 *tw = smem;
+    float2 *A = tw + P;
+    load_twiddles(tw, twg, P);
+    TileWalk w(NG * n);
+    // element e of the thread: (xs, t) = ((tid + 256 e) / T, (tid + 256 e) % T): a wave reads runs of
+    // 64 / T columns of T intermediate rows and writes LDS without bank conflicts
+    float2 v[E];
+    auto fetch = [&](int tile) {
+        const int p = tile / NG, k0 = (tile - p * NG) * T;
+        const float2 *I = inter + (size_t)p * (H + 1) * IP;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int idx = threadIdx.x + 256 * e;
+            const int t = idx % T, xs = idx / T;
+            const int k = k0 + t;
+            int c = xs - off - H;  // image column of padded column xs
+            if (c < 0) c += P;
+            v[e] = (idx < T * P && k <= H && c < B) ? I[(size_t)k * IP + c] : make_float2(0.f, 0.f);
+        }
+    };
+    if (w.t < w.end) fetch(w.t);
+    for (; w.t < w.end; w.t += w.step) {
+        const int p = w.t / NG, k0 = (w.t - p * NG) * T;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int idx = threadIdx.x + 256 * e;
+            if (E * 256 == T * P || idx < T * P) A[idx] = v[e];  // A[xs T + t]
+        }
+        __syncthreads();
+        if (w.t + w.step < w.end) fetch(w.t + w.step);
+        fft_fixed<T, NF>(A, tw);
+        const float df = defocus ? defocus[p] : 0.f;
+        const float df2 = defocus2 ? defocus2[p] : 0.f;
+        float2 *sl = slices + (size_t)p * P * (H + 1);
+        for (int idx = threadIdx.x; idx < T * (H + 1); idx += 256) {
+            const int t = idx % T;
+            const int kx = idx / T;
+            const int ky = k0 + t;
+            if (ky > H) continue;
+            const float hyp = defocus ? sqrt_int_rn(kx * kx + ky * ky) : 0.f;  // Util::hypot_fast(x, y)
+            float ss = defocus ? ctf_sign(hyp, df, cc) * scale : scale;
+            if (defocus2) ss = ss * ctf_sign(hyp, df2, cc);
+            const float2 u = A[kx * T + t];
+            sl[(size_t)ky * (H + 1) + kx] = make_float2(u.x * ss, u.y * ss);
+            if (ky > 0 && ky < H) {
+                const float2 q = A[(kx == 0 ? 0 : P - kx) * T + t];
+                sl[(size_t)(P - ky) * (H + 1) + kx] = make_float2(q.x * ss, -q.y * ss);
+            }
+        }
+        __syncthreads();  // the tile is read before the next one is stored
+    }
+}
+
+// Persistent grid of a column-first pass: resident blocks per CU times the
+// CUs, a multiple of 8 (one share per XCD), at most the tiles rounded up to 8.
+template <typename K>
+static int persistent_grid(K kernel, size_t lds, int ntiles)
+{
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 256, lds) != hipSuccess || per < 1) per = 1;
+    const int g = std::min(device_cus() * per, (ntiles + 7) / 8 * 8);
+    return std::max(8, g / 8 * 8);
+}
+
+// Intermediate row pitch of box B (a multiple of 16 columns: 128-byte rows,
+// and one padding column for odd B).
+int prep_inter_pitch(int B) { return (B + 16) & ~15; }
+
+bool prep_column_first(int P)
+{
+#define FH_CF_CASE(TT, NN) if (P == NN) return true;
+    FH_FIXED_PADS(FH_CF_CASE)
+#undef FH_CF_CASE
+    return false;
+}
+
+// The whole prep of n particles: column-first through `inter` (room for
+// inter_n particles of box P) in sub-batches for the compile-time pads, else
+// the row pass and the column pass in place.
+hipError_t launch_prep2d(const float *imgs, int n, int B, int P, const float *means, float2 *slices,
+                         float2 *inter, int inter_n, const float *defocus, const float *defocus2, CtfConst cc,
+                         float scale, const FftPlan &plan, const float2 *tw, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    if (!inter || inter_n <= 0 || !prep_column_first(P) || B > P) {
+        hipError_t e = launch_prep_rows(imgs, n, B, P, means, slices, plan, tw, s);
+        if (e != hipSuccess) return e;
+        return launch_prep_cols(n, B, P, slices, defocus, defocus2, cc, scale, plan, tw, s);
+    }
+    const int d = B - P;
+    const int fl = d >= 0 ? d / 2 : -((-d + 1) / 2);  // (B - P) // 2, as in the row pass
+    const int off = -fl;
+    const int H = P / 2, IP = prep_inter_pitch(B);
+    // particles per sub-batch: inter holds inter_n particles at the pitch of box P
+    const long long room = (long long)inter_n * prep_inter_pitch(P);
+    const int sub = (int)std::max(1LL, room / IP);
+    for (int i0 = 0; i0 < n; i0 += sub) {
+        const int m = std::min(sub, n - i0);
+        const float *im = imgs + (size_t)i0 * B * B;
+        const float *mn = means + i0;
+        float2 *sl = slices + (size_t)i0 * P * (H + 1);
+        const float *df = defocus ? defocus + i0 : nullptr;
+        const float *df2 = defocus2 ? defocus2 + i0 : nullptr;
+#define FH_CF_LAUNCH(TT, NN)                                                                                      \
+        if (P == NN) {                                                                                            \
+            constexpr int T1 = NN <= 384 ? 8 : 4, T2 = 4;                                                         \
+            const size_t l1 = sizeof(float2) * (T1 + 1) * NN, l2 = sizeof(float2) * (T2 + 1) * NN;                \
+            const int g1 = persistent_grid(k_prep_ycols<T1, NN>, l1, (B + 2 * T1 - 1) / (2 * T1) * m);              \
+            const int g2 = persistent_grid(k_prep_xrows<T2, NN>, l2, (H + T2) / T2 * m);                          \
+            hipLaunchKernelGGL((k_prep_ycols<T1, NN>), dim3(g1), dim3(256), l1, s, im, B, off, m, mn, inter, IP, tw); \
+            hipLaunchKernelGGL((k_prep_xrows<T2, NN>), dim3(g2), dim3(256), l2, s, B, off, m, inter, IP, sl, df, df2, \
+                               cc, scale, tw);                                                                    \
+        }
+        FH_FIXED_PADS(FH_CF_LAUNCH)
+#undef FH_CF_LAUNCH
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 // ---------------------------------------------------------------- normalise
 // ReconstructorVolumeData::normalize_threed (libEM/reconstructor.cpp:315-388).
 // The conjugate-plane loop pairs each weight on the x=0 and x=P/2 planes

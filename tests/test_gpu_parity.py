@@ -79,7 +79,8 @@ def test_prep_ctf_sign_many_particles(oracle):
 
 
 @pytest.mark.parametrize("B,P,ctf", [(64, 96, False), (64, 96, True), (63, 96, True), (96, 96, False),
-                                     (256, 384, True), (192, 288, True), (400, 600, False)])
+                                     (256, 384, True), (192, 288, True), (400, 600, False), (320, 480, True),
+                                     (255, 384, True), (97, 192, False), (100, 100, True)])
 def test_prep_batch(oracle, B, P, ctf):
     rng = np.random.default_rng(B + P)
     n = 5
@@ -91,6 +92,21 @@ def test_prep_batch(oracle, B, P, ctf):
         c = dict(CTF, defocus=float(dfs[i])) if ctf else None
         o = oracle.prep_particle(imgs[i], P, c)
         assert max_rel_err(g[i], o) <= 2e-5, (i, max_rel_err(g[i], o))
+
+
+def test_prep_batch_in_sub_batches():
+    """A batch larger than the prep intermediate (4,096 particles at the pad's
+    own box) runs in sub-batches: the same slices as smaller calls."""
+    B = P = 96
+    n = 5000
+    rng = np.random.default_rng(5)
+    imgs = (rng.standard_normal((n, B, B)) + 1.0).astype(np.float32)
+    dfs = rng.uniform(0.8, 2.5, n).astype(np.float32)
+    r = recon(P)
+    g = r.prep_batch(imgs, defocus=dfs, ctf=CTF)
+    for a, b in ((0, 2500), (2500, n)):
+        h = r.prep_batch(imgs[a:b], defocus=dfs[a:b], ctf=CTF)
+        np.testing.assert_array_equal(g[a:b], h)
 
 
 # ------------------------------------------------------------------ golden fixture
