@@ -21,24 +21,20 @@ __device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.
 // (the single-precision hardware sqrt is only accurate to 1 ulp).  Equals
 // hypot((float)x, (float)y) of the reference for integer x^2 + y^2 < 2^24.
 __device__ __forceinline__ float sqrt_rn(float v) { return (float)sqrt((double)v); }
-// The same for an integer 0 <= n < 2^24 without the double sqrt: the
-// hardware sqrt r is corrected to the float whose rounding interval holds
-// sqrt(n), decided exactly in double ((r +- ulp/2)^2 has at most 50
-// significant bits and never equals an integer); the double sqrt stays as
-// the fallback should r be off by more than one ulp.
+// The same for an integer 0 <= n < 2^24 in single precision: the hardware
+// sqrt s (within 1 ulp) is moved to a neighbour when sqrt(n) lies beyond
+// the midpoint m between them.  With s = M 2^E (E <= -11 here), n - m^2 is
+// an odd multiple of h^2 (h = ulp / 2) and s * s_next = m^2 - h^2 (for a
+// power of two the same holds with a smaller h), so the exactly-signed fma
+// residual n - s * s_next decides n against m^2; no midpoint is ever hit.
 __device__ __forceinline__ float sqrt_int_rn(int n)
 {
-    const float r = __builtin_amdgcn_sqrtf((float)n);
-    const double dn = (double)n, dr = (double)r;
-    const double up = 0.5 * (dr + (double)__int_as_float(__float_as_int(r) + 1));
-    const double dw = 0.5 * (dr + (double)__int_as_float(__float_as_int(r) - 1));
-    if (n == 0 || (dw * dw < dn && dn < up * up)) return r;
-    const float ru = __int_as_float(__float_as_int(r) + 1), rd = __int_as_float(__float_as_int(r) - 1);
-    const double uu = 0.5 * ((double)ru + (double)__int_as_float(__float_as_int(ru) + 1));
-    const double dd = 0.5 * ((double)rd + (double)__int_as_float(__float_as_int(rd) - 1));
-    if (dn > up * up && dn < uu * uu) return ru;
-    if (dn < dw * dw && dn > dd * dd) return rd;
-    return sqrt_rn((float)n);
+    const float x = (float)n;
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sd = __int_as_float(__float_as_int(s) - 1), su = __int_as_float(__float_as_int(s) + 1);
+    float r = __fmaf_rn(-sd, s, x) <= 0.f ? sd : s;
+    r = __fmaf_rn(-su, s, x) > 0.f ? su : r;
+    return n == 0 ? 0.f : r;
 }
 // multiply by -i
 __device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }

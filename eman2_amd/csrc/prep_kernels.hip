@@ -278,13 +278,21 @@ hipError_t launch_prep_rows(const float *imgs, int n, int B, int P, const float 
 // operation order (no contraction); the sign of cos is taken in double.
 // cos(t) < 0 exactly when floor(t / pi - 1/2) is even; away from the
 // zeros of cos (|frac| > 1e-9, far above the ~1e-13 error of u) the parity
-// decides, next to them cos itself is evaluated.
+// decides, next to them cos itself is evaluated.  The single-precision
+// u_f = fma(t, fl(1/pi), -1/2) is within 2 |u| 2^-24 < 5e-4 of u for
+// |u| < 4096, so a fraction of u_f more than 1e-3 from an integer already
+// fixes floor(u); only the rest take the double-precision test.
 __device__ __forceinline__ float ctf_sign(float hyp, float defocus, const CtfConst &cc)
 {
 #pragma clang fp contract(off)
     const float s = hyp * cc.ds;
     const float gam = -cc.g1 * s * s * s * s + cc.g2 * defocus * s * s;
-    const double t = (double)(gam - cc.acac);
+    const float tf = gam - cc.acac;
+    const float uf = __fmaf_rn(tf, 0.318309886f, -0.5f);
+    const float flf = floorf(uf);
+    const float frf = uf - flf;
+    if (fabsf(uf) < 4096.f && frf > 1e-3f && frf < 1.f - 1e-3f) return (((int)flf) & 1) == 0 ? -1.0f : 1.0f;
+    const double t = (double)tf;
     const double u = t * 0.31830988618379067154 - 0.5;
     const double fl = floor(u);
     const double fr = u - fl;
