@@ -597,7 +597,7 @@ hipError_t launch_prep2d(const float *imgs, int n, int B, int P, const float *me
         const float *df2 = defocus2 ? defocus2 + i0 : nullptr;
 #define FH_CF_LAUNCH(TT, NN)                                                                                      \
         if (P == NN) {                                                                                            \
-            constexpr int T1 = NN <= 384 ? 8 : 4, T2 = 4;                                                         \
+            constexpr int T1 = 8, T2 = 4;                                                         \
             const size_t l1 = sizeof(float2) * (T1 + 1) * NN, l2 = sizeof(float2) * (T2 + 1) * NN;                \
             const int g1 = persistent_grid(k_prep_ycols<T1, NN>, l1, (B + 2 * T1 - 1) / (2 * T1) * m);              \
             const int g2 = persistent_grid(k_prep_xrows<T2, NN>, l2, (H + T2) / T2 * m);                          \
