@@ -527,8 +527,8 @@ __global__ void __launch_bounds__(256) k_prep_xrows(int B, int off, int n, const
         const float df2 = defocus2 ? defocus2[p] : 0.f;
         float2 *sl = slices + (size_t)p * P * (H + 1);
         for (int idx = threadIdx.x; idx < T * (H + 1); idx += 256) {
-            const int t = idx % T;
-            const int kx = idx / T;
+            const int t = idx / (H + 1);  // lanes along a slice row: whole-row store runs
+            const int kx = idx - t * (H + 1);
             const int ky = k0 + t;
             if (ky > H) continue;
             const float hyp = defocus ? sqrt_int_rn(kx * kx + ky * ky) : 0.f;  // Util::hypot_fast(x, y)
