@@ -968,13 +968,26 @@ __device__ __forceinline__ void gather_own_offsets(const float4 *pos, const floa
     for (unsigned mm = omask; mm != 0u; mm &= mm - 1u) {  // set bits in increasing order
         const int k = __builtin_ctz(mm);
         const int o = e + (k / 5 - 2) * PITCH + (k % 5 - 2);
+        // position, key and value read together; the weight is corner_weight()'s
+        // arithmetic with the table index clamped for the misses it is computed for
         const float4 q = pos[o];
-        if (((unsigned)(kc - __float_as_int(q.w)) & ~AP_DIGITS) != 0u) continue;
-        const float gg = corner_weight(c.mode, ci, cj, ck, q.x, q.y, q.z, c.h, weight, etab);
         const float2 dt = val[o];
-        re += dt.x * gg;
-        im += dt.y * gg;
-        w += gg;
+        const bool hit = ((unsigned)(kc - __float_as_int(q.w)) & ~AP_DIGITS) == 0u;
+        const float dx = (float)ci - q.x, dy = (float)cj - q.y, dz = (float)ck - q.z;
+        float gg;
+        if (c.mode == 0) {
+            const float r = dx * dx + dy * dy + dz * dz;
+            const float f = r * c.h;
+            gg = etab[min((int)__builtin_fma((double)f, 50.0, 0.5), EXPN - 1)] * weight;
+        } else {
+            gg = (float)((1.0 - (double)fabsf(dx)) * (1.0 - (double)fabsf(dy)) * (1.0 - (double)fabsf(dz)) *
+                         (double)weight);
+        }
+        if (hit) {
+            re += dt.x * gg;
+            im += dt.y * gg;
+            w += gg;
+        }
     }
 }
 
