@@ -28,6 +28,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 ENGINE_CLOCK_HZ = 2.4e9  # MI355X peak engine clock
+LDS_RANDOM_RMW_PER_CLK = 4.2  # plain LDS read-modify-writes per clock per CU at random addresses (microbenchmark)
 CTF = dict(voltage=300.0, cs=2.7, ampcont=10.0, bfactor=80.0)
 
 
@@ -396,6 +397,17 @@ def main():
                 "unit": "G contributions/s", "frac": round(gcs / peak, 4) if peak else None, "traffic": None,
                 "kernel": kname, "avg_launch_ms": round(avg_ms, 3), "contributions_per_launch": int(contrib),
                 "scatter_model_GBs": round(achieved, 1)}
+    # the gauss_2 / trilinear scatter's own limiter is the LDS: three plain
+    # read-modify-writes (re, im, weight) per contribution against the
+    # random-address read-modify-write rate of one CU's LDS that a
+    # microbenchmark on this GPU measured (4.2 per clock)
+    lds = None
+    if not wide and avg_ms > 0:
+        cus = torch.cuda.get_device_properties(0).multi_processor_count
+        rmw = 3 * nsym * npix(P) * 8 * (a.steps * n) / max(1, launches)
+        rate = rmw / (avg_ms * 1e-3) / (cus * ENGINE_CLOCK_HZ)
+        lds = {"rmw_per_clk_per_cu": round(rate, 3), "random_rmw_ceiling": LDS_RANDOM_RMW_PER_CLK,
+               "frac": round(rate / LDS_RANDOM_RMW_PER_CLK, 4), "rmw_per_launch": int(rmw)}
     # whole-pipeline effective bandwidth: the scatter-model bytes plus the 4*B^2 raw read
     pipe_bytes = (4 * B * B + bytes_per_particle) * a.steps * n
     pipeline = {"effective_GBs": round(pipe_bytes / elapsed / 1e9, 1),
@@ -417,6 +429,7 @@ def main():
             n, B, P, a.sym, a.mode, a.snr), "box": B, "pad": P, "particles": n, "sym": a.sym, "mode": a.mode,
             "tuning": r.get_tuning()},
         "roofline": roof,
+        "scatter_lds": lds,
         "pipeline_roofline": pipeline,
         "compulsory_bound": compulsory,
         "job_latency_ms": {"p50": round(statistics.median(lat), 2),
