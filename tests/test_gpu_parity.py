@@ -80,7 +80,8 @@ def test_prep_ctf_sign_many_particles(oracle):
 
 @pytest.mark.parametrize("B,P,ctf", [(64, 96, False), (64, 96, True), (63, 96, True), (96, 96, False),
                                      (256, 384, True), (192, 288, True), (400, 600, False), (320, 480, True),
-                                     (255, 384, True), (97, 192, False), (100, 100, True)])
+                                     (255, 384, True), (97, 192, False), (100, 100, True), (2, 96, True),
+                                     (3, 96, False), (288, 288, True)])
 def test_prep_batch(oracle, B, P, ctf):
     rng = np.random.default_rng(B + P)
     n = 5
