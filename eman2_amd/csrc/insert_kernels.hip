@@ -992,16 +992,18 @@ __device__ __forceinline__ void gather_own_offsets(const float4 *pos, const floa
 }
 
 // Offsets (ox, oy) in [-2, 2]^2 whose step ox * row0 + oy * row1 of slice
-// rows m stays under 1.5 (plus rounding slack) on every axis.
+// rows m stays under 1.5 (plus rounding slack) on every axis: lane k < 25
+// tests offset k, the wave's ballot is the mask (the whole wave active).
 __device__ __forceinline__ unsigned own_offset_mask(const float *m)
 {
-    unsigned mask = 0;
-    for (int k = 0; k < 25; k++) {
+    const int k = threadIdx.x & 63;
+    bool ok = false;
+    if (k < 25) {
         const float ox = (float)(k % 5 - 2), oy = (float)(k / 5 - 2);
         const float d0 = ox * m[0] + oy * m[3], d1 = ox * m[1] + oy * m[4], d2 = ox * m[2] + oy * m[5];
-        if (fmaxf(fabsf(d0), fmaxf(fabsf(d1), fabsf(d2))) < 1.51f) mask |= 1u << k;
+        ok = fmaxf(fabsf(d0), fmaxf(fabsf(d1), fabsf(d2))) < 1.51f;
     }
-    return __builtin_amdgcn_readfirstlane(mask);
+    return (unsigned)__ballot(ok);
 }
 
 // own_contribution() with the slice's own plane gathered from the patch.
