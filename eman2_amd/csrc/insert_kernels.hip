@@ -1068,14 +1068,6 @@ __device__ void own_contribution_tile(const float2 *S, const float *t3dp, int ns
         }
 }
 
-// Sum of a double over the wave (xor butterflies on 32-bit halves).
-__device__ __forceinline__ double wave_sum_f64(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
 // One workgroup per (32 x 16 compare-pixel tile, particle x symmetry op
 // slice); eight double sums per particle accumulated with float64 atomics
 // into acc[p][8].
@@ -1246,16 +1238,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
         const int l = dq[j];
         pixel(tx0 + l % AT_X, ty0 + l / AT_X, true);
     }
-    __shared__ double red[4][8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        const double v = wave_sum_f64(acc[k]);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][k] = v;
-    }
+    // block sums through LDS (the patch's space, no longer read): 32 threads
+    // per sum add 8 of the 256 partials each, then a 32-lane butterfly
     __syncthreads();
-    if (threadIdx.x < 8)
-        unsafeAtomicAdd(&acc_out[(size_t)p * 8 + threadIdx.x],
-                        red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x]);
+    static_assert((sizeof(float4) + sizeof(float2)) * AT_PATCH >= 8 * 256 * sizeof(double), "block sums");
+    double *red = reinterpret_cast<double *>(agree_lds);  // 8 x 256 doubles within the patch's bytes
+#pragma unroll
+    for (int k = 0; k < 8; k++) red[k * 256 + threadIdx.x] = acc[k];
+    __syncthreads();
+    const int k = threadIdx.x >> 5, l = threadIdx.x & 31;
+    double v = 0.0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) v += red[k * 256 + l + 32 * i];
+#pragma unroll
+    for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (l == 0) unsafeAtomicAdd(&acc_out[(size_t)p * 8 + k], v);
 }
 
 hipError_t launch_agree_batch(const float2 *slices, int nsym, const float *t3d, const float *wq, int nslices,
