@@ -486,7 +486,7 @@ static void free_volume(fh_reconstructor *h)
 
 // Particles of box P the column-first prep intermediate holds (the prep runs
 // in sub-batches of at most this many).
-constexpr int PREP_INTER_PARTICLES = 4096;
+constexpr int PREP_INTER_PARTICLES = 2048;
 static size_t prep_inter_bytes(int P, int n)
 {
     return sizeof(float2) * (size_t)n * (P / 2 + 1) * prep_inter_pitch(P);

@@ -96,7 +96,7 @@ def test_prep_batch(oracle, B, P, ctf):
 
 
 def test_prep_batch_in_sub_batches():
-    """A batch larger than the prep intermediate (4,096 particles at the pad's
+    """A batch larger than the prep intermediate (2,048 particles at the pad's
     own box) runs in sub-batches: the same slices as smaller calls."""
     B = P = 96
     n = 5000
