@@ -526,19 +526,25 @@ __global__ void __launch_bounds__(256) k_prep_xrows(int B, int off, int n, const
         const float df = defocus ? defocus[p] : 0.f;
         const float df2 = defocus2 ? defocus2[p] : 0.f;
         float2 *sl = slices + (size_t)p * P * (H + 1);
-        for (int idx = threadIdx.x; idx < T * (H + 1); idx += 256) {
-            const int t = idx / (H + 1);  // lanes along a slice row: whole-row store runs
-            const int kx = idx - t * (H + 1);
-            const int ky = k0 + t;
-            if (ky > H) continue;
-            const float hyp = defocus ? sqrt_int_rn(kx * kx + ky * ky) : 0.f;  // Util::hypot_fast(x, y)
-            float ss = defocus ? ctf_sign(hyp, df, cc) * scale : scale;
-            if (defocus2) ss = ss * ctf_sign(hyp, df2, cc);
-            const float2 u = A[kx * T + t];
-            sl[(size_t)ky * (H + 1) + kx] = make_float2(u.x * ss, u.y * ss);
-            if (ky > 0 && ky < H) {
-                const float2 q = A[(kx == 0 ? 0 : P - kx) * T + t];
-                sl[(size_t)(P - ky) * (H + 1) + kx] = make_float2(q.x * ss, -q.y * ss);
+        // two rows per lane (one 16-byte LDS read per column), lanes along the
+        // rows: whole-row store runs
+        static_assert(T % 2 == 0, "row pairs");
+        for (int idx = threadIdx.x; idx < (T / 2) * (H + 1); idx += 256) {
+            const int t0 = 2 * (idx / (H + 1));
+            const int kx = idx - (t0 / 2) * (H + 1);
+            const float4 u2 = *reinterpret_cast<const float4 *>(A + kx * T + t0);
+            const float4 q2 = *reinterpret_cast<const float4 *>(A + (kx == 0 ? 0 : P - kx) * T + t0);
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int ky = k0 + t0 + h;
+                if (ky > H) continue;
+                const float hyp = defocus ? sqrt_int_rn(kx * kx + ky * ky) : 0.f;  // Util::hypot_fast(x, y)
+                float ss = defocus ? ctf_sign(hyp, df, cc) * scale : scale;
+                if (defocus2) ss = ss * ctf_sign(hyp, df2, cc);
+                const float ux = h ? u2.z : u2.x, uy = h ? u2.w : u2.y;
+                const float qx = h ? q2.z : q2.x, qy = h ? q2.w : q2.y;
+                sl[(size_t)ky * (H + 1) + kx] = make_float2(ux * ss, uy * ss);
+                if (ky > 0 && ky < H) sl[(size_t)(P - ky) * (H + 1) + kx] = make_float2(qx * ss, -qy * ss);
             }
         }
         __syncthreads();  // the tile is read before the next one is stored
