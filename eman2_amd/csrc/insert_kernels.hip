@@ -1227,16 +1227,35 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
         }
         return true;
     };
+    // tile slot l -> pixel; in the x = 0 tiles the first 48 slots are the
+    // columns x < 3 of all 16 rows, so that only one wave runs their mirrored
+    // gather
+    auto slot = [&](int l, int &x, int &y) {
+        if (tx0 == 0) {
+            if (l < 3 * AT_Y) {
+                x = l % 3;
+                y = ty0 + l / 3;
+            } else {
+                x = 3 + (l - 3 * AT_Y) % (AT_X - 3);
+                y = ty0 + (l - 3 * AT_Y) / (AT_X - 3);
+            }
+        } else {
+            x = tx0 + l % AT_X;
+            y = ty0 + l / AT_X;
+        }
+    };
     for (int it = 0; it < AT_Y * AT_X / 256; it++) {
         const int l = threadIdx.x + it * 256;
-        const int x = tx0 + l % AT_X, y = ty0 + l / AT_X;
+        int x, y;
+        slot(l, x, y);
         if (x >= W || y >= H || (x == 0 && y == 0)) continue;
         if (!pixel(x, y, false)) dq[atomicAdd(&dn, 1)] = (unsigned short)l;
     }
     __syncthreads();
     for (int j = threadIdx.x; j < dn; j += blockDim.x) {
-        const int l = dq[j];
-        pixel(tx0 + l % AT_X, ty0 + l / AT_X, true);
+        int x, y;
+        slot(dq[j], x, y);
+        pixel(x, y, true);
     }
     // block sums through LDS (the patch's space, no longer read): 32 threads
     // per sum add 8 of the 256 partials each, then a 32-lane butterfly
